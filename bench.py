@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Throughput of the fused compositing path on one MI355X.
+
+Default workload: the 7680x4320 fused colour + 'over' pipeline (RGBA8 sRGB in
+-> Rec.709-to-Display-P3 -> Oklab -> chroma x1.2 -> back -> 'over' an
+RGBA16F layer -> RGBA8 out), inputs resident in HBM, one step = one launch
+of the compiled program over one frame.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpix/s + HBM GB/s (% peak); surrogate vs plain color chain at ΔE_OK<0.005"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def device_inputs(wl, ctx, torch, seed=0):
+    """Synthetic inputs written straight into HBM (uniform random bytes for
+    RGBA8, uniform [0, 1] values for RGBA16F), wrapped as backend images."""
+    import zosimos_amd as z
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    keep, imgs = [], {}
+    for reg, d in wl.inputs.items():
+        n = d.nbytes
+        if d.texel.bits == z.SampleBits.Float16x4:
+            t = torch.rand(n // 2, device="cuda", dtype=torch.float32, generator=g).to(torch.float16)
+        else:
+            t = torch.randint(0, 256, (n,), device="cuda", dtype=torch.uint8, generator=g)
+        keep.append(t)
+        imgs[reg] = ctx.wrap(d, t.data_ptr(), n)
+    outs = {}
+    for reg, d in wl.outputs.items():
+        t = torch.empty(d.nbytes, device="cuda", dtype=torch.uint8)
+        keep.append(t)
+        outs[reg] = ctx.wrap(d, t.data_ptr(), d.nbytes)
+    torch.cuda.synchronize()
+    return imgs, outs, keep
+
+
+def cpu_baseline(config: str):
+    """The FP64 numpy oracle (a port of the path, one host core) timed on a
+    bounded sample of the same workload."""
+    from oracle import pipelines as PL
+    rng = np.random.default_rng(0)
+    if config == "c2":
+        w, h = 1024, 512
+        src = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        layer = rng.random((h, w, 4)).astype(np.float16).astype(np.float64)
+        fn = lambda: PL.c2(src, layer)
+        sample = f"{w}x{h} pixels of the workload's chain (FP64 numpy oracle)"
+    elif config == "c1":
+        w, h = 1024, 768
+        a = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        b = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        from zosimos_amd import workloads as W
+        m = W.about_centre(w, h, 0.75, np.radians(15.0)).transformation
+        fn = lambda: PL.c1(a, b, m, (h, w))
+        sample = "one full 1024x768 frame (FP64 numpy oracle)"
+    else:
+        w, h = 1024, 512
+        src = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        fn = lambda: PL.c4(src)
+        sample = f"{w}x{h} pixels of the colour chain (FP64 numpy oracle)"
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        fn()
+        reps += 1
+        if time.perf_counter() - t0 > 10.0:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(reps * w * h / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} x {sample}, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import zosimos_amd as z
+    from zosimos_amd import workloads as W
+
+    wl = W.CONFIGS[args.config]()
+    stream = torch.cuda.Stream()  # default priority
+    ctx = z.Context(0, stream=stream.cuda_stream)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    launches, alg_bytes = exe.stats()
+    imgs, outs, keep = device_inputs(wl, ctx, torch)
+
+    for _ in range(args.warmup):
+        exe.launch(imgs, outs)
+    ctx.sync()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(args.steps):
+            exe.launch(imgs, outs)
+        end.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end) / args.steps
+    value = wl.pixels / (ms * 1e-3) / 1e6
+
+    # Dominant kernel: HIP events around every launch unit on the context's
+    # stream, averaged over the same number of launches.
+    unit_ms = exe.time_stages(imgs, outs, iterations=max(10, args.steps))
+    dom = int(np.argmax(unit_ms))
+    text = exe.describe()
+    stage_lines = [l for l in text.splitlines() if l.startswith("stage ")]
+    # algorithmic bytes of the dominant unit (all stages when batched)
+    if launches == 1:
+        dom_bytes = alg_bytes
+    else:
+        dom_bytes = alg_bytes // launches
+    achieved = dom_bytes / (unit_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(wl.name)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "pixels_per_step": wl.pixels, "launches_per_step": launches,
+                   "algorithmic_bytes_per_step": alg_bytes},
+        "hbm_gbps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": stage_lines[0] if stage_lines else "", "kernel_ms": round(unit_ms[dom], 5)},
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.config)
+    print(json.dumps(line), flush=True)
+    del imgs, outs
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

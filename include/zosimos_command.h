@@ -1,0 +1,82 @@
+/* zosimos_command.h -- C ABI of the C++ mirror of zosimos's front end
+ * (CommandBuffer, Linker::compile, Program::lower_to).
+ *
+ * The reference builds programs in Rust (lib/zosimos/src/command.rs) and
+ * hands the linked High IR to the backend.  No Rust toolchain exists in this
+ * build's image, so the front end is mirrored here in C++ with the same
+ * operation names, argument meaning and error classes; it emits exactly the
+ * zh_op list of include/zosimos_hip.h that a Rust adapter would.  Register
+ * handles are plain uint32_t indices (command.rs:31 `Register(usize)`).
+ */
+#ifndef ZOSIMOS_COMMAND_H
+#define ZOSIMOS_COMMAND_H
+
+#include "zosimos_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zs_command zs_command; /* CommandBuffer, command.rs:60-68 */
+typedef struct zs_linked zs_linked;   /* linked Program, program.rs:24-39 */
+
+/* AffineSample, command.rs:313-327 */
+#define ZS_AFFINE_NEAREST 0
+#define ZS_AFFINE_BILINEAR 1
+/* Blend, command.rs:287-291 */
+#define ZS_BLEND_ALPHA 0
+
+zh_status zs_command_create(zs_command** out);
+zh_status zs_command_destroy(zs_command* cmd);
+
+/* CommandBuffer::input, command.rs:743-751 */
+zh_status zs_command_input(zs_command* cmd, const zh_descriptor* desc, uint32_t* reg);
+/* CommandBuffer::output, command.rs:1707-1712; `outformat` may be NULL */
+zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat);
+/* CommandBuffer::crop, command.rs:971-978 (result keeps the source size) */
+zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg);
+/* CommandBuffer::color_convert, command.rs:986-1107 */
+zh_status zs_command_color_convert(zs_command* cmd, uint32_t src, const zh_color* color, const zh_texel* texel,
+                                   uint32_t* reg);
+/* CommandBuffer::inscribe, command.rs:1177-1218 */
+zh_status zs_command_inscribe(zs_command* cmd, uint32_t below, zh_rect rect, uint32_t above, uint32_t* reg);
+/* CommandBuffer::transmute, command.rs:1276-1350 */
+zh_status zs_command_transmute(zs_command* cmd, uint32_t src, const zh_descriptor* target, uint32_t* reg);
+/* CommandBuffer::affine, command.rs:1636-1673.  `matrix` is Affine::transformation
+ * (row-major, command.rs:287-296).  BiLinear (UNIMPLEMENTED in the reference)
+ * samples premultiplied texels bilinearly with a transparent border and
+ * composites them 'over' `below`. */
+zh_status zs_command_affine(zs_command* cmd, uint32_t below, const float matrix[9], uint32_t sampling,
+                            uint32_t above, uint32_t* reg);
+/* CommandBuffer::blend, command.rs:1510-1519 (UNIMPLEMENTED in the reference):
+ * Porter-Duff 'over' of `above` placed at `rect` onto `below`. */
+zh_status zs_command_blend(zs_command* cmd, uint32_t below, zh_rect rect, uint32_t above, uint32_t blend,
+                           uint32_t* reg);
+/* CommandBuffer::solid_rgba, command.rs:1524-1551 */
+zh_status zs_command_solid_rgba(zs_command* cmd, const zh_descriptor* desc, const float color[4], uint32_t* reg);
+/* Oklab chroma scale and hue rotation (radians) on an Oklab register; no
+ * counterpart in the reference. */
+zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg);
+/* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
+ * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob. */
+zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);
+/* Descriptor of a register (RegisterDescription::Texture). */
+zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc);
+
+/* Linker::compile, command.rs:2069-2071 and link_in's High emission. */
+zh_status zs_command_compile(zs_command* cmd, zs_linked** out);
+zh_status zs_linked_destroy(zs_linked* prog);
+/* Borrow the High list (valid until zs_linked_destroy). */
+zh_status zs_linked_high(zs_linked* prog, const zh_descriptor** textures, uint32_t* num_textures, const zh_op** ops,
+                         uint32_t* num_ops);
+/* Binding slot of an input / output register. */
+zh_status zs_linked_input_slot(zs_linked* prog, uint32_t reg, uint32_t* slot);
+zh_status zs_linked_output_slot(zs_linked* prog, uint32_t reg, uint32_t* slot);
+/* Program::lower_to (program.rs:1304): plan for `ctx` (NULL = plan only). */
+zh_status zs_linked_lower(zs_linked* prog, zh_context* ctx, uint32_t flags, zh_program** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZOSIMOS_COMMAND_H */

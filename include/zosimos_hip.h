@@ -1,0 +1,251 @@
+/* zosimos_hip.h -- C ABI of the MI355X execution backend for zosimos programs.
+ *
+ * This is the seam a `zosimos-hip-sys` crate binds.  It replaces, below the
+ * High IR, the reference's lowering and wgpu execution:
+ *   Program::lower_to                 lib/zosimos/src/program.rs:1304-1423
+ *   Encoder (High -> Low for wgpu)    lib/zosimos/src/program/encoder.rs
+ *   Executable/Environment/Execution  lib/zosimos/src/run.rs:38-1499
+ *   SyncPoint::block_on               lib/zosimos/src/run.rs:3015-3029
+ * The op list handed to zh_program_build is the High IR of program.rs:88-240
+ * (one zh_op per High instruction), with the std140 parameter blobs that
+ * FragmentShaderData::binary_data writes (shaders.rs:63-85).
+ *
+ * All functions return a zh_status (0 = ZH_OK).  Nothing throws or aborts
+ * across the boundary; zh_last_error() returns the message of the last
+ * failure on the calling thread.  Contexts are externally synchronised: one
+ * host thread at a time per context, like stepping an Execution.
+ */
+#ifndef ZOSIMOS_HIP_H
+#define ZOSIMOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t zh_status;
+#define ZH_OK 0
+#define ZH_ERR_INVALID_ARGUMENT 1   /* bad pointer, size or id */
+#define ZH_ERR_UNSUPPORTED 2        /* valid request the backend cannot run */
+#define ZH_ERR_HIP 3                /* a HIP runtime call failed */
+#define ZH_ERR_MISMATCHED_DESCRIPTOR 4 /* StartError::MismatchedDescriptor, run.rs:369-392 */
+#define ZH_ERR_MISSING_KEY 5        /* StartError::MissingKey: an input is unbound */
+#define ZH_ERR_OUT_OF_MEMORY 6
+#define ZH_ERR_TYPE 7               /* CommandError type errors, command.rs:3617-3630 */
+#define ZH_ERR_UNIMPLEMENTED 8      /* CommandError::UNIMPLEMENTED */
+#define ZH_ERR_KNOB_LENGTH 9        /* knob length mismatch, run.rs:1295-1298 */
+#define ZH_ERR_NO_DEVICE 10         /* a device call on a context-free program */
+#define ZH_ERR_OTHER 11             /* CommandError::OTHER */
+
+/* ---- identifiers ------------------------------------------------------
+ * SAMPLE_BITS ids   = lib/std/src/stage.frag:151-170 (std140 XyzParameter.z)
+ * SAMPLE_PARTS ids  = lib/std/src/stage.frag:125-145 (XyzParameter.y)
+ * TRANSFER ids      = lib/std/src/stage.frag:107-119 (XyzParameter.x)
+ */
+#define ZH_BITS_INT8 0
+#define ZH_BITS_INT16 3
+#define ZH_BITS_INT8X4 10
+#define ZH_BITS_INT16X4 13
+#define ZH_BITS_FLOAT16X4 18
+#define ZH_BITS_FLOAT32X4 19
+
+#define ZH_PARTS_RGBA 8
+#define ZH_PARTS_BGRA 10
+#define ZH_PARTS_LCHA 20
+
+#define ZH_TRANSFER_BT709 0
+#define ZH_TRANSFER_LINEAR 4
+#define ZH_TRANSFER_SRGB 5
+#define ZH_TRANSFER_LABLCH 0x100
+
+/* Color models (image-canvas Color, lib/lang/src/serde_bindings.rs:16-32). */
+#define ZH_COLOR_RGB 0
+#define ZH_COLOR_OKLAB 1
+#define ZH_COLOR_SCALARS 2
+
+/* Primaries: image-canvas order (serde_bindings.rs:65-75) + Display-P3. */
+#define ZH_PRIMARIES_XYZ 0
+#define ZH_PRIMARIES_BT601_525 1
+#define ZH_PRIMARIES_BT601_625 2
+#define ZH_PRIMARIES_BT709 3
+#define ZH_PRIMARIES_SMPTE240 4
+#define ZH_PRIMARIES_BT2020 5
+#define ZH_PRIMARIES_BT2100 6
+#define ZH_PRIMARIES_DISPLAY_P3 7
+
+/* White points (image-canvas Whitepoint incl. D50 used at tests/blend.rs:180). */
+#define ZH_WHITEPOINT_D50 3
+#define ZH_WHITEPOINT_D65 5
+
+typedef struct zh_texel {
+  uint32_t bits;   /* SAMPLE_BITS id */
+  uint32_t parts;  /* SAMPLE_PARTS id */
+} zh_texel;
+
+typedef struct zh_color {
+  uint32_t model;      /* ZH_COLOR_* */
+  uint32_t primaries;  /* ZH_PRIMARIES_* (RGB only) */
+  uint32_t transfer;   /* TRANSFER id (RGB and Scalars) */
+  uint32_t whitepoint; /* ZH_WHITEPOINT_* (RGB only) */
+  uint32_t luminance;  /* image-canvas Luminance, carried only */
+} zh_color;
+
+/* Descriptor, lib/zosimos/src/buffer.rs:13-30.  row_stride 0 = tight rows. */
+typedef struct zh_descriptor {
+  uint32_t width;
+  uint32_t height;
+  uint64_t row_stride;
+  uint32_t texel_stride; /* bytes per texel (SampleBits::bytes) */
+  zh_texel texel;
+  zh_color color;
+} zh_descriptor;
+
+/* Rectangle, command.rs:279-285: [x, max_x) x [y, max_y). */
+typedef struct zh_rect {
+  uint32_t x, y, max_x, max_y;
+} zh_rect;
+
+/* ---- context and images ------------------------------------------------ */
+typedef struct zh_context zh_context;
+typedef struct zh_image zh_image;
+typedef struct zh_program zh_program;
+
+/* One HIP device and one stream.  `stream` may be NULL (the context creates
+ * its own default-priority stream) or an existing hipStream_t to launch on. */
+zh_status zh_context_create(int32_t device, void* stream, zh_context** out);
+zh_status zh_context_destroy(zh_context* ctx);
+/* Block until all work queued on the context's stream is done
+ * (SyncPoint::block_on, run.rs:3019-3029). */
+zh_status zh_sync(zh_context* ctx);
+/* 1 if work is still queued, 0 if idle (Execution::is_running, run.rs:1352). */
+zh_status zh_query(zh_context* ctx, int32_t* running);
+/* The context's hipStream_t. */
+void* zh_context_stream(zh_context* ctx);
+
+/* Device image with the layout of `desc` (row_stride 0 = tight).  The
+ * context owns the memory (pool.rs:122-156 ImageData::GpuBuffer analogue). */
+zh_status zh_image_alloc(zh_context* ctx, const zh_descriptor* desc, zh_image** out);
+/* Wrap caller-owned device memory (e.g. a torch tensor); not freed by us. */
+zh_status zh_image_wrap(zh_context* ctx, const zh_descriptor* desc, void* device_ptr,
+                        size_t bytes, zh_image** out);
+zh_status zh_image_free(zh_image* img);
+zh_status zh_image_device_ptr(zh_image* img, void** ptr, size_t* bytes);
+/* Host <-> device copies with pitch (replaces the row-by-row memcpy of
+ * copy_host_to_buffer, run.rs:3282-3309).  Asynchronous on the context
+ * stream; host memory must stay valid until zh_sync. */
+zh_status zh_image_upload(zh_image* img, const void* host, uint64_t host_row_stride);
+zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride);
+
+/* ---- High IR --------------------------------------------------------- */
+/* High variants, program.rs:88-139. */
+#define ZH_HIGH_INPUT 0   /* High::Input: texture `dst` <- binding slot `src` */
+#define ZH_HIGH_OUTPUT 1  /* High::Output: binding slot `dst` <- texture `src` */
+#define ZH_HIGH_DRAW 2    /* High::PushOperand* + High::DrawInto */
+#define ZH_HIGH_COPY 3    /* High::Copy: byte copy `src` -> `dst` */
+#define ZH_HIGH_UNINIT 4  /* High::Uninit */
+
+/* Target, program.rs:142-148. */
+#define ZH_TARGET_DISCARD 0 /* cleared first (to opaque blue, program.rs:1495-1499) */
+#define ZH_TARGET_LOAD 1
+
+/* Initializer, program.rs:176-240. */
+#define ZH_INIT_PAINT_FULLSCREEN 0
+#define ZH_INIT_PAINT_TO_SELECTION 1
+
+/* Shader keys: FragmentShaderKey (shaders.rs:116-143) and the ops the
+ * reference lacks.  Parameter blobs (std140):
+ *  PAINT_COPY            none (copy.frag)
+ *  LINEAR_COLOR_MATRIX   mat3 std140, 48 B (color_matrix.rs:151-161)
+ *  OKLAB_ENCODE/DECODE   mat3 std140, 48 B (shaders/oklab.rs:36-39)
+ *  SOLID_RGB             vec4, 16 B (solid_rgb.frag)
+ *  OKLAB_ADJUST          vec4 {chroma scale, hue rotation (rad), 0, 0}
+ *  BLEND_OVER            none; operands (below, above)
+ *  AFFINE_BILINEAR       mat3 std140 of the row-major Affine matrix, 48 B;
+ *                        operand = layer, composited over the target (Load)
+ *  PREMULTIPLY, UNPREMULTIPLY  none
+ */
+#define ZH_SHADER_PAINT_COPY 0
+#define ZH_SHADER_LINEAR_COLOR_MATRIX 1
+#define ZH_SHADER_OKLAB_ENCODE 2
+#define ZH_SHADER_OKLAB_DECODE 3
+#define ZH_SHADER_SOLID_RGB 4
+#define ZH_SHADER_OKLAB_ADJUST 16
+#define ZH_SHADER_BLEND_OVER 17
+#define ZH_SHADER_AFFINE_BILINEAR 18
+#define ZH_SHADER_PREMULTIPLY 19
+#define ZH_SHADER_UNPREMULTIPLY 20
+
+typedef struct zh_op {
+  uint32_t high;        /* ZH_HIGH_* */
+  uint32_t dst;         /* texture id written (OUTPUT: output binding slot) */
+  uint32_t src;         /* INPUT: input binding slot; OUTPUT/COPY: texture id */
+  uint32_t target;      /* ZH_TARGET_* (DRAW) */
+  uint32_t initializer; /* ZH_INIT_* (DRAW) */
+  uint32_t shader;      /* ZH_SHADER_* (DRAW) */
+  uint32_t num_operands;
+  uint32_t operands[2]; /* texture ids pushed by High::PushOperand */
+  int32_t knob;         /* -1, or the knob id that may override `params` */
+  const void* params;   /* std140 blob, copied at build time */
+  uint32_t params_len;
+  /* PaintToSelection (program.rs:191-205): source selection in texels of
+   * operands[0], target quad corners in target pixels (clockwise from
+   * top-left; QuadTarget::Absolute, program.rs:254-258) and viewport. */
+  zh_rect selection;
+  float quad[8];
+  zh_rect viewport;
+} zh_op;
+
+/* Plan flags. */
+#define ZH_PLAN_DEFAULT 0u
+/* Quantise every register to its declared texel format, as the reference's
+ * per-op staging does (encoder.rs:871-956), instead of keeping fused
+ * intermediates in FP32. */
+#define ZH_PLAN_STRICT_REGISTERS 1u
+/* Do not capture the program's launches into a hipGraph. */
+#define ZH_PLAN_NO_GRAPH 2u
+
+/* Lower a High program (Program::lower_to, program.rs:1304).  `textures`
+ * holds the descriptor of every texture id.  `ctx` may be NULL: the plan is
+ * then built for inspection only (zh_program_describe) and cannot launch. */
+zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures,
+                           const zh_op* ops, uint32_t num_ops, uint32_t flags, zh_program** out);
+zh_status zh_program_destroy(zh_program* prog);
+/* Human-readable fused plan (stages, kernels, micro-ops, bytes). */
+zh_status zh_program_describe(zh_program* prog, char* buf, size_t len, size_t* needed);
+/* Number of kernel launches per program launch, and algorithmic HBM bytes
+ * (each input read once, each output written once). */
+zh_status zh_program_stats(zh_program* prog, uint32_t* launches, uint64_t* algorithmic_bytes);
+
+/* Knobs (Environment::knob, run.rs:1292-1306): override the parameter blob of
+ * every op built with this knob id; the length must equal the original. */
+zh_status zh_program_set_knob(zh_program* prog, int32_t knob, const void* data, uint32_t len);
+
+typedef struct zh_binding {
+  uint32_t slot;   /* input slot (ZH_HIGH_INPUT.src) or output slot (ZH_HIGH_OUTPUT.dst) */
+  uint32_t is_output;
+  zh_image* image;
+} zh_binding;
+
+/* Bind and launch asynchronously (Executable::launch + Execution::step
+ * until done, run.rs:1016-1464).  Every input and output slot must be bound
+ * to an image whose descriptor matches. */
+zh_status zh_program_launch(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings);
+/* Average device time (ms) of each stage over the last `zh_program_time`
+ * call: launches the program `iterations` times and records HIP events
+ * around every stage on the context stream. */
+zh_status zh_program_time(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings,
+                          uint32_t iterations, float* stage_ms, uint32_t max_stages,
+                          uint32_t* num_stages);
+
+/* Message of the last failed call on this thread (never NULL). */
+const char* zh_last_error(void);
+/* Library version string. */
+const char* zh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZOSIMOS_HIP_H */

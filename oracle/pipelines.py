@@ -1,0 +1,181 @@
+"""FP64 oracle of the whole workloads and of the reference's paint draws.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  Built from the
+per-pixel functions of ``oracle.color``; nothing here imports the product.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import color as C
+
+
+# --------------------------------------------------------------------------
+# byte <-> value helpers
+# --------------------------------------------------------------------------
+def u8_image(data, w, h):
+    return np.asarray(data, dtype=np.uint8).reshape(h, w, 4)
+
+
+def f16_image(data, w, h):
+    return np.asarray(data).view(np.float16).reshape(h, w, 4).astype(np.float64)
+
+
+def f32_image(data, w, h):
+    return np.asarray(data).view(np.float32).reshape(h, w, 4).astype(np.float64)
+
+
+def mat_f32(m):
+    """The reference keeps Affine and colour matrices in f32; the oracle
+    starts from the same f32 values and works in FP64 from there."""
+    return np.asarray(m, dtype=np.float32).astype(np.float64).reshape(3, 3)
+
+
+def apply3(m, rgba):
+    out = np.array(rgba, dtype=np.float64)
+    out[..., :3] = np.einsum("ij,...j->...i", np.asarray(m, dtype=np.float64), rgba[..., :3])
+    return out
+
+
+# --------------------------------------------------------------------------
+# paint draws (High::DrawInto PaintToSelection with the copy shader)
+# --------------------------------------------------------------------------
+def selection_affine(selection, quad, viewport, target_wh):
+    """Target pixel -> source texel affine map of a PaintToSelection draw.
+
+    Quad corners (clockwise from top-left) are scaled by the viewport onto the
+    render target (``to_screenspace_coords``, program.rs:1908-1935, then
+    box.vert:141-148); each corner carries its selection corner as uv
+    (encoder.rs:1831-1849), and the rasteriser interpolates uv affinely
+    across the parallelogram."""
+    W, H = target_wh
+    vx, vy, vx1, vy1 = viewport
+    q = np.asarray(quad, dtype=np.float64).reshape(4, 2)
+    P = np.stack([(q[:, 0] - vx) * W / (vx1 - vx), (q[:, 1] - vy) * H / (vy1 - vy)], axis=1)
+    x0, y0, x1, y1 = [float(v) for v in selection]
+    S = np.array([[x0, y0], [x1, y0], [x1, y1], [x0, y1]])
+    A = np.column_stack([P[1] - P[0], P[3] - P[0]])
+    B = np.column_stack([S[1] - S[0], S[3] - S[0]])
+    lin = B @ np.linalg.inv(A)
+    off = S[0] - lin @ P[0]
+    return lin, off
+
+
+def paint_nearest(base, src, selection, quad, viewport, mode="replace"):
+    """Composite a nearest-sampled selection of ``src`` onto ``base`` (H, W, 4)
+    where the target pixel centre falls inside the quad (texel = floor of the
+    interpolated texel coordinate).  ``mode`` 'over' composites premultiplied
+    values instead of replacing."""
+    H, W = base.shape[:2]
+    lin, off = selection_affine(selection, quad, viewport, (W, H))
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    u = lin[0, 0] * (xs + 0.5) + lin[0, 1] * (ys + 0.5) + off[0]
+    v = lin[1, 0] * (xs + 0.5) + lin[1, 1] * (ys + 0.5) + off[1]
+    x0, y0, x1, y1 = selection
+    cov = (u >= x0) & (u < x1) & (v >= y0) & (v < y1)
+    ti = np.clip(np.floor(u).astype(np.int64), 0, src.shape[1] - 1)
+    tj = np.clip(np.floor(v).astype(np.int64), 0, src.shape[0] - 1)
+    out = np.array(base, dtype=np.float64)
+    vals = src[tj, ti]
+    if mode == "replace":
+        out[cov] = vals[cov]
+    else:
+        out[cov] = C.over(vals[cov], out[cov])
+    return out
+
+
+def inscribe_quad(rect):
+    """Placement quad of ``inscribe`` after ``Rectangle::normalize`` as the
+    reference writes it (command.rs:3536-3543: max_y = y + width)."""
+    x, y, mx, my = rect
+    w = mx - x
+    return [x, y, x + w, y, x + w, y + w, x, y + w]
+
+
+def crop(src, rect):
+    """``crop`` (command.rs:2507-2526): the selection painted over a quad of the
+    selection's size inside a viewport of the same size, i.e. stretched over
+    the whole output, which keeps the source size (command.rs:972-977)."""
+    H, W = src.shape[:2]
+    x0, y0, x1, y1 = rect
+    rw, rh = x1 - x0, y1 - y0
+    base = np.zeros_like(src, dtype=np.float64)
+    base[..., 2] = 1.0
+    base[..., 3] = 1.0
+    return paint_nearest(base, src, rect, [0, 0, rw, 0, rw, rh, 0, rh], (0, 0, rw, rh))
+
+
+def affine_quad(m, aw, ah):
+    """``QuadTarget::affine`` (program.rs:1897-1906) of an aw x ah image."""
+    m = np.asarray(m, dtype=np.float64).reshape(3, 3)
+    pts = []
+    for (x, y) in [(0, 0), (aw, 0), (aw, ah), (0, ah)]:
+        p = m @ np.array([x, y, 1.0])
+        pts += [p[0] / p[2], p[1] / p[2]]
+    return np.asarray(pts, dtype=np.float32).astype(np.float64)
+
+
+# --------------------------------------------------------------------------
+# workloads (see zosimos_amd/workloads.py for their definitions)
+# --------------------------------------------------------------------------
+def c1(a_u8, b_u8, matrix, out_hw):
+    """premultiply, B 'over' A, bilinear affine onto a transparent canvas,
+    un-premultiply; returns (linear straight RGBA, encoded u8)."""
+    A = C.premultiply(C.decode_u8(a_u8, 5))
+    B = C.premultiply(C.decode_u8(b_u8, 5))
+    ab = C.over(B, A)
+    res = C.bilinear_affine(ab, mat_f32(matrix), out_hw)
+    lin = C.unpremultiply(res)
+    return lin, C.encode_u8(lin, 5)
+
+
+def c2(src_u8, layer_f16):
+    """Rec.709 -> Display-P3 -> Oklab -> chroma x1.2 -> back -> 'over' of the
+    RGBA16F layer -> RGBA8 (P3 primaries, sRGB transfer)."""
+    lin = C.decode_u8(src_u8, 5)
+    p3 = apply3(C.convert_matrix(3, 7, 5), lin)
+    lab = C.rgb_to_oklab(p3[..., :3], 7)
+    lab = C.lch_adjust(lab, 1.2, 0.0)
+    back = p3.copy()
+    back[..., :3] = C.oklab_to_rgb(lab, 7, clamp=True)
+    res = C.unpremultiply(C.over(C.premultiply(layer_f16), C.premultiply(back)))
+    return res, C.encode_u8(res, 5)
+
+
+def chain(lin_rgba, hue_deg=30.0, chroma=1.2):
+    """sRGB-primaries linear RGB -> XYZ -> Oklab -> hue -> chroma -> back
+    (with the Oklab decode clamp), alpha untouched."""
+    lab = C.rgb_to_oklab(lin_rgba[..., :3], 3)
+    lab = C.lch_adjust(lab, 1.0, hue_deg)
+    lab = C.lch_adjust(lab, chroma, 0.0)
+    out = np.array(lin_rgba, dtype=np.float64)
+    out[..., :3] = C.oklab_to_rgb(lab, 3, clamp=True)
+    return out
+
+
+def c4(src_u8):
+    res = chain(C.decode_u8(src_u8, 5))
+    return res, C.encode_u8(res, 5)
+
+
+def c3_image(data, fmt, w, h, ow, oh):
+    """decode -> Rec.709 to Display-P3 -> bilinear resample to (ow, oh) ->
+    encode in the input's format.  Returns (linear, encoded bytes)."""
+    if fmt == 0:
+        lin = C.decode_u8(u8_image(data, w, h), 5)
+    else:
+        lin = f16_image(data, w, h)
+    p3 = apply3(C.convert_matrix(3, 7, 5), lin)
+    m = np.diag([np.float32(ow / w), np.float32(oh / h), 1.0])
+    res = C.unpremultiply(C.bilinear_affine(C.premultiply(p3), m, (oh, ow)))
+    if fmt == 0:
+        return res, C.encode_u8(res, 5)
+    return res, C.encode_f16(res)
+
+
+def c5(layers_f16, matrices, out_hw):
+    acc = np.zeros(out_hw + (4,), dtype=np.float64)
+    for lay, m in zip(layers_f16, matrices):
+        acc = C.over(C.bilinear_affine(C.premultiply(lay), mat_f32(m), out_hw), acc)
+    res = chain(C.unpremultiply(acc))
+    return res, C.encode_u8(res, 5)

@@ -1,0 +1,389 @@
+"""GPU parity of the HIP backend against the FP64 oracle and the FP32
+stage.frag restatement.  Every call goes through the C ABI.
+
+Bars: integer layout work (copies, transmute, inscribe, crop, same-format
+round trips of u8/u16/packed texels) bit-exact; 8-bit float outputs within
+1 LSB of the FP64 oracle (colour compared where the 8-bit alpha is non-zero:
+un-premultiplying divides by alpha); f16 outputs within 1 f16 ulp; linear f32
+outputs within 1e-5 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import zosimos_amd as z
+from zosimos_amd import workloads as W
+from oracle import color as C
+from oracle import pipelines as PL
+from oracle import stage as S
+
+from gpu_util import compare_u8, delta_e_u8, report, run
+
+pytestmark = pytest.mark.gpu
+
+RGBA8 = z.Texel(z.SampleBits.UInt8x4, z.SampleParts.RgbA)
+
+
+def rand_u8(shape, seed):
+    return np.random.default_rng(seed).integers(0, 256, size=shape, dtype=np.uint8)
+
+
+# ---- byte-exact layout paths ----------------------------------------------------------
+@pytest.mark.parametrize("desc", [z.Descriptor.rgba8_srgb(37, 13),
+                                  z.Descriptor(20, 9, z.Texel(z.SampleBits.UInt16x4, z.SampleParts.RgbA)),
+                                  z.Descriptor.rgba16f_linear(33, 7)])
+def test_passthrough_bit_exact(ctx, desc):
+    """tests/color_support.rs: input -> output is a byte copy."""
+    cmd = z.CommandBuffer()
+    a = cmd.input(desc)
+    o, _ = cmd.output(a)
+    data = rand_u8((desc.height, desc.width * desc.texel.bytes), 1)
+    out, _ = run(ctx, cmd, {a: desc}, {a: data}, {o: desc})
+    assert np.array_equal(out[o], data)
+
+
+def test_transmute_rgba8_to_lumaa16_bit_exact(ctx):
+    """tests/blend.rs:340-375 (assert at 372): the transmuted bytes equal the input."""
+    d = z.Descriptor.rgba8_srgb(64, 40)
+    t = z.Descriptor(64, 40, z.Texel(z.SampleBits.UInt16x2, z.SampleParts.LumaA), z.Color.SRGB)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    r = cmd.transmute(a, t)
+    o, _ = cmd.output(r)
+    data = rand_u8((40, 256), 2)
+    out, exe = run(ctx, cmd, {a: d}, {a: data}, {o: t})
+    assert "copy" in exe.describe()
+    assert np.array_equal(out[o], data)
+
+
+def test_srgb_u8_round_trip_through_float(ctx):
+    """decode -> identity conversion -> encode returns every code."""
+    d = z.Descriptor.rgba8_srgb(64, 16)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    c = cmd.color_convert(a, z.Color.SRGB, RGBA8)
+    o, _ = cmd.output(c)
+    data = np.tile(np.arange(256, dtype=np.uint8), 16).reshape(16, 256)
+    out, exe = run(ctx, cmd, {a: d}, {a: data}, {o: d})
+    assert np.array_equal(out[o], data)
+
+
+@pytest.mark.parametrize("bits,parts,transfer,nbits", [
+    (z.SampleBits.UInt8, z.SampleParts.Luma, z.Transfer.Srgb, 8),
+    (z.SampleBits.UInt16, z.SampleParts.Luma, z.Transfer.Linear, 16),
+    (z.SampleBits.UInt565, z.SampleParts.Rgb, z.Transfer.Linear, 16),
+    (z.SampleBits.UInt4x4, z.SampleParts.RgbA, z.Transfer.Linear, 16),
+    (z.SampleBits.UInt332, z.SampleParts.Rgb, z.Transfer.Linear, 8),
+    (z.SampleBits.UInt8x2, z.SampleParts.LumaA, z.Transfer.Linear, 16),
+    (z.SampleBits.UInt1010102, z.SampleParts.RgbA, z.Transfer.Linear, 32),
+])
+def test_staged_codec_round_trip_and_decode(ctx, bits, parts, transfer, nbits):
+    """Staged texels (stage.frag decode/encode) to RGBA32F and back: the
+    decoded values equal the FP32 C restatement bit for bit, and the encode
+    of the decode returns every code."""
+    n = 1 << min(nbits, 16)
+    bpp = z.BITS_BYTES[bits]
+    w = 256
+    h = n // w
+    color = z.Color(transfer=transfer)
+    src = z.Descriptor(w, h, z.Texel(bits, parts), color)
+    f32 = z.Descriptor(w, h, z.Texel(z.SampleBits.Float32x4, z.SampleParts.RgbA), z.Color(transfer=z.Transfer.Linear))
+    if nbits <= 16:
+        codes = np.arange(n, dtype=np.uint32)
+    else:
+        codes = np.random.default_rng(5).integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    raw = codes.astype(np.uint32).view(np.uint8).reshape(n, 4)[:, :bpp].reshape(h, w * bpp)
+    # decode to f32 (the conversion keeps primaries; only the texel changes)
+    cmd = z.CommandBuffer()
+    a = cmd.input(src)
+    c = cmd.color_convert(a, f32.color, f32.texel)
+    o, _ = cmd.output(c)
+    out, _ = run(ctx, cmd, {a: src}, {a: raw}, {o: f32})
+    got = out[o].view(np.float32).reshape(-1, 4)
+    want = S.stage_decode(codes, bits, parts, transfer)
+    if transfer == z.Transfer.Srgb:
+        # the device pow and the C library's powf differ in the last ulps
+        assert np.allclose(got, want, rtol=2e-6, atol=1e-7)
+        return
+    assert np.array_equal(got, want)
+    # encode back (linear transfers are exact round trips)
+    cmd = z.CommandBuffer()
+    a = cmd.input(f32)
+    c = cmd.color_convert(a, src.color, src.texel)
+    o, _ = cmd.output(c)
+    out, _ = run(ctx, cmd, {a: f32}, {a: got.view(np.uint8).reshape(h, w * 16)}, {o: src})
+    assert np.array_equal(out[o], raw)
+
+
+def test_f16_codec_exhaustive(ctx):
+    """RGBA16F -> RGBA32F over all 65536 half values, then f32 -> f16 of
+    random floats, against numpy's IEEE conversions."""
+    halfs = np.arange(65536, dtype=np.uint16)
+    w, h = 128, 128
+    d16 = z.Descriptor.rgba16f_linear(w, h)
+    d32 = z.Descriptor.rgba32f_linear(w, h)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d16)
+    c = cmd.color_convert(a, d32.color, d32.texel)
+    o, _ = cmd.output(c)
+    out, _ = run(ctx, cmd, {a: d16}, {a: halfs.view(np.uint8).reshape(h, w * 8)}, {o: d32})
+    got = out[o].view(np.float32).ravel()
+    want = halfs.view(np.float16).astype(np.float32)
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all()
+    vals = (np.random.default_rng(3).standard_normal(w * h * 4) * 300).astype(np.float32)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d32)
+    c = cmd.color_convert(a, d16.color, d16.texel)
+    o, _ = cmd.output(c)
+    out, _ = run(ctx, cmd, {a: d32}, {a: vals.view(np.uint8).reshape(h, w * 16)}, {o: d16})
+    assert np.array_equal(out[o].view(np.uint16).ravel(), vals.astype(np.float16).view(np.uint16))
+
+
+# ---- paints ------------------------------------------------------------------------
+def test_inscribe_bit_exact(ctx):
+    """tests/blend.rs:83-120 ('composed'): a 157x151 foreground inscribed at the
+    origin of a 512x512 background, with the reference's normalize quirk."""
+    bg_d, fg_d = z.Descriptor.rgba8_srgb(512, 512), z.Descriptor.rgba8_srgb(157, 151)
+    cmd = z.CommandBuffer()
+    bg = cmd.input(bg_d)
+    fg = cmd.input(fg_d)
+    r = cmd.inscribe(bg, z.Rectangle(0, 0, 157, 151), fg)
+    o, _ = cmd.output(r)
+    bgd, fgd = rand_u8((512, 2048), 11), rand_u8((151, 628), 12)
+    out, _ = run(ctx, cmd, {bg: bg_d, fg: fg_d}, {bg: bgd, fg: fgd}, {o: bg_d})
+    want = PL.paint_nearest(PL.u8_image(bgd, 512, 512).astype(np.float64), PL.u8_image(fgd, 157, 151).astype(np.float64),
+                            (0, 0, 157, 151), PL.inscribe_quad((0, 0, 157, 151)), (0, 0, 512, 512))
+    assert np.array_equal(out[o].reshape(512, 512, 4), want.astype(np.uint8))
+
+
+def test_inscribe_at_offset_is_a_slice_copy(ctx):
+    bg_d, fg_d = z.Descriptor.rgba8_srgb(100, 80), z.Descriptor.rgba8_srgb(24, 24)
+    cmd = z.CommandBuffer()
+    bg = cmd.input(bg_d)
+    fg = cmd.input(fg_d)
+    r = cmd.inscribe(bg, z.Rectangle(31, 17, 55, 41), fg)
+    o, _ = cmd.output(r)
+    bgd, fgd = rand_u8((80, 400), 13), rand_u8((24, 96), 14)
+    out, _ = run(ctx, cmd, {bg: bg_d, fg: fg_d}, {bg: bgd, fg: fgd}, {o: bg_d})
+    want = bgd.reshape(80, 100, 4).copy()
+    want[17:41, 31:55] = fgd.reshape(24, 24, 4)
+    assert np.array_equal(out[o].reshape(80, 100, 4), want)
+
+
+def test_crop_stretch_bit_exact(ctx):
+    d = z.Descriptor.rgba8_srgb(64, 48)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    r = cmd.crop(a, z.Rectangle(8, 4, 40, 28))
+    o, _ = cmd.output(r)
+    data = rand_u8((48, 256), 15)
+    out, _ = run(ctx, cmd, {a: d}, {a: data}, {o: d})
+    want = PL.crop(PL.u8_image(data, 64, 48).astype(np.float64), (8, 4, 40, 28))
+    assert np.array_equal(out[o].reshape(48, 64, 4), want.astype(np.uint8))
+
+
+def test_nearest_affine_rotate45(ctx):
+    """tests/blend.rs:122-167 ('affine'): foreground centred, rotated 45 deg,
+    painted over the background with nearest sampling."""
+    bg_d, fg_d = z.Descriptor.rgba8_srgb(512, 512), z.Descriptor.rgba8_srgb(157, 151)
+    aff = (z.Affine.new(z.AffineSample.Nearest).shift(-(157 // 2), -(151 // 2)).rotate(math.pi / 4)
+           .shift(256, 256))
+    cmd = z.CommandBuffer()
+    bg = cmd.input(bg_d)
+    fg = cmd.input(fg_d)
+    r = cmd.affine(bg, aff, fg)
+    o, _ = cmd.output(r)
+    bgd, fgd = rand_u8((512, 2048), 16), rand_u8((151, 628), 17)
+    out, _ = run(ctx, cmd, {bg: bg_d, fg: fg_d}, {bg: bgd, fg: fgd}, {o: bg_d})
+    quad = PL.affine_quad(np.float32(aff.transformation), 157, 151)
+    want = PL.paint_nearest(PL.u8_image(bgd, 512, 512).astype(np.float64),
+                            PL.u8_image(fgd, 157, 151).astype(np.float64), (0, 0, 157, 151), quad, (0, 0, 512, 512))
+    got = out[o].reshape(512, 512, 4)
+    mismatch = np.any(got != want.astype(np.uint8), axis=-1)
+    report("nearest_affine_rotate45", mismatched_pixels=int(mismatch.sum()))
+    # pixel centres exactly on a quad edge may round either way
+    assert mismatch.sum() <= 8
+
+
+def test_blend_at_offset(ctx):
+    bg_d, fg_d = z.Descriptor.rgba8_srgb(50, 40), z.Descriptor.rgba8_srgb(20, 10)
+    cmd = z.CommandBuffer()
+    bg = cmd.input(bg_d)
+    fg = cmd.input(fg_d)
+    r = cmd.blend(bg, z.Rectangle(12, 20, 32, 30), fg)
+    o, _ = cmd.output(r)
+    bgd, fgd = rand_u8((40, 200), 18), rand_u8((10, 80), 19)
+    out, _ = run(ctx, cmd, {bg: bg_d, fg: fg_d}, {bg: bgd, fg: fgd}, {o: bg_d})
+    A = C.premultiply(C.decode_u8(PL.u8_image(bgd, 50, 40), 5))
+    B = C.premultiply(C.decode_u8(PL.u8_image(fgd, 20, 10), 5))
+    res = A.copy()
+    res[20:30, 12:32] = C.over(B, A[20:30, 12:32])
+    want = C.encode_u8(C.unpremultiply(res), 5)
+    mr, ma, _ = compare_u8("blend_at_offset", out[o], want, alpha_ok=want[..., 3] > 0)
+    assert mr <= 1 and ma <= 1
+
+
+# ---- float chains against the FP64 oracle --------------------------------------------
+def test_c2_chain_small(ctx):
+    wl = W.c2(256, 96, seed=1)
+    host = wl.make_inputs()
+    (o, od), = wl.outputs.items()
+    out, _ = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    src_r, layer_r = list(wl.inputs)
+    lin, want = PL.c2(PL.u8_image(host[src_r], 256, 96), PL.f16_image(host[layer_r], 256, 96))
+    mr, ma, exact = compare_u8("c2_chain", out[o], want, alpha_ok=want[..., 3] > 0)
+    report("c2_chain", max_delta_e_ok=f"{delta_e_u8(out[o].reshape(96, 256, 4), lin, 7):.5f}")
+    assert mr <= 1 and ma <= 1 and exact > 0.99
+
+
+def test_c4_chain_small(ctx):
+    wl = W.c4(images=2, width=160, height=90, seed=2)
+    host = wl.make_inputs()
+    out, _ = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    for (ri, ro) in zip(wl.inputs, wl.outputs):
+        lin, want = PL.c4(PL.u8_image(host[ri], 160, 90))
+        mr, ma, exact = compare_u8("c4_chain", out[ro], want)
+        report("c4_chain", max_delta_e_ok=f"{delta_e_u8(out[ro].reshape(90, 160, 4), lin):.5f}")
+        assert mr <= 1 and ma == 0 and exact > 0.99
+
+
+def test_linear_f32_output_within_1e5(ctx):
+    """A linear RGBA32F chain (P3 conversion, Oklab round trip with chroma and
+    hue) against FP64: <= 1e-5 relative (plus 1e-6 absolute near zero)."""
+    w, h = 128, 64
+    d = z.Descriptor.rgba32f_linear(w, h)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    r = W.color_chain(cmd, a, out_texel=z.Texel(z.SampleBits.Float32x4, z.SampleParts.RgbA))
+    # color_chain ends in sRGB-encoded RGB; convert back to linear f32
+    lin_out = z.Color(transfer=z.Transfer.Linear)
+    r = cmd.color_convert(r, lin_out, z.Texel(z.SampleBits.Float32x4, z.SampleParts.RgbA))
+    o, od = cmd.output(r)
+    vals = np.random.default_rng(4).random((h, w, 4), dtype=np.float32)
+    out, _ = run(ctx, cmd, {a: d}, {a: vals.view(np.uint8).reshape(h, w * 16)}, {o: od})
+    got = out[o].view(np.float32).reshape(h, w, 4).astype(np.float64)
+    want = PL.chain(vals.astype(np.float64))
+    err = np.abs(got - want)
+    rel = float(np.max(err / (np.abs(want) + 1e-6)))
+    report("linear_f32_chain", max_rel=f"{rel:.3e}", max_abs=f"{err.max():.3e}")
+    assert np.all(err <= 1e-5 * np.abs(want) + 1e-6)
+
+
+def test_c1_over_affine_full_size(ctx):
+    """The CPU-runnable configuration at its real size (2 x 1024x768)."""
+    wl = W.c1(seed=3)
+    host = wl.make_inputs()
+    (o, od), = wl.outputs.items()
+    out, exe = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    a, b = list(wl.inputs)
+    lin, want = PL.c1(PL.u8_image(host[a], 1024, 768), PL.u8_image(host[b], 1024, 768), wl.meta["affine"], (768, 1024))
+    mr, ma, exact = compare_u8("c1_over_affine", out[o], want, alpha_ok=want[..., 3] > 0)
+    report("c1_over_affine", max_delta_e_ok=f"{delta_e_u8(out[o].reshape(768, 1024, 4), lin):.5f}")
+    assert mr <= 1 and ma <= 1 and exact > 0.99
+
+
+def test_c3_batch_small(ctx):
+    wl = W.c3(n=10, seed=4, max_side=300)
+    host = wl.make_inputs()
+    out, exe = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    assert "tile queue" in exe.describe()
+    for k, ((ri, din), (ro, dout)) in enumerate(zip(wl.inputs.items(), wl.outputs.items())):
+        w, h, fmt, s = wl.meta["shapes"][k]
+        lin, want = PL.c3_image(host[ri], fmt, w, h, dout.width, dout.height)
+        if fmt == 0:
+            mr, ma, _ = compare_u8(f"c3_img{k}_u8", out[ro], want, alpha_ok=want[..., 3] > 0)
+            assert mr <= 1 and ma <= 1
+        else:
+            got = out[ro].view(np.float16).reshape(dout.height, dout.width, 4)
+            a_ok = want[..., 3:4] > 1e-3
+            ulp = np.abs(got.view(np.uint16).astype(np.int32) - want.view(np.uint16).astype(np.int32))
+            ok = np.broadcast_to(a_ok, ulp.shape) | (np.arange(4) == 3)
+            report(f"c3_img{k}_f16", max_ulp=int(ulp[ok].max(initial=0)))
+            assert ulp[ok].max(initial=0) <= 1
+
+
+def test_c5_layers_small(ctx):
+    wl = W.c5(layers=8, width=192, height=128, seed=5)
+    host = wl.make_inputs()
+    (o, od), = wl.outputs.items()
+    out, exe = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    layers = [PL.f16_image(host[r], 192, 128) for r in wl.inputs]
+    lin, want = PL.c5(layers, wl.meta["matrices"], (128, 192))
+    mr, ma, exact = compare_u8("c5_layers", out[o], want, alpha_ok=want[..., 3] > 0)
+    report("c5_layers", max_delta_e_ok=f"{delta_e_u8(out[o].reshape(128, 192, 4), lin):.5f}")
+    assert mr <= 1 and ma <= 1
+
+
+# ---- runtime behaviour ------------------------------------------------------------
+def test_graph_replays_match_eager(ctx):
+    """The captured hipGraph replayed several times, with eager work and
+    synchronisation between the replays, gives the eager result each time."""
+    wl = W.c1(128, 96, seed=6)
+    host = wl.make_inputs()
+    eager, _ = run(ctx, wl.commands, wl.inputs, host, wl.outputs, flags=z.PLAN_NO_GRAPH)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    ins = {}
+    for r, d in wl.inputs.items():
+        ins[r] = ctx.image(d)
+        ins[r].upload(host[r])
+    (o, od), = wl.outputs.items()
+    out = ctx.image(od)
+    for i in range(4):
+        exe.launch(ins, {o: out})
+        ctx.sync()
+        assert np.array_equal(out.download(), eager[o]), f"replay {i}"
+        # eager work on the same stream between replays
+        scratch = ctx.image(od)
+        scratch.upload(np.zeros((96, 128 * 4), np.uint8))
+
+
+def test_knob_matches_rebuilt_program(ctx):
+    """tests/knobs.rs: one executable, several parameterisations through a
+    knob, each equal to a program built with that parameter."""
+    d = z.Descriptor.rgba8_srgb(64, 32)
+    data = rand_u8((32, 256), 20)
+
+    def build(mat):
+        cmd = z.CommandBuffer()
+        a = cmd.input(d)
+        canvas = cmd.solid_rgba(d, (0, 0, 0, 0))
+        r = cmd.affine(canvas, z.Affine(tuple(mat), z.AffineSample.BiLinear), a)
+        k = cmd.knob(r)
+        o, _ = cmd.output(r)
+        return cmd, a, o, k
+
+    m0 = z.workloads.about_centre(64, 32, 0.8, 0.3).transformation
+    cmd, a, o, k = build(m0)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    ia = ctx.image(d)
+    ia.upload(data)
+    io = ctx.image(d)
+    for s, r in [(0.8, 0.3), (1.3, -0.7), (0.6, 1.1)]:
+        m = z.workloads.about_centre(64, 32, s, r).transformation
+        f = np.asarray(m, np.float32)
+        blob = np.array([f[0], f[3], f[6], 0, f[1], f[4], f[7], 0, f[2], f[5], f[8], 0], np.float32).tobytes()
+        exe.knob(k, blob)
+        exe.launch({a: ia}, {o: io})
+        got = io.download()
+        cmd2, a2, o2, _ = build(m)
+        ref, _ = run(ctx, cmd2, {a2: d}, {a2: data}, {o2: d})
+        assert np.array_equal(got, ref[o2])
+
+
+def test_bind_errors(ctx):
+    d = z.Descriptor.rgba8_srgb(16, 16)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    o, _ = cmd.output(cmd.color_convert(a, z.Color(primaries=z.Primaries.DisplayP3), RGBA8))
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    good, wrong, out = ctx.image(d), ctx.image(z.Descriptor.rgba8_srgb(16, 8)), ctx.image(d)
+    with pytest.raises(z.ZosimosError) as e:
+        exe.launch({a: wrong}, {o: out})
+    assert e.value.code == z.Status.MISMATCHED_DESCRIPTOR
+    with pytest.raises(z.ZosimosError) as e:
+        exe.launch({}, {o: out})
+    assert e.value.code == z.Status.MISSING_KEY
+    exe.launch({a: good}, {o: out})
+    ctx.sync()

@@ -1,0 +1,147 @@
+"""Host-side logic of the backend (no GPU): the C ABI exports, the C++
+front end's type checks (command.rs error classes), the High emission and
+the fused plans."""
+import ctypes
+import math
+import os
+import re
+
+import pytest
+
+import zosimos_amd as z
+from zosimos_amd import _abi
+from zosimos_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = []
+    for h in ("zosimos_hip.h", "zosimos_command.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names += re.findall(r"^\s*(?:zh_status|void\*|const char\*)\s+(z[hs]_\w+)\(", text, re.M)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_abi.exported_symbols())
+    assert "gfx950" in z.version()
+
+
+def rgba8(w=16, h=8):
+    return z.Descriptor.rgba8_srgb(w, h)
+
+
+def expect(code, fn, *args):
+    with pytest.raises(z.ZosimosError) as e:
+        fn(*args)
+    assert e.value.code == code, e.value
+    return e.value
+
+
+def test_command_type_errors():
+    cmd = z.CommandBuffer()
+    a = cmd.input(rgba8(16, 8))
+    b = cmd.input(rgba8(8, 8))
+    c = cmd.input(z.Descriptor.rgba16f_linear(8, 8))
+    # inscribe: rect must be the above image (command.rs:1196-1198)
+    expect(z.Status.OTHER, cmd.inscribe, a, z.Rectangle(0, 0, 4, 4), b)
+    # ... inside the below image
+    expect(z.Status.OTHER, cmd.inscribe, a, z.Rectangle(10, 0, 18, 8), b)
+    # conflicting texels
+    err = expect(z.Status.TYPE, cmd.inscribe, a, z.Rectangle(0, 0, 8, 8), c)
+    assert err.is_type_err()
+    # transmute with mismatched size / texel bytes
+    expect(z.Status.TYPE, cmd.transmute, a, rgba8(8, 8))
+    expect(z.Status.TYPE, cmd.transmute, a, z.Descriptor.rgba16f_linear(16, 8))
+    # singular affine (command.rs:1648-1656)
+    expect(z.Status.OTHER, cmd.affine, a, z.Affine.new(z.AffineSample.Nearest).scale(0.0, 1.0), a)
+    # no conversion Oklab -> Oklab
+    lab = cmd.color_convert(a, z.Color.OKLAB, z.Texel(z.SampleBits.Float16x4, z.SampleParts.LabA))
+    expect(z.Status.TYPE, cmd.color_convert, lab, z.Color.OKLAB, z.Texel(z.SampleBits.Float16x4, z.SampleParts.LabA))
+    # blend rect size / position
+    expect(z.Status.OTHER, cmd.blend, a, z.Rectangle(0, 0, 4, 8), b)
+    expect(z.Status.OTHER, cmd.blend, a, z.Rectangle(12, 0, 20, 8), b)
+    # bad register
+    expect(z.Status.OTHER, cmd.crop, 99, z.Rectangle(0, 0, 1, 1))
+    # inconsistent descriptor
+    expect(z.Status.TYPE, cmd.input, z.Descriptor(0, 8, z.Texel(10, 8)))
+
+
+def test_inscribe_high_emission_keeps_normalize_quirk():
+    cmd = z.CommandBuffer()
+    bg = cmd.input(rgba8(32, 32))
+    fg = cmd.input(rgba8(10, 6))
+    r = cmd.inscribe(bg, z.Rectangle(0, 0, 10, 6), fg)
+    cmd.output(r)
+    tex, ops = z.Linker.from_included().compile(cmd).high()
+    draws = [o for o in ops if o.high == 2]
+    assert len(draws) == 2
+    assert draws[0].target == 0 and draws[1].target == 1  # Discard then Load
+    # placement quad is 10 x 10 (Rectangle::normalize, command.rs:3541)
+    assert list(draws[1].quad) == [0, 0, 10, 0, 10, 10, 0, 10]
+    assert (draws[1].selection.max_x, draws[1].selection.max_y) == (10, 6)
+
+
+def plan(wl_or_cmd, flags=0):
+    cmd = wl_or_cmd.commands if hasattr(wl_or_cmd, "commands") else wl_or_cmd
+    return z.Linker.from_included().compile(cmd).lower_to(None, flags)
+
+
+def test_plan_c2_is_one_fused_kernel():
+    e = plan(W.c2(64, 32))
+    text = e.describe()
+    assert text.count("stage ") == 1
+    assert "pointwise 64x32 flat: load#0 mat3 cbrt mat3 cube mat3 clamp01 premul load#1 premul over unpremul store#2" in text
+    assert e.stats() == (1, 64 * 32 * 16)
+
+
+def test_plan_c1_c5_resample_layers():
+    t1 = plan(W.c1(64, 48)).describe()
+    assert "resample 64x48" in t1 and "layer0(bilinear-over 64x48)[load#0 premul load#1 premul over]" in t1
+    t5 = plan(W.c5(8, 64, 48)).describe()
+    assert t5.count("stage ") == 1 and "layer7(bilinear-over" in t5
+    assert "post[unpremul mat3 cbrt mat3 cube mat3 clamp01 store#8]" in t5
+
+
+def test_plan_c3_batches_into_one_tile_queue():
+    e = plan(W.c3(12, max_side=300))
+    text = e.describe()
+    assert "batch: stages 0..11 as one tile queue" in text
+    assert e.stats()[0] == 1
+
+
+def test_plan_strict_registers_quantises():
+    cmd = z.CommandBuffer()
+    a = cmd.input(rgba8())
+    lab = cmd.color_convert(a, z.Color.OKLAB, z.Texel(z.SampleBits.UInt8x4, z.SampleParts.LabA))
+    back = cmd.color_convert(lab, z.Color.SRGB, z.Texel(10, 8))
+    cmd.output(back)
+    assert "quant" not in plan(cmd).describe()
+    assert "quant" in plan(cmd, z.PLAN_STRICT_REGISTERS).describe()
+
+
+def test_knob_length_and_replan_without_device():
+    cmd = z.CommandBuffer()
+    a = cmd.input(rgba8())
+    c = cmd.color_convert(a, z.Color(primaries=z.Primaries.DisplayP3), z.Texel(10, 8))
+    k = cmd.knob(c)
+    cmd.output(c)
+    e = plan(cmd)
+    expect(z.Status.KNOB_LENGTH, e.knob, k, b"\0" * 4)
+    ident = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    e.knob(k, bytes((ctypes.c_float * 12)(*ident)))
+    assert "mat3" not in e.describe()  # identity folds away
+    expect(z.Status.INVALID_ARGUMENT, e.knob, k + 7, b"\0" * 48)
+
+
+def test_launch_without_device_fails_loudly():
+    e = plan(W.c2(8, 8))
+    with pytest.raises(z.ZosimosError) as err:
+        e.launch({}, {})
+    assert err.value.code == z.Status.NO_DEVICE

@@ -1,0 +1,199 @@
+"""Pins of the CPU oracle against published values, library routines and
+invariants (no GPU).  The oracle is the checker of every GPU parity test, so
+each of its functions is checked here by something other than its own
+formula."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import color as C
+from oracle import pipelines as PL
+from oracle import stage as S
+
+
+# ---- colour matrices: published values ---------------------------------------
+def test_srgb_to_xyz_matches_iec_61966_2_1():
+    # IEC 61966-2-1:1999 Table (4-decimal matrix).
+    published = np.array([[0.4124, 0.3576, 0.1805], [0.2126, 0.7152, 0.0722], [0.0193, 0.1192, 0.9505]])
+    assert np.allclose(C.rgb_to_xyz_matrix(3, 5), published, atol=6e-5)
+
+
+def test_display_p3_to_xyz_matches_published():
+    # Display P3 (SMPTE EG 432-1 primaries, D65), 4-decimal published matrix.
+    published = np.array([[0.4866, 0.2657, 0.1982], [0.2290, 0.6917, 0.0793], [0.0000, 0.0451, 1.0439]])
+    assert np.allclose(C.rgb_to_xyz_matrix(7, 5), published, atol=6e-5)
+
+
+def test_bt2020_luma_coefficients():
+    # ITU-R BT.2020-2: Y = 0.2627 R + 0.6780 G + 0.0593 B.
+    assert np.allclose(C.rgb_to_xyz_matrix(5, 5)[1], [0.2627, 0.6780, 0.0593], atol=6e-5)
+
+
+def test_white_maps_to_white_and_inverse():
+    for p in (1, 2, 3, 4, 5, 7):
+        m = C.rgb_to_xyz_matrix(p, 5)
+        assert np.allclose(m @ np.ones(3), C.whitepoint_xyz(5), atol=1e-12)
+    m = C.convert_matrix(3, 7, 5)
+    assert np.allclose(C.convert_matrix(7, 3, 5) @ m, np.eye(3), atol=1e-12)
+    assert np.allclose(C.convert_matrix(3, 3, 5), np.eye(3), atol=1e-12)
+
+
+# ---- Oklab: Ottosson's table and CSS Color 4 ---------------------------------
+@pytest.mark.parametrize("xyz,lab", [
+    ((0.950, 1.000, 1.089), (1.000, 0.000, 0.000)),
+    ((1.000, 0.000, 0.000), (0.450, 1.236, -0.019)),
+    ((0.000, 1.000, 0.000), (0.922, -0.671, 0.263)),
+    ((0.000, 0.000, 1.000), (0.153, -1.415, -0.449)),
+])
+def test_oklab_known_answers(xyz, lab):
+    # Table of XYZ -> Oklab values published with the Oklab definition.
+    assert np.allclose(C.xyz_to_oklab(np.array(xyz)), lab, atol=1.5e-3)
+
+
+def test_oklab_srgb_red_css_color_4():
+    # CSS Color 4: red = oklab(0.6279554 0.2248631 0.1258463).
+    assert np.allclose(C.rgb_to_oklab(np.array([1.0, 0, 0]), 3), [0.6279554, 0.2248631, 0.1258463], atol=5e-4)
+
+
+def test_oklab_round_trip_and_adjust():
+    rng = np.random.default_rng(1)
+    rgb = rng.random((1000, 3))
+    lab = C.rgb_to_oklab(rgb, 3)
+    assert np.allclose(C.oklab_to_rgb(lab, 3, clamp=False), rgb, atol=1e-9)
+    # chroma scaling multiplies the polar radius; hue rotation preserves it
+    adj = C.lch_adjust(lab, 1.2, 30.0)
+    assert np.allclose(np.hypot(adj[:, 1], adj[:, 2]), 1.2 * np.hypot(lab[:, 1], lab[:, 2]), atol=1e-12)
+    ang = np.arctan2(adj[:, 2], adj[:, 1]) - np.arctan2(lab[:, 2], lab[:, 1])
+    ang = (ang + np.pi) % (2 * np.pi) - np.pi
+    ok = np.hypot(lab[:, 1], lab[:, 2]) > 1e-6
+    assert np.allclose(ang[ok], math.radians(30.0), atol=1e-9)
+    assert np.allclose(adj[:, 0], lab[:, 0])
+
+
+# ---- transfers ---------------------------------------------------------------------
+def test_srgb_known_values():
+    assert C.eo_srgb(np.array(0.5)) == pytest.approx(0.214041140, abs=1e-9)
+    assert C.eo_srgb(np.array(128 / 255)) == pytest.approx(0.215860500, abs=1e-8)
+    assert C.oe_srgb(np.array(0.5)) == pytest.approx(0.735356983, abs=1e-9)
+    assert C.eo_srgb(np.array(1.0)) == pytest.approx(1.0)
+    # sign-symmetric extension (stage.frag:335-352)
+    assert C.eo_srgb(np.array(-0.5)) == pytest.approx(-0.214041140, abs=1e-9)
+
+
+def test_transfer_round_trips():
+    v = np.linspace(0.0, 1.0, 10001)
+    assert np.allclose(C.eo_srgb(C.oe_srgb(v)), v, atol=1e-12)
+    assert np.allclose(C.eo_bt709(C.oe_bt709(v)), v, atol=1e-12)
+    assert np.allclose(C.eo_smpte2084(C.eo_inv_smpte2084(v)), v, atol=1e-9)
+    # PQ end points (BT.2100 Table 4)
+    assert C.eo_smpte2084(np.array(1.0)) == pytest.approx(1.0)
+    assert C.eo_smpte2084(np.array(0.0)) == pytest.approx(0.0)
+
+
+def test_srgb_u8_encode_decode_identity():
+    codes = np.arange(256, dtype=np.uint8)
+    px = np.stack([codes, codes, codes, codes], axis=-1)
+    assert np.array_equal(C.encode_u8(C.decode_u8(px, 5), 5), px)
+    assert np.array_equal(C.encode_u8(C.decode_u8(px, 4), 4), px)
+
+
+# ---- stage.frag restatement (FP32 C) -----------------------------------------------
+def test_stage_u8_u16_exhaustive_round_trip():
+    # SURVEY hard part 2: correctly rounded division round-trips every code.
+    v8 = np.arange(256, dtype=np.uint32)
+    assert np.array_equal(S.stage_encode(S.stage_decode(v8, 0, 4, 4), 0, 4, 4), v8)
+    v16 = np.arange(65536, dtype=np.uint32)
+    assert np.array_equal(S.stage_encode(S.stage_decode(v16, 3, 4, 4), 3, 4, 4), v16)
+
+
+@pytest.mark.parametrize("bits,nbits", [(1, 8), (2, 8), (4, 16), (7, 16), (8, 16), (14, 32), (15, 32)])
+def test_stage_packed_round_trip(bits, nbits):
+    rng = np.random.default_rng(bits)
+    v = np.arange(1 << nbits, dtype=np.uint32) if nbits <= 16 else rng.integers(0, 1 << 32, 200000, dtype=np.uint64).astype(np.uint32)
+    parts = 8 if bits in (4, 14, 15) else (5 if bits == 8 else 6)
+    dec = S.stage_decode(v, bits, parts, 4)
+    assert np.array_equal(S.stage_encode(dec, bits, parts, 4), v)
+
+
+def test_stage_demux_known_fields():
+    # Int565 (stage.frag:549-550): low 5 bits are the first channel.
+    out = S.stage_decode(np.array([0x001F, 0x07E0, 0xF800], dtype=np.uint32), 7, 6, 4)
+    assert np.allclose(out[:, :3], [[1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    # Int8x4 RGBA: bytes in memory order
+    out = S.stage_decode(np.array([0x80402010], dtype=np.uint32), 10, 8, 4)
+    assert np.allclose(out[0], np.array([0x10, 0x20, 0x40, 0x80]) / 255.0)
+    # truncating encode: 0.999 * 255 -> 254
+    assert S.stage_encode(np.array([[0.999, 0.999, 0.999, 0.999]], np.float32), 0, 4, 4)[0] == 254
+
+
+def test_stage_srgb_matches_fp64():
+    v = np.arange(256, dtype=np.uint32)
+    got = S.stage_decode(v, 0, 4, 5)[:, 0]
+    assert np.allclose(got, C.srgb_u8_decode_table(), atol=2e-7)
+
+
+# ---- bilinear resampling ------------------------------------------------------------
+def test_bilinear_matches_scipy_grid_constant():
+    rng = np.random.default_rng(7)
+    src = rng.random((23, 31, 4))
+    m = PL.mat_f32(C.affine_matrix((-15.5, -11.5), (0.8, 1.3), 0.4, (20.0, 14.0)))
+    out = C.bilinear_affine(src, m, (29, 41))
+    inv = np.linalg.inv(m)
+    ys, xs = np.mgrid[0:29, 0:41].astype(np.float64)
+    q = inv @ np.stack([xs.ravel() + 0.5, ys.ravel() + 0.5, np.ones(xs.size)])
+    u, v = q[0] / q[2] - 0.5, q[1] / q[2] - 0.5
+    for ch in range(4):
+        ref = ndimage.map_coordinates(src[..., ch], [v, u], order=1, mode="grid-constant", cval=0.0)
+        assert np.allclose(out[..., ch].ravel(), ref, atol=1e-12)
+
+
+def test_bilinear_invariants():
+    rng = np.random.default_rng(8)
+    src = rng.random((16, 12, 4))
+    assert np.allclose(C.bilinear_affine(src, np.eye(3), (16, 12)), src)
+    shifted = C.bilinear_affine(src, C.affine_matrix(shift_post=(3, 2)), (16, 12))
+    assert np.allclose(shifted[2:, 3:], src[:14, :9])
+    assert np.allclose(shifted[:2], 0) and np.allclose(shifted[:, :3], 0)
+    # rotation by 90 degrees about the centre of a square image is a permutation
+    sq = rng.random((10, 10, 4))
+    m = C.affine_matrix((-5, -5), (1, 1), math.pi / 2, (5, 5))
+    rot = C.bilinear_affine(sq, m, (10, 10))
+    assert np.allclose(np.sort(rot.ravel()), np.sort(sq.ravel()))
+
+
+def test_porter_duff_invariants():
+    rng = np.random.default_rng(9)
+    a = C.premultiply(rng.random((50, 4)))
+    b = C.premultiply(rng.random((50, 4)))
+    opaque = a.copy(); opaque[:, 3] = 1.0
+    assert np.allclose(C.over(opaque, b), opaque)
+    clear = np.zeros_like(a)
+    assert np.allclose(C.over(clear, b), b)
+    straight = rng.random((50, 4)) + 0.01
+    assert np.allclose(C.unpremultiply(C.premultiply(straight)), straight)
+
+
+# ---- paint draws ---------------------------------------------------------------------
+def test_paint_identity_and_offset_copy():
+    rng = np.random.default_rng(10)
+    base = rng.random((20, 30, 4))
+    src = rng.random((8, 8, 4))
+    out = PL.paint_nearest(base, src, (0, 0, 8, 8), PL.inscribe_quad((5, 7, 13, 15)), (0, 0, 30, 20))
+    ref = base.copy()
+    ref[7:15, 5:13] = src
+    assert np.array_equal(out, ref)
+
+
+def test_inscribe_normalize_quirk_stretches_height():
+    # A 6x4 foreground becomes a 6x6 quad (max_y = y + width, command.rs:3541).
+    assert PL.inscribe_quad((0, 0, 6, 4)) == [0, 0, 6, 0, 6, 6, 0, 6]
+
+
+def test_crop_stretches_selection():
+    src = np.arange(8 * 8 * 4, dtype=np.float64).reshape(8, 8, 4)
+    out = PL.crop(src, (2, 2, 6, 6))
+    # every 2x2 block of the output shows one texel of the 4x4 selection
+    assert np.array_equal(out[0, 0], src[2, 2]) and np.array_equal(out[7, 7], src[5, 5])
+    assert np.array_equal(out[1, 1], src[2, 2]) and np.array_equal(out[2, 2], src[3, 3])

@@ -1,0 +1,473 @@
+"""zosimos_amd -- MI355X (gfx950) execution backend for zosimos programs.
+
+Python mirror of the reference's front end (``CommandBuffer``, ``Linker``,
+``Program::lower_to``, ``Executable::launch``; lib/zosimos/src/command.rs,
+program.rs, run.rs) over the C ABI of ``include/zosimos_hip.h`` and
+``include/zosimos_command.h``.  All pixel work runs in the HIP kernels of
+``csrc/``; importing the compute classes without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Iterable, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _abi
+from ._abi import BindingC, ColorC, DescriptorC, RectC, Texel as TexelC
+
+# ---- identifiers (lib/std/src/stage.frag:107-170) ---------------------------
+class Transfer:
+    Bt709 = 0
+    Bt470M = 1
+    Bt601 = 2
+    Smpte240 = 3
+    Linear = 4
+    Srgb = 5
+    Bt2020_10bit = 6
+    Bt2020_12bit = 7
+    Smpte2084 = 8
+    LabLch = 0x100
+
+
+class SampleParts:
+    A, R, G, B, Luma, LumaA, Rgb, Bgr, RgbA, Rgbx, BgrA = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
+    Bgrx, ARgb, Xrgb, ABgr, Xbgr, Yuv, Lab, LabA, Lch, LchA = 11, 12, 13, 14, 15, 16, 17, 18, 19, 20
+
+
+class SampleBits:
+    UInt8, UInt332, UInt233, UInt16, UInt4x4 = 0, 1, 2, 3, 4
+    UInt565, UInt8x2, UInt8x3, UInt8x4, UInt16x2, UInt16x3, UInt16x4 = 7, 8, 9, 10, 11, 12, 13
+    UInt1010102, UInt2101010, Float16x4, Float32x4 = 14, 15, 18, 19
+
+
+BITS_BYTES = {0: 1, 1: 1, 2: 1, 3: 2, 4: 2, 5: 2, 6: 2, 7: 2, 8: 2, 9: 3, 10: 4, 11: 4, 12: 6, 13: 8,
+              14: 4, 15: 4, 16: 4, 17: 4, 18: 8, 19: 16}
+
+
+class Primaries:
+    Xyz, Bt601_525, Bt601_625, Bt709, Smpte240, Bt2020, Bt2100, DisplayP3 = range(8)
+
+
+class Whitepoint:
+    A, B, C, D50, D55, D65, D75, E, F2, F7, F11 = range(11)
+
+
+class ColorModel:
+    Rgb, Oklab, Scalars = 0, 1, 2
+
+
+class Status:
+    OK, INVALID_ARGUMENT, UNSUPPORTED, HIP, MISMATCHED_DESCRIPTOR, MISSING_KEY = 0, 1, 2, 3, 4, 5
+    OUT_OF_MEMORY, TYPE, UNIMPLEMENTED, KNOB_LENGTH, NO_DEVICE, OTHER = 6, 7, 8, 9, 10, 11
+
+
+class ZosimosError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+    def is_type_err(self) -> bool:
+        """CommandError::is_type_err (command.rs:3632-3639)."""
+        return self.code == Status.TYPE
+
+
+def _check(code: int):
+    if code != 0:
+        raise ZosimosError(code, _abi.lib().zh_last_error().decode())
+
+
+# ---- descriptors (lib/zosimos/src/buffer.rs:13-30) ---------------------------
+@dataclass(frozen=True)
+class Color:
+    model: int = ColorModel.Rgb
+    primaries: int = Primaries.Bt709
+    transfer: int = Transfer.Srgb
+    whitepoint: int = Whitepoint.D65
+    luminance: int = 0
+
+    def c(self) -> ColorC:
+        return ColorC(self.model, self.primaries, self.transfer, self.whitepoint, self.luminance)
+
+
+Color.SRGB = Color()
+Color.BT709_RGB = Color(transfer=Transfer.Bt709)
+Color.OKLAB = Color(model=ColorModel.Oklab, primaries=0, transfer=Transfer.Linear, whitepoint=Whitepoint.D65)
+
+
+@dataclass(frozen=True)
+class Texel:
+    bits: int
+    parts: int
+
+    @staticmethod
+    def new_u8(parts: int) -> "Texel":
+        return Texel(SampleBits.UInt8x4 if parts in (SampleParts.RgbA, SampleParts.BgrA) else SampleBits.UInt8, parts)
+
+    def c(self) -> TexelC:
+        return TexelC(self.bits, self.parts)
+
+    @property
+    def bytes(self) -> int:
+        return BITS_BYTES[self.bits]
+
+
+@dataclass(frozen=True)
+class Descriptor:
+    width: int
+    height: int
+    texel: Texel
+    color: Color = Color.SRGB
+    row_stride: int = 0
+
+    @staticmethod
+    def rgba8_srgb(w: int, h: int) -> "Descriptor":
+        """``Descriptor::with_srgb_image`` of an RGBA8 image (buffer.rs:201-210)."""
+        return Descriptor(w, h, Texel(SampleBits.UInt8x4, SampleParts.RgbA), Color.SRGB)
+
+    @staticmethod
+    def rgba16f_linear(w: int, h: int, primaries: int = Primaries.Bt709) -> "Descriptor":
+        return Descriptor(w, h, Texel(SampleBits.Float16x4, SampleParts.RgbA),
+                          Color(primaries=primaries, transfer=Transfer.Linear))
+
+    @staticmethod
+    def rgba32f_linear(w: int, h: int, primaries: int = Primaries.Bt709) -> "Descriptor":
+        return Descriptor(w, h, Texel(SampleBits.Float32x4, SampleParts.RgbA),
+                          Color(primaries=primaries, transfer=Transfer.Linear))
+
+    def with_(self, **kw) -> "Descriptor":
+        d = dict(width=self.width, height=self.height, texel=self.texel, color=self.color, row_stride=self.row_stride)
+        d.update(kw)
+        return Descriptor(**d)
+
+    @property
+    def stride(self) -> int:
+        return self.row_stride or self.width * self.texel.bytes
+
+    @property
+    def nbytes(self) -> int:
+        return self.stride * self.height
+
+    def c(self) -> DescriptorC:
+        return DescriptorC(self.width, self.height, self.row_stride, self.texel.bytes, self.texel.c(), self.color.c())
+
+    @staticmethod
+    def from_c(d: DescriptorC) -> "Descriptor":
+        return Descriptor(d.width, d.height, Texel(d.texel.bits, d.texel.parts),
+                          Color(d.color.model, d.color.primaries, d.color.transfer, d.color.whitepoint,
+                                d.color.luminance), int(d.row_stride))
+
+
+@dataclass(frozen=True)
+class Rectangle:
+    """``Rectangle`` (command.rs:279-285)."""
+    x: int
+    y: int
+    max_x: int
+    max_y: int
+
+    @staticmethod
+    def with_width_height(w: int, h: int) -> "Rectangle":
+        return Rectangle(0, 0, w, h)
+
+    def c(self) -> RectC:
+        return RectC(self.x, self.y, self.max_x, self.max_y)
+
+
+class AffineSample:
+    Nearest = 0
+    BiLinear = 1
+
+
+class Blend:
+    Alpha = 0
+
+
+def _mul3(a, b):
+    return [sum(a[i * 3 + k] * b[k * 3 + j] for k in range(3)) for i in range(3) for j in range(3)]
+
+
+@dataclass(frozen=True)
+class Affine:
+    """``Affine`` (command.rs:287-296, builders 3420-3485): a row-major
+    homogeneous matrix; each builder left-multiplies."""
+    transformation: Tuple[float, ...] = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    sampling: int = AffineSample.Nearest
+
+    @staticmethod
+    def new(sampling: int) -> "Affine":
+        return Affine(sampling=sampling)
+
+    def _post(self, m) -> "Affine":
+        return Affine(tuple(_mul3(m, list(self.transformation))), self.sampling)
+
+    def scale(self, x: float, y: float) -> "Affine":
+        return self._post([x, 0, 0, 0, y, 0, 0, 0, 1.0])
+
+    def rotate(self, rad: float) -> "Affine":
+        c, s = math.cos(rad), math.sin(rad)
+        return self._post([c, s, 0, -s, c, 0, 0, 0, 1.0])
+
+    def shift(self, x: float, y: float) -> "Affine":
+        return self._post([1, 0, x, 0, 1, y, 0, 0, 1.0])
+
+
+# ---- CommandBuffer (command.rs:60-68, 739-1863) ----------------------------------
+class CommandBuffer:
+    def __init__(self):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.zs_command_create(ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zs_command_destroy(self._h)
+            self._h = None
+
+    def _reg(self, fn, *args) -> int:
+        r = ctypes.c_uint32()
+        _check(fn(self._h, *args, ctypes.byref(r)))
+        return r.value
+
+    def input(self, desc: Descriptor) -> int:
+        return self._reg(self._lib.zs_command_input, ctypes.byref(desc.c()))
+
+    def output(self, src: int):
+        r = ctypes.c_uint32()
+        d = DescriptorC()
+        _check(self._lib.zs_command_output(self._h, src, ctypes.byref(r), ctypes.byref(d)))
+        return r.value, Descriptor.from_c(d)
+
+    def crop(self, src: int, rect: Rectangle) -> int:
+        return self._reg(self._lib.zs_command_crop, src, rect.c())
+
+    def color_convert(self, src: int, color: Color, texel: Texel) -> int:
+        return self._reg(self._lib.zs_command_color_convert, src, ctypes.byref(color.c()), ctypes.byref(texel.c()))
+
+    def inscribe(self, below: int, rect: Rectangle, above: int) -> int:
+        return self._reg(self._lib.zs_command_inscribe, below, rect.c(), above)
+
+    def transmute(self, src: int, target: Descriptor) -> int:
+        return self._reg(self._lib.zs_command_transmute, src, ctypes.byref(target.c()))
+
+    def affine(self, below: int, affine: Affine, above: int) -> int:
+        m = (ctypes.c_float * 9)(*affine.transformation)
+        return self._reg(self._lib.zs_command_affine, below, m, affine.sampling, above)
+
+    def blend(self, below: int, rect: Rectangle, above: int, blend: int = Blend.Alpha) -> int:
+        return self._reg(self._lib.zs_command_blend, below, rect.c(), above, blend)
+
+    def solid_rgba(self, desc: Descriptor, color: Sequence[float]) -> int:
+        c = (ctypes.c_float * 4)(*color)
+        return self._reg(self._lib.zs_command_solid_rgba, ctypes.byref(desc.c()), c)
+
+    def oklab_adjust(self, src: int, chroma: float = 1.0, hue_rad: float = 0.0) -> int:
+        return self._reg(self._lib.zs_command_oklab_adjust, src, chroma, hue_rad)
+
+    def knob(self, reg: int) -> int:
+        k = ctypes.c_int32()
+        _check(self._lib.zs_command_knob(self._h, reg, ctypes.byref(k)))
+        return k.value
+
+    def describe(self, reg: int) -> Descriptor:
+        d = DescriptorC()
+        _check(self._lib.zs_command_describe(self._h, reg, ctypes.byref(d)))
+        return Descriptor.from_c(d)
+
+
+class Linker:
+    """``Linker::from_included().compile(&commands)`` (command.rs:2063-2185)."""
+
+    @staticmethod
+    def from_included() -> "Linker":
+        return Linker()
+
+    def compile(self, commands: CommandBuffer) -> "Program":
+        h = ctypes.c_void_p()
+        _check(_abi.lib().zs_command_compile(commands._h, ctypes.byref(h)))
+        return Program(h)
+
+
+# Plan flags (zosimos_hip.h)
+PLAN_DEFAULT = 0
+PLAN_STRICT_REGISTERS = 1
+PLAN_NO_GRAPH = 2
+
+
+class Program:
+    """A linked program: the High IR list (program.rs:24-39)."""
+
+    def __init__(self, handle):
+        self._lib = _abi.lib()
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zs_linked_destroy(self._h)
+            self._h = None
+
+    def high(self):
+        tex = ctypes.POINTER(DescriptorC)()
+        ops = ctypes.POINTER(_abi.OpC)()
+        nt, no = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.zs_linked_high(self._h, ctypes.byref(tex), ctypes.byref(nt), ctypes.byref(ops),
+                                        ctypes.byref(no)))
+        return [tex[i] for i in range(nt.value)], [ops[i] for i in range(no.value)]
+
+    def input_slot(self, reg: int) -> int:
+        s = ctypes.c_uint32()
+        _check(self._lib.zs_linked_input_slot(self._h, reg, ctypes.byref(s)))
+        return s.value
+
+    def output_slot(self, reg: int) -> int:
+        s = ctypes.c_uint32()
+        _check(self._lib.zs_linked_output_slot(self._h, reg, ctypes.byref(s)))
+        return s.value
+
+    def lower_to(self, ctx: Optional["Context"] = None, flags: int = PLAN_DEFAULT) -> "Executable":
+        """``Program::lower_to`` (program.rs:1304).  ``ctx=None`` plans only."""
+        h = ctypes.c_void_p()
+        _check(self._lib.zs_linked_lower(self._h, ctx._h if ctx else None, flags, ctypes.byref(h)))
+        return Executable(h, self, ctx)
+
+
+class Executable:
+    """``run::Executable`` (run.rs:38-62) of one lowered program."""
+
+    def __init__(self, handle, program: Program, ctx: Optional["Context"]):
+        self._lib = _abi.lib()
+        self._h = handle
+        self.program = program
+        self.ctx = ctx
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zh_program_destroy(self._h)
+            self._h = None
+
+    def describe(self) -> str:
+        need = ctypes.c_size_t()
+        _check(self._lib.zh_program_describe(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self._lib.zh_program_describe(self._h, buf, need.value, None))
+        return buf.value.decode()
+
+    def stats(self) -> Tuple[int, int]:
+        n, b = ctypes.c_uint32(), ctypes.c_uint64()
+        _check(self._lib.zh_program_stats(self._h, ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
+    def knob(self, knob: int, data: bytes):
+        """``Environment::knob`` (run.rs:1292-1306)."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(self._lib.zh_program_set_knob(self._h, knob, buf, len(data)))
+
+    def _bindings(self, inputs: Dict[int, "Image"], outputs: Dict[int, "Image"]):
+        binds = []
+        for reg, img in inputs.items():
+            binds.append(BindingC(self.program.input_slot(reg), 0, img._h))
+        for reg, img in outputs.items():
+            binds.append(BindingC(self.program.output_slot(reg), 1, img._h))
+        arr = (BindingC * len(binds))(*binds)
+        return arr, len(binds)
+
+    def launch(self, inputs: Dict[int, "Image"], outputs: Dict[int, "Image"]):
+        """Bind registers to images and launch asynchronously
+        (``Environment::bind`` + ``Executable::launch`` + stepping)."""
+        arr, n = self._bindings(inputs, outputs)
+        _check(self._lib.zh_program_launch(self._h, arr, n))
+
+    def time_stages(self, inputs, outputs, iterations: int = 10):
+        arr, n = self._bindings(inputs, outputs)
+        out = (ctypes.c_float * 64)()
+        cnt = ctypes.c_uint32()
+        _check(self._lib.zh_program_time(self._h, arr, n, iterations, out, 64, ctypes.byref(cnt)))
+        return [out[i] for i in range(cnt.value)]
+
+
+class Context:
+    """One HIP device and stream (the analogue of a ``Pool`` device)."""
+
+    def __init__(self, device: int = 0, stream: Optional[int] = None):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_context_create(device, stream, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.zh_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return self._lib.zh_context_stream(self._h)
+
+    def sync(self):
+        _check(self._lib.zh_sync(self._h))
+
+    def is_running(self) -> bool:
+        r = ctypes.c_int32()
+        _check(self._lib.zh_query(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def image(self, desc: Descriptor) -> "Image":
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_image_alloc(self._h, ctypes.byref(desc.c()), ctypes.byref(h)))
+        return Image(self, desc, h)
+
+    def wrap(self, desc: Descriptor, device_ptr: int, nbytes: int) -> "Image":
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_image_wrap(self._h, ctypes.byref(desc.c()), device_ptr, nbytes, ctypes.byref(h)))
+        return Image(self, desc, h, keep=None)
+
+
+class Image:
+    def __init__(self, ctx: Context, desc: Descriptor, handle, keep=None):
+        self.ctx = ctx
+        self.desc = desc
+        self._h = handle
+        self._keep = keep
+
+    def free(self):
+        if getattr(self, "_h", None):
+            _abi.lib().zh_image_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+    @property
+    def device_ptr(self) -> int:
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_abi.lib().zh_image_device_ptr(self._h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value
+
+    def upload(self, data: np.ndarray, sync: bool = True):
+        """Copy tightly packed host rows into the image."""
+        arr = np.ascontiguousarray(data)
+        row = self.desc.width * self.desc.texel.bytes
+        if arr.nbytes != row * self.desc.height:
+            raise ValueError(f"expected {row * self.desc.height} bytes, got {arr.nbytes}")
+        _check(_abi.lib().zh_image_upload(self._h, arr.ctypes.data, row))
+        if sync:
+            self.ctx.sync()
+
+    def download(self) -> np.ndarray:
+        """Tightly packed bytes of the image (uint8, shape (H, W * bpp))."""
+        row = self.desc.width * self.desc.texel.bytes
+        out = np.empty((self.desc.height, row), dtype=np.uint8)
+        _check(_abi.lib().zh_image_download(self._h, out.ctypes.data, row))
+        self.ctx.sync()
+        return out
+
+
+def version() -> str:
+    return _abi.lib().zh_version().decode()

@@ -1,0 +1,115 @@
+"""ctypes declarations of ``include/zosimos_hip.h`` and ``include/zosimos_command.h``.
+
+Loading fails loudly when the built library is missing: there is no CPU
+fallback behind these entry points.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzosimos_hip.so")
+
+u32, i32, u64, f32 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_float
+vp, sz = ctypes.c_void_p, ctypes.c_size_t
+
+
+class Texel(ctypes.Structure):
+    _fields_ = [("bits", u32), ("parts", u32)]
+
+
+class ColorC(ctypes.Structure):
+    _fields_ = [("model", u32), ("primaries", u32), ("transfer", u32), ("whitepoint", u32), ("luminance", u32)]
+
+
+class DescriptorC(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("row_stride", u64), ("texel_stride", u32), ("texel", Texel),
+                ("color", ColorC)]
+
+
+class RectC(ctypes.Structure):
+    _fields_ = [("x", u32), ("y", u32), ("max_x", u32), ("max_y", u32)]
+
+
+class OpC(ctypes.Structure):
+    _fields_ = [("high", u32), ("dst", u32), ("src", u32), ("target", u32), ("initializer", u32), ("shader", u32),
+                ("num_operands", u32), ("operands", u32 * 2), ("knob", i32), ("params", vp), ("params_len", u32),
+                ("selection", RectC), ("quad", f32 * 8), ("viewport", RectC)]
+
+
+class BindingC(ctypes.Structure):
+    _fields_ = [("slot", u32), ("is_output", u32), ("image", vp)]
+
+
+# name: (restype, argtypes)
+_SIGNATURES = {
+    # zosimos_hip.h
+    "zh_context_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
+    "zh_context_destroy": (i32, [vp]),
+    "zh_sync": (i32, [vp]),
+    "zh_query": (i32, [vp, ctypes.POINTER(i32)]),
+    "zh_context_stream": (vp, [vp]),
+    "zh_image_alloc": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(vp)]),
+    "zh_image_wrap": (i32, [vp, ctypes.POINTER(DescriptorC), vp, sz, ctypes.POINTER(vp)]),
+    "zh_image_free": (i32, [vp]),
+    "zh_image_device_ptr": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+    "zh_image_upload": (i32, [vp, vp, u64]),
+    "zh_image_download": (i32, [vp, vp, u64]),
+    "zh_program_build": (i32, [vp, ctypes.POINTER(DescriptorC), u32, ctypes.POINTER(OpC), u32, u32,
+                               ctypes.POINTER(vp)]),
+    "zh_program_destroy": (i32, [vp]),
+    "zh_program_describe": (i32, [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]),
+    "zh_program_stats": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
+    "zh_program_set_knob": (i32, [vp, i32, vp, u32]),
+    "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
+    "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
+                              ctypes.POINTER(u32)]),
+    "zh_last_error": (ctypes.c_char_p, []),
+    "zh_version": (ctypes.c_char_p, []),
+    # zosimos_command.h
+    "zs_command_create": (i32, [ctypes.POINTER(vp)]),
+    "zs_command_destroy": (i32, [vp]),
+    "zs_command_input": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
+    "zs_command_output": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(DescriptorC)]),
+    "zs_command_crop": (i32, [vp, u32, RectC, ctypes.POINTER(u32)]),
+    "zs_command_color_convert": (i32, [vp, u32, ctypes.POINTER(ColorC), ctypes.POINTER(Texel), ctypes.POINTER(u32)]),
+    "zs_command_inscribe": (i32, [vp, u32, RectC, u32, ctypes.POINTER(u32)]),
+    "zs_command_transmute": (i32, [vp, u32, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
+    "zs_command_affine": (i32, [vp, u32, ctypes.POINTER(f32), u32, u32, ctypes.POINTER(u32)]),
+    "zs_command_blend": (i32, [vp, u32, RectC, u32, u32, ctypes.POINTER(u32)]),
+    "zs_command_solid_rgba": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(f32), ctypes.POINTER(u32)]),
+    "zs_command_oklab_adjust": (i32, [vp, u32, f32, f32, ctypes.POINTER(u32)]),
+    "zs_command_knob": (i32, [vp, u32, ctypes.POINTER(i32)]),
+    "zs_command_describe": (i32, [vp, u32, ctypes.POINTER(DescriptorC)]),
+    "zs_command_compile": (i32, [vp, ctypes.POINTER(vp)]),
+    "zs_linked_destroy": (i32, [vp]),
+    "zs_linked_high": (i32, [vp, ctypes.POINTER(ctypes.POINTER(DescriptorC)), ctypes.POINTER(u32),
+                             ctypes.POINTER(ctypes.POINTER(OpC)), ctypes.POINTER(u32)]),
+    "zs_linked_input_slot": (i32, [vp, u32, ctypes.POINTER(u32)]),
+    "zs_linked_output_slot": (i32, [vp, u32, ctypes.POINTER(u32)]),
+    "zs_linked_lower": (i32, [vp, vp, u32, ctypes.POINTER(vp)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded backend library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m zosimos_amd.build` "
+                "(the HIP backend has no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)

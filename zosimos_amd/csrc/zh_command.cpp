@@ -1,0 +1,546 @@
+// zh_command.cpp -- C++ mirror of zosimos's CommandBuffer and Linker.
+//
+// Each constructor validates its operands the way lib/zosimos/src/command.rs
+// does and records an SSA op; zs_command_compile emits the High list the way
+// Linker::link_in does (command.rs:2345-2880), one texture per register.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/zosimos_command.h"
+#include "zh_internal.h"
+
+using namespace zh;
+
+namespace {
+
+enum OpKind { K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST };
+
+struct CmdOp {
+  OpKind kind;
+  zh_descriptor desc{};  // register descriptor (outputs: the source's)
+  uint32_t src = 0, above = 0;
+  zh_rect rect{};
+  uint32_t shader = 0;                  // colour conversion shader key
+  std::vector<uint8_t> blob;            // std140 parameters
+  float matrix[9] = {0};
+  uint32_t sampling = 0;
+  int32_t knob = -1;
+};
+
+// RGB -> XYZ from chromaticities and white point (textbook derivation, as
+// image-canvas Primaries::to_xyz_row_matrix provides it to command.rs:1023).
+bool rgb_to_xyz(uint32_t primaries, uint32_t wp, Mat3* out) {
+  static const double chroma[8][3][2] = {
+      {{0, 0}, {0, 0}, {0, 0}},
+      {{0.630, 0.340}, {0.310, 0.595}, {0.155, 0.070}},  // BT.601 525 (SMPTE C)
+      {{0.640, 0.330}, {0.290, 0.600}, {0.150, 0.060}},  // BT.601 625 (EBU)
+      {{0.640, 0.330}, {0.300, 0.600}, {0.150, 0.060}},  // BT.709 / sRGB
+      {{0.630, 0.340}, {0.310, 0.595}, {0.155, 0.070}},  // SMPTE 240M
+      {{0.708, 0.292}, {0.170, 0.797}, {0.131, 0.046}},  // BT.2020
+      {{0.708, 0.292}, {0.170, 0.797}, {0.131, 0.046}},  // BT.2100
+      {{0.680, 0.320}, {0.265, 0.690}, {0.150, 0.060}},  // Display P3 (SMPTE EG 432-1)
+  };
+  static const double white[11][2] = {{0.44757, 0.40745}, {0.34840, 0.35160}, {0.31006, 0.31616},
+                                      {0.3457, 0.3585},   {0.33242, 0.34743}, {0.3127, 0.3290},
+                                      {0.29902, 0.31485}, {1.0 / 3, 1.0 / 3}, {0.37208, 0.37529},
+                                      {0.31292, 0.32933}, {0.38052, 0.37713}};
+  if (primaries > 7 || wp > 10) return false;
+  if (primaries == 0) { *out = mat_identity(); return true; }
+  Mat3 P;
+  for (int c = 0; c < 3; ++c) {
+    const double x = chroma[primaries][c][0], y = chroma[primaries][c][1];
+    P[0 * 3 + c] = x / y;
+    P[1 * 3 + c] = 1.0;
+    P[2 * 3 + c] = (1.0 - x - y) / y;
+  }
+  const double wx = white[wp][0], wy = white[wp][1];
+  const double W[3] = {wx / wy, 1.0, (1.0 - wx - wy) / wy};
+  Mat3 Pi;
+  if (!mat_inv(P, &Pi)) return false;
+  double s[3];
+  for (int i = 0; i < 3; ++i) s[i] = Pi[i * 3] * W[0] + Pi[i * 3 + 1] * W[1] + Pi[i * 3 + 2] * W[2];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) P[r * 3 + c] *= s[c];
+  *out = P;
+  return true;
+}
+
+// RowMatrix::into_mat3x3_std140 (color_matrix.rs:151-161).
+std::vector<uint8_t> std140(const Mat3& m) {
+  const float f[12] = {(float)m[0], (float)m[3], (float)m[6], 0.0f, (float)m[1], (float)m[4],
+                       (float)m[7], 0.0f,        (float)m[2], (float)m[5], (float)m[8], 0.0f};
+  std::vector<uint8_t> b(sizeof(f));
+  std::memcpy(b.data(), f, sizeof(f));
+  return b;
+}
+
+bool chroma_equal(const zh_descriptor& a, const zh_descriptor& b) {
+  return a.texel.bits == b.texel.bits && a.texel.parts == b.texel.parts && a.color.model == b.color.model &&
+         (a.color.model == ZH_COLOR_OKLAB ||
+          (a.color.transfer == b.color.transfer &&
+           (a.color.model != ZH_COLOR_RGB ||
+            (a.color.primaries == b.color.primaries && a.color.whitepoint == b.color.whitepoint))));
+}
+
+zh_rect with_size(uint32_t w, uint32_t h) { return zh_rect{0, 0, w, h}; }
+uint32_t rw(zh_rect r) { return r.max_x > r.x ? r.max_x - r.x : 0; }
+uint32_t rh(zh_rect r) { return r.max_y > r.y ? r.max_y - r.y : 0; }
+bool rect_eq(zh_rect a, zh_rect b) { return a.x == b.x && a.y == b.y && a.max_x == b.max_x && a.max_y == b.max_y; }
+// Rectangle::contains, command.rs:3560-3571.
+bool contains(zh_rect s, zh_rect o) {
+  if (!(s.x <= o.x && s.y <= o.y)) return false;
+  const uint32_t ox = o.x - s.x, oy = o.y - s.y;
+  if (rw(s) < ox || rh(s) < oy) return false;
+  return rw(s) - ox >= rw(o) && rh(s) - oy >= rh(o);
+}
+// Rectangle::normalize as written at command.rs:3536-3543: the height of the
+// result is the width (kept: inscribe's draws depend on it).
+zh_rect normalize(zh_rect r) { return zh_rect{r.x, r.y, r.x + rw(r), r.y + rw(r)}; }
+
+void quad_of_rect(zh_rect r, float q[8]) {
+  const float x0 = (float)r.x, y0 = (float)r.y, x1 = (float)r.max_x, y1 = (float)r.max_y;
+  const float v[8] = {x0, y0, x1, y0, x1, y1, x0, y1};
+  std::memcpy(q, v, sizeof(v));
+}
+
+}  // namespace
+
+struct zs_command {
+  std::vector<CmdOp> ops;
+  int32_t next_knob = 0;
+};
+
+struct zs_linked {
+  std::vector<zh_descriptor> textures;
+  std::vector<zh_op> ops;
+  std::vector<std::vector<uint8_t>> blobs;
+  std::vector<int64_t> input_slot, output_slot;  // per register
+};
+
+static zh_status reg_desc(zs_command* c, uint32_t r, zh_descriptor* d) {
+  if (!c || r >= c->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
+  if (c->ops[r].kind == K_OUTPUT) return fail(ZH_ERR_TYPE, "register is an output, not a texture");
+  *d = c->ops[r].desc;
+  return ZH_OK;
+}
+
+static uint32_t push(zs_command* c, CmdOp op) {
+  c->ops.push_back(std::move(op));
+  return (uint32_t)c->ops.size() - 1;
+}
+
+static zh_descriptor normalized(zh_descriptor d) {
+  d.texel_stride = bits_bytes(d.texel.bits);
+  return d;
+}
+
+extern "C" {
+
+zh_status zs_command_create(zs_command** out) {
+  if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null output");
+  *out = new zs_command();
+  return ZH_OK;
+}
+
+zh_status zs_command_destroy(zs_command* cmd) {
+  delete cmd;
+  return ZH_OK;
+}
+
+zh_status zs_command_input(zs_command* cmd, const zh_descriptor* desc, uint32_t* reg) {
+  if (!cmd || !desc || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent input declared: " + why);
+  CmdOp op;
+  op.kind = K_INPUT;
+  op.desc = normalized(*desc);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat) {
+  if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor d;
+  zh_status st = reg_desc(cmd, src, &d);
+  if (st != ZH_OK) return st;
+  CmdOp op;
+  op.kind = K_OUTPUT;
+  op.src = src;
+  op.desc = d;
+  *reg = push(cmd, op);
+  if (outformat) *outformat = d;
+  return ZH_OK;
+}
+
+zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor d;
+  zh_status st = reg_desc(cmd, src, &d);
+  if (st != ZH_OK) return st;
+  CmdOp op;
+  op.kind = K_CROP;
+  op.src = src;
+  op.rect = rect;
+  op.desc = d;  // command.rs:972-977 keeps the source descriptor
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_color_convert(zs_command* cmd, uint32_t src, const zh_color* color, const zh_texel* texel,
+                                   uint32_t* reg) {
+  if (!color || !texel || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor s;
+  zh_status st = reg_desc(cmd, src, &s);
+  if (st != ZH_OK) return st;
+  CmdOp op;
+  op.kind = K_CONVERT;
+  op.src = src;
+  const zh_color& a = s.color;
+  const zh_color& b = *color;
+  Mat3 ps, pd;
+  if (a.model == ZH_COLOR_RGB && b.model == ZH_COLOR_RGB && a.whitepoint == b.whitepoint) {
+    // Linear RGB(src) -> XYZ -> linear RGB(dst): P_dst^-1 P_src.
+    Mat3 pdi;
+    if (!rgb_to_xyz(a.primaries, a.whitepoint, &ps) || !rgb_to_xyz(b.primaries, b.whitepoint, &pd) ||
+        !mat_inv(pd, &pdi))
+      return fail(ZH_ERR_TYPE, "unknown primaries or white point");
+    op.shader = ZH_SHADER_LINEAR_COLOR_MATRIX;
+    op.blob = std140(mat_mul(pdi, ps));
+  } else if (a.model == ZH_COLOR_RGB && a.whitepoint == ZH_WHITEPOINT_D65 && b.model == ZH_COLOR_OKLAB) {
+    if (!rgb_to_xyz(a.primaries, ZH_WHITEPOINT_D65, &ps)) return fail(ZH_ERR_TYPE, "unknown primaries");
+    op.shader = ZH_SHADER_OKLAB_ENCODE;
+    op.blob = std140(ps);
+  } else if (a.model == ZH_COLOR_OKLAB && b.model == ZH_COLOR_RGB && b.whitepoint == ZH_WHITEPOINT_D65) {
+    Mat3 pdi;
+    if (!rgb_to_xyz(b.primaries, ZH_WHITEPOINT_D65, &pd) || !mat_inv(pd, &pdi))
+      return fail(ZH_ERR_TYPE, "unknown primaries");
+    op.shader = ZH_SHADER_OKLAB_DECODE;
+    op.blob = std140(pdi);
+  } else {
+    return fail(ZH_ERR_TYPE, "No conversion");
+  }
+  zh_descriptor d = s;
+  d.color = *color;
+  d.texel = *texel;
+  d.row_stride = 0;
+  d = normalized(d);
+  std::string why;
+  if (!descriptor_valid(d, &why)) return fail(ZH_ERR_TYPE, "conversion target: " + why);
+  op.desc = d;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_inscribe(zs_command* cmd, uint32_t below, zh_rect rect, uint32_t above, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor db, da;
+  zh_status st = reg_desc(cmd, below, &db);
+  if (st == ZH_OK) st = reg_desc(cmd, above, &da);
+  if (st != ZH_OK) return st;
+  if (!chroma_equal(db, da)) return fail(ZH_ERR_TYPE, "ConflictingTypes: inscribe of a different texel/colour");
+  if (!rect_eq(with_size(da.width, da.height), rect)) return fail(ZH_ERR_OTHER, "inscribe rect must be the above image");
+  if (!contains(with_size(db.width, db.height), rect)) return fail(ZH_ERR_OTHER, "inscribe rect outside the image");
+  CmdOp op;
+  op.kind = K_INSCRIBE;
+  op.src = below;
+  op.above = above;
+  op.rect = normalize(rect);
+  op.desc = db;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_transmute(zs_command* cmd, uint32_t src, const zh_descriptor* target, uint32_t* reg) {
+  if (!target || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor s;
+  zh_status st = reg_desc(cmd, src, &s);
+  if (st != ZH_OK) return st;
+  if (s.width != target->width || s.height != target->height)
+    return fail(ZH_ERR_TYPE, "invalid transmute with mismatched size");
+  if (bits_bytes(s.texel.bits) != bits_bytes(target->texel.bits))
+    return fail(ZH_ERR_TYPE, "ConflictingTypes: transmute between texels of different size");
+  std::string why;
+  if (!descriptor_valid(*target, &why)) return fail(ZH_ERR_TYPE, "invalid transmute with inconsistent result");
+  CmdOp op;
+  op.kind = K_TRANSMUTE;
+  op.src = src;
+  op.desc = normalized(*target);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_affine(zs_command* cmd, uint32_t below, const float matrix[9], uint32_t sampling,
+                            uint32_t above, uint32_t* reg) {
+  if (!matrix || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor db, da;
+  zh_status st = reg_desc(cmd, below, &db);
+  if (st == ZH_OK) st = reg_desc(cmd, above, &da);
+  if (st != ZH_OK) return st;
+  if (!chroma_equal(db, da)) return fail(ZH_ERR_TYPE, "affine of a different texel/colour");
+  // RowMatrix::det in f32 against f32::EPSILON (command.rs:1648-1656).
+  const float* m = matrix;
+  const float det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                    m[2] * (m[3] * m[7] - m[4] * m[6]);
+  if (!(std::fabs(det) >= 1.1920929e-07f)) return fail(ZH_ERR_OTHER, "singular affine transformation");
+  if (sampling != ZS_AFFINE_NEAREST && sampling != ZS_AFFINE_BILINEAR)
+    return fail(ZH_ERR_INVALID_ARGUMENT, "unknown sampling");
+  CmdOp op;
+  op.kind = K_AFFINE;
+  op.src = below;
+  op.above = above;
+  std::memcpy(op.matrix, matrix, sizeof(op.matrix));
+  op.sampling = sampling;
+  op.desc = db;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_blend(zs_command* cmd, uint32_t below, zh_rect rect, uint32_t above, uint32_t blend,
+                           uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (blend != ZS_BLEND_ALPHA) return fail(ZH_ERR_UNIMPLEMENTED, "unknown blend mode");
+  zh_descriptor db, da;
+  zh_status st = reg_desc(cmd, below, &db);
+  if (st == ZH_OK) st = reg_desc(cmd, above, &da);
+  if (st != ZH_OK) return st;
+  if (!chroma_equal(db, da)) return fail(ZH_ERR_TYPE, "blend of a different texel/colour");
+  if (rw(rect) != da.width || rh(rect) != da.height) return fail(ZH_ERR_OTHER, "blend rect must have the above size");
+  if (!contains(with_size(db.width, db.height), rect)) return fail(ZH_ERR_OTHER, "blend rect outside the image");
+  CmdOp op;
+  op.kind = K_BLEND;
+  op.src = below;
+  op.above = above;
+  op.rect = rect;
+  op.desc = db;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_solid_rgba(zs_command* cmd, const zh_descriptor* desc, const float color[4], uint32_t* reg) {
+  if (!cmd || !desc || !color || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent constant color image created");
+  CmdOp op;
+  op.kind = K_SOLID;
+  op.desc = normalized(*desc);
+  op.blob.resize(16);
+  std::memcpy(op.blob.data(), color, 16);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor s;
+  zh_status st = reg_desc(cmd, src, &s);
+  if (st != ZH_OK) return st;
+  if (s.color.model != ZH_COLOR_OKLAB) return fail(ZH_ERR_TYPE, "oklab_adjust needs an Oklab register");
+  CmdOp op;
+  op.kind = K_ADJUST;
+  op.src = src;
+  op.desc = s;
+  const float p[4] = {chroma, hue_rad, 0.0f, 0.0f};
+  op.blob.resize(16);
+  std::memcpy(op.blob.data(), p, 16);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob) {
+  if (!cmd || !knob || reg >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
+  CmdOp& op = cmd->ops[reg];
+  if (op.kind == K_INPUT || op.kind == K_OUTPUT || op.kind == K_TRANSMUTE || op.kind == K_CROP ||
+      op.kind == K_INSCRIBE || op.kind == K_BLEND)
+    return fail(ZH_ERR_TYPE, "operation has no parameter data to override");
+  if (op.knob < 0) op.knob = cmd->next_knob++;
+  *knob = op.knob;
+  return ZH_OK;
+}
+
+zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc) {
+  if (!desc) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  return reg_desc(cmd, reg, desc);
+}
+
+zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
+  if (!cmd || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zs_linked* L = new zs_linked();
+  const size_t n = cmd->ops.size();
+  L->textures.resize(n);
+  L->input_slot.assign(n, -1);
+  L->output_slot.assign(n, -1);
+  uint32_t inputs = 0, outputs = 0;
+  auto draw = [&](uint32_t dst, uint32_t target, uint32_t init, uint32_t shader, std::vector<uint32_t> operands,
+                  std::vector<uint8_t> blob, int32_t knob) -> zh_op& {
+    zh_op op;
+    std::memset(&op, 0, sizeof(op));
+    op.high = ZH_HIGH_DRAW;
+    op.dst = dst;
+    op.target = target;
+    op.initializer = init;
+    op.shader = shader;
+    op.num_operands = (uint32_t)operands.size();
+    for (size_t k = 0; k < operands.size() && k < 2; ++k) op.operands[k] = operands[k];
+    op.knob = knob;
+    L->ops.push_back(op);
+    L->blobs.push_back(std::move(blob));
+    return L->ops.back();
+  };
+  // A full copy of `src` into `dst` (the Discard draw that opens inscribe,
+  // affine and blend, command.rs:2646-2661).
+  auto copy_below = [&](uint32_t dst, uint32_t src) {
+    const zh_descriptor& d = cmd->ops[src].desc;
+    zh_op& op = draw(dst, ZH_TARGET_DISCARD, ZH_INIT_PAINT_TO_SELECTION, ZH_SHADER_PAINT_COPY, {src}, {}, -1);
+    op.selection = with_size(d.width, d.height);
+    quad_of_rect(op.selection, op.quad);
+    op.viewport = op.selection;
+  };
+  for (uint32_t r = 0; r < n; ++r) {
+    const CmdOp& c = cmd->ops[r];
+    L->textures[r] = c.desc;
+    switch (c.kind) {
+      case K_INPUT: {
+        zh_op op;
+        std::memset(&op, 0, sizeof(op));
+        op.high = ZH_HIGH_INPUT;
+        op.dst = r;
+        op.src = inputs;
+        op.knob = -1;
+        L->input_slot[r] = inputs++;
+        L->ops.push_back(op);
+        L->blobs.emplace_back();
+        break;
+      }
+      case K_OUTPUT: {
+        zh_op op;
+        std::memset(&op, 0, sizeof(op));
+        op.high = ZH_HIGH_OUTPUT;
+        op.src = c.src;
+        op.dst = outputs;
+        op.knob = -1;
+        L->output_slot[r] = outputs++;
+        L->ops.push_back(op);
+        L->blobs.emplace_back();
+        break;
+      }
+      case K_CROP: {
+        // command.rs:2507-2526: selection = region, target = viewport = the
+        // region's size at the origin.
+        const zh_rect t = with_size(rw(c.rect), rh(c.rect));
+        zh_op& op = draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_TO_SELECTION, ZH_SHADER_PAINT_COPY, {c.src}, {}, c.knob);
+        op.selection = c.rect;
+        quad_of_rect(t, op.quad);
+        op.viewport = t;
+        break;
+      }
+      case K_CONVERT:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, c.shader, {c.src}, c.blob, c.knob);
+        break;
+      case K_ADJUST:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_OKLAB_ADJUST, {c.src}, c.blob, c.knob);
+        break;
+      case K_SOLID:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_SOLID_RGB, {}, c.blob, c.knob);
+        break;
+      case K_TRANSMUTE: {
+        zh_op op;
+        std::memset(&op, 0, sizeof(op));
+        op.high = ZH_HIGH_COPY;
+        op.src = c.src;
+        op.dst = r;
+        op.knob = -1;
+        L->ops.push_back(op);
+        L->blobs.emplace_back();
+        break;
+      }
+      case K_INSCRIBE: {
+        copy_below(r, c.src);
+        const zh_descriptor& a = cmd->ops[c.above].desc;
+        const zh_descriptor& b = cmd->ops[c.src].desc;
+        zh_op& op = draw(r, ZH_TARGET_LOAD, ZH_INIT_PAINT_TO_SELECTION, ZH_SHADER_PAINT_COPY, {c.above}, {}, c.knob);
+        op.selection = with_size(a.width, a.height);
+        quad_of_rect(c.rect, op.quad);
+        op.viewport = with_size(b.width, b.height);
+        break;
+      }
+      case K_BLEND: {
+        copy_below(r, c.src);
+        const zh_descriptor& a = cmd->ops[c.above].desc;
+        const zh_descriptor& b = cmd->ops[c.src].desc;
+        zh_op& op = draw(r, ZH_TARGET_LOAD, ZH_INIT_PAINT_TO_SELECTION, ZH_SHADER_BLEND_OVER, {c.above}, {}, c.knob);
+        op.selection = with_size(a.width, a.height);
+        quad_of_rect(c.rect, op.quad);
+        op.viewport = with_size(b.width, b.height);
+        break;
+      }
+      case K_AFFINE: {
+        copy_below(r, c.src);
+        const zh_descriptor& a = cmd->ops[c.above].desc;
+        const zh_descriptor& b = cmd->ops[c.src].desc;
+        Mat3 m;
+        for (int k = 0; k < 9; ++k) m[k] = c.matrix[k];
+        const bool bil = c.sampling == ZS_AFFINE_BILINEAR;
+        zh_op& op = draw(r, ZH_TARGET_LOAD, ZH_INIT_PAINT_TO_SELECTION,
+                         bil ? ZH_SHADER_AFFINE_BILINEAR : ZH_SHADER_PAINT_COPY, {c.above},
+                         bil ? std140(m) : std::vector<uint8_t>(), c.knob);
+        op.selection = with_size(a.width, a.height);
+        // QuadTarget::affine (program.rs:1897-1906): the unit quad of the
+        // above image in its pixels, each corner through multiply_point.
+        const double cx[4] = {0.0, (double)a.width, (double)a.width, 0.0};
+        const double cy[4] = {0.0, 0.0, (double)a.height, (double)a.height};
+        for (int k = 0; k < 4; ++k) {
+          const double q = m[6] * cx[k] + m[7] * cy[k] + m[8];
+          op.quad[2 * k] = (float)((m[0] * cx[k] + m[1] * cy[k] + m[2]) / q);
+          op.quad[2 * k + 1] = (float)((m[3] * cx[k] + m[4] * cy[k] + m[5]) / q);
+        }
+        op.viewport = with_size(b.width, b.height);
+        break;
+      }
+    }
+  }
+  for (size_t i = 0; i < L->ops.size(); ++i) {
+    L->ops[i].params = L->blobs[i].empty() ? nullptr : L->blobs[i].data();
+    L->ops[i].params_len = (uint32_t)L->blobs[i].size();
+  }
+  *out = L;
+  return ZH_OK;
+}
+
+zh_status zs_linked_destroy(zs_linked* prog) {
+  delete prog;
+  return ZH_OK;
+}
+
+zh_status zs_linked_high(zs_linked* L, const zh_descriptor** textures, uint32_t* num_textures, const zh_op** ops,
+                         uint32_t* num_ops) {
+  if (!L) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (textures) *textures = L->textures.data();
+  if (num_textures) *num_textures = (uint32_t)L->textures.size();
+  if (ops) *ops = L->ops.data();
+  if (num_ops) *num_ops = (uint32_t)L->ops.size();
+  return ZH_OK;
+}
+
+zh_status zs_linked_input_slot(zs_linked* L, uint32_t reg, uint32_t* slot) {
+  if (!L || !slot || reg >= L->input_slot.size() || L->input_slot[reg] < 0)
+    return fail(ZH_ERR_INVALID_ARGUMENT, "register is not an input");
+  *slot = (uint32_t)L->input_slot[reg];
+  return ZH_OK;
+}
+
+zh_status zs_linked_output_slot(zs_linked* L, uint32_t reg, uint32_t* slot) {
+  if (!L || !slot || reg >= L->output_slot.size() || L->output_slot[reg] < 0)
+    return fail(ZH_ERR_INVALID_ARGUMENT, "register is not an output");
+  *slot = (uint32_t)L->output_slot[reg];
+  return ZH_OK;
+}
+
+zh_status zs_linked_lower(zs_linked* L, zh_context* ctx, uint32_t flags, zh_program** out) {
+  if (!L) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  return zh_program_build(ctx, L->textures.data(), (uint32_t)L->textures.size(), L->ops.data(), (uint32_t)L->ops.size(),
+                          flags, out);
+}
+
+}  // extern "C"

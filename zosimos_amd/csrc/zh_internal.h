@@ -1,0 +1,82 @@
+// zh_internal.h -- host-side structures shared by the runtime, planner and
+// command front end.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/zosimos_hip.h"
+#include "zh_stage.h"
+
+namespace zh {
+
+zh_status fail(zh_status code, const std::string& msg);
+
+// ---- texel format helpers ------------------------------------------------
+uint32_t bits_bytes(uint32_t bits);  // SampleBits::bytes for a stage.frag id, 0 if unknown
+// RGBA8/BGRA8 with sRGB or linear transfer on an RGB colour: the formats the
+// reference hands to the texture unit (program.rs:794-838).
+bool is_native(const zh_descriptor& d);
+// Transfer applied when staging this descriptor (ImageDescriptor::new,
+// program.rs:840-899): the RGB/Scalars transfer, or LabLch / Linear for Oklab.
+uint32_t stage_transfer(const zh_descriptor& d);
+uint32_t stage_parts(const zh_descriptor& d);
+uint64_t tight_stride(const zh_descriptor& d);
+uint64_t row_stride_of(const zh_descriptor& d);
+bool descriptor_valid(const zh_descriptor& d, std::string* why);
+bool same_layout(const zh_descriptor& a, const zh_descriptor& b);
+
+using Mat3 = std::array<double, 9>;  // row-major
+Mat3 mat_mul(const Mat3& a, const Mat3& b);
+bool mat_inv(const Mat3& a, Mat3* out);
+Mat3 mat_identity();
+Mat3 mat_from_std140(const void* blob);  // 12 floats, column-major with padding
+
+// ---- planned program -------------------------------------------------------
+struct StagePlan {
+  StageDesc desc;                 // host copy; slot ptrs patched per launch
+  int tile_mode = 0;              // resample: TILE_* of zh_kernels.hip
+  uint32_t lds_bytes = 0;
+  uint64_t alg_bytes = 0;         // algorithmic HBM bytes (reads once + writes once)
+  // slot -> what it refers to: >=0 input slot, or ~output slot (<0), or
+  // INTERMEDIATE_BASE + buffer id.
+  std::vector<int64_t> slot_ref;
+  std::string text;               // description
+};
+
+constexpr int64_t kIntermediateBase = 1ll << 40;
+
+struct Plan {
+  std::vector<StagePlan> stages;
+  std::vector<zh_descriptor> intermediates;  // device buffers owned by the program
+  std::vector<zh_descriptor> input_desc;     // per input slot (declared texture desc)
+  std::vector<zh_descriptor> output_desc;    // per output slot
+  std::vector<bool> input_used, output_used;
+  bool batched = false;                      // resample stages run as one tile queue
+  std::string text;
+};
+
+// Build a fused plan from a High op list.
+zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::vector<zh_op>& ops,
+                       const std::vector<std::vector<uint8_t>>& params, uint32_t flags, Plan* out);
+
+}  // namespace zh
+
+struct zh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  float* d_lut = nullptr;        // 256 floats
+  uint32_t* d_error = nullptr;   // kernel error flag
+};
+
+struct zh_image {
+  zh_context* ctx = nullptr;
+  zh_descriptor desc{};
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  bool owned = false;
+};

@@ -1,0 +1,640 @@
+// zh_kernels.hip -- gfx950 kernels of the fused compositing stages.
+//
+// One lane owns kPix = 4 horizontally adjacent pixels.  A stage's micro-op
+// program (zh_stage.h) is read through the constant address space, so every
+// instruction fetch and dispatch is scalar; the per-pixel colour work runs on
+// the vector ALUs.  Contiguous, 16-byte aligned groups move through 128-bit
+// global loads/stores (RGBA8: one dwordx4 per lane, RGBA16F: two, RGBA32F:
+// four), so each wave touches 1-4 KiB of consecutive HBM per instruction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zh_codec.h"
+#include "zh_stage.h"
+
+namespace zh {
+
+typedef const __attribute__((address_space(4))) StageDesc* CDesc;
+typedef const __attribute__((address_space(4))) BatchDesc* CBatch;
+
+__device__ __forceinline__ CDesc as_const(const StageDesc* p) { return (CDesc)(uintptr_t)p; }
+
+// Copy a descriptor record out of the constant address space (scalar loads).
+template <typename T>
+__device__ __forceinline__ T ldc(const __attribute__((address_space(4))) T* p) {
+  static_assert(sizeof(T) % 4 == 0, "records are dword multiples");
+  T r;
+  const __attribute__((address_space(4))) uint32_t* src = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return r;
+}
+
+// The pixels a lane works on: either kPix contiguous pixels of one row
+// starting at (x0, y), of which the first n exist, or kPix gathered texel
+// coordinates with their own validity.
+struct Ctx {
+  int x0, y, n;
+  int dx, dy;       // OP_OFFSET displacement for loads
+  bool gather;
+  int gx[kPix], gy[kPix];
+  bool gv[kPix];
+};
+
+typedef float4 Px[kPix];
+
+__device__ __forceinline__ const uint8_t* texel_addr(const Slot& s, int x, int y) {
+  return reinterpret_cast<const uint8_t*>(s.ptr) + (uint64_t)y * s.row_stride + (uint64_t)x * s.bpp;
+}
+
+// Fetch one texel's bytes (bpp in {1, 2, 3, 4, 6, 8, 16}).
+__device__ __forceinline__ uint4 fetch_one(const Slot& s, int x, int y) {
+  const uint8_t* p = texel_addr(s, x, y);
+  switch (s.bpp) {
+    case 1: return make_uint4(*p, 0u, 0u, 0u);
+    case 2: return make_uint4(*reinterpret_cast<const uint16_t*>(p), 0u, 0u, 0u);
+    case 3: return make_uint4((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16), 0u, 0u, 0u);
+    case 4: return make_uint4(*reinterpret_cast<const uint32_t*>(p), 0u, 0u, 0u);
+    case 6: {
+      const uint16_t* q = reinterpret_cast<const uint16_t*>(p);
+      return make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], 0u, 0u);
+    }
+    case 8: {
+      const uint2 v = *reinterpret_cast<const uint2*>(p);
+      return make_uint4(v.x, v.y, 0u, 0u);
+    }
+    default: return *reinterpret_cast<const uint4*>(p);
+  }
+}
+
+__device__ __forceinline__ void store_one(const Slot& s, int x, int y, uint4 v) {
+  uint8_t* p = const_cast<uint8_t*>(texel_addr(s, x, y));
+  switch (s.bpp) {
+    case 1: *p = (uint8_t)v.x; break;
+    case 2: *reinterpret_cast<uint16_t*>(p) = (uint16_t)v.x; break;
+    case 3: p[0] = (uint8_t)v.x; p[1] = (uint8_t)(v.x >> 8); p[2] = (uint8_t)(v.x >> 16); break;
+    case 4: *reinterpret_cast<uint32_t*>(p) = v.x; break;
+    case 6: {
+      uint16_t* q = reinterpret_cast<uint16_t*>(p);
+      q[0] = (uint16_t)v.x; q[1] = (uint16_t)(v.x >> 16); q[2] = (uint16_t)v.y;
+      break;
+    }
+    case 8: *reinterpret_cast<uint2*>(p) = make_uint2(v.x, v.y); break;
+    default: *reinterpret_cast<uint4*>(p) = v; break;
+  }
+}
+
+__device__ __forceinline__ bool in_slot(const Slot& s, int x, int y) {
+  return x >= 0 && y >= 0 && x < (int)s.width && y < (int)s.height;
+}
+
+// X[i] <- decode(slot) at the lane's pixels; missing pixels read zero.
+__device__ __forceinline__ void load_px(const Slot& s, const Ctx& c, Px& X, const float* lut) {
+  if (!c.gather) {
+    const int x = c.x0 + c.dx, y = c.y + c.dy;
+    const bool full = c.n == kPix && s.vec_ok && (((uint32_t)x * s.bpp) & 15u) == 0 && in_slot(s, x, y) &&
+                      in_slot(s, x + kPix - 1, y);
+    if (full && s.bpp == 4) {
+      const uint4 w = *reinterpret_cast<const uint4*>(texel_addr(s, x, y));
+      X[0] = decode_texel(s, make_uint4(w.x, 0u, 0u, 0u), lut);
+      X[1] = decode_texel(s, make_uint4(w.y, 0u, 0u, 0u), lut);
+      X[2] = decode_texel(s, make_uint4(w.z, 0u, 0u, 0u), lut);
+      X[3] = decode_texel(s, make_uint4(w.w, 0u, 0u, 0u), lut);
+      return;
+    }
+    if (full && s.bpp == 8) {
+      const uint4* p = reinterpret_cast<const uint4*>(texel_addr(s, x, y));
+      const uint4 a = p[0], b = p[1];
+      X[0] = decode_texel(s, make_uint4(a.x, a.y, 0u, 0u), lut);
+      X[1] = decode_texel(s, make_uint4(a.z, a.w, 0u, 0u), lut);
+      X[2] = decode_texel(s, make_uint4(b.x, b.y, 0u, 0u), lut);
+      X[3] = decode_texel(s, make_uint4(b.z, b.w, 0u, 0u), lut);
+      return;
+    }
+    if (full && s.bpp == 16) {
+      const uint4* p = reinterpret_cast<const uint4*>(texel_addr(s, x, y));
+#pragma unroll
+      for (int i = 0; i < kPix; ++i) X[i] = decode_texel(s, p[i], lut);
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < kPix; ++i) {
+      const bool ok = i < c.n && in_slot(s, x + i, y);
+      X[i] = ok ? decode_texel(s, fetch_one(s, x + i, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) {
+    const int x = c.gx[i] + c.dx, y = c.gy[i] + c.dy;
+    const bool ok = c.gv[i] && in_slot(s, x, y);
+    X[i] = ok ? decode_texel(s, fetch_one(s, x, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+__device__ __forceinline__ void store_px(const Slot& s, const Ctx& c, const Px& X) {
+  const int x = c.x0, y = c.y;
+  if (c.n == kPix && s.vec_ok && (((uint32_t)x * s.bpp) & 15u) == 0) {
+    if (s.bpp == 4) {
+      const uint4 w = make_uint4(encode_texel(s, X[0]).x, encode_texel(s, X[1]).x, encode_texel(s, X[2]).x,
+                                 encode_texel(s, X[3]).x);
+      *reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y))) = w;
+      return;
+    }
+    if (s.bpp == 8) {
+      const uint4 a0 = encode_texel(s, X[0]), a1 = encode_texel(s, X[1]);
+      const uint4 a2 = encode_texel(s, X[2]), a3 = encode_texel(s, X[3]);
+      uint4* p = reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y)));
+      p[0] = make_uint4(a0.x, a0.y, a1.x, a1.y);
+      p[1] = make_uint4(a2.x, a2.y, a3.x, a3.y);
+      return;
+    }
+    if (s.bpp == 16) {
+      uint4* p = reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y)));
+#pragma unroll
+      for (int i = 0; i < kPix; ++i) p[i] = encode_texel(s, X[i]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPix; ++i)
+    if (i < c.n) store_one(s, x + i, y, encode_texel(s, X[i]));
+}
+
+__device__ __forceinline__ float3 mat3(const __attribute__((address_space(4))) float* m, float3 v) {
+  return make_float3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                     m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// Signed cube root: sign(x) 2^(log2|x| / 3), refined by one Newton step.
+__device__ __forceinline__ float cbrt_s(float x) {
+  const float a = fabsf(x);
+  float y = pow_pos(a, 1.0f / 3.0f);
+  y = a > 0.0f ? y - (y * y * y - a) / (3.0f * y * y) : 0.0f;
+  return copysignf(y, x);
+}
+
+// Evaluate instructions [begin, end) of the stage program.
+__device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& c, Px& X, Px& Y, Px& Z, const float* lut) {
+  for (uint32_t pc = begin; pc < end; ++pc) {
+    const Instr in = ldc(&d->code[pc]);
+    switch (in.op) {
+      case OP_LOAD:
+      case OP_SOLID: {
+        const Slot sl = ldc(&d->slot[in.a]);
+        if (in.b) {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
+        }
+        if (in.op == OP_LOAD) {
+          load_px(sl, c, X, lut);
+        } else {
+          const float4 v = make_float4(d->params[in.arg], d->params[in.arg + 1], d->params[in.arg + 2],
+                                       d->params[in.arg + 3]);
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) X[i] = v;
+        }
+        break;
+      }
+      case OP_MAT3: {
+        const __attribute__((address_space(4))) float* m = d->params + in.arg;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float3 r = mat3(m, make_float3(X[i].x, X[i].y, X[i].z));
+          X[i] = make_float4(r.x, r.y, r.z, X[i].w);
+        }
+        break;
+      }
+      case OP_CBRT:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = make_float4(cbrt_s(X[i].x), cbrt_s(X[i].y), cbrt_s(X[i].z), X[i].w);
+        break;
+      case OP_CUBE:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i)
+          X[i] = make_float4(X[i].x * X[i].x * X[i].x, X[i].y * X[i].y * X[i].y, X[i].z * X[i].z * X[i].z, X[i].w);
+        break;
+      case OP_CLAMP01:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = make_float4(clamp01(X[i].x), clamp01(X[i].y), clamp01(X[i].z), X[i].w);
+        break;
+      case OP_PREMUL:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = make_float4(X[i].x * X[i].w, X[i].y * X[i].w, X[i].z * X[i].w, X[i].w);
+        break;
+      case OP_UNPREMUL:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float a = X[i].w;
+          const float r = a > 0.0f ? 1.0f / a : 0.0f;
+          X[i] = make_float4(X[i].x * r, X[i].y * r, X[i].z * r, a);
+        }
+        break;
+      case OP_OVER:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float k = 1.0f - X[i].w;
+          X[i] = make_float4(fmaf(Y[i].x, k, X[i].x), fmaf(Y[i].y, k, X[i].y), fmaf(Y[i].z, k, X[i].z),
+                             fmaf(Y[i].w, k, X[i].w));
+          Y[i] = Z[i];
+        }
+        break;
+      case OP_SWAP:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) { const float4 t = X[i]; X[i] = Y[i]; Y[i] = t; }
+        break;
+      case OP_EOTF:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = untransfer(X[i], in.arg);
+        break;
+      case OP_OETF:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = transfer(X[i], in.arg);
+        break;
+      case OP_QUANT: {
+        const Slot s = ldc(&d->slot[in.a]);
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = decode_texel(s, encode_texel(s, X[i]), lut);
+        break;
+      }
+      case OP_STORE: {
+        const Slot s = ldc(&d->slot[in.a]);
+        store_px(s, c, X);
+        break;
+      }
+      case OP_OFFSET:
+        c.dx = in.a ? (int)d->params[in.arg] : 0;
+        c.dy = in.a ? (int)d->params[in.arg + 1] : 0;
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+__device__ __forceinline__ void load_lut(CDesc d, float* lut) {
+  if (threadIdx.x < 256) lut[threadIdx.x] = d->srgb_lut[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// pointwise stage: one lane per kPix pixels, one pass over the images.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pointwise(const StageDesc* dd) {
+  const CDesc d = as_const(dd);
+  __shared__ float lut[256];
+  load_lut(d, lut);
+  __syncthreads();
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  Ctx c;
+  c.dx = c.dy = 0;
+  c.gather = false;
+  if (d->flat) {
+    const uint64_t total = (uint64_t)d->width * d->height;
+    const uint64_t x0 = (uint64_t)gid * kPix;
+    c.x0 = (int)x0;
+    c.y = 0;
+    c.n = x0 < total ? (int)min((uint64_t)kPix, total - x0) : 0;
+  } else {
+    const uint32_t gpr = (d->width + kPix - 1) / kPix;
+    const uint32_t y = gid / gpr;
+    c.y = (int)y;
+    c.x0 = (int)((gid - y * gpr) * kPix);
+    c.n = y < d->height ? min(kPix, (int)d->width - c.x0) : 0;
+  }
+  if (c.n <= 0) return;
+  Px X, Y, Z;
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) X[i] = Y[i] = Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  run(d, 0, d->n_instr, c, X, Y, Z, lut);
+}
+
+// ---------------------------------------------------------------------------
+// resample stage
+// ---------------------------------------------------------------------------
+// Map a pixel centre through a layer's inverse transform (FP64).
+__device__ __forceinline__ double2 map_pt(const double* m, double px, double py) {
+  const double q = m[6] * px + m[7] * py + m[8];
+  return make_double2((m[0] * px + m[1] * py + m[2]) / q, (m[3] * px + m[4] * py + m[5]) / q);
+}
+
+// Split a mapped coordinate into the texel left of the sample and the
+// bilinear weight of the right texel (texel centres at i + 0.5).
+__device__ __forceinline__ void split_tap(double u, int& i, float& f) {
+  const double t = u - 0.5;
+  const double fl = floor(t);
+  i = (int)fl;
+  f = (float)(t - fl);
+}
+
+__device__ __forceinline__ float4 f4_fma(float4 a, float k, float4 b) {
+  return make_float4(fmaf(a.x, k, b.x), fmaf(a.y, k, b.y), fmaf(a.z, k, b.z), fmaf(a.w, k, b.w));
+}
+__device__ __forceinline__ float4 f4_scale(float4 a, float k) {
+  return make_float4(a.x * k, a.y * k, a.z * k, a.w * k);
+}
+__device__ __forceinline__ float4 over4(float4 top, float4 below) {
+  return f4_fma(below, 1.0f - top.w, top);
+}
+
+enum TileMode { TILE_STAGED = 0, TILE_DIRECT = 1, TILE_NEAREST = 2 };
+
+// Source box of every bilinear tap of the tile's pixel centres (one texel of
+// margin against rounding), clipped to the source grid.  Returns false when
+// the tile's footprint misses the source.
+__device__ __forceinline__ bool tap_box(const Layer& L, double ex0, double ey0, double ex1, double ey1, int& bx0,
+                                        int& by0, int& bw, int& bh) {
+  const double2 p0 = map_pt(L.inv, ex0, ey0), p1 = map_pt(L.inv, ex1, ey0);
+  const double2 p2 = map_pt(L.inv, ex0, ey1), p3 = map_pt(L.inv, ex1, ey1);
+  const double umin = fmin(fmin(p0.x, p1.x), fmin(p2.x, p3.x)) - 0.5;
+  const double umax = fmax(fmax(p0.x, p1.x), fmax(p2.x, p3.x)) - 0.5;
+  const double vmin = fmin(fmin(p0.y, p1.y), fmin(p2.y, p3.y)) - 0.5;
+  const double vmax = fmax(fmax(p0.y, p1.y), fmax(p2.y, p3.y)) - 0.5;
+  const int sw = (int)L.src_w, sh = (int)L.src_h;
+  if (!(umax > -1.0 && vmax > -1.0 && umin < (double)sw && vmin < (double)sh)) return false;
+  bx0 = max(0, (int)floor(umin) - 1);
+  by0 = max(0, (int)floor(vmin) - 1);
+  const int bx1 = min(sw - 1, (int)floor(umax) + 2), by1 = min(sh - 1, (int)floor(vmax) + 2);
+  bw = bx1 - bx0 + 1;
+  bh = by1 - by0 + 1;
+  return bw > 0 && bh > 0;
+}
+
+// One output tile of a resample stage.  All lanes of the workgroup take the
+// same branches at every barrier (tile and layer decisions are uniform).
+template <int MODE>
+__device__ __forceinline__ void process_tile(CDesc d, int tx, int ty, float4* tile, const float* lut) {
+  const int TW = (int)d->tile_w, TH = (int)d->tile_h;
+  const int tpr = TW / kPix;
+  const int lx = (int)threadIdx.x % tpr, ly = (int)threadIdx.x / tpr;
+  const int W = (int)d->width, H = (int)d->height;
+  Ctx c;
+  c.dx = c.dy = 0;
+  c.gather = false;
+  c.x0 = tx * TW + lx * kPix;
+  c.y = ty * TH + ly;
+  c.n = (ly < TH && c.y < H) ? max(0, min(kPix, W - c.x0)) : 0;
+
+  Px X, Y, Z;
+#pragma unroll
+  for (int i = 0; i < kPix; ++i) X[i] = Y[i] = Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (d->base_end > d->base_begin && c.n > 0) run(d, d->base_begin, d->base_end, c, X, Y, Z, lut);
+
+  // Pixel-centre extent of the tile (clipped to the image).
+  const double ex0 = tx * TW + 0.5, ey0 = ty * TH + 0.5;
+  const double ex1 = min(tx * TW + TW, W) - 0.5, ey1 = min(ty * TH + TH, H) - 0.5;
+
+  for (uint32_t l = 0; l < d->n_layers; ++l) {
+    const Layer L = ldc(&d->layer[l]);
+    if (MODE == TILE_STAGED) {
+     // The tile is processed in bands of band_h rows whose source boxes fit
+     // the LDS budget (strong downscales and rotations widen the box).
+     const int BH = (int)d->band_h;
+     for (int band = 0; band < TH; band += BH) {
+      const double by_lo = ty * TH + band + 0.5;
+      const double by_hi = min(ty * TH + band + BH, H) - 0.5;
+      if (by_lo > by_hi) break;
+      int bx0, by0, bw, bh;
+      // A band outside the layer contributes nothing: 'over' a transparent
+      // sample is the identity.
+      if (!tap_box(L, ex0, by_lo, ex1, by_hi, bx0, by0, bw, bh)) continue;
+      if ((uint32_t)(bw * bh) > d->lds_texels) {
+        // The planner sizes bands so this cannot happen; flag it loudly.
+        if (threadIdx.x == 0 && d->error_flag) *d->error_flag = 1u;
+        bh = (int)d->lds_texels / bw;
+      }
+      // Cooperative fill: the layer's source program runs once per texel of
+      // the box, kPix texels per lane, and lands premultiplied in LDS.
+      const int gprow = (bw + kPix - 1) / kPix;
+      const int ngroups = gprow * bh;
+      for (int g = (int)threadIdx.x; g < ngroups; g += kBlock) {
+        const int r = g / gprow;
+        const int cx = (g - r * gprow) * kPix;
+        Ctx s;
+        s.dx = s.dy = 0;
+        s.gather = false;
+        s.x0 = bx0 + cx;
+        s.y = by0 + r;
+        s.n = min(kPix, bw - cx);
+        Px V, V1, V2;
+        run(d, L.prog_begin, L.prog_end, s, V, V1, V2, lut);
+#pragma unroll
+        for (int i = 0; i < kPix; ++i)
+          if (i < s.n) tile[r * bw + cx + i] = V[i];
+      }
+      __syncthreads();
+      if (c.n > 0 && ly >= band && ly < band + BH) {
+        const int sw = (int)L.src_w, sh = (int)L.src_h;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          if (i >= c.n) continue;
+          const double2 uv = map_pt(L.inv, c.x0 + i + 0.5, c.y + 0.5);
+          int iu, iv;
+          float ax, ay;
+          split_tap(uv.x, iu, ax);
+          split_tap(uv.y, iv, ay);
+          float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          // Taps outside the source are transparent.
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int xx = iu + (t & 1), yy = iv + (t >> 1);
+            const float w = ((t & 1) ? ax : 1.0f - ax) * ((t >> 1) ? ay : 1.0f - ay);
+            const int ty_ = yy - by0, tx_ = xx - bx0;
+            if (xx >= 0 && yy >= 0 && xx < sw && yy < sh && tx_ >= 0 && tx_ < bw && ty_ >= 0 && ty_ < bh)
+              sm = f4_fma(tile[ty_ * bw + tx_], w, sm);
+          }
+          X[i] = over4(sm, X[i]);
+        }
+      }
+      __syncthreads();
+     }
+    } else if (MODE == TILE_DIRECT) {
+      // Four gathered taps per pixel, each running the source program at its
+      // texel (served by L2); used when a tile's source box exceeds LDS.
+      if (c.n > 0) {
+        const int sw = (int)L.src_w, sh = (int)L.src_h;
+        float ax[kPix], ay[kPix];
+        int iu[kPix], iv[kPix];
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const double2 uv = map_pt(L.inv, c.x0 + i + 0.5, c.y + 0.5);
+          split_tap(uv.x, iu[i], ax[i]);
+          split_tap(uv.y, iv[i], ay[i]);
+          Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        for (int t = 0; t < 4; ++t) {
+          Ctx s;
+          s.dx = s.dy = 0;
+          s.gather = true;
+          s.x0 = s.y = 0;
+          s.n = kPix;
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) {
+            s.gx[i] = iu[i] + (t & 1);
+            s.gy[i] = iv[i] + (t >> 1);
+            s.gv[i] = i < c.n && s.gx[i] >= 0 && s.gy[i] >= 0 && s.gx[i] < sw && s.gy[i] < sh;
+          }
+          Px V, V1, V2;
+          run(d, L.prog_begin, L.prog_end, s, V, V1, V2, lut);
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) {
+            const float w = ((t & 1) ? ax[i] : 1.0f - ax[i]) * ((t >> 1) ? ay[i] : 1.0f - ay[i]);
+            if (s.gv[i]) Z[i] = f4_fma(V[i], w, Z[i]);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < kPix; ++i)
+          if (i < c.n) X[i] = over4(Z[i], X[i]);
+      }
+    } else {
+      // Nearest modes: the pixel is covered when its centre maps inside the
+      // selection (the rasteriser's coverage of the painted quad); the
+      // sampled texel is the floor of the mapped coordinate.
+      if (c.n > 0) {
+        Ctx s;
+        s.dx = s.dy = 0;
+        s.gather = true;
+        s.x0 = s.y = 0;
+        s.n = kPix;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const double2 uv = map_pt(L.inv, c.x0 + i + 0.5, c.y + 0.5);
+          s.gv[i] = i < c.n && uv.x >= L.cover[0] && uv.x < L.cover[2] && uv.y >= L.cover[1] && uv.y < L.cover[3];
+          s.gx[i] = (int)floor(uv.x);
+          s.gy[i] = (int)floor(uv.y);
+        }
+        Px V, V1, V2;
+        run(d, L.prog_begin, L.prog_end, s, V, V1, V2, lut);
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          if (!s.gv[i]) continue;
+          X[i] = L.mode == LAYER_NEAREST_REPLACE ? V[i] : over4(V[i], X[i]);
+        }
+      }
+    }
+  }
+
+  if (c.n > 0) run(d, d->post_begin, d->post_end, c, X, Y, Z, lut);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_resample(const StageDesc* dd) {
+  const CDesc d = as_const(dd);
+  extern __shared__ float4 tile[];
+  __shared__ float lut[256];
+  load_lut(d, lut);
+  __syncthreads();
+  process_tile<MODE>(d, (int)blockIdx.x, (int)blockIdx.y, tile, lut);
+}
+
+// Persistent tile queue over many LDS-staged resample stages (one launch for a
+// whole batch of images of uneven sizes and formats).  Every workgroup takes
+// tiles from a global counter until it passes the total, so the grid drains.
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* bb) {
+  const CBatch b = (CBatch)(uintptr_t)bb;
+  extern __shared__ float4 tile[];
+  __shared__ float lut[256];
+  __shared__ uint32_t job;
+  load_lut(as_const(b->stages), lut);
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) job = atomicAdd(b->counter, 1u);
+    __syncthreads();
+    const uint32_t j = job;
+    if (j >= b->total_tiles) break;
+    // Stage of tile j: the last s with prefix[s] <= j (uniform binary search).
+    uint32_t lo = 0, hi = b->n_stages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
+    }
+    const CDesc d = as_const(b->stages + lo);
+    const uint32_t t = j - b->tile_prefix[lo];
+    process_tile<TILE_STAGED>(d, (int)(t % d->tiles_x), (int)(t / d->tiles_x), tile, lut);
+  }
+}
+
+__global__ void k_reset_counter(uint32_t* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// byte copy of rows (High::Copy, same-format inscribe): 16 bytes per lane.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_copy(const StageDesc* dd) {
+  const CDesc d = as_const(dd);
+  const uint64_t chunks_per_row = (d->copy_row_bytes + 15) / 16;
+  const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t row = gid / chunks_per_row;
+  if (row >= d->copy_rows) return;
+  const uint64_t off = (gid - row * chunks_per_row) * 16;
+  const uint8_t* src = reinterpret_cast<const uint8_t*>(d->copy_src) + row * d->copy_src_stride + off;
+  uint8_t* dst = reinterpret_cast<uint8_t*>(d->copy_dst) + row * d->copy_dst_stride + off;
+  const uint64_t n = min((uint64_t)16, d->copy_row_bytes - off);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+  if (n == 16 && aligned) {
+    *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+  } else {
+    for (uint64_t i = 0; i < n; ++i) dst[i] = src[i];
+  }
+}
+
+}  // namespace zh
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers (C linkage inside the library)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat,
+                               hipStream_t stream) {
+  uint64_t groups;
+  if (flat) {
+    groups = ((uint64_t)width * height + zh::kPix - 1) / zh::kPix;
+  } else {
+    groups = (uint64_t)((width + zh::kPix - 1) / zh::kPix) * height;
+  }
+  const uint64_t blocks = (groups + zh::kBlock - 1) / zh::kBlock;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(zh::k_pointwise, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  return hipGetLastError();
+}
+
+hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
+                              uint32_t lds_bytes, hipStream_t stream) {
+  if (tiles_x == 0 || tiles_y == 0) return hipSuccess;
+  const dim3 grid(tiles_x, tiles_y), block(zh::kBlock);
+  switch (mode) {
+    case zh::TILE_STAGED: hipLaunchKernelGGL(zh::k_resample<zh::TILE_STAGED>, grid, block, lds_bytes, stream, d_dev); break;
+    case zh::TILE_DIRECT: hipLaunchKernelGGL(zh::k_resample<zh::TILE_DIRECT>, grid, block, 0, stream, d_dev); break;
+    default: hipLaunchKernelGGL(zh::k_resample<zh::TILE_NEAREST>, grid, block, 0, stream, d_dev); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(zh::k_reset_counter, dim3(1), dim3(64), 0, stream, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(zh::k_batch, dim3(grid), dim3(zh::kBlock), lds_bytes, stream, b_dev);
+  return hipGetLastError();
+}
+
+hipError_t zh_launch_copy(const zh::StageDesc* d_dev, uint64_t row_bytes, uint32_t rows, hipStream_t stream) {
+  const uint64_t chunks = ((row_bytes + 15) / 16) * rows;
+  const uint64_t blocks = (chunks + zh::kBlock - 1) / zh::kBlock;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(zh::k_copy, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  return hipGetLastError();
+}
+
+hipError_t zh_set_resample_lds(uint32_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(zh::k_resample<zh::TILE_STAGED>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(zh::k_batch), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+}  // extern "C"

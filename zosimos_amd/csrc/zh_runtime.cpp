@@ -1,0 +1,639 @@
+// zh_runtime.cpp -- contexts, images, programs and launches of the C ABI in
+// include/zosimos_hip.h.
+//
+// A program owns its planned stages, one device copy of every stage
+// descriptor, the intermediate buffers of the plan and (unless disabled) a
+// hipGraph of its launches, captured at the first launch and replayed after.
+// Launch-time binding patches image addresses into the host copies of the
+// descriptors and re-uploads only when they changed, so steady-state launches
+// are one graph launch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+
+#include "zh_internal.h"
+
+extern "C" {
+hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat,
+                               hipStream_t stream);
+hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
+                              uint32_t lds_bytes, hipStream_t stream);
+hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
+                           hipStream_t stream);
+hipError_t zh_launch_copy(const zh::StageDesc* d_dev, uint64_t row_bytes, uint32_t rows, hipStream_t stream);
+hipError_t zh_set_resample_lds(uint32_t bytes);
+}
+
+namespace zh {
+
+static thread_local std::string g_error = "no error";
+
+zh_status fail(zh_status code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+static zh_status hip_fail(hipError_t e, const char* what) {
+  return fail(ZH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define ZH_HIP(call)                                 \
+  do {                                               \
+    hipError_t _e = (call);                          \
+    if (_e != hipSuccess) return hip_fail(_e, #call); \
+  } while (0)
+
+// Exact sRGB EOTF of every 8-bit code (IEC 61966-2-1 in FP64, rounded once).
+static void srgb_table(float* t) {
+  for (int i = 0; i < 256; ++i) {
+    const double v = i / 255.0;
+    const double l = v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4);
+    t[i] = (float)l;
+  }
+}
+
+}  // namespace zh
+
+using namespace zh;
+
+// One launch unit: a single stage, or a batch of independent staged
+// resample stages run by one persistent tile-queue kernel.
+struct Unit {
+  bool batch = false;
+  size_t first = 0, count = 1;   // stage range
+  BatchDesc* d_batch = nullptr;  // device
+  uint32_t* d_counter = nullptr;
+  uint32_t* d_prefix = nullptr;
+  uint32_t grid = 0, lds = 0;
+};
+
+struct zh_program {
+  zh_context* ctx = nullptr;
+  uint32_t flags = 0;
+  std::vector<zh_descriptor> textures;
+  std::vector<zh_op> ops;
+  std::vector<std::vector<uint8_t>> params;
+  Plan plan;
+  std::vector<Unit> units;
+  StageDesc* d_stages = nullptr;       // device copies, one per stage
+  StageDesc* h_stages = nullptr;       // pinned host staging
+  std::vector<void*> d_intermediates;
+  std::vector<void*> bound_inputs, bound_outputs;
+  bool uploaded = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_failed = false;
+  uint32_t max_lds = 0;
+};
+
+extern "C" {
+
+const char* zh_last_error(void) { return g_error.c_str(); }
+const char* zh_version(void) { return "zosimos-hip 0.1.0 (gfx950)"; }
+
+zh_status zh_context_create(int32_t device, void* stream, zh_context** out) {
+  if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null output pointer");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(ZH_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(ZH_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+  ZH_HIP(hipSetDevice(device));
+  zh_context* ctx = new zh_context();
+  ctx->device = device;
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+  } else {
+    // Default priority on purpose (streams of other priorities are avoided).
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete ctx; return hip_fail(e, "hipStreamCreate"); }
+    ctx->own_stream = true;
+  }
+  float lut[256];
+  srgb_table(lut);
+  e = hipMalloc(&ctx->d_lut, sizeof(lut));
+  if (e == hipSuccess) e = hipMalloc(&ctx->d_error, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(ctx->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+  uint32_t zero = 0;
+  if (e == hipSuccess) e = hipMemcpy(ctx->d_error, &zero, sizeof(zero), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = zh_set_resample_lds(kTileLdsBytes);
+  if (e != hipSuccess) {
+    zh_context_destroy(ctx);
+    return hip_fail(e, "context setup");
+  }
+  *out = ctx;
+  return ZH_OK;
+}
+
+zh_status zh_context_destroy(zh_context* ctx) {
+  if (!ctx) return ZH_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->d_lut) hipFree(ctx->d_lut);
+  if (ctx->d_error) hipFree(ctx->d_error);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ZH_OK;
+}
+
+void* zh_context_stream(zh_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+zh_status zh_sync(zh_context* ctx) {
+  if (!ctx) return fail(ZH_ERR_INVALID_ARGUMENT, "null context");
+  ZH_HIP(hipSetDevice(ctx->device));
+  ZH_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t flag = 0;
+  ZH_HIP(hipMemcpy(&flag, ctx->d_error, sizeof(flag), hipMemcpyDeviceToHost));
+  if (flag) {
+    uint32_t zero = 0;
+    hipMemcpy(ctx->d_error, &zero, sizeof(zero), hipMemcpyHostToDevice);
+    return fail(ZH_ERR_UNSUPPORTED, "a kernel met a tile larger than its planned LDS box");
+  }
+  return ZH_OK;
+}
+
+zh_status zh_query(zh_context* ctx, int32_t* running) {
+  if (!ctx || !running) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) { *running = 0; return ZH_OK; }
+  if (e == hipErrorNotReady) { *running = 1; return ZH_OK; }
+  return hip_fail(e, "hipStreamQuery");
+}
+
+// ---- images -----------------------------------------------------------------
+static uint64_t image_bytes(const zh_descriptor& d) { return row_stride_of(d) * d.height; }
+
+zh_status zh_image_alloc(zh_context* ctx, const zh_descriptor* desc, zh_image** out) {
+  if (!ctx || !desc || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_INVALID_ARGUMENT, "descriptor: " + why);
+  ZH_HIP(hipSetDevice(ctx->device));
+  zh_image* img = new zh_image();
+  img->ctx = ctx;
+  img->desc = *desc;
+  img->desc.texel_stride = bits_bytes(desc->texel.bits);
+  img->bytes = image_bytes(*desc);
+  hipError_t e = hipMalloc(&img->ptr, img->bytes);
+  if (e != hipSuccess) {
+    delete img;
+    return fail(ZH_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  img->owned = true;
+  *out = img;
+  return ZH_OK;
+}
+
+zh_status zh_image_wrap(zh_context* ctx, const zh_descriptor* desc, void* device_ptr, size_t bytes,
+                        zh_image** out) {
+  if (!ctx || !desc || !out || !device_ptr) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_INVALID_ARGUMENT, "descriptor: " + why);
+  if (bytes < image_bytes(*desc)) return fail(ZH_ERR_INVALID_ARGUMENT, "wrapped buffer smaller than the image");
+  zh_image* img = new zh_image();
+  img->ctx = ctx;
+  img->desc = *desc;
+  img->desc.texel_stride = bits_bytes(desc->texel.bits);
+  img->ptr = device_ptr;
+  img->bytes = bytes;
+  img->owned = false;
+  *out = img;
+  return ZH_OK;
+}
+
+zh_status zh_image_free(zh_image* img) {
+  if (!img) return ZH_OK;
+  if (img->owned && img->ptr) {
+    hipSetDevice(img->ctx->device);
+    hipStreamSynchronize(img->ctx->stream);
+    hipFree(img->ptr);
+  }
+  delete img;
+  return ZH_OK;
+}
+
+zh_status zh_image_device_ptr(zh_image* img, void** ptr, size_t* bytes) {
+  if (!img) return fail(ZH_ERR_INVALID_ARGUMENT, "null image");
+  if (ptr) *ptr = img->ptr;
+  if (bytes) *bytes = img->bytes;
+  return ZH_OK;
+}
+
+zh_status zh_image_upload(zh_image* img, const void* host, uint64_t host_row_stride) {
+  if (!img || !host) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  const uint64_t row = tight_stride(img->desc);
+  if (host_row_stride == 0) host_row_stride = row;
+  if (host_row_stride < row) return fail(ZH_ERR_INVALID_ARGUMENT, "host row stride shorter than a row");
+  ZH_HIP(hipSetDevice(img->ctx->device));
+  ZH_HIP(hipMemcpy2DAsync(img->ptr, row_stride_of(img->desc), host, host_row_stride, row, img->desc.height,
+                          hipMemcpyHostToDevice, img->ctx->stream));
+  return ZH_OK;
+}
+
+zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride) {
+  if (!img || !host) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  const uint64_t row = tight_stride(img->desc);
+  if (host_row_stride == 0) host_row_stride = row;
+  if (host_row_stride < row) return fail(ZH_ERR_INVALID_ARGUMENT, "host row stride shorter than a row");
+  ZH_HIP(hipSetDevice(img->ctx->device));
+  ZH_HIP(hipMemcpy2DAsync(host, host_row_stride, img->ptr, row_stride_of(img->desc), row, img->desc.height,
+                          hipMemcpyDeviceToHost, img->ctx->stream));
+  return ZH_OK;
+}
+
+// ---- programs -----------------------------------------------------------------
+static void free_device(zh_program* p) {
+  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  for (Unit& u : p->units) {
+    if (u.d_batch) hipFree(u.d_batch);
+    if (u.d_counter) hipFree(u.d_counter);
+    if (u.d_prefix) hipFree(u.d_prefix);
+  }
+  p->units.clear();
+  for (void* b : p->d_intermediates) hipFree(b);
+  p->d_intermediates.clear();
+  if (p->d_stages) hipFree(p->d_stages);
+  if (p->h_stages) hipHostFree(p->h_stages);
+  p->d_stages = nullptr;
+  p->h_stages = nullptr;
+}
+
+// Group stages into launch units: maximal runs of independent staged
+// resample stages become one batch (tile queue), everything else launches
+// alone.  A stage depends on an earlier one when it reads what that wrote.
+static void make_units(zh_program* p) {
+  p->units.clear();
+  const auto& st = p->plan.stages;
+  size_t i = 0;
+  while (i < st.size()) {
+    Unit u;
+    u.first = i;
+    u.count = 1;
+    if (st[i].desc.kind == STAGE_RESAMPLE && st[i].tile_mode == 0) {
+      std::vector<int64_t> written;
+      for (size_t k = 0; k < st[i].slot_ref.size(); ++k)
+        if (st[i].slot_ref[k] < 0 || st[i].slot_ref[k] >= kIntermediateBase) written.push_back(st[i].slot_ref[k]);
+      size_t j = i + 1;
+      while (j < st.size() && st[j].desc.kind == STAGE_RESAMPLE && st[j].tile_mode == 0) {
+        bool dep = false;
+        for (int64_t r : st[j].slot_ref)
+          for (int64_t w : written)
+            if (r == w && r >= kIntermediateBase) dep = true;
+        if (dep) break;
+        for (int64_t r : st[j].slot_ref)
+          if (r < 0 || r >= kIntermediateBase) written.push_back(r);
+        ++j;
+      }
+      if (j - i >= 2) {
+        u.batch = true;
+        u.count = j - i;
+      }
+    }
+    p->units.push_back(u);
+    i += u.count;
+  }
+}
+
+static zh_status alloc_device(zh_program* p) {
+  zh_context* ctx = p->ctx;
+  ZH_HIP(hipSetDevice(ctx->device));
+  const size_t n = p->plan.stages.size();
+  ZH_HIP(hipMalloc(&p->d_stages, std::max<size_t>(1, n) * sizeof(StageDesc)));
+  ZH_HIP(hipHostMalloc(&p->h_stages, std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
+  for (const zh_descriptor& d : p->plan.intermediates) {
+    void* b = nullptr;
+    hipError_t e = hipMalloc(&b, image_bytes(d));
+    if (e != hipSuccess) return fail(ZH_ERR_OUT_OF_MEMORY, "intermediate buffer allocation failed");
+    p->d_intermediates.push_back(b);
+  }
+  make_units(p);
+  for (Unit& u : p->units) {
+    if (!u.batch) continue;
+    BatchDesc b{};
+    std::vector<uint32_t> prefix(u.count + 1, 0);
+    for (size_t k = 0; k < u.count; ++k) {
+      const StageDesc& d = p->plan.stages[u.first + k].desc;
+      prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
+      u.lds = std::max(u.lds, p->plan.stages[u.first + k].lds_bytes);
+    }
+    ZH_HIP(hipMalloc(&u.d_batch, sizeof(BatchDesc)));
+    ZH_HIP(hipMalloc(&u.d_counter, sizeof(uint32_t)));
+    ZH_HIP(hipMalloc(&u.d_prefix, prefix.size() * sizeof(uint32_t)));
+    ZH_HIP(hipMemcpy(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    b.n_stages = (uint32_t)u.count;
+    b.total_tiles = prefix.back();
+    b.counter = u.d_counter;
+    b.stages = p->d_stages + u.first;
+    b.tile_prefix = u.d_prefix;
+    ZH_HIP(hipMemcpy(u.d_batch, &b, sizeof(b), hipMemcpyHostToDevice));
+    // Persistent grid: enough workgroups to fill every CU at the kernel's
+    // occupancy (2 per CU with a 64 KiB tile), capped by the work.
+    hipDeviceProp_t prop;
+    ZH_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    u.grid = std::min<uint32_t>(b.total_tiles, (uint32_t)prop.multiProcessorCount * 2);
+    if (u.grid == 0) u.grid = 1;
+  }
+  return ZH_OK;
+}
+
+zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures, const zh_op* ops,
+                           uint32_t num_ops, uint32_t flags, zh_program** out) {
+  if (!out || (num_textures && !textures) || (num_ops && !ops)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_program* p = new zh_program();
+  p->ctx = ctx;
+  p->flags = flags;
+  p->textures.assign(textures, textures + num_textures);
+  p->ops.assign(ops, ops + num_ops);
+  p->params.resize(num_ops);
+  for (uint32_t i = 0; i < num_ops; ++i) {
+    if (ops[i].params_len) {
+      if (!ops[i].params) { delete p; return fail(ZH_ERR_INVALID_ARGUMENT, "params pointer is null"); }
+      const uint8_t* b = static_cast<const uint8_t*>(ops[i].params);
+      p->params[i].assign(b, b + ops[i].params_len);
+    }
+    p->ops[i].params = nullptr;
+  }
+  zh_status st = plan_program(p->textures, p->ops, p->params, flags, &p->plan);
+  if (st != ZH_OK) { delete p; return st; }
+  if (ctx) {
+    st = alloc_device(p);
+    if (st != ZH_OK) { free_device(p); delete p; return st; }
+  } else {
+    make_units(p);
+  }
+  *out = p;
+  return ZH_OK;
+}
+
+zh_status zh_program_destroy(zh_program* p) {
+  if (!p) return ZH_OK;
+  if (p->ctx) {
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    free_device(p);
+  }
+  delete p;
+  return ZH_OK;
+}
+
+zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* needed) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  std::ostringstream o;
+  o << p->plan.text;
+  for (size_t i = 0; i < p->units.size(); ++i)
+    if (p->units[i].batch)
+      o << "batch: stages " << p->units[i].first << ".." << p->units[i].first + p->units[i].count - 1
+        << " as one tile queue\n";
+  for (size_t i = 0; i < p->plan.intermediates.size(); ++i) {
+    const zh_descriptor& d = p->plan.intermediates[i];
+    o << "intermediate " << i << ": " << d.width << "x" << d.height << " bits " << d.texel.bits << "\n";
+  }
+  const std::string s = o.str();
+  if (needed) *needed = s.size() + 1;
+  if (buf && len) {
+    const size_t n = std::min(len - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return ZH_OK;
+}
+
+zh_status zh_program_stats(zh_program* p, uint32_t* launches, uint64_t* algorithmic_bytes) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (launches) *launches = (uint32_t)p->units.size();
+  if (algorithmic_bytes) {
+    uint64_t b = 0;
+    for (const StagePlan& s : p->plan.stages) b += s.alg_bytes;
+    *algorithmic_bytes = b;
+  }
+  return ZH_OK;
+}
+
+zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uint32_t len) {
+  if (!p || (!data && len)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  bool found = false;
+  for (size_t i = 0; i < p->ops.size(); ++i) {
+    if (p->ops[i].knob != knob) continue;
+    if (p->params[i].size() != len)
+      return fail(ZH_ERR_KNOB_LENGTH, "knob data length " + std::to_string(len) + " differs from " +
+                                          std::to_string(p->params[i].size()));
+    found = true;
+  }
+  if (!found) return fail(ZH_ERR_INVALID_ARGUMENT, "no op uses knob " + std::to_string(knob));
+  std::vector<std::vector<uint8_t>> params = p->params;
+  for (size_t i = 0; i < p->ops.size(); ++i)
+    if (p->ops[i].knob == knob) {
+      const uint8_t* b = static_cast<const uint8_t*>(data);
+      params[i].assign(b, b + len);
+    }
+  Plan np;
+  zh_status st = plan_program(p->textures, p->ops, params, p->flags, &np);
+  if (st != ZH_OK) return st;
+  // Same structure (stage kinds, tiles, slots): keep buffers and the graph and
+  // refresh the descriptors at the next launch.
+  bool same = np.stages.size() == p->plan.stages.size() && np.intermediates.size() == p->plan.intermediates.size();
+  for (size_t i = 0; same && i < np.stages.size(); ++i) {
+    const StagePlan& a = np.stages[i];
+    const StagePlan& b = p->plan.stages[i];
+    same = a.desc.kind == b.desc.kind && a.tile_mode == b.tile_mode && a.desc.tiles_x == b.desc.tiles_x &&
+           a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes;
+  }
+  p->params = params;
+  if (same) {
+    for (size_t i = 0; i < np.stages.size(); ++i) np.stages[i].lds_bytes = p->plan.stages[i].lds_bytes;
+    p->plan = std::move(np);
+    p->uploaded = false;
+    return ZH_OK;
+  }
+  if (p->ctx) {
+    hipStreamSynchronize(p->ctx->stream);
+    free_device(p);
+  }
+  p->plan = std::move(np);
+  p->uploaded = false;
+  p->bound_inputs.clear();
+  p->bound_outputs.clear();
+  if (p->ctx) return alloc_device(p);
+  make_units(p);
+  return ZH_OK;
+}
+
+// Resolve bindings, patch slot addresses and upload descriptors if needed.
+static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
+  const Plan& plan = p->plan;
+  std::vector<zh_image*> in(plan.input_desc.size(), nullptr), outs(plan.output_desc.size(), nullptr);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!b[i].image) return fail(ZH_ERR_INVALID_ARGUMENT, "binding without image");
+    if (b[i].image->ctx != p->ctx) return fail(ZH_ERR_INVALID_ARGUMENT, "image belongs to another context");
+    auto& vec = b[i].is_output ? outs : in;
+    const auto& descs = b[i].is_output ? plan.output_desc : plan.input_desc;
+    if (b[i].slot >= vec.size()) return fail(ZH_ERR_INVALID_ARGUMENT, "binding slot out of range");
+    const zh_descriptor& want = descs[b[i].slot];
+    const zh_descriptor& have = b[i].image->desc;
+    // Environment::bind requires the pool image's descriptor to equal the
+    // register's (run.rs:1187-1193); row layout included.
+    if (!same_layout(want, have))
+      return fail(ZH_ERR_MISMATCHED_DESCRIPTOR, std::string(b[i].is_output ? "output" : "input") + " slot " +
+                                                    std::to_string(b[i].slot) + " bound to a mismatched image");
+    vec[b[i].slot] = b[i].image;
+  }
+  for (size_t i = 0; i < in.size(); ++i)
+    if (plan.input_used[i] && !in[i]) return fail(ZH_ERR_MISSING_KEY, "input slot " + std::to_string(i) + " unbound");
+  for (size_t i = 0; i < outs.size(); ++i)
+    if (plan.output_used[i] && !outs[i]) return fail(ZH_ERR_MISSING_KEY, "output slot " + std::to_string(i) + " unbound");
+  std::vector<void*> ip(in.size()), op(outs.size());
+  for (size_t i = 0; i < in.size(); ++i) ip[i] = in[i] ? in[i]->ptr : nullptr;
+  for (size_t i = 0; i < outs.size(); ++i) op[i] = outs[i] ? outs[i]->ptr : nullptr;
+  if (p->uploaded && ip == p->bound_inputs && op == p->bound_outputs) return ZH_OK;
+
+  for (size_t s = 0; s < plan.stages.size(); ++s) {
+    StageDesc d = plan.stages[s].desc;
+    d.srgb_lut = p->ctx->d_lut;
+    d.error_flag = p->ctx->d_error;
+    const auto& refs = plan.stages[s].slot_ref;
+    auto resolve = [&](int64_t ref, uint64_t* stride) -> uint64_t {
+      if (ref >= kIntermediateBase) {
+        const size_t k = (size_t)(ref - kIntermediateBase);
+        *stride = row_stride_of(plan.intermediates[k]);
+        return (uint64_t)(uintptr_t)p->d_intermediates[k];
+      }
+      if (ref >= 0) { *stride = row_stride_of(in[ref]->desc); return (uint64_t)(uintptr_t)in[ref]->ptr; }
+      if (ref > -1000000) {
+        const size_t k = (size_t)(~ref);
+        *stride = row_stride_of(outs[k]->desc);
+        return (uint64_t)(uintptr_t)outs[k]->ptr;
+      }
+      *stride = 0;
+      return 0;  // format-only slot (strict quantisation)
+    };
+    if (d.kind == STAGE_COPY) {
+      uint64_t s0, s1;
+      d.copy_src = resolve(refs[0], &s0);
+      d.copy_dst = resolve(refs[1], &s1);
+    } else {
+      for (size_t k = 0; k < refs.size(); ++k) {
+        Slot& sl = d.slot[k];
+        uint64_t stride = 0;
+        sl.ptr = resolve(refs[k], &stride);
+        sl.vec_ok = ((sl.ptr & 15) == 0 && (d.flat || (sl.row_stride & 15) == 0)) ? 1u : 0u;
+      }
+    }
+    p->h_stages[s] = d;
+  }
+  ZH_HIP(hipMemcpyAsync(p->d_stages, p->h_stages, plan.stages.size() * sizeof(StageDesc), hipMemcpyHostToDevice,
+                        p->ctx->stream));
+  p->bound_inputs = ip;
+  p->bound_outputs = op;
+  p->uploaded = true;
+  return ZH_OK;
+}
+
+static zh_status launch_unit(zh_program* p, const Unit& u) {
+  hipStream_t s = p->ctx->stream;
+  if (u.batch) {
+    ZH_HIP(zh_launch_batch(u.d_batch, u.d_counter, u.grid, u.lds, s));
+    return ZH_OK;
+  }
+  const StagePlan& sp = p->plan.stages[u.first];
+  const StageDesc* d = p->d_stages + u.first;
+  switch (sp.desc.kind) {
+    case STAGE_POINTWISE:
+      ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, s));
+      break;
+    case STAGE_RESAMPLE:
+      ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
+      break;
+    case STAGE_COPY:
+      ZH_HIP(zh_launch_copy(d, sp.desc.copy_row_bytes, sp.desc.copy_rows, s));
+      break;
+  }
+  return ZH_OK;
+}
+
+static zh_status launch_all(zh_program* p) {
+  for (const Unit& u : p->units) {
+    zh_status st = launch_unit(p, u);
+    if (st != ZH_OK) return st;
+  }
+  return ZH_OK;
+}
+
+zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t num_bindings) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  zh_status st = bind(p, bindings, num_bindings);
+  if (st != ZH_OK) return st;
+  const bool use_graph = !(p->flags & ZH_PLAN_NO_GRAPH) && !p->graph_failed && p->units.size() > 0;
+  if (use_graph && !p->graph_exec) {
+    // Capture once; descriptor contents change through their device copies,
+    // never through the graph, so the executable graph stays valid.
+    hipError_t e = hipStreamBeginCapture(p->ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      st = launch_all(p);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(p->ctx->stream, &g);
+      if (st != ZH_OK) { if (g) hipGraphDestroy(g); return st; }
+      if (e == hipSuccess && g) {
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+          p->graph = g;
+          p->graph_exec = ge;
+        } else {
+          hipGraphDestroy(g);
+        }
+      }
+    }
+    if (!p->graph_exec) {
+      (void)hipGetLastError();
+      p->graph_failed = true;
+    }
+  }
+  if (p->graph_exec) {
+    ZH_HIP(hipGraphLaunch(p->graph_exec, p->ctx->stream));
+    return ZH_OK;
+  }
+  return launch_all(p);
+}
+
+zh_status zh_program_time(zh_program* p, const zh_binding* bindings, uint32_t num_bindings, uint32_t iterations,
+                          float* stage_ms, uint32_t max_stages, uint32_t* num_stages) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  if (iterations == 0) iterations = 1;
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  zh_status st = bind(p, bindings, num_bindings);
+  if (st != ZH_OK) return st;
+  const size_t n = p->units.size();
+  if (num_stages) *num_stages = (uint32_t)n;
+  std::vector<hipEvent_t> ev(n + 1);
+  for (auto& e : ev) ZH_HIP(hipEventCreate(&e));
+  std::vector<double> acc(n, 0.0);
+  for (uint32_t it = 0; it < iterations; ++it) {
+    ZH_HIP(hipEventRecord(ev[0], p->ctx->stream));
+    for (size_t u = 0; u < n; ++u) {
+      st = launch_unit(p, p->units[u]);
+      if (st != ZH_OK) break;
+      ZH_HIP(hipEventRecord(ev[u + 1], p->ctx->stream));
+    }
+    if (st != ZH_OK) break;
+    ZH_HIP(hipEventSynchronize(ev[n]));
+    for (size_t u = 0; u < n; ++u) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev[u], ev[u + 1]);
+      acc[u] += ms;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  if (st != ZH_OK) return st;
+  for (size_t u = 0; u < n && u < max_stages; ++u) stage_ms[u] = (float)(acc[u] / iterations);
+  return ZH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// zh_stage.h -- the compiled form of one fused stage, shared by the host
+// planner and the gfx950 kernels.
+//
+// A stage writes one output image.  Its per-pixel work is a short program of
+// micro-ops over a three-entry register stack (X, Y, Z of float4 per pixel),
+// evaluated for P = 4 horizontally adjacent pixels per lane.  Every
+// instruction, slot and parameter is uniform across the grid, so the
+// interpreter's dispatch is scalar; the vector work is the colour math.
+//
+// Stage kinds
+//   STAGE_POINTWISE  out(p) = prog(p): loads of same-sized images at p.
+//   STAGE_RESAMPLE   out(p) = post(compose_l layer_l(p) onto base(p)); a
+//                    layer is a source program evaluated on a source grid
+//                    and sampled through an affine map (bilinear, staged in
+//                    LDS per output tile, or nearest with quad coverage).
+//   STAGE_COPY       byte copy of rows (transmute / same-format inscribe).
+#pragma once
+#include <stdint.h>
+
+namespace zh {
+
+constexpr int kPix = 4;          // pixels per lane
+constexpr int kMaxSlots = 12;    // images referenced by one stage
+constexpr int kMaxInstr = 64;
+constexpr int kMaxParams = 256;  // floats
+constexpr int kMaxLayers = 8;
+constexpr int kBlock = 256;      // threads per workgroup
+// LDS bytes available to one resample tile (float4 texels).
+constexpr int kTileLdsBytes = 64 * 1024;
+
+enum Opcode : uint8_t {
+  OP_NOP = 0,
+  OP_LOAD,      // a = slot; b = push.  X <- decode(slot at the pixel)
+  OP_SOLID,     // arg = param index of 4 floats; b = push
+  OP_MAT3,      // arg = param index of 9 floats (row-major); X.rgb <- M X.rgb
+  OP_CBRT,      // X.rgb <- sign(x) |x|^(1/3)
+  OP_CUBE,      // X.rgb <- x^3
+  OP_CLAMP01,   // X.rgb <- clamp(x, 0, 1)
+  OP_PREMUL,    // X.rgb <- X.rgb * X.a
+  OP_UNPREMUL,  // X.rgb <- X.a > 0 ? X.rgb / X.a : 0
+  OP_OVER,      // X <- X + Y (1 - X.a); pop (Y <- Z)
+  OP_SWAP,      // X <-> Y
+  OP_EOTF,      // arg = transfer id; X.rgb electrical -> linear
+  OP_OETF,      // arg = transfer id; X.rgb linear -> electrical
+  OP_QUANT,     // a = slot whose format X is rounded through (encode+decode)
+  OP_STORE,     // a = slot; encode X at the pixel
+  OP_OFFSET,    // a = 1: following loads read at (x + dx, y + dy); arg = params (dx, dy); a = 0 resets
+};
+
+struct Instr {
+  uint8_t op, a, b, c;
+  uint32_t arg;
+};
+
+struct Slot {
+  uint64_t ptr;         // device address of texel (0, 0)
+  uint64_t row_stride;  // bytes
+  uint32_t width, height;
+  uint32_t bits, parts, transfer;
+  uint32_t bpp;         // bytes per texel
+  uint32_t native;      // 1: native unorm (round to nearest, proper BGRA swizzle)
+  uint32_t vec_ok;      // 1: ptr and row_stride are 16-byte aligned
+};
+
+enum LayerMode : uint32_t {
+  LAYER_BILINEAR_OVER = 0,  // premultiplied bilinear sample, transparent border, 'over'
+  LAYER_NEAREST_REPLACE = 1,  // nearest texel where the pixel centre is inside the quad
+  LAYER_NEAREST_OVER = 2,     // nearest texel, composited 'over'
+};
+
+struct Layer {
+  uint32_t mode;
+  uint32_t prog_begin, prog_end;  // source program (evaluated on the source grid)
+  uint32_t src_w, src_h;          // source grid
+  uint32_t pad;
+  // Output pixel centre (x + 0.5, y + 0.5) -> source texel space, in FP64
+  // (at 8K an FP32 ulp is 5e-4 px):
+  //   u = (m0 px + m1 py + m2) / (m6 px + m7 py + m8), v likewise with m3..m5.
+  double inv[9];
+  // Nearest modes: coverage rectangle in source texel space [x0, x1) x [y0, y1).
+  double cover[4];
+};
+
+enum StageKind : uint32_t { STAGE_POINTWISE = 0, STAGE_RESAMPLE = 1, STAGE_COPY = 2 };
+
+struct StageDesc {
+  uint32_t kind;
+  uint32_t width, height;     // output grid
+  uint32_t flat;              // pointwise: all slots tight and same size -> 1-D
+  uint32_t n_instr;
+  uint32_t base_begin, base_end;  // resample: base program (empty = transparent)
+  uint32_t post_begin, post_end;  // resample: post program (ends with a STORE)
+  uint32_t n_layers;
+  uint32_t tile_w, tile_h;        // resample tile (tile_w * tile_h == kBlock * kPix)
+  uint32_t tiles_x, tiles_y;
+  uint32_t lds_texels;        // resample: float4 texels of LDS available per tile
+  uint32_t band_h;            // resample: rows per LDS fill (divides tile_h)
+  // copy stage
+  uint64_t copy_src, copy_dst, copy_src_stride, copy_dst_stride, copy_row_bytes;
+  uint32_t copy_rows, copy_pad;
+  const float* srgb_lut;      // 256 floats: exact sRGB EOTF of each 8-bit code
+  uint32_t* error_flag;       // set to 1 by a kernel that met an impossible case
+  Layer layer[kMaxLayers];
+  Slot slot[kMaxSlots];
+  Instr code[kMaxInstr];
+  float params[kMaxParams];
+};
+
+// A batch of resample stages executed by one persistent tile-queue launch.
+struct BatchDesc {
+  uint32_t n_stages;
+  uint32_t total_tiles;
+  uint32_t* counter;            // reset by a one-lane kernel before each launch
+  const StageDesc* stages;      // n_stages descriptors
+  const uint32_t* tile_prefix;  // n_stages + 1 exclusive prefix of tile counts
+};
+
+}  // namespace zh
