@@ -71,6 +71,10 @@ struct zh_context {
   bool own_stream = false;
   float* d_lut = nullptr;        // 256 floats
   uint32_t* d_error = nullptr;   // kernel error flag
+  // Images and programs hold the context; zh_context_destroy of a context
+  // with live objects defers the teardown to the release of the last one.
+  int refs = 0;
+  bool closing = false;
 };
 
 struct zh_image {

@@ -772,8 +772,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         count++;
         const int base = L.a;
         const Node& bn = B.nodes[base];
-        if (bn.kind == N_PAINT && !bn.mat && bn.uses == 1 && is_layer_class_bilinear(bn.mode) == bil &&
-            bn.w == L.w && bn.h == L.h && count < kMaxLayers) {
+        if (bn.kind == N_PAINT && !bn.mat && bn.uses == 1 && bn.mode == L.mode && bn.w == L.w && bn.h == L.h &&
+            count < kMaxLayers) {
           layer = base;
           continue;
         }
@@ -948,8 +948,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       layers.push_back(base);
       const Node& L = B.nodes[base];
       const Node& bn = B.nodes[L.a];
-      if ((int)layers.size() < kMaxLayers && bn.kind == N_PAINT && !bn.mat && bn.uses == 1 &&
-          is_layer_class_bilinear(bn.mode) == bil && bn.w == L.w && bn.h == L.h) {
+      if ((int)layers.size() < kMaxLayers && bn.kind == N_PAINT && !bn.mat && bn.uses == 1 && bn.mode == L.mode &&
+          bn.w == L.w && bn.h == L.h) {
         base = L.a;
         continue;
       }
@@ -957,10 +957,18 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       break;
     }
     std::reverse(layers.begin(), layers.end());
+    // Replacing paints move texels unchanged, so they stay in straight
+    // alpha (a premultiply round trip would lose the colour of alpha = 0
+    // texels); compositing layers work premultiplied.
+    const bool straight = B.nodes[paint].mode == LAYER_NEAREST_REPLACE;
+    auto represent = [&](bool pm_now) {
+      if (straight && pm_now) E.op(OP_UNPREMUL);
+      if (!straight && !pm_now) E.op(OP_PREMUL);
+    };
     // base program
     sp.desc.base_begin = 0;
     E.depth = 0;
-    if (!E.value(base, -1)) E.op(OP_PREMUL);
+    represent(E.value(base, -1));
     sp.desc.base_end = (uint32_t)E.code.size();
     // layer source programs
     sp.desc.n_layers = (uint32_t)layers.size();
@@ -1025,7 +1033,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       ly.prog_begin = (uint32_t)E.code.size();
       E.depth = 0;
       E.pending_mat_valid = false;
-      if (!E.value(L.b, -1)) E.op(OP_PREMUL);
+      represent(E.value(L.b, -1));
       ly.prog_end = (uint32_t)E.code.size();
       // algorithmic bytes: the visible part of each source read once
       const double frac = L.mode == LAYER_BILINEAR_OVER ? visible_fraction(L.inv, L.src_w, L.src_h, n.w, n.h) : 1.0;
@@ -1040,7 +1048,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     sp.desc.post_begin = (uint32_t)E.code.size();
     E.depth = 1;
     E.pending_mat_valid = false;
-    bool pm = true;
+    bool pm = !straight;
     for (auto it = chain.rbegin(); it != chain.rend(); ++it) {
       const Node& u = B.nodes[*it];
       for (const Micro& m : u.micro) {

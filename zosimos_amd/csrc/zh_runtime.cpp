@@ -128,14 +128,28 @@ zh_status zh_context_create(int32_t device, void* stream, zh_context** out) {
   return ZH_OK;
 }
 
-zh_status zh_context_destroy(zh_context* ctx) {
-  if (!ctx) return ZH_OK;
+static void context_teardown(zh_context* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->d_lut) hipFree(ctx->d_lut);
   if (ctx->d_error) hipFree(ctx->d_error);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+static void context_retain(zh_context* ctx) {
+  if (ctx) ctx->refs++;
+}
+
+static void context_release(zh_context* ctx) {
+  if (!ctx) return;
+  if (--ctx->refs == 0 && ctx->closing) context_teardown(ctx);
+}
+
+zh_status zh_context_destroy(zh_context* ctx) {
+  if (!ctx || ctx->closing) return ZH_OK;
+  ctx->closing = true;
+  if (ctx->refs == 0) context_teardown(ctx);
   return ZH_OK;
 }
 
@@ -168,6 +182,7 @@ static uint64_t image_bytes(const zh_descriptor& d) { return row_stride_of(d) * 
 
 zh_status zh_image_alloc(zh_context* ctx, const zh_descriptor* desc, zh_image** out) {
   if (!ctx || !desc || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->closing) return fail(ZH_ERR_INVALID_ARGUMENT, "context is being destroyed");
   std::string why;
   if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_INVALID_ARGUMENT, "descriptor: " + why);
   ZH_HIP(hipSetDevice(ctx->device));
@@ -182,6 +197,7 @@ zh_status zh_image_alloc(zh_context* ctx, const zh_descriptor* desc, zh_image** 
     return fail(ZH_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   img->owned = true;
+  context_retain(ctx);
   *out = img;
   return ZH_OK;
 }
@@ -199,6 +215,7 @@ zh_status zh_image_wrap(zh_context* ctx, const zh_descriptor* desc, void* device
   img->ptr = device_ptr;
   img->bytes = bytes;
   img->owned = false;
+  context_retain(ctx);
   *out = img;
   return ZH_OK;
 }
@@ -210,7 +227,9 @@ zh_status zh_image_free(zh_image* img) {
     hipStreamSynchronize(img->ctx->stream);
     hipFree(img->ptr);
   }
+  zh_context* ctx = img->ctx;
   delete img;
+  context_release(ctx);
   return ZH_OK;
 }
 
@@ -361,8 +380,10 @@ zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint3
   zh_status st = plan_program(p->textures, p->ops, p->params, flags, &p->plan);
   if (st != ZH_OK) { delete p; return st; }
   if (ctx) {
+    if (ctx->closing) { delete p; return fail(ZH_ERR_INVALID_ARGUMENT, "context is being destroyed"); }
     st = alloc_device(p);
     if (st != ZH_OK) { free_device(p); delete p; return st; }
+    context_retain(ctx);
   } else {
     make_units(p);
   }
@@ -377,7 +398,9 @@ zh_status zh_program_destroy(zh_program* p) {
     hipStreamSynchronize(p->ctx->stream);
     free_device(p);
   }
+  zh_context* ctx = p->ctx;
   delete p;
+  context_release(ctx);
   return ZH_OK;
 }
 
