@@ -239,6 +239,11 @@ zh_status zh_program_time(zh_program* prog, const zh_binding* bindings, uint32_t
                           uint32_t iterations, float* stage_ms, uint32_t max_stages,
                           uint32_t* num_stages);
 
+/* Host-only: the 8-bit sRGB tables the kernels stage in LDS (decode values,
+ * bucket codes and code thresholds; 6,160 bytes), for inspection and tests.
+ * No counterpart in the reference, whose sRGB codec is the texture unit's. */
+zh_status zh_srgb_tables(void* out, size_t len);
+
 /* Message of the last failed call on this thread (never NULL). */
 const char* zh_last_error(void);
 /* Library version string. */

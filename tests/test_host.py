@@ -145,3 +145,44 @@ def test_launch_without_device_fails_loudly():
     with pytest.raises(z.ZosimosError) as err:
         e.launch({}, {})
     assert err.value.code == z.Status.NO_DEVICE
+
+
+def srgb_tables():
+    import numpy as np
+    buf = ctypes.create_string_buffer(6160)
+    assert _abi.lib().zh_srgb_tables(buf, 6160) == 0
+    raw = np.frombuffer(buf.raw, dtype=np.uint8)
+    dec = raw[:1024].view(np.float32)
+    thr = raw[1024:1024 + 1040].view(np.float32)
+    idx = raw[2064:2064 + 4096]
+    return dec, thr, idx
+
+
+def table_encode(x, thr, idx):
+    """The kernels' lookup: bucket code, then one threshold comparison."""
+    import numpy as np
+    c = np.minimum(np.maximum(np.nan_to_num(x.astype(np.float32), nan=0.0), np.float32(0)), np.float32(1))
+    i = np.minimum((c * np.float32(4096)).astype(np.int64), 4095)
+    k = idx[i].astype(np.int64)
+    return (k + (c >= thr[k + 1])).astype(np.uint8)
+
+
+def test_srgb_tables_are_exact():
+    """The table-based 8-bit sRGB encode equals round(255 * OETF(x)) in FP64
+    for every tested f32 x: all thresholds +-3 ulps, all bucket edges +-1 ulp,
+    and 4M random values; the decode table is the FP64 EOTF rounded once."""
+    import numpy as np
+    from oracle import color as C
+    dec, thr, idx = srgb_tables()
+    assert np.array_equal(dec, C.srgb_u8_decode_table().astype(np.float32))
+    t = thr[1:256].view(np.uint32).astype(np.int64)
+    near = np.concatenate([t + d for d in range(-3, 4)]).astype(np.uint32).view(np.float32)
+    edges = (np.arange(4097, dtype=np.float32) / np.float32(4096)).view(np.uint32).astype(np.int64)
+    edges = np.concatenate([edges + d for d in (-1, 0, 1)]).astype(np.uint32).view(np.float32)
+    rnd = np.random.default_rng(0).random(4_000_000, dtype=np.float32)
+    x = np.concatenate([near, edges, rnd, np.float32([0, 1, -1, 2, np.inf, -np.inf, np.nan])])
+    want = C.encode_u8(np.stack([x, x, x, np.ones_like(x)], -1).astype(np.float64), 5)[:, 0]
+    assert np.array_equal(table_encode(x, thr, idx), want)
+    # thresholds are increasing and a 1/4096 bucket holds at most one of them
+    assert np.all(np.diff(thr[1:256]) > 0)
+    assert len(np.unique(np.floor(thr[1:256] * 4096))) == 255

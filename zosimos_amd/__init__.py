@@ -438,7 +438,8 @@ class Image:
 
     def free(self):
         if getattr(self, "_h", None):
-            _abi.lib().zh_image_free(self._h)
+            if _abi is not None:  # module globals are cleared at interpreter exit
+                _abi.lib().zh_image_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -65,6 +65,7 @@ _SIGNATURES = {
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
     "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
                               ctypes.POINTER(u32)]),
+    "zh_srgb_tables": (i32, [vp, sz]),
     "zh_last_error": (ctypes.c_char_p, []),
     "zh_version": (ctypes.c_char_p, []),
     # zosimos_command.h

@@ -20,6 +20,26 @@
 namespace zh {
 
 // ---------------------------------------------------------------------------
+// 8-bit sRGB tables, built on the host in FP64 and staged in LDS per block.
+//   dec[k]   linear value of code k (IEC 61966-2-1, rounded once to f32)
+//   idx[i]   code of the linear value i / 4096
+//   thr[k]   smallest f32 x whose code is >= k (thr[256] = +inf)
+// Adjacent codes are at least 1 / (255 * 12.92) = 3.0e-4 apart in linear
+// light, more than a 1/4096 bucket, so a bucket holds at most one threshold:
+// code(x) = idx[i] + (x >= thr[idx[i] + 1]), exact for every f32 x.
+// ---------------------------------------------------------------------------
+// (struct Luts is declared in zh_stage.h, shared with the host builder.)
+
+__device__ __forceinline__ float clamp01_(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+__device__ __forceinline__ uint32_t srgb_encode8(float v, const Luts* L) {
+  const float c = clamp01_(v);  // NaN -> 0
+  const int i = min((int)(c * 4096.0f), 4095);
+  const uint32_t k = L->idx[i];
+  return k + (c >= L->thr[k + 1] ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
 // transfer functions
 // ---------------------------------------------------------------------------
 // x^e for x > 0 through the hardware log2/exp2 (v_log_f32 / v_exp_f32, ~1 ulp).
@@ -259,7 +279,12 @@ __device__ __noinline__ uint4 encode_generic(uint32_t bits, uint32_t parts, uint
 // Parts whose normalisation is the identity on rgba.
 __device__ __forceinline__ bool parts_identity(uint32_t p) { return p == 8 || p == 18 || p == 20; }
 
-__device__ __forceinline__ float4 decode_texel(const Slot& s, uint4 raw, const float* lut) {
+// FAST instantiations only handle the formats with an in-line path (native
+// RGBA8/BGRA8, linear RGBA16F/RGBA32F); the planner routes a stage there only
+// when every slot has one of them.  Leaving the out-of-line generic codecs
+// out keeps their register budget out of the kernel.
+template <bool FAST = false>
+__device__ __forceinline__ float4 decode_texel(const Slot& s, uint4 raw, const Luts* lut) {
   const uint32_t bits = s.bits;
   if (bits == 10 && s.native) {
     const uint32_t w = raw.x;
@@ -267,23 +292,33 @@ __device__ __forceinline__ float4 decode_texel(const Slot& s, uint4 raw, const f
     const uint32_t r = s.parts == 10 ? b2 : b0;
     const uint32_t b = s.parts == 10 ? b0 : b2;
     const float a = (float)b3 * (1.0f / 255.0f);
-    if (s.transfer == 5) return make_float4(lut[r], lut[b1], lut[b], a);
+    if (s.transfer == 5) return make_float4(lut->dec[r], lut->dec[b1], lut->dec[b], a);
     return make_float4((float)r * (1.0f / 255.0f), (float)b1 * (1.0f / 255.0f), (float)b * (1.0f / 255.0f), a);
   }
   if (bits == 18 && parts_identity(s.parts) && s.transfer == 4)
     return make_float4(h2f(raw.x & 0xffffu), h2f(raw.x >> 16), h2f(raw.y & 0xffffu), h2f(raw.y >> 16));
   if (bits == 19 && parts_identity(s.parts) && s.transfer == 4)
     return make_float4(__uint_as_float(raw.x), __uint_as_float(raw.y), __uint_as_float(raw.z), __uint_as_float(raw.w));
+  if (FAST) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   return decode_generic(bits, s.parts, s.transfer, raw);
 }
 
-__device__ __forceinline__ uint4 encode_texel(const Slot& s, float4 v) {
+template <bool FAST = false>
+__device__ __forceinline__ uint4 encode_texel(const Slot& s, float4 v, const Luts* lut) {
   const uint32_t bits = s.bits;
   if (bits == 10 && s.native) {
-    const float4 e = s.transfer == 5 ? make_float4(oe_srgb(v.x), oe_srgb(v.y), oe_srgb(v.z), v.w) : v;
     // clamp01 maps NaN to 0 (fmaxf returns the non-NaN operand).
-    const uint32_t r = unorm8(clamp01(e.x)), g = unorm8(clamp01(e.y)), b = unorm8(clamp01(e.z)),
-                   a = unorm8(clamp01(e.w));
+    uint32_t r, g, b;
+    if (s.transfer == 5) {
+      r = srgb_encode8(v.x, lut);
+      g = srgb_encode8(v.y, lut);
+      b = srgb_encode8(v.z, lut);
+    } else {
+      r = unorm8(clamp01(v.x));
+      g = unorm8(clamp01(v.y));
+      b = unorm8(clamp01(v.z));
+    }
+    const uint32_t a = unorm8(clamp01(v.w));
     const uint32_t lo = s.parts == 10 ? b : r, hi = s.parts == 10 ? r : b;
     return make_uint4(lo | (g << 8) | (hi << 16) | (a << 24), 0u, 0u, 0u);
   }
@@ -291,6 +326,7 @@ __device__ __forceinline__ uint4 encode_texel(const Slot& s, float4 v) {
     return make_uint4(f2h(v.x) | (f2h(v.y) << 16), f2h(v.z) | (f2h(v.w) << 16), 0u, 0u);
   if (bits == 19 && parts_identity(s.parts) && s.transfer == 4)
     return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w));
+  if (FAST) return make_uint4(0u, 0u, 0u, 0u);
   return encode_generic(bits, s.parts, s.transfer, v);
 }
 

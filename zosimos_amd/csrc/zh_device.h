@@ -1,0 +1,122 @@
+// zh_device.h -- device helpers shared by the interpreter kernels
+// (zh_kernels.hip) and the specialised pointwise kernels (zh_spec.hip):
+// constant-space descriptor access, global-space 128-bit streaming accesses
+// and the in-line codecs of the common formats.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zh_codec.h"
+#include "zh_stage.h"
+
+namespace zh {
+
+typedef const __attribute__((address_space(4))) StageDesc* CDesc;
+typedef const __attribute__((address_space(4))) BatchDesc* CBatch;
+
+__device__ __forceinline__ CDesc as_const(const StageDesc* p) { return (CDesc)(uintptr_t)p; }
+
+// Copy a descriptor record out of the constant address space (scalar loads).
+template <typename T>
+__device__ __forceinline__ T ldc(const __attribute__((address_space(4))) T* p) {
+  static_assert(sizeof(T) % 4 == 0, "records are dword multiples");
+  T r;
+  const __attribute__((address_space(4))) uint32_t* src = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return r;
+}
+
+typedef float4 Px[kPix];
+
+// Image memory is addressed in the global address space (addrspace 1): the
+// uniform slot base stays in SGPRs and loads/stores are global_* instructions
+// counted by vmcnt alone, so they overlap with LDS traffic (generic FLAT
+// accesses would also hold lgkmcnt and serialise with every LDS read).
+typedef __attribute__((address_space(1))) uint8_t gbyte;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ T gld(const gbyte* p) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) T*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void gst(gbyte* p, T v) {
+  *reinterpret_cast<__attribute__((address_space(1))) T*>(p) = v;
+}
+
+// Streaming (non-temporal) 128-bit accesses: every texel is read and written
+// exactly once, so they do not claim cache residency.
+__device__ __forceinline__ uint4 nt_load(const gbyte* p) {
+  const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) u32x4*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void nt_store(uint4 v, gbyte* p) {
+  const u32x4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<__attribute__((address_space(1))) u32x4*>(p));
+}
+__device__ __forceinline__ uint4 ld16(const gbyte* p) {
+  const u32x4 v = gld<u32x4>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ gbyte* texel_addr(const Slot& s, int x, int y) {
+  return (gbyte*)s.ptr + ((uint64_t)y * s.row_stride + (uint64_t)x * s.bpp);
+}
+
+// ---- in-line codecs of the FmtCode formats -----------------------------------
+template <uint32_t F>
+__device__ __forceinline__ float4 dec_u8(uint32_t w, const Luts* L) {
+  constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
+  constexpr bool srgb = F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA;
+  const uint32_t b0 = w & 255u, b1 = (w >> 8) & 255u, b2 = (w >> 16) & 255u, b3 = w >> 24;
+  const uint32_t r = bgra ? b2 : b0, b = bgra ? b0 : b2;
+  const float a = (float)b3 * (1.0f / 255.0f);
+  if (srgb) return make_float4(L->dec[r], L->dec[b1], L->dec[b], a);
+  return make_float4((float)r * (1.0f / 255.0f), (float)b1 * (1.0f / 255.0f), (float)b * (1.0f / 255.0f), a);
+}
+
+template <uint32_t F>
+__device__ __forceinline__ uint32_t enc_u8(float4 v, const Luts* L) {
+  constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
+  constexpr bool srgb = F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA;
+  uint32_t r, g, b;
+  if (srgb) {
+    r = srgb_encode8(v.x, L);
+    g = srgb_encode8(v.y, L);
+    b = srgb_encode8(v.z, L);
+  } else {
+    r = unorm8(clamp01(v.x));
+    g = unorm8(clamp01(v.y));
+    b = unorm8(clamp01(v.z));
+  }
+  const uint32_t a = unorm8(clamp01(v.w));
+  const uint32_t lo = bgra ? b : r, hi = bgra ? r : b;
+  return lo | (g << 8) | (hi << 16) | (a << 24);
+}
+
+__device__ __forceinline__ float4 dec_f16(uint32_t lo, uint32_t hi) {
+  return make_float4(h2f(lo & 0xffffu), h2f(lo >> 16), h2f(hi & 0xffffu), h2f(hi >> 16));
+}
+__device__ __forceinline__ uint32_t pack_f16(float x, float y) { return f2h(x) | (f2h(y) << 16); }
+
+__device__ __forceinline__ float3 mat3(const __attribute__((address_space(4))) float* m, float3 v) {
+  return make_float3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                     m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// Signed cube root: sign(x) 2^(log2|x| / 3) through the hardware log2/exp2
+// (relative error ~1e-7 over the colour range; cbrt(0) = 0 since
+// exp2(-inf) = 0).
+__device__ __forceinline__ float cbrt_s(float x) { return copysignf(pow_pos(fabsf(x), 1.0f / 3.0f), x); }
+
+// Copy the context's sRGB tables (zh_codec.h Luts) into LDS.
+__device__ __forceinline__ void load_lut(CDesc d, Luts* lut) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(d->srgb_lut);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(lut);
+  for (uint32_t i = threadIdx.x; i < sizeof(Luts) / 4; i += kBlock) dst[i] = src[i];
+}
+
+}  // namespace zh

@@ -39,6 +39,8 @@ Mat3 mat_from_std140(const void* blob);  // 12 floats, column-major with padding
 struct StagePlan {
   StageDesc desc;                 // host copy; slot ptrs patched per launch
   int tile_mode = 0;              // resample: TILE_* of zh_kernels.hip
+  bool fast = false;              // pointwise: every slot has an in-line codec (k_pointwise<true>)
+  int spec = -1;                  // pointwise: index of the specialised kernel (zh_spec.h), -1 = none
   uint32_t lds_bytes = 0;
   uint64_t alg_bytes = 0;         // algorithmic HBM bytes (reads once + writes once)
   // slot -> what it refers to: >=0 input slot, or ~output slot (<0), or
@@ -69,7 +71,8 @@ struct zh_context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  float* d_lut = nullptr;        // 256 floats
+  void* d_lut = nullptr;         // zh::Luts (8-bit sRGB tables)
+  int cus = 256;                 // compute units of the device
   uint32_t* d_error = nullptr;   // kernel error flag
   // Images and programs hold the context; zh_context_destroy of a context
   // with live objects defers the teardown to the release of the last one.

@@ -9,27 +9,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "zh_codec.h"
-#include "zh_stage.h"
+#include <algorithm>
+
+#include "zh_device.h"
 
 namespace zh {
-
-typedef const __attribute__((address_space(4))) StageDesc* CDesc;
-typedef const __attribute__((address_space(4))) BatchDesc* CBatch;
-
-__device__ __forceinline__ CDesc as_const(const StageDesc* p) { return (CDesc)(uintptr_t)p; }
-
-// Copy a descriptor record out of the constant address space (scalar loads).
-template <typename T>
-__device__ __forceinline__ T ldc(const __attribute__((address_space(4))) T* p) {
-  static_assert(sizeof(T) % 4 == 0, "records are dword multiples");
-  T r;
-  const __attribute__((address_space(4))) uint32_t* src = (const __attribute__((address_space(4))) uint32_t*)p;
-  uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
-  return r;
-}
 
 // The pixels a lane works on: either kPix contiguous pixels of one row
 // starting at (x0, y), of which the first n exist, or kPix gathered texel
@@ -42,46 +26,44 @@ struct Ctx {
   bool gv[kPix];
 };
 
-typedef float4 Px[kPix];
-
-__device__ __forceinline__ const uint8_t* texel_addr(const Slot& s, int x, int y) {
-  return reinterpret_cast<const uint8_t*>(s.ptr) + (uint64_t)y * s.row_stride + (uint64_t)x * s.bpp;
-}
-
 // Fetch one texel's bytes (bpp in {1, 2, 3, 4, 6, 8, 16}).
 __device__ __forceinline__ uint4 fetch_one(const Slot& s, int x, int y) {
-  const uint8_t* p = texel_addr(s, x, y);
+  const gbyte* p = texel_addr(s, x, y);
   switch (s.bpp) {
-    case 1: return make_uint4(*p, 0u, 0u, 0u);
-    case 2: return make_uint4(*reinterpret_cast<const uint16_t*>(p), 0u, 0u, 0u);
-    case 3: return make_uint4((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16), 0u, 0u, 0u);
-    case 4: return make_uint4(*reinterpret_cast<const uint32_t*>(p), 0u, 0u, 0u);
-    case 6: {
-      const uint16_t* q = reinterpret_cast<const uint16_t*>(p);
-      return make_uint4((uint32_t)q[0] | ((uint32_t)q[1] << 16), q[2], 0u, 0u);
-    }
+    case 1: return make_uint4(gld<uint8_t>(p), 0u, 0u, 0u);
+    case 2: return make_uint4(gld<uint16_t>(p), 0u, 0u, 0u);
+    case 3: return make_uint4((uint32_t)gld<uint8_t>(p) | ((uint32_t)gld<uint8_t>(p + 1) << 8) |
+                                  ((uint32_t)gld<uint8_t>(p + 2) << 16), 0u, 0u, 0u);
+    case 4: return make_uint4(gld<uint32_t>(p), 0u, 0u, 0u);
+    case 6:
+      return make_uint4((uint32_t)gld<uint16_t>(p) | ((uint32_t)gld<uint16_t>(p + 2) << 16), gld<uint16_t>(p + 4),
+                        0u, 0u);
     case 8: {
-      const uint2 v = *reinterpret_cast<const uint2*>(p);
+      const u32x2 v = gld<u32x2>(p);
       return make_uint4(v.x, v.y, 0u, 0u);
     }
-    default: return *reinterpret_cast<const uint4*>(p);
+    default: return ld16(p);
   }
 }
 
 __device__ __forceinline__ void store_one(const Slot& s, int x, int y, uint4 v) {
-  uint8_t* p = const_cast<uint8_t*>(texel_addr(s, x, y));
+  gbyte* p = texel_addr(s, x, y);
   switch (s.bpp) {
-    case 1: *p = (uint8_t)v.x; break;
-    case 2: *reinterpret_cast<uint16_t*>(p) = (uint16_t)v.x; break;
-    case 3: p[0] = (uint8_t)v.x; p[1] = (uint8_t)(v.x >> 8); p[2] = (uint8_t)(v.x >> 16); break;
-    case 4: *reinterpret_cast<uint32_t*>(p) = v.x; break;
-    case 6: {
-      uint16_t* q = reinterpret_cast<uint16_t*>(p);
-      q[0] = (uint16_t)v.x; q[1] = (uint16_t)(v.x >> 16); q[2] = (uint16_t)v.y;
+    case 1: gst<uint8_t>(p, (uint8_t)v.x); break;
+    case 2: gst<uint16_t>(p, (uint16_t)v.x); break;
+    case 3:
+      gst<uint8_t>(p, (uint8_t)v.x);
+      gst<uint8_t>(p + 1, (uint8_t)(v.x >> 8));
+      gst<uint8_t>(p + 2, (uint8_t)(v.x >> 16));
       break;
-    }
-    case 8: *reinterpret_cast<uint2*>(p) = make_uint2(v.x, v.y); break;
-    default: *reinterpret_cast<uint4*>(p) = v; break;
+    case 4: gst<uint32_t>(p, v.x); break;
+    case 6:
+      gst<uint16_t>(p, (uint16_t)v.x);
+      gst<uint16_t>(p + 2, (uint16_t)(v.x >> 16));
+      gst<uint16_t>(p + 4, (uint16_t)v.y);
+      break;
+    case 8: gst<u32x2>(p, u32x2{v.x, v.y}); break;
+    default: gst<u32x4>(p, u32x4{v.x, v.y, v.z, v.w}); break;
   }
 }
 
@@ -89,39 +71,108 @@ __device__ __forceinline__ bool in_slot(const Slot& s, int x, int y) {
   return x >= 0 && y >= 0 && x < (int)s.width && y < (int)s.height;
 }
 
+// Decode a full group of kPix texels of 4 or 8 bytes from its raw 16-byte
+// words (bpp 4: a; bpp 8: a, b).
+template <bool FAST = false>
+__device__ __forceinline__ void decode_group(const Slot& s, uint4 a, uint4 b, Px& X, const Luts* lut) {
+  if (s.bpp == 4) {
+    X[0] = decode_texel<FAST>(s, make_uint4(a.x, 0u, 0u, 0u), lut);
+    X[1] = decode_texel<FAST>(s, make_uint4(a.y, 0u, 0u, 0u), lut);
+    X[2] = decode_texel<FAST>(s, make_uint4(a.z, 0u, 0u, 0u), lut);
+    X[3] = decode_texel<FAST>(s, make_uint4(a.w, 0u, 0u, 0u), lut);
+  } else {
+    X[0] = decode_texel<FAST>(s, make_uint4(a.x, a.y, 0u, 0u), lut);
+    X[1] = decode_texel<FAST>(s, make_uint4(a.z, a.w, 0u, 0u), lut);
+    X[2] = decode_texel<FAST>(s, make_uint4(b.x, b.y, 0u, 0u), lut);
+    X[3] = decode_texel<FAST>(s, make_uint4(b.z, b.w, 0u, 0u), lut);
+  }
+}
+
+// Decode the lane's full group of slot s from raw words a, b (4-byte texels:
+// a; 8-byte: a, b); 16-byte texels are read here.
+__device__ __forceinline__ void dec_group_fmt(const Slot& s, int x0, uint4 a, uint4 b, Px& X, const Luts* L) {
+  switch (s.fmt) {
+#define ZH_U8(F)                        \
+  case F:                               \
+    X[0] = dec_u8<F>(a.x, L);           \
+    X[1] = dec_u8<F>(a.y, L);           \
+    X[2] = dec_u8<F>(a.z, L);           \
+    X[3] = dec_u8<F>(a.w, L);           \
+    break;
+    ZH_U8(FMT_U8_SRGB_RGBA)
+    ZH_U8(FMT_U8_SRGB_BGRA)
+    ZH_U8(FMT_U8_LIN_RGBA)
+    ZH_U8(FMT_U8_LIN_BGRA)
+#undef ZH_U8
+    case FMT_F16:
+      X[0] = dec_f16(a.x, a.y);
+      X[1] = dec_f16(a.z, a.w);
+      X[2] = dec_f16(b.x, b.y);
+      X[3] = dec_f16(b.z, b.w);
+      break;
+    default: {  // FMT_F32
+      const gbyte* q = texel_addr(s, x0, 0);
+#pragma unroll
+      for (int i = 0; i < kPix; ++i) {
+        const uint4 v = nt_load(q + 16 * i);
+        X[i] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+      }
+      break;
+    }
+  }
+}
+
+__device__ __forceinline__ void enc_group_fmt(const Slot& s, int x0, const Px& X, const Luts* L) {
+  gbyte* q = texel_addr(s, x0, 0);
+  switch (s.fmt) {
+#define ZH_U8(F)                                                                                         \
+  case F:                                                                                                \
+    nt_store(make_uint4(enc_u8<F>(X[0], L), enc_u8<F>(X[1], L), enc_u8<F>(X[2], L), enc_u8<F>(X[3], L)), q); \
+    break;
+    ZH_U8(FMT_U8_SRGB_RGBA)
+    ZH_U8(FMT_U8_SRGB_BGRA)
+    ZH_U8(FMT_U8_LIN_RGBA)
+    ZH_U8(FMT_U8_LIN_BGRA)
+#undef ZH_U8
+    case FMT_F16:
+      nt_store(make_uint4(pack_f16(X[0].x, X[0].y), pack_f16(X[0].z, X[0].w), pack_f16(X[1].x, X[1].y),
+                          pack_f16(X[1].z, X[1].w)), q);
+      nt_store(make_uint4(pack_f16(X[2].x, X[2].y), pack_f16(X[2].z, X[2].w), pack_f16(X[3].x, X[3].y),
+                          pack_f16(X[3].z, X[3].w)), q + 16);
+      break;
+    default:  // FMT_F32
+#pragma unroll
+      for (int i = 0; i < kPix; ++i)
+        nt_store(make_uint4(__float_as_uint(X[i].x), __float_as_uint(X[i].y), __float_as_uint(X[i].z),
+                            __float_as_uint(X[i].w)), q + 16 * i);
+      break;
+  }
+}
+
 // X[i] <- decode(slot) at the lane's pixels; missing pixels read zero.
-__device__ __forceinline__ void load_px(const Slot& s, const Ctx& c, Px& X, const float* lut) {
+template <bool FAST = false>
+__device__ __forceinline__ void load_px(const Slot& s, const Ctx& c, Px& X, const Luts* lut) {
   if (!c.gather) {
     const int x = c.x0 + c.dx, y = c.y + c.dy;
     const bool full = c.n == kPix && s.vec_ok && (((uint32_t)x * s.bpp) & 15u) == 0 && in_slot(s, x, y) &&
                       in_slot(s, x + kPix - 1, y);
-    if (full && s.bpp == 4) {
-      const uint4 w = *reinterpret_cast<const uint4*>(texel_addr(s, x, y));
-      X[0] = decode_texel(s, make_uint4(w.x, 0u, 0u, 0u), lut);
-      X[1] = decode_texel(s, make_uint4(w.y, 0u, 0u, 0u), lut);
-      X[2] = decode_texel(s, make_uint4(w.z, 0u, 0u, 0u), lut);
-      X[3] = decode_texel(s, make_uint4(w.w, 0u, 0u, 0u), lut);
-      return;
-    }
-    if (full && s.bpp == 8) {
-      const uint4* p = reinterpret_cast<const uint4*>(texel_addr(s, x, y));
-      const uint4 a = p[0], b = p[1];
-      X[0] = decode_texel(s, make_uint4(a.x, a.y, 0u, 0u), lut);
-      X[1] = decode_texel(s, make_uint4(a.z, a.w, 0u, 0u), lut);
-      X[2] = decode_texel(s, make_uint4(b.x, b.y, 0u, 0u), lut);
-      X[3] = decode_texel(s, make_uint4(b.z, b.w, 0u, 0u), lut);
+    if (full && (s.bpp == 4 || s.bpp == 8)) {
+      const gbyte* p = texel_addr(s, x, y);
+      const uint4 a = nt_load(p);
+      const uint4 b = s.bpp == 8 ? nt_load(p + 16) : make_uint4(0u, 0u, 0u, 0u);
+      decode_group<FAST>(s, a, b, X, lut);
       return;
     }
     if (full && s.bpp == 16) {
-      const uint4* p = reinterpret_cast<const uint4*>(texel_addr(s, x, y));
+      const gbyte* p = texel_addr(s, x, y);
 #pragma unroll
-      for (int i = 0; i < kPix; ++i) X[i] = decode_texel(s, p[i], lut);
+      for (int i = 0; i < kPix; ++i) X[i] = decode_texel<FAST>(s, nt_load(p + 16 * i), lut);
       return;
     }
 #pragma unroll
     for (int i = 0; i < kPix; ++i) {
       const bool ok = i < c.n && in_slot(s, x + i, y);
-      X[i] = ok ? decode_texel(s, fetch_one(s, x + i, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      X[i] = ok ? decode_texel<FAST>(s, fetch_one(s, x + i, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     return;
   }
@@ -129,54 +180,53 @@ __device__ __forceinline__ void load_px(const Slot& s, const Ctx& c, Px& X, cons
   for (int i = 0; i < kPix; ++i) {
     const int x = c.gx[i] + c.dx, y = c.gy[i] + c.dy;
     const bool ok = c.gv[i] && in_slot(s, x, y);
-    X[i] = ok ? decode_texel(s, fetch_one(s, x, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    X[i] = ok ? decode_texel<FAST>(s, fetch_one(s, x, y), lut) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
 
-__device__ __forceinline__ void store_px(const Slot& s, const Ctx& c, const Px& X) {
+template <bool FAST = false>
+__device__ __forceinline__ void store_px(const Slot& s, const Ctx& c, const Px& X, const Luts* lut) {
   const int x = c.x0, y = c.y;
   if (c.n == kPix && s.vec_ok && (((uint32_t)x * s.bpp) & 15u) == 0) {
     if (s.bpp == 4) {
-      const uint4 w = make_uint4(encode_texel(s, X[0]).x, encode_texel(s, X[1]).x, encode_texel(s, X[2]).x,
-                                 encode_texel(s, X[3]).x);
-      *reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y))) = w;
+      const uint4 w = make_uint4(encode_texel<FAST>(s, X[0], lut).x, encode_texel<FAST>(s, X[1], lut).x,
+                                 encode_texel<FAST>(s, X[2], lut).x, encode_texel<FAST>(s, X[3], lut).x);
+      nt_store(w, texel_addr(s, x, y));
       return;
     }
     if (s.bpp == 8) {
-      const uint4 a0 = encode_texel(s, X[0]), a1 = encode_texel(s, X[1]);
-      const uint4 a2 = encode_texel(s, X[2]), a3 = encode_texel(s, X[3]);
-      uint4* p = reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y)));
-      p[0] = make_uint4(a0.x, a0.y, a1.x, a1.y);
-      p[1] = make_uint4(a2.x, a2.y, a3.x, a3.y);
+      const uint4 a0 = encode_texel<FAST>(s, X[0], lut), a1 = encode_texel<FAST>(s, X[1], lut);
+      const uint4 a2 = encode_texel<FAST>(s, X[2], lut), a3 = encode_texel<FAST>(s, X[3], lut);
+      gbyte* p = texel_addr(s, x, y);
+      nt_store(make_uint4(a0.x, a0.y, a1.x, a1.y), p);
+      nt_store(make_uint4(a2.x, a2.y, a3.x, a3.y), p + 16);
       return;
     }
     if (s.bpp == 16) {
-      uint4* p = reinterpret_cast<uint4*>(const_cast<uint8_t*>(texel_addr(s, x, y)));
+      gbyte* p = texel_addr(s, x, y);
 #pragma unroll
-      for (int i = 0; i < kPix; ++i) p[i] = encode_texel(s, X[i]);
+      for (int i = 0; i < kPix; ++i) nt_store(encode_texel<FAST>(s, X[i], lut), p + 16 * i);
       return;
     }
   }
 #pragma unroll
   for (int i = 0; i < kPix; ++i)
-    if (i < c.n) store_one(s, x + i, y, encode_texel(s, X[i]));
+    if (i < c.n) store_one(s, x + i, y, encode_texel<FAST>(s, X[i], lut));
 }
 
-__device__ __forceinline__ float3 mat3(const __attribute__((address_space(4))) float* m, float3 v) {
-  return make_float3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
-                     m[6] * v.x + m[7] * v.y + m[8] * v.z);
-}
+// Raw texels of a lane's group for up to two contiguous slots, loaded one
+// grid-stride iteration ahead of their use so the loads of the next group
+// are in flight while this group's colour math runs.
+struct Pre {
+  uint4 a0, b0;  // slot pre_slot[0]
+  uint4 a1, b1;  // slot pre_slot[1]
+};
 
-// Signed cube root: sign(x) 2^(log2|x| / 3), refined by one Newton step.
-__device__ __forceinline__ float cbrt_s(float x) {
-  const float a = fabsf(x);
-  float y = pow_pos(a, 1.0f / 3.0f);
-  y = a > 0.0f ? y - (y * y * y - a) / (3.0f * y * y) : 0.0f;
-  return copysignf(y, x);
-}
-
-// Evaluate instructions [begin, end) of the stage program.
-__device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& c, Px& X, Px& Y, Px& Z, const float* lut) {
+// Evaluate instructions [begin, end) of the stage program.  FAST programs
+// (see decode_texel) take their contiguous loads from the prefetched words.
+template <bool FAST = false>
+__device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& c, Px& X, Px& Y, Px& Z,
+                                    const Luts* lut, Pre pre = Pre{}, bool has_pre = false, bool full = false) {
   for (uint32_t pc = begin; pc < end; ++pc) {
     const Instr in = ldc(&d->code[pc]);
     switch (in.op) {
@@ -188,7 +238,25 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
           for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
         }
         if (in.op == OP_LOAD) {
-          load_px(sl, c, X, lut);
+          if (FAST && full) {
+            // Full aligned group of a FmtCode format: the slot tests are
+            // uniform; take the prefetched words or load them, then one
+            // specialised decode.
+            const bool p0 = has_pre && (int)in.a == d->pre_slot[0];
+            const bool p1 = has_pre && (int)in.a == d->pre_slot[1];
+            uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
+            if (p0 || p1) {
+              a = p0 ? pre.a0 : pre.a1;
+              b = p0 ? pre.b0 : pre.b1;
+            } else if (sl.fmt != FMT_F32) {
+              const gbyte* q = texel_addr(sl, c.x0, 0);
+              a = nt_load(q);
+              if (sl.fmt == FMT_F16) b = nt_load(q + 16);
+            }
+            dec_group_fmt(sl, c.x0, a, b, X, lut);
+          } else {
+            load_px<FAST>(sl, c, X, lut);
+          }
         } else {
           const float4 v = make_float4(d->params[in.arg], d->params[in.arg + 1], d->params[in.arg + 2],
                                        d->params[in.arg + 3]);
@@ -227,7 +295,7 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
 #pragma unroll
         for (int i = 0; i < kPix; ++i) {
           const float a = X[i].w;
-          const float r = a > 0.0f ? 1.0f / a : 0.0f;
+          const float r = a > 0.0f ? __builtin_amdgcn_rcpf(a) : 0.0f;
           X[i] = make_float4(X[i].x * r, X[i].y * r, X[i].z * r, a);
         }
         break;
@@ -245,22 +313,31 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         for (int i = 0; i < kPix; ++i) { const float4 t = X[i]; X[i] = Y[i]; Y[i] = t; }
         break;
       case OP_EOTF:
+        if constexpr (!FAST) {
 #pragma unroll
-        for (int i = 0; i < kPix; ++i) X[i] = untransfer(X[i], in.arg);
+          for (int i = 0; i < kPix; ++i) X[i] = untransfer(X[i], in.arg);
+        }
         break;
       case OP_OETF:
+        if constexpr (!FAST) {
 #pragma unroll
-        for (int i = 0; i < kPix; ++i) X[i] = transfer(X[i], in.arg);
+          for (int i = 0; i < kPix; ++i) X[i] = transfer(X[i], in.arg);
+        }
         break;
       case OP_QUANT: {
-        const Slot s = ldc(&d->slot[in.a]);
+        if constexpr (!FAST) {
+          const Slot s = ldc(&d->slot[in.a]);
 #pragma unroll
-        for (int i = 0; i < kPix; ++i) X[i] = decode_texel(s, encode_texel(s, X[i]), lut);
+          for (int i = 0; i < kPix; ++i) X[i] = decode_texel<FAST>(s, encode_texel<FAST>(s, X[i], lut), lut);
+        }
         break;
       }
       case OP_STORE: {
         const Slot s = ldc(&d->slot[in.a]);
-        store_px(s, c, X);
+        if (FAST && full)
+          enc_group_fmt(s, c.x0, X, lut);
+        else
+          store_px<FAST>(s, c, X, lut);
         break;
       }
       case OP_OFFSET:
@@ -273,40 +350,109 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
   }
 }
 
-__device__ __forceinline__ void load_lut(CDesc d, float* lut) {
-  if (threadIdx.x < 256) lut[threadIdx.x] = d->srgb_lut[threadIdx.x];
+// Raw loads of the full group g for the prefetched slots (flat stages).
+__device__ __forceinline__ void issue(CDesc d, uint64_t g, Pre& p) {
+  const uint64_t x0 = g * kPix;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int sl = d->pre_slot[k];
+    if (sl < 0) continue;
+    const uint64_t ptr = d->slot[sl].ptr;
+    const uint32_t bpp = d->slot[sl].bpp;
+    const gbyte* q = (const gbyte*)ptr + x0 * bpp;
+    const uint4 a = nt_load(q);
+    const uint4 b = bpp == 8 ? nt_load(q + 16) : make_uint4(0u, 0u, 0u, 0u);
+    if (k == 0) { p.a0 = a; p.b0 = b; } else { p.a1 = a; p.b1 = b; }
+  }
 }
 
-// ---------------------------------------------------------------------------
-// pointwise stage: one lane per kPix pixels, one pass over the images.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_pointwise(const StageDesc* dd) {
-  const CDesc d = as_const(dd);
-  __shared__ float lut[256];
-  load_lut(d, lut);
-  __syncthreads();
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ Ctx flat_ctx(uint64_t g, int n) {
   Ctx c;
   c.dx = c.dy = 0;
   c.gather = false;
-  if (d->flat) {
-    const uint64_t total = (uint64_t)d->width * d->height;
-    const uint64_t x0 = (uint64_t)gid * kPix;
-    c.x0 = (int)x0;
-    c.y = 0;
-    c.n = x0 < total ? (int)min((uint64_t)kPix, total - x0) : 0;
-  } else {
-    const uint32_t gpr = (d->width + kPix - 1) / kPix;
-    const uint32_t y = gid / gpr;
-    c.y = (int)y;
-    c.x0 = (int)((gid - y * gpr) * kPix);
-    c.n = y < d->height ? min(kPix, (int)d->width - c.x0) : 0;
-  }
-  if (c.n <= 0) return;
-  Px X, Y, Z;
+  c.x0 = (int)(g * kPix);
+  c.y = 0;
+  c.n = n;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise stage: a persistent grid-stride loop; each lane owns kPix pixels
+// per step.
+//   FAST (flat, in-line codecs): full groups with the raw texels of the next
+//   step already in flight (the address of the last step is reused instead
+//   of branching, so the loop body has no divergent control flow), then the
+//   one partial group of the image tail.
+//   Otherwise: the general interpreter, one step at a time.
+// ---------------------------------------------------------------------------
+template <bool FAST>
+__global__ __launch_bounds__(kBlock) void k_pointwise(const StageDesc* dd) {
+  const CDesc d = as_const(dd);
+  __shared__ Luts luts;
+  load_lut(d, &luts);
+  __syncthreads();
+  const bool flat = d->flat != 0;
+  const uint64_t total = (uint64_t)d->width * d->height;
+  const uint32_t gpr = (d->width + kPix - 1) / kPix;
+  const uint64_t ngroups = flat ? (total + kPix - 1) / kPix : (uint64_t)gpr * d->height;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if constexpr (FAST) {
+    const uint64_t nfull = total / kPix;
+    const bool prefetch = d->pre_slot[0] >= 0;
+    // Two register buffers in ping-pong: the step on buffer A runs while B's
+    // loads are in flight and vice versa (no copy between them, which would
+    // make the step wait for the newest loads).
+    Pre A, B;
+    if (prefetch && g < nfull) issue(d, g, A);
+    // (Programs write a register before reading it, so the stack needs no
+    // initialisation here.)
+    while (g < nfull) {
+      if (prefetch) issue(d, min(g + stride, nfull - 1), B);
+      {
+        Ctx c = flat_ctx(g, kPix);
+        Px X, Y, Z;
+        run<true>(d, 0, d->n_instr, c, X, Y, Z, &luts, A, prefetch, true);
+      }
+      g += stride;
+      if (g >= nfull) break;
+      if (prefetch) issue(d, min(g + stride, nfull - 1), A);
+      {
+        Ctx c = flat_ctx(g, kPix);
+        Px X, Y, Z;
+        run<true>(d, 0, d->n_instr, c, X, Y, Z, &luts, B, prefetch, true);
+      }
+      g += stride;
+    }
+    if (g == nfull && nfull * kPix < total) {
+      Ctx c = flat_ctx(g, (int)(total - nfull * kPix));
+      Px X, Y, Z;
 #pragma unroll
-  for (int i = 0; i < kPix; ++i) X[i] = Y[i] = Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  run(d, 0, d->n_instr, c, X, Y, Z, lut);
+      for (int i = 0; i < kPix; ++i) X[i] = Y[i] = Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      run<true>(d, 0, d->n_instr, c, X, Y, Z, &luts);
+    }
+    return;
+  }
+  for (; g < ngroups; g += stride) {
+    Ctx c;
+    c.dx = c.dy = 0;
+    c.gather = false;
+    if (flat) {
+      const uint64_t x0 = g * kPix;
+      c.x0 = (int)x0;
+      c.y = 0;
+      c.n = (int)min((uint64_t)kPix, total - x0);
+    } else {
+      const uint32_t y = (uint32_t)(g / gpr);
+      c.y = (int)y;
+      c.x0 = (int)((g - (uint64_t)y * gpr) * kPix);
+      c.n = min(kPix, (int)d->width - c.x0);
+    }
+    Px X, Y, Z;
+#pragma unroll
+    for (int i = 0; i < kPix; ++i) X[i] = Y[i] = Z[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    run<false>(d, 0, d->n_instr, c, X, Y, Z, &luts);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -363,7 +509,7 @@ __device__ __forceinline__ bool tap_box(const Layer& L, double ex0, double ey0, 
 // One output tile of a resample stage.  All lanes of the workgroup take the
 // same branches at every barrier (tile and layer decisions are uniform).
 template <int MODE>
-__device__ __forceinline__ void process_tile(CDesc d, int tx, int ty, float4* tile, const float* lut) {
+__device__ __forceinline__ void process_tile(CDesc d, int tx, int ty, float4* tile, const Luts* lut) {
   const int TW = (int)d->tile_w, TH = (int)d->tile_h;
   const int tpr = TW / kPix;
   const int lx = (int)threadIdx.x % tpr, ly = (int)threadIdx.x / tpr;
@@ -521,7 +667,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_resample(const StageDesc* dd) {
   const CDesc d = as_const(dd);
   extern __shared__ float4 tile[];
-  __shared__ float lut[256];
+  __shared__ Luts lut[1];
   load_lut(d, lut);
   __syncthreads();
   process_tile<MODE>(d, (int)blockIdx.x, (int)blockIdx.y, tile, lut);
@@ -533,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_resample(const StageDesc* dd) {
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
   extern __shared__ float4 tile[];
-  __shared__ float lut[256];
+  __shared__ Luts lut[1];
   __shared__ uint32_t job;
   load_lut(as_const(b->stages), lut);
   for (;;) {
@@ -568,14 +714,15 @@ __global__ __launch_bounds__(kBlock) void k_copy(const StageDesc* dd) {
   const uint64_t row = gid / chunks_per_row;
   if (row >= d->copy_rows) return;
   const uint64_t off = (gid - row * chunks_per_row) * 16;
-  const uint8_t* src = reinterpret_cast<const uint8_t*>(d->copy_src) + row * d->copy_src_stride + off;
-  uint8_t* dst = reinterpret_cast<uint8_t*>(d->copy_dst) + row * d->copy_dst_stride + off;
+  const uint64_t s_off = row * d->copy_src_stride + off, d_off = row * d->copy_dst_stride + off;
+  const gbyte* src = (const gbyte*)d->copy_src + s_off;
+  gbyte* dst = (gbyte*)d->copy_dst + d_off;
   const uint64_t n = min((uint64_t)16, d->copy_row_bytes - off);
-  const bool aligned = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+  const bool aligned = (((d->copy_src + s_off) | (d->copy_dst + d_off)) & 15) == 0;
   if (n == 16 && aligned) {
-    *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+    nt_store(nt_load(src), dst);
   } else {
-    for (uint64_t i = 0; i < n; ++i) dst[i] = src[i];
+    for (uint64_t i = 0; i < n; ++i) gst<uint8_t>(dst + i, gld<uint8_t>(src + i));
   }
 }
 
@@ -586,17 +733,21 @@ __global__ __launch_bounds__(kBlock) void k_copy(const StageDesc* dd) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat,
-                               hipStream_t stream) {
+hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat, int fast,
+                               uint32_t max_blocks, hipStream_t stream) {
   uint64_t groups;
   if (flat) {
     groups = ((uint64_t)width * height + zh::kPix - 1) / zh::kPix;
   } else {
     groups = (uint64_t)((width + zh::kPix - 1) / zh::kPix) * height;
   }
-  const uint64_t blocks = (groups + zh::kBlock - 1) / zh::kBlock;
+  // Persistent grid: at most max_blocks workgroups stride over the groups.
+  const uint64_t blocks = std::min<uint64_t>((groups + zh::kBlock - 1) / zh::kBlock, max_blocks);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(zh::k_pointwise, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  if (fast)
+    hipLaunchKernelGGL(zh::k_pointwise<true>, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  else
+    hipLaunchKernelGGL(zh::k_pointwise<false>, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();
 }
 

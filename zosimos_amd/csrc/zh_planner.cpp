@@ -20,6 +20,7 @@
 #include <sstream>
 
 #include "zh_internal.h"
+#include "zh_spec.h"
 
 namespace zh {
 
@@ -464,6 +465,14 @@ struct Emitter {
     s.bpp = bits_bytes(d.texel.bits);
     s.native = is_native(d) ? 1u : 0u;
     s.vec_ok = 0;  // set at bind time from the actual pointer
+    s.fmt = FMT_GENERIC;
+    if (s.native) {
+      const bool bgra = d.texel.parts == ZH_PARTS_BGRA, srgb = s.transfer == ZH_TRANSFER_SRGB;
+      s.fmt = srgb ? (bgra ? FMT_U8_SRGB_BGRA : FMT_U8_SRGB_RGBA) : (bgra ? FMT_U8_LIN_BGRA : FMT_U8_LIN_RGBA);
+    } else if ((s.bits == ZH_BITS_FLOAT16X4 || s.bits == ZH_BITS_FLOAT32X4) && s.transfer == ZH_TRANSFER_LINEAR &&
+               (s.parts == ZH_PARTS_RGBA || s.parts == 18 || s.parts == ZH_PARTS_LCHA)) {
+      s.fmt = s.bits == ZH_BITS_FLOAT16X4 ? FMT_F16 : FMT_F32;
+    }
     slot_index[key] = k;
     return k;
   }
@@ -816,6 +825,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     if (!live[id] || !n.mat) continue;
     StagePlan sp;
     std::memset(&sp.desc, 0, sizeof(sp.desc));
+    sp.desc.pre_slot[0] = sp.desc.pre_slot[1] = -1;
     sp.desc.width = n.w;
     sp.desc.height = n.h;
     Emitter E(B, plan, sp, strict);
@@ -875,6 +885,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         const zh_descriptor& od = plan.output_desc[slot_out];
         StagePlan cp;
         std::memset(&cp.desc, 0, sizeof(cp.desc));
+        cp.desc.pre_slot[0] = cp.desc.pre_slot[1] = -1;
         if (same_layout(n.desc, od)) {
           cp.desc.kind = STAGE_COPY;
           cp.slot_ref = {n.input_slot, ~(int64_t)slot_out};
@@ -913,6 +924,37 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         if (s.width != n.w || s.height != n.h || s.row_stride != (uint64_t)s.width * s.bpp) flat = false;
       }
       sp.desc.flat = flat ? 1u : 0u;
+      // In-line codecs only (k_pointwise<true>): every slot has a FmtCode
+      // and no out-of-line transfer ops.
+      bool fast = flat;
+      for (size_t k = 0; k < sp.slot_ref.size(); ++k) fast = fast && sp.desc.slot[k].fmt != FMT_GENERIC;
+      for (uint32_t i = 0; i < sp.desc.n_instr; ++i) {
+        const uint8_t o = sp.desc.code[i].op;
+        if (o == OP_EOTF || o == OP_OETF || o == OP_QUANT || o == OP_OFFSET) fast = false;
+      }
+      sp.fast = fast;
+      // A registered signature runs as a specialised kernel (zh_spec.h).
+      sp.spec = -1;
+      if (fast) {
+        uint16_t sig[kMaxInstr];
+        for (uint32_t i = 0; i < sp.desc.n_instr; ++i) {
+          const Instr& in = sp.desc.code[i];
+          const bool io = in.op == OP_LOAD || in.op == OP_STORE;
+          sig[i] = sg(in.op, io ? sp.desc.slot[in.a].fmt : 0u, in.op == OP_LOAD && in.b ? 1u : 0u);
+        }
+        sp.spec = zh_spec_lookup(sig, (int)sp.desc.n_instr);
+      }
+      if (flat) {
+        // The first two distinct 4- or 8-byte load slots are prefetched one
+        // grid-stride step ahead.
+        int np = 0;
+        for (uint32_t i = 0; i < sp.desc.n_instr && np < 2; ++i) {
+          const Instr& in = sp.desc.code[i];
+          if (in.op != OP_LOAD) continue;
+          const uint32_t bpp = sp.desc.slot[in.a].bpp;
+          if ((bpp == 4 || bpp == 8) && (np == 0 || sp.desc.pre_slot[0] != in.a)) sp.desc.pre_slot[np++] = in.a;
+        }
+      }
       if (flat) {
         for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
           Slot& s = sp.desc.slot[k];
@@ -932,6 +974,10 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       std::ostringstream t;
       t << "pointwise " << n.w << "x" << n.h << (flat ? " flat" : "") << ": "
         << describe_code(sp.desc, 0, sp.desc.n_instr);
+      if (sp.spec >= 0)
+        t << " [kernel " << zh_spec_name(sp.spec) << "]";
+      else if (sp.fast)
+        t << " [kernel interp-inline]";
       sp.text = t.str();
       plan.stages.push_back(sp);
       continue;

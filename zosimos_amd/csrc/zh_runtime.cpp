@@ -18,8 +18,10 @@
 #include "zh_internal.h"
 
 extern "C" {
-hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat,
-                               hipStream_t stream);
+hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat, int fast,
+                               uint32_t max_blocks, hipStream_t stream);
+hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
+                          uint32_t max_blocks, hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -47,13 +49,34 @@ static zh_status hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #call); \
   } while (0)
 
-// Exact sRGB EOTF of every 8-bit code (IEC 61966-2-1 in FP64, rounded once).
-static void srgb_table(float* t) {
-  for (int i = 0; i < 256; ++i) {
-    const double v = i / 255.0;
-    const double l = v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4);
-    t[i] = (float)l;
+// IEC 61966-2-1 in FP64.
+static double srgb_eotf(double v) { return v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4); }
+static double srgb_oetf(double l) { return l <= 0.0031308 ? l * 12.92 : 1.055 * std::pow(l, 1.0 / 2.4) - 0.055; }
+// 8-bit code of a linear value: clamp, OETF, round to nearest (even).
+static uint32_t srgb_code(double l) {
+  if (!(l > 0.0)) return 0;
+  if (l >= 1.0) return 255;
+  return (uint32_t)std::nearbyint(255.0 * srgb_oetf(l));
+}
+
+// The context's 8-bit sRGB tables (zh_stage.h Luts).
+static void srgb_tables(Luts* L) {
+  std::memset(L, 0, sizeof(*L));
+  for (int k = 0; k < 256; ++k) L->dec[k] = (float)srgb_eotf(k / 255.0);
+  for (int i = 0; i < 4096; ++i) L->idx[i] = (uint8_t)srgb_code(i / 4096.0);
+  // thr[k]: smallest non-negative f32 whose code is >= k, by bisection over
+  // the ordered bit patterns of non-negative floats.
+  for (int k = 1; k < 256; ++k) {
+    uint32_t lo = 0, hi = 0x3f800000u;  // code(0) < k <= code(1.0f)
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      float f;
+      std::memcpy(&f, &mid, 4);
+      if (srgb_code(f) >= (uint32_t)k) hi = mid; else lo = mid;
+    }
+    std::memcpy(&L->thr[k], &hi, 4);
   }
+  for (int k = 256; k < 260; ++k) L->thr[k] = INFINITY;
 }
 
 }  // namespace zh
@@ -88,11 +111,23 @@ struct zh_program {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_failed = false;
   uint32_t max_lds = 0;
+  // Pointwise kernel per stage for the bound images: 2 specialised, 1
+  // interpreter with in-line codecs, 0 general interpreter (an unaligned
+  // image forces 0).
+  std::vector<int> variant;
 };
 
 extern "C" {
 
 const char* zh_last_error(void) { return g_error.c_str(); }
+
+zh_status zh_srgb_tables(void* out, size_t len) {
+  if (!out || len < sizeof(Luts)) return fail(ZH_ERR_INVALID_ARGUMENT, "buffer smaller than the tables");
+  Luts L;
+  srgb_tables(&L);
+  std::memcpy(out, &L, sizeof(L));
+  return ZH_OK;
+}
 const char* zh_version(void) { return "zosimos-hip 0.1.0 (gfx950)"; }
 
 zh_status zh_context_create(int32_t device, void* stream, zh_context** out) {
@@ -112,11 +147,12 @@ zh_status zh_context_create(int32_t device, void* stream, zh_context** out) {
     if (e != hipSuccess) { delete ctx; return hip_fail(e, "hipStreamCreate"); }
     ctx->own_stream = true;
   }
-  float lut[256];
-  srgb_table(lut);
+  Luts lut;
+  srgb_tables(&lut);
   e = hipMalloc(&ctx->d_lut, sizeof(lut));
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipMalloc(&ctx->d_error, sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemcpy(ctx->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ctx->d_lut, &lut, sizeof(lut), hipMemcpyHostToDevice);
   uint32_t zero = 0;
   if (e == hipSuccess) e = hipMemcpy(ctx->d_error, &zero, sizeof(zero), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = zh_set_resample_lds(kTileLdsBytes);
@@ -464,7 +500,8 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     const StagePlan& a = np.stages[i];
     const StagePlan& b = p->plan.stages[i];
     same = a.desc.kind == b.desc.kind && a.tile_mode == b.tile_mode && a.desc.tiles_x == b.desc.tiles_x &&
-           a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes;
+           a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes &&
+           a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat;
   }
   p->params = params;
   if (same) {
@@ -514,6 +551,7 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
   for (size_t i = 0; i < outs.size(); ++i) op[i] = outs[i] ? outs[i]->ptr : nullptr;
   if (p->uploaded && ip == p->bound_inputs && op == p->bound_outputs) return ZH_OK;
 
+  std::vector<int> variant(plan.stages.size(), 0);
   for (size_t s = 0; s < plan.stages.size(); ++s) {
     StageDesc d = plan.stages[s].desc;
     d.srgb_lut = p->ctx->d_lut;
@@ -545,8 +583,32 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
         sl.ptr = resolve(refs[k], &stride);
         sl.vec_ok = ((sl.ptr & 15) == 0 && (d.flat || (sl.row_stride & 15) == 0)) ? 1u : 0u;
       }
+      // Prefetch only slots whose addresses allow 16-byte loads; keep the
+      // first entry filled so the kernel's "any prefetch" test holds.
+      int32_t pre[2] = {-1, -1};
+      int np = 0;
+      for (int k = 0; k < 2; ++k) {
+        const int32_t sl = d.pre_slot[k];
+        if (sl >= 0 && d.slot[sl].vec_ok) pre[np++] = sl;
+      }
+      d.pre_slot[0] = pre[0];
+      d.pre_slot[1] = pre[1];
+      if (d.kind == STAGE_POINTWISE) {
+        bool aligned = true;
+        for (size_t k = 0; k < refs.size(); ++k) aligned = aligned && d.slot[k].vec_ok;
+        const StagePlan& sp = plan.stages[s];
+        variant[s] = !aligned ? 0 : sp.spec >= 0 ? 2 : sp.fast ? 1 : 0;
+      }
     }
     p->h_stages[s] = d;
+  }
+  if (variant != p->variant) {
+    // A captured graph holds the previous kernels: capture again.
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    p->variant = variant;
   }
   ZH_HIP(hipMemcpyAsync(p->d_stages, p->h_stages, plan.stages.size() * sizeof(StageDesc), hipMemcpyHostToDevice,
                         p->ctx->stream));
@@ -565,9 +627,15 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
   const StagePlan& sp = p->plan.stages[u.first];
   const StageDesc* d = p->d_stages + u.first;
   switch (sp.desc.kind) {
-    case STAGE_POINTWISE:
-      ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, s));
+    case STAGE_POINTWISE: {
+      const int v = u.first < p->variant.size() ? p->variant[u.first] : 0;
+      if (v == 2)
+        ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus * 8, s));
+      else
+        ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, v, (uint32_t)p->ctx->cus * 8,
+                                   s));
       break;
+    }
     case STAGE_RESAMPLE:
       ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
       break;

@@ -52,6 +52,15 @@ struct Instr {
   uint32_t arg;
 };
 
+// 8-bit sRGB tables (built once per context on the host in FP64):
+//   dec[k]  linear value of code k, idx[i] code of i / 4096, thr[k] smallest
+//   f32 whose code is >= k (thr[256..259] = +inf).
+struct Luts {
+  float dec[256];
+  float thr[260];
+  uint8_t idx[4096];
+};
+
 struct Slot {
   uint64_t ptr;         // device address of texel (0, 0)
   uint64_t row_stride;  // bytes
@@ -60,6 +69,20 @@ struct Slot {
   uint32_t bpp;         // bytes per texel
   uint32_t native;      // 1: native unorm (round to nearest, proper BGRA swizzle)
   uint32_t vec_ok;      // 1: ptr and row_stride are 16-byte aligned
+  uint32_t fmt;         // FMT_* code of the in-line codecs, FMT_GENERIC otherwise
+  uint32_t pad;
+};
+
+// Formats with a specialised in-line codec; one uniform switch per micro-op
+// selects the body, so per-pixel code carries no format tests.
+enum FmtCode : uint32_t {
+  FMT_GENERIC = 0,
+  FMT_U8_SRGB_RGBA = 1,
+  FMT_U8_SRGB_BGRA = 2,
+  FMT_U8_LIN_RGBA = 3,
+  FMT_U8_LIN_BGRA = 4,
+  FMT_F16 = 5,  // linear, rgba-identity parts
+  FMT_F32 = 6,
 };
 
 enum LayerMode : uint32_t {
@@ -95,10 +118,11 @@ struct StageDesc {
   uint32_t tiles_x, tiles_y;
   uint32_t lds_texels;        // resample: float4 texels of LDS available per tile
   uint32_t band_h;            // resample: rows per LDS fill (divides tile_h)
+  int32_t pre_slot[2];        // pointwise (flat): slots whose raw texels are prefetched, -1 = none
   // copy stage
   uint64_t copy_src, copy_dst, copy_src_stride, copy_dst_stride, copy_row_bytes;
   uint32_t copy_rows, copy_pad;
-  const float* srgb_lut;      // 256 floats: exact sRGB EOTF of each 8-bit code
+  const void* srgb_lut;       // the context's 8-bit sRGB tables (Luts of zh_codec.h)
   uint32_t* error_flag;       // set to 1 by a kernel that met an impossible case
   Layer layer[kMaxLayers];
   Slot slot[kMaxSlots];
