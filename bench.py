@@ -30,7 +30,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
+                    help="copy: the byte-copy calibration workload of the HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -61,28 +62,46 @@ def device_inputs(wl, ctx, torch, seed=0):
 
 def cpu_baseline(config: str):
     """The FP64 numpy oracle (a port of the path, one host core) timed on a
-    bounded sample of the same workload."""
+    bounded sample of the same workload (about 10 s of CPU work)."""
     from oracle import pipelines as PL
+    from zosimos_amd import workloads as W
     rng = np.random.default_rng(0)
     if config == "c2":
         w, h = 1024, 512
         src = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
         layer = rng.random((h, w, 4)).astype(np.float16).astype(np.float64)
         fn = lambda: PL.c2(src, layer)
-        sample = f"{w}x{h} pixels of the workload's chain (FP64 numpy oracle)"
+        px, sample = w * h, f"{w}x{h} pixels of the workload's chain"
     elif config == "c1":
         w, h = 1024, 768
         a = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
         b = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
-        from zosimos_amd import workloads as W
         m = W.about_centre(w, h, 0.75, np.radians(15.0)).transformation
         fn = lambda: PL.c1(a, b, m, (h, w))
-        sample = "one full 1024x768 frame (FP64 numpy oracle)"
+        px, sample = w * h, "one full 1024x768 frame"
+    elif config == "c3":
+        # The first images of the seeded batch, as many as fit a 10 s budget.
+        shapes = [s_ for s_ in W.c3_shapes() if s_[0] * s_[1] <= 1 << 20][:8]
+        data = [W.random_rgba8(rng, w_, h_) if f_ == 0 else W.random_rgba16f(rng, w_, h_)
+                for (w_, h_, f_, _) in shapes]
+
+        def fn():
+            for (w_, h_, f_, sc), d_ in zip(shapes, data):
+                PL.c3_image(d_, f_, w_, h_, max(1, int(round(w_ * sc))), max(1, int(round(h_ * sc))))
+        px = sum(max(1, int(round(w_ * sc))) * max(1, int(round(h_ * sc))) for (w_, h_, _, sc) in shapes)
+        sample = f"{len(shapes)} images of the seeded batch (output pixels counted)"
+    elif config == "c5":
+        w, h = 480, 270
+        wl = W.c5(8, w, h)
+        host = wl.make_inputs()
+        layers = [PL.f16_image(host[r], w, h) for r in wl.inputs]
+        fn = lambda: PL.c5(layers, wl.meta["matrices"], (h, w))
+        px, sample = w * h, f"8 layers at {w}x{h} (the workload at 1/8 scale per side)"
     else:
         w, h = 1024, 512
         src = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
         fn = lambda: PL.c4(src)
-        sample = f"{w}x{h} pixels of the colour chain (FP64 numpy oracle)"
+        px, sample = w * h, f"{w}x{h} pixels of the colour chain"
     t0 = time.perf_counter()
     reps = 0
     while True:
@@ -91,8 +110,8 @@ def cpu_baseline(config: str):
         if time.perf_counter() - t0 > 10.0:
             break
     dt = time.perf_counter() - t0
-    return {"value": round(reps * w * h / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} x {sample}, {dt:.1f} s"}
+    return {"value": round(reps * px / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} x {sample} (FP64 numpy oracle), {dt:.1f} s"}
 
 
 def main():
@@ -105,7 +124,7 @@ def main():
     import zosimos_amd as z
     from zosimos_amd import workloads as W
 
-    wl = W.CONFIGS[args.config]()
+    wl = W.copy_frame() if args.config == "copy" else W.CONFIGS[args.config]()
     stream = torch.cuda.Stream()  # default priority
     ctx = z.Context(0, stream=stream.cuda_stream)
     exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
@@ -164,10 +183,11 @@ def main():
         "hbm_frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": stage_lines[0] if stage_lines else "", "kernel_ms": round(unit_ms[dom], 5)},
+                     "kernel": stage_lines[dom] if len(stage_lines) == len(unit_ms) else text.splitlines()[0],
+                     "kernel_ms": round(unit_ms[dom], 5)},
         "wall_s": round(wall, 3),
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config != "copy":
         line["cpu_baseline"] = cpu_baseline(args.config)
     print(json.dumps(line), flush=True)
     del imgs, outs

@@ -224,4 +224,19 @@ def c5(layers: int = 8, width: int = 3840, height: int = 2160, seed: int = 0) ->
                     dict(matrices=mats))
 
 
+def copy_frame(width: int = 7680, height: int = 4320, seed: int = 0) -> Workload:
+    """Calibration workload: a byte copy (transmute RGBA8 sRGB -> RGBA8
+    linear) of one frame, width * height * 4 bytes read and written once;
+    hardware byte counters read on it give their scale."""
+    cmd = CommandBuffer()
+    d8 = Descriptor.rgba8_srgb(width, height)
+    a = cmd.input(d8)
+    o, od = cmd.output(cmd.transmute(a, Descriptor(width, height, RGBA8, LINEAR_709)))
+
+    def make():
+        return {a: random_rgba8(np.random.default_rng([seed, 900]), width, height)}
+
+    return Workload(f"copy_rgba8_{width}x{height}", cmd, {a: d8}, {o: od}, make, width * height)
+
+
 CONFIGS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
