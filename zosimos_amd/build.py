@@ -20,13 +20,14 @@ LIB = os.path.join(HERE, "libzosimos_hip.so")
 SOURCES = [
     "zh_kernels.hip",
     "zh_spec.hip",
+    "zh_warp.hip",
     "zh_mlp.hip",
     "zh_runtime.cpp",
     "zh_planner.cpp",
     "zh_command.cpp",
     "zh_surrogate.cpp",
 ]
-HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_internal.h", "zh_mlp.h"]
+HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_internal.h", "zh_mlp.h"]
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) is disabled for every device compile.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

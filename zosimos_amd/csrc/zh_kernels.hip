@@ -458,21 +458,6 @@ __global__ __launch_bounds__(kBlock) void k_pointwise(const StageDesc* dd) {
 // ---------------------------------------------------------------------------
 // resample stage
 // ---------------------------------------------------------------------------
-// Map a pixel centre through a layer's inverse transform (FP64).
-__device__ __forceinline__ double2 map_pt(const double* m, double px, double py) {
-  const double q = m[6] * px + m[7] * py + m[8];
-  return make_double2((m[0] * px + m[1] * py + m[2]) / q, (m[3] * px + m[4] * py + m[5]) / q);
-}
-
-// Split a mapped coordinate into the texel left of the sample and the
-// bilinear weight of the right texel (texel centres at i + 0.5).
-__device__ __forceinline__ void split_tap(double u, int& i, float& f) {
-  const double t = u - 0.5;
-  const double fl = floor(t);
-  i = (int)fl;
-  f = (float)(t - fl);
-}
-
 __device__ __forceinline__ float4 f4_fma(float4 a, float k, float4 b) {
   return make_float4(fmaf(a.x, k, b.x), fmaf(a.y, k, b.y), fmaf(a.z, k, b.z), fmaf(a.w, k, b.w));
 }
@@ -484,27 +469,6 @@ __device__ __forceinline__ float4 over4(float4 top, float4 below) {
 }
 
 enum TileMode { TILE_STAGED = 0, TILE_DIRECT = 1, TILE_NEAREST = 2 };
-
-// Source box of every bilinear tap of the tile's pixel centres (one texel of
-// margin against rounding), clipped to the source grid.  Returns false when
-// the tile's footprint misses the source.
-__device__ __forceinline__ bool tap_box(const Layer& L, double ex0, double ey0, double ex1, double ey1, int& bx0,
-                                        int& by0, int& bw, int& bh) {
-  const double2 p0 = map_pt(L.inv, ex0, ey0), p1 = map_pt(L.inv, ex1, ey0);
-  const double2 p2 = map_pt(L.inv, ex0, ey1), p3 = map_pt(L.inv, ex1, ey1);
-  const double umin = fmin(fmin(p0.x, p1.x), fmin(p2.x, p3.x)) - 0.5;
-  const double umax = fmax(fmax(p0.x, p1.x), fmax(p2.x, p3.x)) - 0.5;
-  const double vmin = fmin(fmin(p0.y, p1.y), fmin(p2.y, p3.y)) - 0.5;
-  const double vmax = fmax(fmax(p0.y, p1.y), fmax(p2.y, p3.y)) - 0.5;
-  const int sw = (int)L.src_w, sh = (int)L.src_h;
-  if (!(umax > -1.0 && vmax > -1.0 && umin < (double)sw && vmin < (double)sh)) return false;
-  bx0 = max(0, (int)floor(umin) - 1);
-  by0 = max(0, (int)floor(vmin) - 1);
-  const int bx1 = min(sw - 1, (int)floor(umax) + 2), by1 = min(sh - 1, (int)floor(vmax) + 2);
-  bw = bx1 - bx0 + 1;
-  bh = by1 - by0 + 1;
-  return bw > 0 && bh > 0;
-}
 
 // One output tile of a resample stage.  All lanes of the workgroup take the
 // same branches at every barrier (tile and layer decisions are uniform).

@@ -14,6 +14,7 @@
 // Everything else fuses into one kernel per materialised value; runs of 3x3
 // matrices (colour conversion, Oklab M1/M2, chroma/hue) fold in FP64 first.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -663,6 +664,63 @@ static uint64_t tile_box_texels(const Mat3& inv, int TW, int TH) {
 
 // Fraction of a layer's source texels whose centre lands on the output
 // canvas (for algorithmic bytes).
+// Registered warp form of a bilinear resample stage (zh_spec.h), or -1:
+// affine layers, a solid (or empty) base, one source form shared by every
+// layer and a registered post program.
+static int warp_variant(const StageDesc& d) {
+  const uint32_t nb = d.base_end - d.base_begin;
+  if (!(nb == 0 || (nb == 2 && d.code[d.base_begin].op == OP_SOLID && d.code[d.base_begin + 1].op == OP_PREMUL)))
+    return -1;
+  uint32_t src = ~0u, fmt = 0, mat = 0;
+  for (uint32_t l = 0; l < d.n_layers; ++l) {
+    const Layer& L = d.layer[l];
+    if (L.mode != LAYER_BILINEAR_OVER) return -1;
+    if (L.inv[6] != 0.0 || L.inv[7] != 0.0 || std::fabs(L.inv[8] - 1.0) > 1e-12) return -1;
+    const Instr* c = d.code + L.prog_begin;
+    const uint32_t n = L.prog_end - L.prog_begin;
+    uint32_t s_, f_, m_ = 0;
+    if (n == 2 && c[0].op == OP_LOAD && c[1].op == OP_PREMUL) {
+      s_ = WSRC_ONE;
+    } else if (n == 3 && c[0].op == OP_LOAD && c[1].op == OP_MAT3 && c[2].op == OP_PREMUL) {
+      s_ = WSRC_ONE;
+      m_ = 1;
+    } else if (n == 5 && c[0].op == OP_LOAD && c[1].op == OP_PREMUL && c[2].op == OP_LOAD && c[2].b &&
+               c[3].op == OP_PREMUL && c[4].op == OP_OVER && d.slot[c[0].a].fmt == d.slot[c[2].a].fmt) {
+      s_ = WSRC_OVER2;
+    } else {
+      return -1;
+    }
+    f_ = d.slot[c[0].a].fmt;
+    if (f_ == FMT_GENERIC) return -1;
+    // The kernel addresses every source over the whole layer grid with
+    // 32-bit byte offsets, one row stride for both sources of WSRC_OVER2.
+    for (uint32_t i = 0; i < n; ++i)
+      if (c[i].op == OP_LOAD) {
+        const Slot& sl = d.slot[c[i].a];
+        if (sl.width != L.src_w || sl.height != L.src_h || sl.row_stride != d.slot[c[0].a].row_stride ||
+            sl.row_stride * sl.height >= (1ull << 32))
+          return -1;
+      }
+    if (src == ~0u) {
+      src = s_;
+      fmt = f_;
+      mat = m_;
+    } else if (src != s_ || fmt != f_ || mat != m_) {
+      return -1;
+    }
+  }
+  if (src == ~0u) return -1;
+  uint16_t sig[kMaxInstr];
+  const uint32_t np = d.post_end - d.post_begin;
+  for (uint32_t i = 0; i < np; ++i) {
+    const Instr& in = d.code[d.post_begin + i];
+    if (in.op == OP_LOAD) return -1;
+    if (in.op == OP_STORE && (d.slot[in.a].width != d.width || d.slot[in.a].height != d.height)) return -1;
+    sig[i] = sg(in.op, in.op == OP_STORE ? d.slot[in.a].fmt : 0u);
+  }
+  return zh_warp_lookup(src, fmt, mat, sig, (int)np);
+}
+
 static double visible_fraction(const Mat3& inv, uint32_t sw, uint32_t sh, uint32_t W, uint32_t H) {
   Mat3 fwd;
   if (!mat_inv(inv, &fwd)) return 1.0;
@@ -826,6 +884,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     StagePlan sp;
     std::memset(&sp.desc, 0, sizeof(sp.desc));
     sp.desc.pre_slot[0] = sp.desc.pre_slot[1] = -1;
+    sp.desc.variant = -1;
     sp.desc.width = n.w;
     sp.desc.height = n.h;
     Emitter E(B, plan, sp, strict);
@@ -886,6 +945,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         StagePlan cp;
         std::memset(&cp.desc, 0, sizeof(cp.desc));
         cp.desc.pre_slot[0] = cp.desc.pre_slot[1] = -1;
+        cp.desc.variant = -1;
         if (same_layout(n.desc, od)) {
           cp.desc.kind = STAGE_COPY;
           cp.slot_ref = {n.input_slot, ~(int64_t)slot_out};
@@ -1020,13 +1080,13 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     sp.desc.n_layers = (uint32_t)layers.size();
     uint64_t max_box = 0;
     int TW = 32, TH = 32, BH = 32;
-    if (bil) {
-      // Tile shape and band height: the candidate with the least LDS fill per
-      // output pixel whose worst band box fits the LDS budget.
-      const int cand[][2] = {{32, 32}, {64, 16}, {16, 64}, {128, 8}, {8, 128}};
+    // Tile shape and band height of the LDS-staged kernel: the candidate with
+    // the least LDS fill per output pixel whose worst band box fits the LDS
+    // budget; no fit -> four gathered taps per pixel (TILE_DIRECT).
+    auto choose_tiles = [&](const std::vector<std::array<int, 2>>& cands) {
       double best = 1e300;
       uint64_t best_box = 0;
-      for (auto& cd : cand) {
+      for (auto& cd : cands) {
         for (int bh = cd[1]; bh >= 1; bh /= 2) {
           uint64_t worst = 0, sum = 0;
           for (int L : layers) {
@@ -1047,12 +1107,16 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       }
       if (best_box == 0) {
         sp.tile_mode = 1;  // TILE_DIRECT
-        TW = 32;
-        TH = 32;
+        TW = cands[0][0];
+        TH = cands[0][1];
+        BH = TH;
       } else {
         sp.tile_mode = 0;  // TILE_STAGED
         max_box = best_box;
       }
+    };
+    if (bil) {
+      choose_tiles({{32, 32}, {64, 16}, {16, 64}, {128, 8}, {8, 128}});
     } else {
       sp.tile_mode = 2;  // TILE_NEAREST
     }
@@ -1116,6 +1180,21 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     E.finish();
     for (uint32_t i = sp.desc.post_begin; i < sp.desc.post_end; ++i)
       if (sp.desc.code[i].op == OP_STORE) sp.alg_bytes += slot_bytes(sp.desc.slot[sp.desc.code[i].a]);
+    // A registered affine layer form runs as a warp kernel (zh_warp.hip) on
+    // 64 x 16 tiles; the interpreter kernels stay valid on the same tiles as
+    // the fallback for images the warp kernel cannot address.
+    sp.desc.variant = bil ? warp_variant(sp.desc) : -1;
+    if (sp.desc.variant >= 0) {
+      max_box = 0;
+      choose_tiles({{64, 16}});
+      sp.desc.tile_w = TW;
+      sp.desc.tile_h = TH;
+      sp.desc.band_h = BH;
+      sp.desc.tiles_x = (n.w + TW - 1) / TW;
+      sp.desc.tiles_y = (n.h + TH - 1) / TH;
+      sp.desc.lds_texels = (uint32_t)std::min<uint64_t>(max_box, kTileLdsBytes / 16);
+      sp.lds_bytes = sp.tile_mode == 0 ? (uint32_t)(sp.desc.lds_texels * 16) : 0;
+    }
     {
       std::ostringstream t2;
       t2 << "resample " << n.w << "x" << n.h << " tile " << TW << "x" << TH << " band " << BH
@@ -1127,6 +1206,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
            << " " << ly.src_w << "x" << ly.src_h << ")[" << describe_code(sp.desc, ly.prog_begin, ly.prog_end) << "]";
       }
       t2 << " post[" << describe_code(sp.desc, sp.desc.post_begin, sp.desc.post_end) << "]";
+      if (sp.desc.variant >= 0) t2 << " [kernel " << zh_warp_name(sp.desc.variant) << "]";
       sp.text = t2.str();
     }
     plan.stages.push_back(sp);

@@ -22,6 +22,10 @@ hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint3
                                uint32_t max_blocks, hipStream_t stream);
 hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
                           uint32_t max_blocks, hipStream_t stream);
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t max_blocks,
+                          hipStream_t stream);
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t max_blocks,
+                                hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -87,11 +91,13 @@ using namespace zh;
 // resample stages run by one persistent tile-queue kernel.
 struct Unit {
   bool batch = false;
+  bool warp = false;             // batch of warp-form stages (k_warp_batch)
   size_t first = 0, count = 1;   // stage range
   BatchDesc* d_batch = nullptr;  // device
   uint32_t* d_counter = nullptr;
   uint32_t* d_prefix = nullptr;
   uint32_t grid = 0, lds = 0;
+  uint32_t total_tiles = 0;
 };
 
 struct zh_program {
@@ -329,12 +335,19 @@ static void make_units(zh_program* p) {
     Unit u;
     u.first = i;
     u.count = 1;
-    if (st[i].desc.kind == STAGE_RESAMPLE && st[i].tile_mode == 0) {
+    // Batches: runs of independent resample stages that are all warp forms,
+    // or all LDS-staged interpreter stages.
+    const bool warp = st[i].desc.kind == STAGE_RESAMPLE && st[i].desc.variant >= 0;
+    auto same_family = [&](const StagePlan& sp) {
+      if (sp.desc.kind != STAGE_RESAMPLE) return false;
+      return warp ? sp.desc.variant >= 0 : (sp.desc.variant < 0 && sp.tile_mode == 0);
+    };
+    if (same_family(st[i])) {
       std::vector<int64_t> written;
       for (size_t k = 0; k < st[i].slot_ref.size(); ++k)
         if (st[i].slot_ref[k] < 0 || st[i].slot_ref[k] >= kIntermediateBase) written.push_back(st[i].slot_ref[k]);
       size_t j = i + 1;
-      while (j < st.size() && st[j].desc.kind == STAGE_RESAMPLE && st[j].tile_mode == 0) {
+      while (j < st.size() && same_family(st[j])) {
         bool dep = false;
         for (int64_t r : st[j].slot_ref)
           for (int64_t w : written)
@@ -346,6 +359,7 @@ static void make_units(zh_program* p) {
       }
       if (j - i >= 2) {
         u.batch = true;
+        u.warp = warp;
         u.count = j - i;
       }
     }
@@ -382,6 +396,7 @@ static zh_status alloc_device(zh_program* p) {
     ZH_HIP(hipMemcpy(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     b.n_stages = (uint32_t)u.count;
     b.total_tiles = prefix.back();
+    u.total_tiles = b.total_tiles;
     b.counter = u.d_counter;
     b.stages = p->d_stages + u.first;
     b.tile_prefix = u.d_prefix;
@@ -501,7 +516,7 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     const StagePlan& b = p->plan.stages[i];
     same = a.desc.kind == b.desc.kind && a.tile_mode == b.tile_mode && a.desc.tiles_x == b.desc.tiles_x &&
            a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes &&
-           a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat;
+           a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat && a.desc.variant == b.desc.variant;
   }
   p->params = params;
   if (same) {
@@ -593,6 +608,14 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
       }
       d.pre_slot[0] = pre[0];
       d.pre_slot[1] = pre[1];
+      if (d.kind == STAGE_RESAMPLE && d.variant >= 0) {
+        // The warp kernel reads and writes whole texels with 4/8/16-byte
+        // accesses: every image must be aligned to its texel size.
+        bool ok = true;
+        for (size_t k = 0; k < refs.size(); ++k)
+          ok = ok && d.slot[k].ptr % d.slot[k].bpp == 0 && d.slot[k].row_stride % d.slot[k].bpp == 0;
+        variant[s] = ok ? 3 : 0;
+      }
       if (d.kind == STAGE_POINTWISE) {
         bool aligned = true;
         for (size_t k = 0; k < refs.size(); ++k) aligned = aligned && d.slot[k].vec_ok;
@@ -621,7 +644,25 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
 static zh_status launch_unit(zh_program* p, const Unit& u) {
   hipStream_t s = p->ctx->stream;
   if (u.batch) {
-    ZH_HIP(zh_launch_batch(u.d_batch, u.d_counter, u.grid, u.lds, s));
+    bool all_warp = u.warp;
+    for (size_t k = 0; k < u.count && all_warp; ++k) all_warp = p->variant[u.first + k] == 3;
+    if (all_warp) {
+      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus * 8, s));
+      return ZH_OK;
+    }
+    bool all_staged = true;
+    for (size_t k = 0; k < u.count; ++k) all_staged = all_staged && p->plan.stages[u.first + k].tile_mode == 0;
+    if (all_staged) {
+      ZH_HIP(zh_launch_batch(u.d_batch, u.d_counter, u.grid, u.lds, s));
+      return ZH_OK;
+    }
+    // Mixed fallback kernels: one launch per stage.
+    for (size_t k = 0; k < u.count; ++k) {
+      Unit one;
+      one.first = u.first + k;
+      zh_status st = launch_unit(p, one);
+      if (st != ZH_OK) return st;
+    }
     return ZH_OK;
   }
   const StagePlan& sp = p->plan.stages[u.first];
@@ -637,7 +678,10 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
       break;
     }
     case STAGE_RESAMPLE:
-      ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
+      if (u.first < p->variant.size() && p->variant[u.first] == 3)
+        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus * 8, s));
+      else
+        ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
       break;
     case STAGE_COPY:
       ZH_HIP(zh_launch_copy(d, sp.desc.copy_row_bytes, sp.desc.copy_rows, s));
@@ -660,7 +704,8 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
   ZH_HIP(hipSetDevice(p->ctx->device));
   zh_status st = bind(p, bindings, num_bindings);
   if (st != ZH_OK) return st;
-  const bool use_graph = !(p->flags & ZH_PLAN_NO_GRAPH) && !p->graph_failed && p->units.size() > 0;
+  // A graph pays off from two launches on: one kernel launches directly.
+  const bool use_graph = !(p->flags & ZH_PLAN_NO_GRAPH) && !p->graph_failed && p->units.size() > 1;
   if (use_graph && !p->graph_exec) {
     // Capture once; descriptor contents change through their device copies,
     // never through the graph, so the executable graph stays valid.

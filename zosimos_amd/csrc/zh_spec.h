@@ -69,6 +69,24 @@ constexpr uint16_t sg(uint32_t op, uint32_t fmt = 0, uint32_t push = 0) {
   X(over_s8, ZH_LD(ZH_S8), ZH_PM, ZH_LDP(ZH_S8), ZH_PM, ZH_OV, ZH_UP, ZH_ST(ZH_S8))                      \
   X(over_h, ZH_LD(ZH_H), ZH_PM, ZH_LDP(ZH_H), ZH_PM, ZH_OV, ZH_UP, ZH_ST(ZH_H))
 
+// ---- affine layer stacks (zh_warp.hip) ----------------------------------------
+// A resample stage whose layers are all bilinear 'over' through affine maps,
+// whose base is a solid colour (or transparent) and whose layer source
+// programs share one form runs as a specialised warp kernel:
+//   WSRC_ONE    layer program [load(F) (mat3) premul]
+//   WSRC_OVER2  layer program [load(F) premul load(F) premul over]
+// X(name, source form, F, mat3 in the source program, post elements...)
+enum WarpSrc : uint32_t { WSRC_ONE = 0, WSRC_OVER2 = 1 };
+
+#define ZH_WARP_LIST(X)                                                                        \
+  X(w_over2_s8, WSRC_OVER2, ZH_S8, 0, ZH_UP, ZH_ST(ZH_S8))                                      \
+  X(w_s8m_s8, WSRC_ONE, ZH_S8, 1, ZH_UP, ZH_ST(ZH_S8))                                          \
+  X(w_hm_h, WSRC_ONE, ZH_H, 1, ZH_UP, ZH_ST(ZH_H))                                              \
+  X(w_s8_s8, WSRC_ONE, ZH_S8, 0, ZH_UP, ZH_ST(ZH_S8))                                           \
+  X(w_h_h, WSRC_ONE, ZH_H, 0, ZH_UP, ZH_ST(ZH_H))                                               \
+  X(w_h_s8, WSRC_ONE, ZH_H, 0, ZH_UP, ZH_ST(ZH_S8))                                             \
+  X(w_h_chain_s8, WSRC_ONE, ZH_H, 0, ZH_UP, ZH_M, ZH_CB, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_S8))
+
 }  // namespace zh
 
 extern "C" {
@@ -76,4 +94,8 @@ extern "C" {
 // defined next to the kernels).
 int zh_spec_lookup(const uint16_t* sig, int n);
 const char* zh_spec_name(int index);
+// Index of the warp kernel for (source form, format, mat3, post signature),
+// or -1.
+int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n);
+const char* zh_warp_name(int index);
 }

@@ -119,6 +119,8 @@ struct StageDesc {
   uint32_t lds_texels;        // resample: float4 texels of LDS available per tile
   uint32_t band_h;            // resample: rows per LDS fill (divides tile_h)
   int32_t pre_slot[2];        // pointwise (flat): slots whose raw texels are prefetched, -1 = none
+  int32_t variant;            // resample: warp kernel index (zh_spec.h), -1 = none
+  uint32_t pad0;
   // copy stage
   uint64_t copy_src, copy_dst, copy_src_stride, copy_dst_stride, copy_row_bytes;
   uint32_t copy_rows, copy_pad;

@@ -1,0 +1,331 @@
+// zh_warp.hip -- specialised kernels of affine layer stacks.
+//
+// A resample stage composites bilinear samples of affine-mapped layers 'over'
+// a solid base and finishes with a per-pixel post program (zh_stage.h).  Its
+// registered forms (zh_spec.h) run here as straight-line code:
+//
+//   * the output is cut into tiles of tile_w x tile_h pixels (one workgroup
+//     of 256 lanes, kPix = 4 horizontally adjacent pixels per lane); a
+//     persistent grid walks the tiles with a stride;
+//   * per tile and layer, the tile's footprint is mapped in FP64 (four
+//     corners) and the layer is skipped when it misses the source; the map of
+//     the tile's first pixel centre is split into an integer texel and an
+//     FP32 remainder, and every pixel adds its FP32 offset (|offset| < 100 px,
+//     so coordinates carry ~1e-5 px error instead of FP32's 5e-4 px at 8K);
+//   * the four taps are read straight from global memory (the L1/L2 caches
+//     serve the overlap between neighbouring pixels), decoded in-line,
+//     premultiplied and weighted; taps outside the source get weight 0
+//     (transparent border) and read a clamped in-bounds texel;
+//   * a 3x3 matrix in the source program is linear, so it is applied once to
+//     the interpolated premultiplied value rather than to every tap.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "zh_prog.h"
+
+namespace zh {
+namespace spec {
+
+// Raw texel of format F as loaded (4, 8 or 16 bytes).
+template <uint32_t F> struct RawT { typedef uint32_t type; };
+template <> struct RawT<FMT_F16> { typedef u32x2 type; };
+template <> struct RawT<FMT_F32> { typedef u32x4 type; };
+
+template <uint32_t F>
+__device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const Luts* L) {
+  if constexpr (F == FMT_F16) {
+    return dec_f16(r.x, r.y);
+  } else if constexpr (F == FMT_F32) {
+    return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w));
+  } else {
+    return dec_u8<F>(r, L);
+  }
+}
+
+__device__ __forceinline__ float4 premul4(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
+
+// Uniform data of one layer, read per tile.  Both sources of WSRC_OVER2 have
+// the same row stride (the planner checks), so one offset serves both.
+struct LayerArgs {
+  const gbyte* src0;
+  const gbyte* src1;  // WSRC_OVER2: the upper source
+  uint32_t stride;
+  int sw, sh;
+  float m[9];  // WSRC_ONE with mat3
+};
+
+template <uint32_t SRC, uint32_t F, bool MAT>
+__device__ __forceinline__ LayerArgs layer_args(CDesc d, const Layer& L) {
+  LayerArgs a;
+  const uint32_t pb = L.prog_begin;
+  const uint32_t s0 = d->code[pb].a;
+  a.src0 = (const gbyte*)d->slot[s0].ptr;
+  a.stride = (uint32_t)d->slot[s0].row_stride;
+  a.src1 = a.src0;
+  if constexpr (SRC == WSRC_OVER2) a.src1 = (const gbyte*)d->slot[d->code[pb + 2].a].ptr;
+  if constexpr (MAT) {
+    const uint32_t q = d->code[pb + 1].arg;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.m[k] = d->params[q + k];
+  }
+  a.sw = (int)L.src_w;
+  a.sh = (int)L.src_h;
+  return a;
+}
+
+// Does any bilinear tap of the tile's pixel centres land in the source?
+// (Affine maps: the corners bound the footprint.)
+__device__ __forceinline__ bool layer_hits(const Layer& L, double ex0, double ey0, double ex1, double ey1) {
+  const double* m = L.inv;
+  const double u00 = m[0] * ex0 + m[1] * ey0 + m[2], u10 = m[0] * ex1 + m[1] * ey0 + m[2];
+  const double u01 = m[0] * ex0 + m[1] * ey1 + m[2], u11 = m[0] * ex1 + m[1] * ey1 + m[2];
+  const double v00 = m[3] * ex0 + m[4] * ey0 + m[5], v10 = m[3] * ex1 + m[4] * ey0 + m[5];
+  const double v01 = m[3] * ex0 + m[4] * ey1 + m[5], v11 = m[3] * ex1 + m[4] * ey1 + m[5];
+  const double umin = fmin(fmin(u00, u10), fmin(u01, u11)) - 0.5, umax = fmax(fmax(u00, u10), fmax(u01, u11)) - 0.5;
+  const double vmin = fmin(fmin(v00, v10), fmin(v01, v11)) - 0.5, vmax = fmax(fmax(v00, v10), fmax(v01, v11)) - 0.5;
+  return umax > -1.0 && vmax > -1.0 && umin < (double)L.src_w && vmin < (double)L.src_h;
+}
+
+// Warp variant: source form, source format, mat3 flag and post program.
+template <uint32_t SRC_, uint32_t F_, bool MAT_, class Post_>
+struct Warp {
+  static constexpr uint32_t SRC = SRC_, F = F_;
+  static constexpr bool MAT = MAT_;
+  typedef Post_ Post;
+};
+
+// One output tile (tx, ty) of stage d.  The tile is tile_w x tile_h pixels;
+// lane (lx, ly) owns pixels x = tx * tile_w + lx + k * lanes_x (k < kPix) of
+// row ty * tile_h + ly, so every load and store instruction covers runs of
+// lanes_x adjacent pixels.
+template <class V>
+__device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, const Luts* L) {
+  typedef typename V::Post Post;
+  typedef typename RawT<V::F>::type R;
+  constexpr uint32_t B = fmt_bpp(V::F);
+  const int TW = (int)d->tile_w, TH = (int)d->tile_h;
+  const int lanes_x = TW / kPix;
+  const int lx = (int)threadIdx.x % lanes_x, ly = (int)threadIdx.x / lanes_x;
+  const int W = (int)d->width, H = (int)d->height;
+  const int x0 = (int)tx * TW + lx, y = (int)ty * TH + ly;
+  const int n = (y < H && x0 < W) ? min(kPix, (W - x0 + lanes_x - 1) / lanes_x) : 0;
+
+  // Base: a premultiplied solid colour (transparent when the program is empty).
+  float4 base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (d->base_end > d->base_begin) {
+    const uint32_t q = d->code[d->base_begin].arg;
+    base = premul4(make_float4(d->params[q], d->params[q + 1], d->params[q + 2], d->params[q + 3]));
+  }
+  Px X;
+#pragma unroll
+  for (int k = 0; k < kPix; ++k) X[k] = base;
+
+  // Pixel-centre extent of the tile (clipped to the image).
+  const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
+  const double ex1 = min((int)tx * TW + TW, W) - 0.5, ey1 = min((int)ty * TH + TH, H) - 0.5;
+  const uint32_t nl = d->n_layers;
+  for (uint32_t l = 0; l < nl; ++l) {
+    const Layer Ly = ldc(&d->layer[l]);
+    if (!layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;  // uniform
+    const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
+    // Map of the tile's first pixel centre: integer texel + FP32 remainder.
+    const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
+    const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
+    const double fuo = floor(uo), fvo = floor(vo);
+    const int iu0 = (int)fuo, iv0 = (int)fvo;
+    const float m0 = (float)Ly.inv[0], m1 = (float)Ly.inv[1], m3 = (float)Ly.inv[3], m4 = (float)Ly.inv[4];
+    const float lu = (float)(uo - fuo) + m0 * (float)lx + m1 * (float)ly - 0.5f;
+    const float lv = (float)(vo - fvo) + m3 * (float)lx + m4 * (float)ly - 0.5f;
+    const float su = m0 * (float)lanes_x, sv = m3 * (float)lanes_x;
+    // Taps of all kPix pixels first (clamped in-bounds offsets, weight 0 for
+    // taps outside the source or missing pixels), then every load, then the
+    // arithmetic: 4 * kPix loads in flight per lane.
+    uint32_t off[kPix][4];
+    float w[kPix][4];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < kPix; ++k) {
+      const float u = lu + su * (float)k, v = lv + sv * (float)k;
+      const float fu = floorf(u), fv = floorf(v);
+      const int i = iu0 + (int)fu, j = iv0 + (int)fv;
+      const float ax = u - fu, ay = v - fv;
+      const bool in = k < n;
+      const bool xa = in && (unsigned)i < (unsigned)a.sw, xb = in && (unsigned)(i + 1) < (unsigned)a.sw;
+      const bool ya = (unsigned)j < (unsigned)a.sh, yb = (unsigned)(j + 1) < (unsigned)a.sh;
+      const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
+      const float wy0 = ya ? 1.0f - ay : 0.0f, wy1 = yb ? ay : 0.0f;
+      w[k][0] = wx0 * wy0;
+      w[k][1] = wx1 * wy0;
+      w[k][2] = wx0 * wy1;
+      w[k][3] = wx1 * wy1;
+      any = any || ((xa || xb) && (ya || yb));
+      const uint32_t i0 = (uint32_t)min(max(i, 0), a.sw - 1), i1 = (uint32_t)min(max(i + 1, 0), a.sw - 1);
+      const uint32_t r0 = (uint32_t)min(max(j, 0), a.sh - 1) * a.stride, r1 = (uint32_t)min(max(j + 1, 0), a.sh - 1) * a.stride;
+      off[k][0] = r0 + i0 * B;
+      off[k][1] = r0 + i1 * B;
+      off[k][2] = r1 + i0 * B;
+      off[k][3] = r1 + i1 * B;
+    }
+    if (!any) continue;
+    R t0[kPix][4], t1[kPix][4];
+#pragma unroll
+    for (int k = 0; k < kPix; ++k)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        t0[k][t] = gld<R>(a.src0 + off[k][t]);
+        if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = gld<R>(a.src1 + off[k][t]);
+      }
+#pragma unroll
+    for (int k = 0; k < kPix; ++k) {
+      float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        float4 c = premul4(raw_dec<V::F>(t0[k][t], L));
+        if constexpr (V::SRC == WSRC_OVER2) {
+          const float4 h = premul4(raw_dec<V::F>(t1[k][t], L));
+          const float q = 1.0f - h.w;
+          c = make_float4(fmaf(c.x, q, h.x), fmaf(c.y, q, h.y), fmaf(c.z, q, h.z), fmaf(c.w, q, h.w));
+        }
+        const float wt = w[k][t];
+        s = make_float4(fmaf(c.x, wt, s.x), fmaf(c.y, wt, s.y), fmaf(c.z, wt, s.z), fmaf(c.w, wt, s.w));
+      }
+      if constexpr (V::MAT) {
+        const float r = a.m[0] * s.x + a.m[1] * s.y + a.m[2] * s.z;
+        const float g = a.m[3] * s.x + a.m[4] * s.y + a.m[5] * s.z;
+        const float b = a.m[6] * s.x + a.m[7] * s.y + a.m[8] * s.z;
+        s = make_float4(r, g, b, s.w);
+      }
+      const float kk = 1.0f - s.w;
+      X[k] = make_float4(fmaf(X[k].x, kk, s.x), fmaf(X[k].y, kk, s.y), fmaf(X[k].z, kk, s.z), fmaf(X[k].w, kk, s.w));
+    }
+  }
+
+  if (n == 0) return;
+  // Post program on the composite; its last element stores the lane's pixels.
+  Uni<Post> u;
+  resolve<Post>(d, u, d->post_begin);
+  const uint32_t so = d->code[d->post_end - 1].a;
+  const uint64_t stride_tx = d->slot[so].row_stride / fmt_bpp(el_fmt(Post::el[Post::n - 1]));
+  const uint64_t xi = (uint64_t)y * stride_tx + (uint64_t)x0;
+  Px Y, Z;
+  exec<Post, false>(u, xi, n, nullptr, X, Y, Z, L, lanes_x);
+}
+
+// Tile t of a stage with tiles_x tiles per row; tiles are spread over the
+// grid so that each XCD (blocks are dealt to the 8 XCDs round-robin) works on
+// a contiguous run of tiles per round and shares their halo texels in its L2.
+__device__ __forceinline__ uint32_t virtual_block() {
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  if (g % 8u != 0u) return b;
+  return (b % 8u) * (g / 8u) + b / 8u;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_warp(const StageDesc* dd) {
+  const CDesc d = as_const(dd);
+  __shared__ Luts luts;
+  load_lut(d, &luts);
+  __syncthreads();
+  const uint32_t tiles = d->tiles_x * d->tiles_y;
+  for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x) warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts);
+}
+
+// ---- registry ------------------------------------------------------------------
+#define ZH_WARP_TYPE(name, src, f, mat, ...)     \
+  typedef Warp<src, f, mat != 0, Prog<__VA_ARGS__>> V_##name; \
+  static const uint16_t WP_##name[] = {__VA_ARGS__};
+ZH_WARP_LIST(ZH_WARP_TYPE)
+#undef ZH_WARP_TYPE
+
+enum {
+#define ZH_WARP_ENUM(name, ...) WI_##name,
+  ZH_WARP_LIST(ZH_WARP_ENUM)
+#undef ZH_WARP_ENUM
+  kNumWarps
+};
+
+// Every stage of a batch, whichever registered form it has: the form is
+// uniform per tile, so the switch costs one scalar branch per tile.
+__device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint32_t ty, const Luts* L) {
+  switch (v) {
+#define ZH_WARP_CASE(name, ...) \
+  case WI_##name: warp_tile<V_##name>(d, tx, ty, L); break;
+    ZH_WARP_LIST(ZH_WARP_CASE)
+#undef ZH_WARP_CASE
+    default: break;
+  }
+}
+
+// Persistent grid over the tiles of a batch of stages (images of uneven sizes
+// and formats): tile j belongs to the last stage s with prefix[s] <= j.
+__global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
+  const CBatch b = (CBatch)(uintptr_t)bb;
+  __shared__ Luts luts;
+  load_lut(as_const(b->stages), &luts);
+  __syncthreads();
+  const uint32_t total = b->total_tiles;
+  for (uint32_t j = virtual_block(); j < total; j += gridDim.x) {
+    uint32_t lo = 0, hi = b->n_stages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
+    }
+    const CDesc d = as_const(b->stages + lo);
+    const uint32_t t = j - b->tile_prefix[lo];
+    warp_dispatch(d->variant, d, t % d->tiles_x, t / d->tiles_x, &luts);
+  }
+}
+
+struct WarpEntry {
+  const char* name;
+  uint32_t src, fmt, mat;
+  const uint16_t* post;
+  int n;
+  void (*fn)(const StageDesc*);
+};
+
+#define ZH_WARP_ENTRY(name, src, f, mat, ...) \
+  {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>},
+static const WarpEntry kWarps[] = {ZH_WARP_LIST(ZH_WARP_ENTRY)};
+#undef ZH_WARP_ENTRY
+
+}  // namespace spec
+}  // namespace zh
+
+extern "C" {
+
+int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n) {
+  for (int i = 0; i < zh::spec::kNumWarps; ++i) {
+    const auto& e = zh::spec::kWarps[i];
+    if (e.src == src && e.fmt == fmt && e.mat == mat && e.n == n &&
+        std::memcmp(e.post, post, sizeof(uint16_t) * (size_t)n) == 0)
+      return i;
+  }
+  return -1;
+}
+
+const char* zh_warp_name(int index) {
+  return index >= 0 && index < zh::spec::kNumWarps ? zh::spec::kWarps[index].name : "";
+}
+
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t max_blocks,
+                          hipStream_t stream) {
+  if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
+  const uint32_t blocks = std::min(tiles, max_blocks);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(zh::spec::kWarps[index].fn, dim3(blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  return hipGetLastError();
+}
+
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t max_blocks,
+                                hipStream_t stream) {
+  const uint32_t blocks = std::min(total_tiles, max_blocks);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(zh::spec::k_warp_batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
+  return hipGetLastError();
+}
+
+}  // extern "C"
