@@ -239,6 +239,58 @@ zh_status zh_program_time(zh_program* prog, const zh_binding* bindings, uint32_t
                           uint32_t iterations, float* stage_ms, uint32_t max_stages,
                           uint32_t* num_stages);
 
+/* ---- learned colour surrogate ---------------------------------------------
+ * No counterpart in the reference (it has no learned colour path).  A
+ * pointwise stage that maps RGBA8 sRGB to RGBA8 sRGB through colour-only
+ * micro-ops (3x3 matrices, cube roots, cubes, clamps) is an R^3 -> R^3
+ * function of the encoded colour; zh_program_learn fits a 3-64-64-3 SiLU
+ * network to it (bfloat16 weights, FP32 accumulation on the matrix cores):
+ * training samples come from the stage's own kernel, training runs on the
+ * GPU, and the network replaces the stage only if its max Oklab Delta-E
+ * against the stage's program evaluated in FP64 on a lattice**3 grid of the
+ * input cube stays below max_delta_e.  Otherwise the plain kernel keeps
+ * running.  Alpha passes through unchanged either way. */
+typedef struct zh_learn_options {
+  uint32_t uniform_points; /* 0: 2^20 samples uniform in the cube */
+  uint32_t face_points;    /* 0: 2^16 samples on the cube's faces */
+  uint32_t steps;          /* 0: 20000 Adam steps */
+  uint32_t batch;          /* 0: 8192 samples per step */
+  float lr;                /* 0: 3e-3, cosine decay to lr / 100 */
+  uint32_t seed;
+  uint32_t lattice;        /* 0: 65 points per axis */
+  float max_delta_e;       /* 0: 0.005 */
+} zh_learn_options;
+
+typedef struct zh_learn_report {
+  uint32_t stage;             /* stage index (zh_program_describe) */
+  uint32_t accepted;          /* 1: the surrogate runs instead of the stage's kernel */
+  uint32_t steps, samples;
+  float initial_loss;         /* MSE of the first step's batch */
+  float final_loss;           /* mean MSE over the last 100 steps */
+  double max_delta_e;         /* trained network on the lattice */
+  double mean_delta_e;
+  double max_delta_e_initial; /* the untrained network on the same lattice */
+  float train_ms;             /* sample generation + training, wall clock */
+} zh_learn_report;
+
+/* Learn a surrogate for every eligible stage; fills up to max_reports
+ * reports (num_reports = eligible stages).  Needs a context. */
+zh_status zh_program_learn(zh_program* prog, const zh_learn_options* options, zh_learn_report* reports,
+                           uint32_t max_reports, uint32_t* num_reports);
+/* 1 (default): accepted surrogates replace their stages; 0: plain kernels;
+ * 2: every fitted surrogate runs, accepted or not (for measurement). */
+zh_status zh_program_use_surrogates(zh_program* prog, int32_t enable);
+/* Master (FP32) weights of a stage's surrogate, 4,611 floats in the order
+ * W1[3][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
+zh_status zh_program_surrogate_params(zh_program* prog, uint32_t stage, float* params, uint32_t len);
+/* Loss (mean squared error over n x 3 values) and its gradient for the given
+ * parameters on n samples x[n][3] -> t[n][3], by the training kernels. */
+zh_status zh_mlp_gradient(zh_context* ctx, const float* params, uint32_t n, const float* x, const float* t,
+                          float* grad, float* loss);
+/* The network on n inputs x[n][3] -> y[n][3] by the matrix-core inference
+ * kernel. */
+zh_status zh_mlp_evaluate(zh_context* ctx, const float* params, uint32_t n, const float* x, float* y);
+
 /* Host-only: the 8-bit sRGB tables the kernels stage in LDS (decode values,
  * bucket codes and code thresholds; 6,160 bytes), for inspection and tests.
  * No counterpart in the reference, whose sRGB codec is the texture unit's. */

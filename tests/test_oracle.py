@@ -197,3 +197,52 @@ def test_crop_stretches_selection():
     # every 2x2 block of the output shows one texel of the 4x4 selection
     assert np.array_equal(out[0, 0], src[2, 2]) and np.array_equal(out[7, 7], src[5, 5])
     assert np.array_equal(out[1, 1], src[2, 2]) and np.array_equal(out[2, 2], src[3, 3])
+
+
+# ---- the learned colour surrogate's oracle (oracle/mlp.py) ------------------------
+def test_mlp_gradient_matches_central_differences():
+    """Reverse-mode gradient of the FP64 network against central
+    differences on every parameter of a small random problem."""
+    from oracle import mlp as M
+    rng = np.random.default_rng(1)
+    p = M.init_params(3) + rng.normal(0, 0.05, M.N_PARAMS)
+    x, t = rng.random((7, 3)), rng.random((7, 3))
+    _, g = M.loss_and_grad(p, x, t, quant=False)
+    h = 1e-6
+    fd = np.empty_like(p)
+    for i in range(M.N_PARAMS):
+        e = np.zeros_like(p)
+        e[i] = h
+        fd[i] = (M.loss_and_grad(p + e, x, t, quant=False)[0] - M.loss_and_grad(p - e, x, t, quant=False)[0]) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 1e-7 * max(1.0, np.max(np.abs(g)))
+
+
+def test_mlp_bf16_rounding_and_silu():
+    from oracle import mlp as M
+    # ties to even at 1 + 2^-8 (halfway between 1 and 1 + 2^-7)
+    assert M.bf16(1.0 + 2 ** -8) == 1.0
+    assert M.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
+    assert M.bf16(-2.0 ** -3 * (1 + 2 ** -9)) == -2.0 ** -3
+    assert M.bf16(3.140625) == 3.140625  # representable
+    a = np.linspace(-8, 8, 101)
+    assert np.allclose(M.silu(a), a / (1 + np.exp(-a)), rtol=1e-14, atol=1e-15)
+    assert np.allclose(M.silu_grad(a), np.gradient(M.silu(a), a), atol=2e-2)
+    assert M.silu(0.0) == 0.0 and M.silu_grad(0.0) == 0.5
+
+
+def test_mlp_forward_layout_and_adam_first_step():
+    """A network whose only path is x -> 2x: W1 picks x into feature 0, W2
+    copies it, W3 reads it back; the first Adam step moves every parameter
+    with a non-zero gradient by -lr * sign(g)."""
+    from oracle import mlp as M
+    w1 = np.zeros((3, 64)); w1[0, 0] = 1.0
+    w2 = np.zeros((64, 64)); w2[0, 0] = 1.0
+    w3 = np.zeros((64, 3)); w3[0, 2] = 2.0
+    p = M.pack(w1, np.zeros(64), w2, np.zeros(64), w3, np.zeros(3))
+    x = np.array([[0.5, 0.1, 0.2]])
+    y, _ = M.forward(p, x, quant=False)
+    assert np.isclose(y[0, 2], 2 * M.silu(M.silu(0.5))) and y[0, 0] == 0 and y[0, 1] == 0
+    _, g = M.loss_and_grad(p, x, np.zeros((1, 3)), quant=False)
+    q, m, v = M.adam_step(p, g, np.zeros_like(p), np.zeros_like(p), 1, 0.01)
+    nz = np.abs(g) > 1e-12
+    assert np.allclose((q - p)[nz], -0.01 * np.sign(g[nz]), rtol=1e-6)

@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes
 import math
 from dataclasses import dataclass, field
-from typing import Dict, Iterable, Optional, Sequence, Tuple
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -388,6 +388,28 @@ class Executable:
         _check(self._lib.zh_program_time(self._h, arr, n, iterations, out, 64, ctypes.byref(cnt)))
         return [out[i] for i in range(cnt.value)]
 
+    def learn(self, **options) -> List[dict]:
+        """Fit the learned colour surrogate of every eligible stage
+        (``zh_program_learn``); returns one report per stage."""
+        o = _abi.LearnOptionsC()
+        for k, v in options.items():
+            setattr(o, k, v)
+        reps = (_abi.LearnReportC * 16)()
+        cnt = ctypes.c_uint32()
+        _check(self._lib.zh_program_learn(self._h, ctypes.byref(o), reps, 16, ctypes.byref(cnt)))
+        return [{f: getattr(reps[i], f) for f, _ in _abi.LearnReportC._fields_} for i in range(min(cnt.value, 16))]
+
+    def use_surrogates(self, mode: int = 1):
+        """0: plain kernels; 1: accepted surrogates; 2: every fitted surrogate."""
+        _check(self._lib.zh_program_use_surrogates(self._h, int(mode)))
+
+    def surrogate_params(self, stage: int):
+        import numpy as np
+        out = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(self._lib.zh_program_surrogate_params(self._h, stage, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                     MLP_PARAMS))
+        return out
+
 
 class Context:
     """One HIP device and stream (the analogue of a ``Pool`` device)."""
@@ -468,6 +490,36 @@ class Image:
         _check(_abi.lib().zh_image_download(self._h, out.ctypes.data, row))
         self.ctx.sync()
         return out
+
+
+MLP_PARAMS = 4611
+
+
+def _f32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_gradient(ctx: Context, params, x, t):
+    """(loss, gradient) of the surrogate's training kernels on samples
+    x[n][3] -> t[n][3] (``zh_mlp_gradient``)."""
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _check(_abi.lib().zh_mlp_gradient(ctx._h, _f32p(p), x.shape[0], _f32p(x), _f32p(t), _f32p(g), _f32p(loss)))
+    return float(loss[0]), g
+
+
+def mlp_evaluate(ctx: Context, params, x):
+    """The surrogate's matrix-core inference on x[n][3] (``zh_mlp_evaluate``)."""
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x)
+    _check(_abi.lib().zh_mlp_evaluate(ctx._h, _f32p(p), x.shape[0], _f32p(x), _f32p(y)))
+    return y
 
 
 def version() -> str:

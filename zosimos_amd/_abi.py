@@ -42,6 +42,17 @@ class BindingC(ctypes.Structure):
     _fields_ = [("slot", u32), ("is_output", u32), ("image", vp)]
 
 
+class LearnOptionsC(ctypes.Structure):
+    _fields_ = [("uniform_points", u32), ("face_points", u32), ("steps", u32), ("batch", u32), ("lr", f32),
+                ("seed", u32), ("lattice", u32), ("max_delta_e", f32)]
+
+
+class LearnReportC(ctypes.Structure):
+    _fields_ = [("stage", u32), ("accepted", u32), ("steps", u32), ("samples", u32), ("initial_loss", f32),
+                ("final_loss", f32), ("max_delta_e", ctypes.c_double), ("mean_delta_e", ctypes.c_double),
+                ("max_delta_e_initial", ctypes.c_double), ("train_ms", f32)]
+
+
 # name: (restype, argtypes)
 _SIGNATURES = {
     # zosimos_hip.h
@@ -65,6 +76,13 @@ _SIGNATURES = {
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
     "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
                               ctypes.POINTER(u32)]),
+    "zh_program_learn": (i32, [vp, ctypes.POINTER(LearnOptionsC), ctypes.POINTER(LearnReportC), u32,
+                               ctypes.POINTER(u32)]),
+    "zh_program_use_surrogates": (i32, [vp, i32]),
+    "zh_program_surrogate_params": (i32, [vp, u32, ctypes.POINTER(f32), u32]),
+    "zh_mlp_gradient": (i32, [vp, ctypes.POINTER(f32), u32, ctypes.POINTER(f32), ctypes.POINTER(f32),
+                              ctypes.POINTER(f32), ctypes.POINTER(f32)]),
+    "zh_mlp_evaluate": (i32, [vp, ctypes.POINTER(f32), u32, ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "zh_srgb_tables": (i32, [vp, sz]),
     "zh_last_error": (ctypes.c_char_p, []),
     "zh_version": (ctypes.c_char_p, []),

@@ -107,6 +107,10 @@ void quad_of_rect(zh_rect r, float q[8]) {
 
 }  // namespace
 
+bool zh::primaries_to_xyz(uint32_t primaries, uint32_t whitepoint, Mat3* out) {
+  return rgb_to_xyz(primaries, whitepoint, out);
+}
+
 struct zs_command {
   std::vector<CmdOp> ops;
   int32_t next_knob = 0;

@@ -15,6 +15,7 @@ namespace zh {
 
 zh_status fail(zh_status code, const std::string& msg);
 
+
 // ---- texel format helpers ------------------------------------------------
 uint32_t bits_bytes(uint32_t bits);  // SampleBits::bytes for a stage.frag id, 0 if unknown
 // RGBA8/BGRA8 with sRGB or linear transfer on an RGB colour: the formats the
@@ -33,6 +34,8 @@ using Mat3 = std::array<double, 9>;  // row-major
 Mat3 mat_mul(const Mat3& a, const Mat3& b);
 bool mat_inv(const Mat3& a, Mat3* out);
 Mat3 mat_identity();
+// RGB -> XYZ matrix of ZH_PRIMARIES_* under a ZH_WHITEPOINT_* (zh_command.cpp).
+bool primaries_to_xyz(uint32_t primaries, uint32_t whitepoint, Mat3* out);
 Mat3 mat_from_std140(const void* blob);  // 12 floats, column-major with padding
 
 // ---- planned program -------------------------------------------------------
@@ -87,3 +90,46 @@ struct zh_image {
   size_t bytes = 0;
   bool owned = false;
 };
+
+// ---- programs (zh_runtime.cpp, zh_surrogate.cpp) --------------------------
+struct Unit {
+  bool batch = false;
+  bool warp = false;             // batch of warp-form stages (k_warp_batch)
+  size_t first = 0, count = 1;   // stage range
+  zh::BatchDesc* d_batch = nullptr;  // device
+  uint32_t* d_counter = nullptr;
+  uint32_t* d_prefix = nullptr;
+  uint32_t grid = 0, lds = 0;
+  uint32_t total_tiles = 0;
+};
+
+struct zh_program {
+  zh_context* ctx = nullptr;
+  uint32_t flags = 0;
+  std::vector<zh_descriptor> textures;
+  std::vector<zh_op> ops;
+  std::vector<std::vector<uint8_t>> params;
+  zh::Plan plan;
+  std::vector<Unit> units;
+  zh::StageDesc* d_stages = nullptr;       // device copies, one per stage
+  zh::StageDesc* h_stages = nullptr;       // pinned host staging
+  std::vector<void*> d_intermediates;
+  std::vector<void*> bound_inputs, bound_outputs;
+  bool uploaded = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_failed = false;
+  uint32_t max_lds = 0;
+  // Kernel per stage for the bound images: 0 general interpreter, 1
+  // interpreter with in-line codecs, 2 specialised pointwise, 3 warp, 5 the
+  // learned surrogate (an image the faster kernel cannot address forces 0).
+  std::vector<int> variant;
+  // Learned colour surrogates per stage (zh_surrogate.cpp): packed matrix-core
+  // operands (nullptr = none), the lattice judge's verdict, master weights.
+  std::vector<uint16_t*> d_surrogate;
+  std::vector<uint8_t> surrogate_ok;
+  std::vector<std::vector<float>> surrogate_params;
+  int use_surrogates = 1;  // 0 plain kernels, 1 accepted surrogates, 2 every fitted surrogate
+};
+
+

@@ -33,6 +33,7 @@ hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32
 hipError_t zh_launch_copy(const zh::StageDesc* d_dev, uint64_t row_bytes, uint32_t rows, hipStream_t stream);
 hipError_t zh_set_resample_lds(uint32_t bytes);
 }
+#include "zh_mlp.h"
 
 namespace zh {
 
@@ -89,40 +90,6 @@ using namespace zh;
 
 // One launch unit: a single stage, or a batch of independent staged
 // resample stages run by one persistent tile-queue kernel.
-struct Unit {
-  bool batch = false;
-  bool warp = false;             // batch of warp-form stages (k_warp_batch)
-  size_t first = 0, count = 1;   // stage range
-  BatchDesc* d_batch = nullptr;  // device
-  uint32_t* d_counter = nullptr;
-  uint32_t* d_prefix = nullptr;
-  uint32_t grid = 0, lds = 0;
-  uint32_t total_tiles = 0;
-};
-
-struct zh_program {
-  zh_context* ctx = nullptr;
-  uint32_t flags = 0;
-  std::vector<zh_descriptor> textures;
-  std::vector<zh_op> ops;
-  std::vector<std::vector<uint8_t>> params;
-  Plan plan;
-  std::vector<Unit> units;
-  StageDesc* d_stages = nullptr;       // device copies, one per stage
-  StageDesc* h_stages = nullptr;       // pinned host staging
-  std::vector<void*> d_intermediates;
-  std::vector<void*> bound_inputs, bound_outputs;
-  bool uploaded = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  bool graph_failed = false;
-  uint32_t max_lds = 0;
-  // Pointwise kernel per stage for the bound images: 2 specialised, 1
-  // interpreter with in-line codecs, 0 general interpreter (an unaligned
-  // image forces 0).
-  std::vector<int> variant;
-};
-
 extern "C" {
 
 const char* zh_last_error(void) { return g_error.c_str(); }
@@ -305,7 +272,16 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 }
 
 // ---- programs -----------------------------------------------------------------
+static void drop_surrogates(zh_program* p) {
+  for (uint16_t*& b : p->d_surrogate)
+    if (b) { hipFree(b); b = nullptr; }
+  p->d_surrogate.clear();
+  p->surrogate_ok.clear();
+  p->surrogate_params.clear();
+}
+
 static void free_device(zh_program* p) {
+  drop_surrogates(p);
   if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
   if (p->graph) hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;
@@ -519,6 +495,9 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
            a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat && a.desc.variant == b.desc.variant;
   }
   p->params = params;
+  // A surrogate was fitted to the old parameters.
+  if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+  drop_surrogates(p);
   if (same) {
     for (size_t i = 0; i < np.stages.size(); ++i) np.stages[i].lds_bytes = p->plan.stages[i].lds_bytes;
     p->plan = std::move(np);
@@ -616,6 +595,17 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
           ok = ok && d.slot[k].ptr % d.slot[k].bpp == 0 && d.slot[k].row_stride % d.slot[k].bpp == 0;
         variant[s] = ok ? 3 : 0;
       }
+      if (d.kind == STAGE_POINTWISE && p->use_surrogates && s < p->d_surrogate.size() && p->d_surrogate[s] &&
+          (p->surrogate_ok[s] || p->use_surrogates == 2)) {
+        // The surrogate reads and writes whole RGBA8 texels.
+        const Slot& si = d.slot[d.code[0].a];
+        const Slot& so = d.slot[d.code[d.n_instr - 1].a];
+        if (si.ptr % 4 == 0 && so.ptr % 4 == 0) {
+          variant[s] = 5;
+          p->h_stages[s] = d;
+          continue;
+        }
+      }
       if (d.kind == STAGE_POINTWISE) {
         bool aligned = true;
         for (size_t k = 0; k < refs.size(); ++k) aligned = aligned && d.slot[k].vec_ok;
@@ -670,7 +660,10 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
   switch (sp.desc.kind) {
     case STAGE_POINTWISE: {
       const int v = u.first < p->variant.size() ? p->variant[u.first] : 0;
-      if (v == 2)
+      if (v == 5)
+        ZH_HIP(zh_mlp_infer_stage(p->d_surrogate[u.first], d, (uint64_t)sp.desc.width * sp.desc.height,
+                                  (uint32_t)p->ctx->cus * 8, s));
+      else if (v == 2)
         ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus * 8, s));
       else
         ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, v, (uint32_t)p->ctx->cus * 8,
