@@ -172,6 +172,9 @@ def main():
     exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
     launches, alg_bytes = exe.stats()
     imgs, outs, keep = device_inputs(wl, ctx, torch)
+    # Tile shapes of the warp kernels, timed on these images (a no-op for
+    # programs without them).
+    exe.autotune(imgs, outs, iterations=3)
 
     for _ in range(args.warmup):
         exe.launch(imgs, outs)

@@ -239,6 +239,15 @@ zh_status zh_program_time(zh_program* prog, const zh_binding* bindings, uint32_t
                           uint32_t iterations, float* stage_ms, uint32_t max_stages,
                           uint32_t* num_stages);
 
+/* Autotuner (no counterpart in the reference, whose tiles are the
+ * rasteriser's): for every warp-kernel launch of the program (affine layer
+ * stacks and their batches), time each tile shape the kernel accepts
+ * (64x16, 32x32, 128x8, 16x64 pixels) on the bound images over `iterations`
+ * launches and keep the fastest.  The choice persists in the program until a
+ * knob re-plans it; zh_program_describe shows the tiles. */
+zh_status zh_program_autotune(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings,
+                              uint32_t iterations);
+
 /* ---- learned colour surrogate ---------------------------------------------
  * No counterpart in the reference (it has no learned colour path).  A
  * pointwise stage that maps RGBA8 sRGB to RGBA8 sRGB through colour-only

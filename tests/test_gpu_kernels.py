@@ -273,3 +273,47 @@ def test_resample_interpreter_fallback(ctx, scale, mode):
     Bv = C.premultiply(C.decode_u8(PL.u8_image(hb, w, h), 4))
     lin = C.unpremultiply(C.bilinear_affine(C.over(Bv, A), PL.mat_f32(aff.transformation), (h, w)))
     check(f"fallback_{mode}", got, lin, "l8", w, h, lin[..., 3] > 0.5 / 255.0)
+
+
+def _bind_and_run(ctx, wl, host, exe):
+    imgs = {}
+    for r, d in wl.inputs.items():
+        imgs[r] = ctx.image(d)
+        imgs[r].upload(host[r])
+    outs = {r: ctx.image(d) for r, d in wl.outputs.items()}
+    return imgs, outs
+
+
+def test_autotune_keeps_parity(ctx):
+    """The autotuner re-tiles warp kernels (single stages and a batch); the
+    tuned programs still match the FP64 oracle and replay their graph."""
+    import re
+    wl = W.c3(n=6, seed=12, max_side=420)
+    host = wl.make_inputs()
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    imgs, outs = _bind_and_run(ctx, wl, host, exe)
+    exe.autotune(imgs, outs, iterations=2)
+    tiles = set(re.findall(r"tile (\d+x\d+)", exe.describe()))
+    assert tiles and tiles <= {"64x16", "32x32", "128x8", "16x64"}, tiles
+    for _ in range(2):
+        exe.launch(imgs, outs)
+    ctx.sync()
+    for k, ((ri, din), (ro, dout)) in enumerate(zip(wl.inputs.items(), wl.outputs.items())):
+        w, h, fmt, s = wl.meta["shapes"][k]
+        lin, want = PL.c3_image(host[ri], fmt, w, h, dout.width, dout.height)
+        got = outs[ro].download().reshape(-1)
+        mask = want[..., 3] > (0.5 / 255.0 if fmt == 0 else 1e-3)
+        check(f"tuned_c3_img{k}", got, lin, "s8" if fmt == 0 else "h", dout.width, dout.height,
+              mask)
+
+    wl = W.c5(layers=4, width=200, height=120, seed=13)
+    host = wl.make_inputs()
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    imgs, outs = _bind_and_run(ctx, wl, host, exe)
+    exe.autotune(imgs, outs, iterations=2)
+    exe.launch(imgs, outs)
+    ctx.sync()
+    (ro, dout), = wl.outputs.items()
+    layers = [PL.f16_image(host[r], 200, 120) for r in wl.inputs]
+    lin, _ = PL.c5(layers, wl.meta["matrices"], (120, 200))
+    check("tuned_c5", outs[ro].download().reshape(-1), lin, "s8", 200, 120, lin[..., 3] > 0.5 / 255.0)

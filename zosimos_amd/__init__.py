@@ -388,6 +388,12 @@ class Executable:
         _check(self._lib.zh_program_time(self._h, arr, n, iterations, out, 64, ctypes.byref(cnt)))
         return [out[i] for i in range(cnt.value)]
 
+    def autotune(self, inputs, outputs, iterations: int = 5):
+        """Time every tile shape of the program's warp kernels on these images
+        and keep the fastest (``zh_program_autotune``)."""
+        arr, n = self._bindings(inputs, outputs)
+        _check(self._lib.zh_program_autotune(self._h, arr, n, iterations))
+
     def learn(self, **options) -> List[dict]:
         """Fit the learned colour surrogate of every eligible stage
         (``zh_program_learn``); returns one report per stage."""

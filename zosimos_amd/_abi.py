@@ -76,6 +76,7 @@ _SIGNATURES = {
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
     "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
                               ctypes.POINTER(u32)]),
+    "zh_program_autotune": (i32, [vp, ctypes.POINTER(BindingC), u32, u32]),
     "zh_program_learn": (i32, [vp, ctypes.POINTER(LearnOptionsC), ctypes.POINTER(LearnReportC), u32,
                                ctypes.POINTER(u32)]),
     "zh_program_use_surrogates": (i32, [vp, i32]),

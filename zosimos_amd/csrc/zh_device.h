@@ -112,6 +112,31 @@ __device__ __forceinline__ float3 mat3(const __attribute__((address_space(4))) f
 // exp2(-inf) = 0).
 __device__ __forceinline__ float cbrt_s(float x) { return copysignf(pow_pos(fabsf(x), 1.0f / 3.0f), x); }
 
+// 8-bit sRGB decode table replicated once per LDS bank: entry e of copy c
+// sits at word e * 32 + c, so lane l reads copy l % 32 from its own bank and
+// a wave's data-dependent lookups never conflict (32 KiB).
+constexpr int kRepDec = 256 * 32;
+__device__ __forceinline__ void load_rep(const Luts* L, float* rep) {
+  for (int i = (int)threadIdx.x; i < kRepDec; i += kBlock) rep[i] = L->dec[i >> 5];
+}
+template <uint32_t F>
+__device__ __forceinline__ float4 dec_u8_rep(uint32_t w, const float* rep) {
+  constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
+  const uint32_t c = threadIdx.x & 31u;
+  const uint32_t b0 = w & 255u, b1 = (w >> 8) & 255u, b2 = (w >> 16) & 255u, b3 = w >> 24;
+  const uint32_t r = bgra ? b2 : b0, b = bgra ? b0 : b2;
+  return make_float4(rep[(r << 5) | c], rep[(b1 << 5) | c], rep[(b << 5) | c], (float)b3 * (1.0f / 255.0f));
+}
+
+// Workgroups of `kernel` (kBlock lanes) resident on a device of `cus` compute
+// units: the grid of a persistent kernel, which then has no second partial
+// round of workgroups.
+inline uint32_t resident_blocks(const void* kernel, uint32_t cus, size_t dyn_lds = 0) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kBlock, dyn_lds) != hipSuccess || n <= 0) n = 1;
+  return (uint32_t)n * cus;
+}
+
 // Copy the context's sRGB tables (zh_codec.h Luts) into LDS.
 __device__ __forceinline__ void load_lut(CDesc d, Luts* lut) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(d->srgb_lut);

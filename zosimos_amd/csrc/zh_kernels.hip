@@ -698,15 +698,19 @@ __global__ __launch_bounds__(kBlock) void k_copy(const StageDesc* dd) {
 extern "C" {
 
 hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat, int fast,
-                               uint32_t max_blocks, hipStream_t stream) {
+                               uint32_t cus, hipStream_t stream) {
   uint64_t groups;
   if (flat) {
     groups = ((uint64_t)width * height + zh::kPix - 1) / zh::kPix;
   } else {
     groups = (uint64_t)((width + zh::kPix - 1) / zh::kPix) * height;
   }
-  // Persistent grid: at most max_blocks workgroups stride over the groups.
-  const uint64_t blocks = std::min<uint64_t>((groups + zh::kBlock - 1) / zh::kBlock, max_blocks);
+  static uint32_t resident[2] = {};
+  const void* fn = fast ? (const void*)zh::k_pointwise<true> : (const void*)zh::k_pointwise<false>;
+  if (!resident[fast ? 1 : 0]) resident[fast ? 1 : 0] = zh::resident_blocks(fn, 1);
+  // Persistent grid: the resident workgroups stride over the groups.
+  const uint64_t blocks =
+      std::min<uint64_t>((groups + zh::kBlock - 1) / zh::kBlock, (uint64_t)resident[fast ? 1 : 0] * cus);
   if (blocks == 0) return hipSuccess;
   if (fast)
     hipLaunchKernelGGL(zh::k_pointwise<true>, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);

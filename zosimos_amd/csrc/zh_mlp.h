@@ -65,8 +65,8 @@ hipError_t zh_mlp_pack(const float* p, uint16_t* ops, hipStream_t s);
 // fmt_in/fmt_out: 1 = RGBA8 sRGB codes (encoded value = code / 255, alpha
 // passed through), 6 = RGBA32F (encoded values, alpha passed through).
 // Surrogate of a flat RGBA8 sRGB stage [load, colour ops, store] (n pixels).
-hipError_t zh_mlp_infer_stage(const uint16_t* ops, const zh::StageDesc* d_dev, uint64_t n, uint32_t max_blocks,
+hipError_t zh_mlp_infer_stage(const uint16_t* ops, const zh::StageDesc* d_dev, uint64_t n, uint32_t cus,
                               hipStream_t s);
 hipError_t zh_mlp_infer(const uint16_t* ops, const void* src, void* dst, uint64_t n, int fmt_in, int fmt_out,
-                        uint32_t max_blocks, hipStream_t s);
+                        uint32_t cus, hipStream_t s);
 }

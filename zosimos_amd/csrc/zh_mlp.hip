@@ -20,7 +20,7 @@
 #include <algorithm>
 
 #include "zh_mlp.h"
-#include "zh_stage.h"
+#include "zh_device.h"
 
 namespace zh {
 namespace mlp {
@@ -398,10 +398,12 @@ hipError_t zh_mlp_pack(const float* p, uint16_t* ops, hipStream_t s) {
 }
 
 hipError_t zh_mlp_infer(const uint16_t* ops, const void* src, void* dst, uint64_t n, int fmt_in, int fmt_out,
-                        uint32_t max_blocks, hipStream_t s) {
+                        uint32_t cus, hipStream_t s) {
   if (n == 0) return hipSuccess;
+  static uint32_t resident = 0;
+  if (!resident) resident = zh::resident_blocks((const void*)k_infer<1, 1>, 1);
   const uint64_t tiles = (n + 31) / 32;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, max_blocks);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)resident * cus);
   if (fmt_in == 1 && fmt_out == 1)
     hipLaunchKernelGGL((k_infer<1, 1>), dim3(blocks), dim3(kBlock), 0, s, ops, src, dst, n);
   else if (fmt_in == 6 && fmt_out == 6)
@@ -413,11 +415,13 @@ hipError_t zh_mlp_infer(const uint16_t* ops, const void* src, void* dst, uint64_
   return hipGetLastError();
 }
 
-hipError_t zh_mlp_infer_stage(const uint16_t* ops, const zh::StageDesc* d_dev, uint64_t n, uint32_t max_blocks,
+hipError_t zh_mlp_infer_stage(const uint16_t* ops, const zh::StageDesc* d_dev, uint64_t n, uint32_t cus,
                               hipStream_t s) {
   if (n == 0) return hipSuccess;
+  static uint32_t resident = 0;
+  if (!resident) resident = zh::resident_blocks((const void*)k_infer_stage, 1);
   const uint64_t tiles = (n + 31) / 32;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, max_blocks);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)resident * cus);
   hipLaunchKernelGGL(k_infer_stage, dim3(blocks), dim3(kBlock), 0, s, ops, d_dev);
   return hipGetLastError();
 }

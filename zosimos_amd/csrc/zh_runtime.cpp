@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -19,13 +20,11 @@
 
 extern "C" {
 hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat, int fast,
-                               uint32_t max_blocks, hipStream_t stream);
+                               uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
-                          uint32_t max_blocks, hipStream_t stream);
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t max_blocks,
-                          hipStream_t stream);
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t max_blocks,
-                                hipStream_t stream);
+                          uint32_t cus, hipStream_t stream);
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, hipStream_t stream);
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -434,7 +433,7 @@ zh_status zh_program_destroy(zh_program* p) {
 zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* needed) {
   if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
   std::ostringstream o;
-  o << p->plan.text;
+  for (size_t i = 0; i < p->plan.stages.size(); ++i) o << "stage " << i << ": " << p->plan.stages[i].text << "\n";
   for (size_t i = 0; i < p->units.size(); ++i)
     if (p->units[i].batch)
       o << "batch: stages " << p->units[i].first << ".." << p->units[i].first + p->units[i].count - 1
@@ -637,7 +636,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     bool all_warp = u.warp;
     for (size_t k = 0; k < u.count && all_warp; ++k) all_warp = p->variant[u.first + k] == 3;
     if (all_warp) {
-      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus * 8, s));
+      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, s));
       return ZH_OK;
     }
     bool all_staged = true;
@@ -662,17 +661,17 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
       const int v = u.first < p->variant.size() ? p->variant[u.first] : 0;
       if (v == 5)
         ZH_HIP(zh_mlp_infer_stage(p->d_surrogate[u.first], d, (uint64_t)sp.desc.width * sp.desc.height,
-                                  (uint32_t)p->ctx->cus * 8, s));
+                                  (uint32_t)p->ctx->cus, s));
       else if (v == 2)
-        ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus * 8, s));
+        ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus, s));
       else
-        ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, v, (uint32_t)p->ctx->cus * 8,
+        ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, v, (uint32_t)p->ctx->cus,
                                    s));
       break;
     }
     case STAGE_RESAMPLE:
       if (u.first < p->variant.size() && p->variant[u.first] == 3)
-        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus * 8, s));
+        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus, s));
       else
         ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
       break;
@@ -729,6 +728,109 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
     return ZH_OK;
   }
   return launch_all(p);
+}
+
+// ---- autotuner ---------------------------------------------------------------
+// Tile shapes the warp kernels accept (tile_w / kPix * tile_h == kBlock).
+static const uint32_t kWarpTiles[][2] = {{64, 16}, {32, 32}, {128, 8}, {16, 64}};
+
+static bool warp_unit(const zh_program* p, const Unit& u) {
+  for (size_t k = 0; k < u.count; ++k) {
+    const size_t s = u.first + k;
+    if (p->plan.stages[s].desc.kind != STAGE_RESAMPLE || s >= p->variant.size() || p->variant[s] != 3) return false;
+  }
+  return true;
+}
+
+// Give every stage of a warp unit the tile shape tw x th (descriptors, device
+// copies and, for a batch, the tile prefix).  The interpreter fallback of a
+// re-tiled stage gathers its taps directly (TILE_DIRECT accepts any tile).
+static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th) {
+  std::vector<uint32_t> prefix(u.count + 1, 0);
+  for (size_t k = 0; k < u.count; ++k) {
+    const size_t s = u.first + k;
+    StagePlan& sp = p->plan.stages[s];
+    StageDesc& d = sp.desc;
+    if (d.tile_w != tw || d.tile_h != th) {
+      d.tile_w = tw;
+      d.tile_h = th;
+      d.band_h = th;
+      d.tiles_x = (d.width + tw - 1) / tw;
+      d.tiles_y = (d.height + th - 1) / th;
+      sp.tile_mode = 1;  // TILE_DIRECT
+      sp.lds_bytes = 0;
+      const size_t a = sp.text.find(" tile "), b = a == std::string::npos ? a : sp.text.find(' ', a + 6);
+      if (b != std::string::npos) sp.text.replace(a + 6, b - (a + 6), std::to_string(tw) + "x" + std::to_string(th));
+    }
+    StageDesc& h = p->h_stages[s];
+    h.tile_w = d.tile_w;
+    h.tile_h = d.tile_h;
+    h.band_h = d.band_h;
+    h.tiles_x = d.tiles_x;
+    h.tiles_y = d.tiles_y;
+    prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
+  }
+  hipStream_t st = p->ctx->stream;
+  ZH_HIP(hipMemcpyAsync(p->d_stages + u.first, p->h_stages + u.first, u.count * sizeof(StageDesc),
+                        hipMemcpyHostToDevice, st));
+  if (u.batch) {
+    ZH_HIP(hipMemcpyAsync(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    u.total_tiles = prefix.back();
+    ZH_HIP(hipMemcpyAsync((char*)u.d_batch + offsetof(BatchDesc, total_tiles), &u.total_tiles, sizeof(uint32_t),
+                          hipMemcpyHostToDevice, st));
+  }
+  ZH_HIP(hipStreamSynchronize(st));
+  return ZH_OK;
+}
+
+zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_t num_bindings, uint32_t iterations) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  if (iterations == 0) iterations = 5;
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  zh_status st = bind(p, bindings, num_bindings);
+  if (st != ZH_OK) return st;
+  hipEvent_t e0, e1;
+  ZH_HIP(hipEventCreate(&e0));
+  ZH_HIP(hipEventCreate(&e1));
+  bool changed = false;
+  for (Unit& u : p->units) {
+    if (!warp_unit(p, u)) continue;
+    const uint32_t tw0 = p->plan.stages[u.first].desc.tile_w, th0 = p->plan.stages[u.first].desc.tile_h;
+    float best = 1e30f;
+    uint32_t bw = tw0, bh = th0;
+    for (const auto& c : kWarpTiles) {
+      st = set_warp_tile(p, u, c[0], c[1]);
+      if (st == ZH_OK) st = launch_unit(p, u);  // warm-up
+      if (st != ZH_OK) break;
+      hipEventRecord(e0, p->ctx->stream);
+      for (uint32_t i = 0; i < iterations && st == ZH_OK; ++i) st = launch_unit(p, u);
+      hipEventRecord(e1, p->ctx->stream);
+      if (st != ZH_OK) break;
+      hipEventSynchronize(e1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bw = c[0];
+        bh = c[1];
+      }
+    }
+    if (st == ZH_OK) st = set_warp_tile(p, u, bw, bh);
+    if (st != ZH_OK) break;
+    changed = changed || bw != tw0 || bh != th0;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st != ZH_OK) return st;
+  // Tiles live in the descriptors, but a captured graph holds the grid
+  // sizes: capture again at the next launch.
+  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  (void)changed;
+  return ZH_OK;
 }
 
 zh_status zh_program_time(zh_program* p, const zh_binding* bindings, uint32_t num_bindings, uint32_t iterations,

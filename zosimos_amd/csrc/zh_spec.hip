@@ -100,11 +100,13 @@ const char* zh_spec_name(int index) {
   return index >= 0 && index < zh::spec::kNumEntries ? zh::spec::kEntries[index].name : "";
 }
 
-hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
-                          uint32_t max_blocks, hipStream_t stream) {
+hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height, uint32_t cus,
+                          hipStream_t stream) {
   if (index < 0 || index >= zh::spec::kNumEntries) return hipErrorInvalidValue;
+  static uint32_t resident[zh::spec::kNumEntries] = {};
+  if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kEntries[index].fn, 1);
   const uint64_t groups = ((uint64_t)width * height + zh::kPix - 1) / zh::kPix;
-  const uint64_t blocks = std::min<uint64_t>((groups + zh::kBlock - 1) / zh::kBlock, max_blocks);
+  const uint64_t blocks = std::min<uint64_t>((groups + zh::kBlock - 1) / zh::kBlock, (uint64_t)resident[index] * cus);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::spec::kEntries[index].fn, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();

@@ -25,7 +25,7 @@
 
 extern "C" {
 hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint32_t height, int flat, int fast,
-                               uint32_t max_blocks, hipStream_t stream);
+                               uint32_t cus, hipStream_t stream);
 }
 
 using namespace zh;
@@ -158,7 +158,7 @@ struct Trainer {
 // Surrogate outputs on n float4 inputs (device), downloaded as float4.
 zh_status infer_host(hipStream_t s, uint32_t cus, const uint16_t* d_ops, const float* d_in, float* d_out, uint64_t n,
                      std::vector<float>& host) {
-  ZS_HIP(zh_mlp_infer(d_ops, d_in, d_out, n, 6, 6, cus * 8, s));
+  ZS_HIP(zh_mlp_infer(d_ops, d_in, d_out, n, 6, 6, cus, s));
   host.resize(n * 4);
   ZS_HIP(hipMemcpyAsync(host.data(), d_out, n * 16, hipMemcpyDeviceToHost, s));
   ZS_HIP(hipStreamSynchronize(s));
@@ -218,7 +218,7 @@ zh_status train_stage(zh_program* p, uint32_t stage, const zh_learn_options& o, 
   StageDesc* d_td = nullptr;
   ZS_HIP(T.alloc(&d_td, 1));
   ZS_HIP(hipMemcpyAsync(d_td, &td, sizeof(td), hipMemcpyHostToDevice, s));
-  ZS_HIP(zh_launch_pointwise(d_td, M, 1, 1, 0, cus * 8, s));
+  ZS_HIP(zh_launch_pointwise(d_td, M, 1, 1, 0, cus, s));
 
   // 2. training.
   const uint32_t B = std::min(o.batch, M);

@@ -35,20 +35,24 @@ template <> struct RawT<FMT_F16> { typedef u32x2 type; };
 template <> struct RawT<FMT_F32> { typedef u32x4 type; };
 
 template <uint32_t F>
-__device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const Luts* L) {
+__device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const float* rep) {
   if constexpr (F == FMT_F16) {
     return dec_f16(r.x, r.y);
   } else if constexpr (F == FMT_F32) {
     return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w));
+  } else if constexpr (F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA) {
+    return dec_u8_rep<F>(r, rep);
   } else {
-    return dec_u8<F>(r, L);
+    return dec_u8<F>(r, nullptr);
   }
 }
 
 __device__ __forceinline__ float4 premul4(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
 
 // Uniform data of one layer, read per tile.  Both sources of WSRC_OVER2 have
-// the same row stride (the planner checks), so one offset serves both.
+// the same row stride (the planner checks), so one offset serves both.  Row
+// offsets use 24-bit multiplies: the planner admits only sources under 4 GiB
+// with fewer than 2^24 rows of fewer than 2^24 bytes.
 struct LayerArgs {
   const gbyte* src0;
   const gbyte* src1;  // WSRC_OVER2: the upper source
@@ -97,12 +101,122 @@ struct Warp {
   typedef Post_ Post;
 };
 
+// Composite one layer's bilinear samples over the lane's kPix pixels.
+// (lu, lv): the first pixel's map minus the half-texel, relative to the
+// integer texel (iu0, iv0); (su, sv): the map step between the lane's pixels.
+// Taps of all pixels are addressed first (clamped in-bounds offsets, weight 0
+// for taps outside the source or missing pixels), then loaded, then blended:
+// 4 * kPix loads in flight per lane.  Straight-alpha sources are weighted by
+// w * alpha (premultiplying inside the weighted sum).
+template <class V, bool SEP>
+__device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0, float lu, float lv, float su, float sv,
+                                           int n, Px& X, const float* rep) {
+  typedef typename RawT<V::F>::type R;
+  constexpr uint32_t B = fmt_bpp(V::F);
+  uint32_t off[kPix][4];
+  float w[kPix][4];
+  bool any = false;
+  // Rows: shared by the lane's pixels when the map is separable.
+  float wy0s = 0.0f, wy1s = 0.0f;
+  uint32_t r0s = 0, r1s = 0;
+  bool yas = false, ybs = false;
+  if constexpr (SEP) {
+    const float fv = floorf(lv);
+    const int j = iv0 + (int)fv;
+    const float ay = lv - fv;
+    yas = (unsigned)j < (unsigned)a.sh;
+    ybs = (unsigned)(j + 1) < (unsigned)a.sh;
+    wy0s = yas ? 1.0f - ay : 0.0f;
+    wy1s = ybs ? ay : 0.0f;
+    r0s = __umul24((uint32_t)min(max(j, 0), a.sh - 1), a.stride);
+    r1s = __umul24((uint32_t)min(max(j + 1, 0), a.sh - 1), a.stride);
+  }
+#pragma unroll
+  for (int k = 0; k < kPix; ++k) {
+    const float u = lu + su * (float)k;
+    const float fu = floorf(u);
+    const int i = iu0 + (int)fu;
+    const float ax = u - fu;
+    const bool in = k < n;
+    const bool xa = in && (unsigned)i < (unsigned)a.sw, xb = in && (unsigned)(i + 1) < (unsigned)a.sw;
+    bool ya, yb;
+    float wy0, wy1;
+    uint32_t r0, r1;
+    if constexpr (SEP) {
+      ya = yas;
+      yb = ybs;
+      wy0 = wy0s;
+      wy1 = wy1s;
+      r0 = r0s;
+      r1 = r1s;
+    } else {
+      const float v = lv + sv * (float)k;
+      const float fv = floorf(v);
+      const int j = iv0 + (int)fv;
+      const float ay = v - fv;
+      ya = (unsigned)j < (unsigned)a.sh;
+      yb = (unsigned)(j + 1) < (unsigned)a.sh;
+      wy0 = ya ? 1.0f - ay : 0.0f;
+      wy1 = yb ? ay : 0.0f;
+      r0 = __umul24((uint32_t)min(max(j, 0), a.sh - 1), a.stride);
+      r1 = __umul24((uint32_t)min(max(j + 1, 0), a.sh - 1), a.stride);
+    }
+    const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
+    w[k][0] = wx0 * wy0;
+    w[k][1] = wx1 * wy0;
+    w[k][2] = wx0 * wy1;
+    w[k][3] = wx1 * wy1;
+    any = any || ((xa || xb) && (ya || yb));
+    const uint32_t i0 = (uint32_t)min(max(i, 0), a.sw - 1) * B, i1 = (uint32_t)min(max(i + 1, 0), a.sw - 1) * B;
+    off[k][0] = r0 + i0;
+    off[k][1] = r0 + i1;
+    off[k][2] = r1 + i0;
+    off[k][3] = r1 + i1;
+  }
+  if (!any) return;
+  R t0[kPix][4], t1[kPix][4];
+#pragma unroll
+  for (int k = 0; k < kPix; ++k)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      t0[k][t] = gld<R>(a.src0 + off[k][t]);
+      if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = gld<R>(a.src1 + off[k][t]);
+    }
+#pragma unroll
+  for (int k = 0; k < kPix; ++k) {
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float4 c = raw_dec<V::F>(t0[k][t], rep);
+      if constexpr (V::SRC == WSRC_OVER2) {
+        // Premultiplied 'over' of the two sources, then weighted.
+        const float4 lo = premul4(c);
+        const float4 h = premul4(raw_dec<V::F>(t1[k][t], rep));
+        const float q = 1.0f - h.w, wt = w[k][t];
+        const float4 m = make_float4(fmaf(lo.x, q, h.x), fmaf(lo.y, q, h.y), fmaf(lo.z, q, h.z), fmaf(lo.w, q, h.w));
+        s = make_float4(fmaf(m.x, wt, s.x), fmaf(m.y, wt, s.y), fmaf(m.z, wt, s.z), fmaf(m.w, wt, s.w));
+      } else {
+        const float wa = w[k][t] * c.w;
+        s = make_float4(fmaf(c.x, wa, s.x), fmaf(c.y, wa, s.y), fmaf(c.z, wa, s.z), s.w + wa);
+      }
+    }
+    if constexpr (V::MAT) {
+      const float r = a.m[0] * s.x + a.m[1] * s.y + a.m[2] * s.z;
+      const float g = a.m[3] * s.x + a.m[4] * s.y + a.m[5] * s.z;
+      const float b = a.m[6] * s.x + a.m[7] * s.y + a.m[8] * s.z;
+      s = make_float4(r, g, b, s.w);
+    }
+    const float kk = 1.0f - s.w;
+    X[k] = make_float4(fmaf(X[k].x, kk, s.x), fmaf(X[k].y, kk, s.y), fmaf(X[k].z, kk, s.z), fmaf(X[k].w, kk, s.w));
+  }
+}
+
 // One output tile (tx, ty) of stage d.  The tile is tile_w x tile_h pixels;
 // lane (lx, ly) owns pixels x = tx * tile_w + lx + k * lanes_x (k < kPix) of
 // row ty * tile_h + ly, so every load and store instruction covers runs of
 // lanes_x adjacent pixels.
 template <class V>
-__device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, const Luts* L) {
+__device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, const Luts* L, const float* rep) {
   typedef typename V::Post Post;
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
@@ -140,67 +254,12 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
     const float lu = (float)(uo - fuo) + m0 * (float)lx + m1 * (float)ly - 0.5f;
     const float lv = (float)(vo - fvo) + m3 * (float)lx + m4 * (float)ly - 0.5f;
     const float su = m0 * (float)lanes_x, sv = m3 * (float)lanes_x;
-    // Taps of all kPix pixels first (clamped in-bounds offsets, weight 0 for
-    // taps outside the source or missing pixels), then every load, then the
-    // arithmetic: 4 * kPix loads in flight per lane.
-    uint32_t off[kPix][4];
-    float w[kPix][4];
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < kPix; ++k) {
-      const float u = lu + su * (float)k, v = lv + sv * (float)k;
-      const float fu = floorf(u), fv = floorf(v);
-      const int i = iu0 + (int)fu, j = iv0 + (int)fv;
-      const float ax = u - fu, ay = v - fv;
-      const bool in = k < n;
-      const bool xa = in && (unsigned)i < (unsigned)a.sw, xb = in && (unsigned)(i + 1) < (unsigned)a.sw;
-      const bool ya = (unsigned)j < (unsigned)a.sh, yb = (unsigned)(j + 1) < (unsigned)a.sh;
-      const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
-      const float wy0 = ya ? 1.0f - ay : 0.0f, wy1 = yb ? ay : 0.0f;
-      w[k][0] = wx0 * wy0;
-      w[k][1] = wx1 * wy0;
-      w[k][2] = wx0 * wy1;
-      w[k][3] = wx1 * wy1;
-      any = any || ((xa || xb) && (ya || yb));
-      const uint32_t i0 = (uint32_t)min(max(i, 0), a.sw - 1), i1 = (uint32_t)min(max(i + 1, 0), a.sw - 1);
-      const uint32_t r0 = (uint32_t)min(max(j, 0), a.sh - 1) * a.stride, r1 = (uint32_t)min(max(j + 1, 0), a.sh - 1) * a.stride;
-      off[k][0] = r0 + i0 * B;
-      off[k][1] = r0 + i1 * B;
-      off[k][2] = r1 + i0 * B;
-      off[k][3] = r1 + i1 * B;
-    }
-    if (!any) continue;
-    R t0[kPix][4], t1[kPix][4];
-#pragma unroll
-    for (int k = 0; k < kPix; ++k)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        t0[k][t] = gld<R>(a.src0 + off[k][t]);
-        if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = gld<R>(a.src1 + off[k][t]);
-      }
-#pragma unroll
-    for (int k = 0; k < kPix; ++k) {
-      float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        float4 c = premul4(raw_dec<V::F>(t0[k][t], L));
-        if constexpr (V::SRC == WSRC_OVER2) {
-          const float4 h = premul4(raw_dec<V::F>(t1[k][t], L));
-          const float q = 1.0f - h.w;
-          c = make_float4(fmaf(c.x, q, h.x), fmaf(c.y, q, h.y), fmaf(c.z, q, h.z), fmaf(c.w, q, h.w));
-        }
-        const float wt = w[k][t];
-        s = make_float4(fmaf(c.x, wt, s.x), fmaf(c.y, wt, s.y), fmaf(c.z, wt, s.z), fmaf(c.w, wt, s.w));
-      }
-      if constexpr (V::MAT) {
-        const float r = a.m[0] * s.x + a.m[1] * s.y + a.m[2] * s.z;
-        const float g = a.m[3] * s.x + a.m[4] * s.y + a.m[5] * s.z;
-        const float b = a.m[6] * s.x + a.m[7] * s.y + a.m[8] * s.z;
-        s = make_float4(r, g, b, s.w);
-      }
-      const float kk = 1.0f - s.w;
-      X[k] = make_float4(fmaf(X[k].x, kk, s.x), fmaf(X[k].y, kk, s.y), fmaf(X[k].z, kk, s.z), fmaf(X[k].w, kk, s.w));
-    }
+    // Axis-aligned maps (pure scale and shift) are separable: the lane's
+    // pixels share one source row pair, computed once.
+    if (m1 == 0.0f && m3 == 0.0f)
+      layer_taps<V, true>(a, iu0, iv0, lu, lv, su, sv, n, X, rep);
+    else
+      layer_taps<V, false>(a, iu0, iv0, lu, lv, su, sv, n, X, rep);
   }
 
   if (n == 0) return;
@@ -223,14 +282,26 @@ __device__ __forceinline__ uint32_t virtual_block() {
   return (b % 8u) * (g / 8u) + b / 8u;
 }
 
+// Replicated decode table only for 8-bit sRGB sources.
+template <class V>
+constexpr int rep_words() {
+  return (V::F == FMT_U8_SRGB_RGBA || V::F == FMT_U8_SRGB_BGRA) ? kRepDec : 1;
+}
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp(const StageDesc* dd) {
   const CDesc d = as_const(dd);
   __shared__ Luts luts;
+  __shared__ float rep[rep_words<V>()];
   load_lut(d, &luts);
   __syncthreads();
+  if constexpr (rep_words<V>() > 1) {
+    load_rep(&luts, rep);
+    __syncthreads();
+  }
   const uint32_t tiles = d->tiles_x * d->tiles_y;
-  for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x) warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts);
+  for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x)
+    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
 }
 
 // ---- registry ------------------------------------------------------------------
@@ -249,10 +320,11 @@ enum {
 
 // Every stage of a batch, whichever registered form it has: the form is
 // uniform per tile, so the switch costs one scalar branch per tile.
-__device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint32_t ty, const Luts* L) {
+__device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint32_t ty, const Luts* L,
+                                              const float* rep) {
   switch (v) {
 #define ZH_WARP_CASE(name, ...) \
-  case WI_##name: warp_tile<V_##name>(d, tx, ty, L); break;
+  case WI_##name: warp_tile<V_##name>(d, tx, ty, L, rep); break;
     ZH_WARP_LIST(ZH_WARP_CASE)
 #undef ZH_WARP_CASE
     default: break;
@@ -264,7 +336,10 @@ __device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint3
 __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
   __shared__ Luts luts;
+  __shared__ float rep[kRepDec];
   load_lut(as_const(b->stages), &luts);
+  __syncthreads();
+  load_rep(&luts, rep);
   __syncthreads();
   const uint32_t total = b->total_tiles;
   for (uint32_t j = virtual_block(); j < total; j += gridDim.x) {
@@ -275,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
     }
     const CDesc d = as_const(b->stages + lo);
     const uint32_t t = j - b->tile_prefix[lo];
-    warp_dispatch(d->variant, d, t % d->tiles_x, t / d->tiles_x, &luts);
+    warp_dispatch(d->variant, d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
   }
 }
 
@@ -311,18 +386,20 @@ const char* zh_warp_name(int index) {
   return index >= 0 && index < zh::spec::kNumWarps ? zh::spec::kWarps[index].name : "";
 }
 
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t max_blocks,
-                          hipStream_t stream) {
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, hipStream_t stream) {
   if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
-  const uint32_t blocks = std::min(tiles, max_blocks);
+  static uint32_t resident[zh::spec::kNumWarps] = {};
+  if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kWarps[index].fn, 1);
+  const uint32_t blocks = std::min(tiles, resident[index] * cus);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::spec::kWarps[index].fn, dim3(blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();
 }
 
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t max_blocks,
-                                hipStream_t stream) {
-  const uint32_t blocks = std::min(total_tiles, max_blocks);
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, hipStream_t stream) {
+  static uint32_t resident = 0;
+  if (!resident) resident = zh::resident_blocks((const void*)zh::spec::k_warp_batch, 1);
+  const uint32_t blocks = std::min(total_tiles, resident * cus);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::spec::k_warp_batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
   return hipGetLastError();
