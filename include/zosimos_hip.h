@@ -242,8 +242,9 @@ zh_status zh_program_time(zh_program* prog, const zh_binding* bindings, uint32_t
 /* Autotuner (no counterpart in the reference, whose tiles are the
  * rasteriser's): for every warp-kernel launch of the program (affine layer
  * stacks and their batches), time each tile shape the kernel accepts
- * (64x16, 32x32, 128x8, 16x64 pixels) on the bound images over `iterations`
- * launches and keep the fastest.  The choice persists in the program until a
+ * (64x16, 32x32, 128x8, 16x64 pixels) with each persistent-grid width
+ * (all resident workgroups, or 4, 3, 2, 1 per CU) on the bound images over
+ * `iterations` launches and keep the fastest.  The choice persists in the program until a
  * knob re-plans it; zh_program_describe shows the tiles. */
 zh_status zh_program_autotune(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings,
                               uint32_t iterations);

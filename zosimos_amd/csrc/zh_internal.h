@@ -101,6 +101,7 @@ struct Unit {
   uint32_t* d_prefix = nullptr;
   uint32_t grid = 0, lds = 0;
   uint32_t total_tiles = 0;
+  uint32_t blocks_per_cu = 0;    // persistent warp grids: 0 = as many as are resident
 };
 
 struct zh_program {

@@ -718,7 +718,11 @@ static int warp_variant(const StageDesc& d) {
     if (in.op == OP_STORE && (d.slot[in.a].width != d.width || d.slot[in.a].height != d.height)) return -1;
     sig[i] = sg(in.op, in.op == OP_STORE ? d.slot[in.a].fmt : 0u);
   }
-  return zh_warp_lookup(src, fmt, mat, sig, (int)np);
+  const int v = zh_warp_lookup(src, fmt, mat, sig, (int)np);
+  if (v < 0) return -1;
+  bool sep = true;
+  for (uint32_t l = 0; l < d.n_layers; ++l) sep = sep && d.layer[l].inv[1] == 0.0 && d.layer[l].inv[3] == 0.0;
+  return sep ? v + 1 : v;
 }
 
 static double visible_fraction(const Mat3& inv, uint32_t sw, uint32_t sh, uint32_t W, uint32_t H) {

@@ -23,8 +23,10 @@ hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint3
                                uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
                           uint32_t cus, hipStream_t stream);
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, hipStream_t stream);
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, hipStream_t stream);
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
+                          hipStream_t stream);
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, uint32_t per_cu,
+                                hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -434,10 +436,13 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
   if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
   std::ostringstream o;
   for (size_t i = 0; i < p->plan.stages.size(); ++i) o << "stage " << i << ": " << p->plan.stages[i].text << "\n";
-  for (size_t i = 0; i < p->units.size(); ++i)
+  for (size_t i = 0; i < p->units.size(); ++i) {
     if (p->units[i].batch)
       o << "batch: stages " << p->units[i].first << ".." << p->units[i].first + p->units[i].count - 1
         << " as one tile queue\n";
+    if (p->units[i].blocks_per_cu)
+      o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
+  }
   for (size_t i = 0; i < p->plan.intermediates.size(); ++i) {
     const zh_descriptor& d = p->plan.intermediates[i];
     o << "intermediate " << i << ": " << d.width << "x" << d.height << " bits " << d.texel.bits << "\n";
@@ -636,7 +641,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     bool all_warp = u.warp;
     for (size_t k = 0; k < u.count && all_warp; ++k) all_warp = p->variant[u.first + k] == 3;
     if (all_warp) {
-      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, s));
+      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
       return ZH_OK;
     }
     bool all_staged = true;
@@ -671,7 +676,8 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     }
     case STAGE_RESAMPLE:
       if (u.first < p->variant.size() && p->variant[u.first] == 3)
-        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus, s));
+        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus,
+                              u.blocks_per_cu, s));
       else
         ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
       break;
@@ -798,24 +804,32 @@ zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_
     if (!warp_unit(p, u)) continue;
     const uint32_t tw0 = p->plan.stages[u.first].desc.tile_w, th0 = p->plan.stages[u.first].desc.tile_h;
     float best = 1e30f;
-    uint32_t bw = tw0, bh = th0;
+    uint32_t bw = tw0, bh = th0, bb = u.blocks_per_cu;
+    static const uint32_t kPerCu[] = {0, 4, 3, 2, 1};
     for (const auto& c : kWarpTiles) {
       st = set_warp_tile(p, u, c[0], c[1]);
-      if (st == ZH_OK) st = launch_unit(p, u);  // warm-up
-      if (st != ZH_OK) break;
-      hipEventRecord(e0, p->ctx->stream);
-      for (uint32_t i = 0; i < iterations && st == ZH_OK; ++i) st = launch_unit(p, u);
-      hipEventRecord(e1, p->ctx->stream);
-      if (st != ZH_OK) break;
-      hipEventSynchronize(e1);
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) {
-        best = ms;
-        bw = c[0];
-        bh = c[1];
+      for (uint32_t bpc : kPerCu) {
+        if (st != ZH_OK) break;
+        u.blocks_per_cu = bpc;
+        st = launch_unit(p, u);  // warm-up
+        if (st != ZH_OK) break;
+        hipEventRecord(e0, p->ctx->stream);
+        for (uint32_t i = 0; i < iterations && st == ZH_OK; ++i) st = launch_unit(p, u);
+        hipEventRecord(e1, p->ctx->stream);
+        if (st != ZH_OK) break;
+        hipEventSynchronize(e1);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          bw = c[0];
+          bh = c[1];
+          bb = bpc;
+        }
       }
+      if (st != ZH_OK) break;
     }
+    u.blocks_per_cu = bb;
     if (st == ZH_OK) st = set_warp_tile(p, u, bw, bh);
     if (st != ZH_OK) break;
     changed = changed || bw != tw0 || bh != th0;
@@ -843,23 +857,28 @@ zh_status zh_program_time(zh_program* p, const zh_binding* bindings, uint32_t nu
   if (st != ZH_OK) return st;
   const size_t n = p->units.size();
   if (num_stages) *num_stages = (uint32_t)n;
-  std::vector<hipEvent_t> ev(n + 1);
+  // Every iteration's launches and events are queued before one wait, so
+  // the intervals measure the GPU back to back, not the host's launch latency.
+  std::vector<hipEvent_t> ev((n + 1) * iterations);
   for (auto& e : ev) ZH_HIP(hipEventCreate(&e));
-  std::vector<double> acc(n, 0.0);
-  for (uint32_t it = 0; it < iterations; ++it) {
-    ZH_HIP(hipEventRecord(ev[0], p->ctx->stream));
-    for (size_t u = 0; u < n; ++u) {
+  st = launch_all(p);  // warm-up
+  for (uint32_t it = 0; it < iterations && st == ZH_OK; ++it) {
+    hipEvent_t* e = &ev[(size_t)it * (n + 1)];
+    ZH_HIP(hipEventRecord(e[0], p->ctx->stream));
+    for (size_t u = 0; u < n && st == ZH_OK; ++u) {
       st = launch_unit(p, p->units[u]);
-      if (st != ZH_OK) break;
-      ZH_HIP(hipEventRecord(ev[u + 1], p->ctx->stream));
+      if (st == ZH_OK) ZH_HIP(hipEventRecord(e[u + 1], p->ctx->stream));
     }
-    if (st != ZH_OK) break;
-    ZH_HIP(hipEventSynchronize(ev[n]));
-    for (size_t u = 0; u < n; ++u) {
-      float ms = 0;
-      hipEventElapsedTime(&ms, ev[u], ev[u + 1]);
-      acc[u] += ms;
-    }
+  }
+  std::vector<double> acc(n, 0.0);
+  if (st == ZH_OK) {
+    ZH_HIP(hipEventSynchronize(ev.back()));
+    for (uint32_t it = 0; it < iterations; ++it)
+      for (size_t u = 0; u < n; ++u) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[(size_t)it * (n + 1) + u], ev[(size_t)it * (n + 1) + u + 1]);
+        acc[u] += ms;
+      }
   }
   for (auto& e : ev) hipEventDestroy(e);
   if (st != ZH_OK) return st;

@@ -94,8 +94,9 @@ extern "C" {
 // defined next to the kernels).
 int zh_spec_lookup(const uint16_t* sig, int n);
 const char* zh_spec_name(int index);
-// Index of the warp kernel for (source form, format, mat3, post signature),
-// or -1.
+// Index of the warp kernel for (source form, format, mat3, post signature)
+// with general affine maps, or -1; the next index is the same form's kernel
+// for axis-aligned maps.
 int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n);
 const char* zh_warp_name(int index);
 }

@@ -8,10 +8,13 @@
 //     of 256 lanes, kPix = 4 horizontally adjacent pixels per lane); a
 //     persistent grid walks the tiles with a stride;
 //   * per tile and layer, the tile's footprint is mapped in FP64 (four
-//     corners) and the layer is skipped when it misses the source; the map of
-//     the tile's first pixel centre is split into an integer texel and an
-//     FP32 remainder, and every pixel adds its FP32 offset (|offset| < 100 px,
-//     so coordinates carry ~1e-5 px error instead of FP32's 5e-4 px at 8K);
+//     corners) and the layer is skipped when it misses the source; for 8-bit
+//     outputs the map of the tile's first pixel centre is split into an
+//     integer texel and an FP32 remainder and every pixel adds its FP32
+//     offset (|offset| < 100 px: ~4e-6 px error instead of FP32's 5e-4 px at
+//     8K); float outputs map every pixel in FP64, since a bilinear weight
+//     error of ~4e-6 reaches a few f16 ulps after un-premultiplying a small
+//     alpha;
 //   * the four taps are read straight from global memory (the L1/L2 caches
 //     serve the overlap between neighbouring pixels), decoded in-line,
 //     premultiplied and weighted; taps outside the source get weight 0
@@ -93,24 +96,35 @@ __device__ __forceinline__ bool layer_hits(const Layer& L, double ex0, double ey
   return umax > -1.0 && vmax > -1.0 && umin < (double)L.src_w && vmin < (double)L.src_h;
 }
 
-// Warp variant: source form, source format, mat3 flag and post program.
-template <uint32_t SRC_, uint32_t F_, bool MAT_, class Post_>
+// Warp variant: source form, source format, mat3 flag, post program, and
+// whether every layer's map is axis-aligned (separable taps).
+template <uint32_t SRC_, uint32_t F_, bool MAT_, class Post_, bool SEP_>
 struct Warp {
   static constexpr uint32_t SRC = SRC_, F = F_;
-  static constexpr bool MAT = MAT_;
+  static constexpr bool MAT = MAT_, SEP = SEP_;
   typedef Post_ Post;
 };
 
+// Map of a lane's pixels into one layer's source.  FP32 form: (lu, lv) is the
+// lane's first pixel centre mapped, minus the half texel, relative to the
+// integer texel (iu0, iv0), and (su, sv) the step between the lane's pixels
+// (~4e-6 px error: enough for 8-bit outputs).  FP64 form (float outputs,
+// where a weight error of that size shows after un-premultiplying a small
+// alpha): every pixel is mapped from (du, dv) = the lane's first pixel centre
+// mapped minus the half texel, steps (eu, ev).
+struct LaneMap {
+  int iu0, iv0;
+  float lu, lv, su, sv;
+  double du, dv, eu, ev;
+};
+
 // Composite one layer's bilinear samples over the lane's kPix pixels.
-// (lu, lv): the first pixel's map minus the half-texel, relative to the
-// integer texel (iu0, iv0); (su, sv): the map step between the lane's pixels.
 // Taps of all pixels are addressed first (clamped in-bounds offsets, weight 0
 // for taps outside the source or missing pixels), then loaded, then blended:
 // 4 * kPix loads in flight per lane.  Straight-alpha sources are weighted by
 // w * alpha (premultiplying inside the weighted sum).
-template <class V, bool SEP>
-__device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0, float lu, float lv, float su, float sv,
-                                           int n, Px& X, const float* rep) {
+template <class V, bool SEP, bool P64>
+__device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp, int n, Px& X, const float* rep) {
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
   uint32_t off[kPix][4];
@@ -121,9 +135,17 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0,
   uint32_t r0s = 0, r1s = 0;
   bool yas = false, ybs = false;
   if constexpr (SEP) {
-    const float fv = floorf(lv);
-    const int j = iv0 + (int)fv;
-    const float ay = lv - fv;
+    int j;
+    float ay;
+    if constexpr (P64) {
+      const double fv = floor(mp.dv);
+      j = (int)fv;
+      ay = (float)(mp.dv - fv);
+    } else {
+      const float fv = floorf(mp.lv);
+      j = mp.iv0 + (int)fv;
+      ay = mp.lv - fv;
+    }
     yas = (unsigned)j < (unsigned)a.sh;
     ybs = (unsigned)(j + 1) < (unsigned)a.sh;
     wy0s = yas ? 1.0f - ay : 0.0f;
@@ -133,10 +155,19 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0,
   }
 #pragma unroll
   for (int k = 0; k < kPix; ++k) {
-    const float u = lu + su * (float)k;
-    const float fu = floorf(u);
-    const int i = iu0 + (int)fu;
-    const float ax = u - fu;
+    int i;
+    float ax;
+    if constexpr (P64) {
+      const double u = fma(mp.eu, (double)k, mp.du);
+      const double fu = floor(u);
+      i = (int)fu;
+      ax = (float)(u - fu);
+    } else {
+      const float u = mp.lu + mp.su * (float)k;
+      const float fu = floorf(u);
+      i = mp.iu0 + (int)fu;
+      ax = u - fu;
+    }
     const bool in = k < n;
     const bool xa = in && (unsigned)i < (unsigned)a.sw, xb = in && (unsigned)(i + 1) < (unsigned)a.sw;
     bool ya, yb;
@@ -150,10 +181,19 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0,
       r0 = r0s;
       r1 = r1s;
     } else {
-      const float v = lv + sv * (float)k;
-      const float fv = floorf(v);
-      const int j = iv0 + (int)fv;
-      const float ay = v - fv;
+      int j;
+      float ay;
+      if constexpr (P64) {
+        const double v = fma(mp.ev, (double)k, mp.dv);
+        const double fv = floor(v);
+        j = (int)fv;
+        ay = (float)(v - fv);
+      } else {
+        const float v = mp.lv + mp.sv * (float)k;
+        const float fv = floorf(v);
+        j = mp.iv0 + (int)fv;
+        ay = v - fv;
+      }
       ya = (unsigned)j < (unsigned)a.sh;
       yb = (unsigned)(j + 1) < (unsigned)a.sh;
       wy0 = ya ? 1.0f - ay : 0.0f;
@@ -182,6 +222,9 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, int iu0, int iv0,
       t0[k][t] = gld<R>(a.src0 + off[k][t]);
       if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = gld<R>(a.src1 + off[k][t]);
     }
+  // Keep every load above this point: left free, the scheduler may trade
+  // the loads in flight for registers and wait on each tap in turn.
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int k = 0; k < kPix; ++k) {
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -245,21 +288,31 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
     const Layer Ly = ldc(&d->layer[l]);
     if (!layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;  // uniform
     const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
-    // Map of the tile's first pixel centre: integer texel + FP32 remainder.
+    // Map of the tile's first pixel centre: integer texel + FP32 remainder
+    // (8-bit outputs), or the lane's first pixel in FP64 (float outputs).
+    constexpr bool P64 = el_fmt(Post::el[Post::n - 1]) == FMT_F16 || el_fmt(Post::el[Post::n - 1]) == FMT_F32;
     const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
     const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
-    const double fuo = floor(uo), fvo = floor(vo);
-    const int iu0 = (int)fuo, iv0 = (int)fvo;
-    const float m0 = (float)Ly.inv[0], m1 = (float)Ly.inv[1], m3 = (float)Ly.inv[3], m4 = (float)Ly.inv[4];
-    const float lu = (float)(uo - fuo) + m0 * (float)lx + m1 * (float)ly - 0.5f;
-    const float lv = (float)(vo - fvo) + m3 * (float)lx + m4 * (float)ly - 0.5f;
-    const float su = m0 * (float)lanes_x, sv = m3 * (float)lanes_x;
-    // Axis-aligned maps (pure scale and shift) are separable: the lane's
+    LaneMap mp{};
+    if constexpr (P64) {
+      mp.du = fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5;
+      mp.dv = fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5;
+      mp.eu = Ly.inv[0] * (double)lanes_x;
+      mp.ev = Ly.inv[3] * (double)lanes_x;
+    } else {
+      const double fuo = floor(uo), fvo = floor(vo);
+      mp.iu0 = (int)fuo;
+      mp.iv0 = (int)fvo;
+      const float m0 = (float)Ly.inv[0], m1 = (float)Ly.inv[1], m3 = (float)Ly.inv[3], m4 = (float)Ly.inv[4];
+      mp.lu = (float)(uo - fuo) + m0 * (float)lx + m1 * (float)ly - 0.5f;
+      mp.lv = (float)(vo - fvo) + m3 * (float)lx + m4 * (float)ly - 0.5f;
+      mp.su = m0 * (float)lanes_x;
+      mp.sv = m3 * (float)lanes_x;
+    }
+    // Axis-aligned maps (pure scale and shift; the planner picks the SEP
+    // variant only when every layer has one) are separable: the lane's
     // pixels share one source row pair, computed once.
-    if (m1 == 0.0f && m3 == 0.0f)
-      layer_taps<V, true>(a, iu0, iv0, lu, lv, su, sv, n, X, rep);
-    else
-      layer_taps<V, false>(a, iu0, iv0, lu, lv, su, sv, n, X, rep);
+    layer_taps<V, V::SEP, P64>(a, mp, n, X, rep);
   }
 
   if (n == 0) return;
@@ -305,14 +358,16 @@ __global__ __launch_bounds__(kBlock) void k_warp(const StageDesc* dd) {
 }
 
 // ---- registry ------------------------------------------------------------------
-#define ZH_WARP_TYPE(name, src, f, mat, ...)     \
-  typedef Warp<src, f, mat != 0, Prog<__VA_ARGS__>> V_##name; \
+#define ZH_WARP_TYPE(name, src, f, mat, ...)                         \
+  typedef Warp<src, f, mat != 0, Prog<__VA_ARGS__>, false> V_##name;    \
+  typedef Warp<src, f, mat != 0, Prog<__VA_ARGS__>, true> VS_##name;    \
   static const uint16_t WP_##name[] = {__VA_ARGS__};
 ZH_WARP_LIST(ZH_WARP_TYPE)
 #undef ZH_WARP_TYPE
 
+// Kernel index 2 i: form i with general affine maps; 2 i + 1: separable.
 enum {
-#define ZH_WARP_ENUM(name, ...) WI_##name,
+#define ZH_WARP_ENUM(name, ...) WI_##name, WIS_##name,
   ZH_WARP_LIST(ZH_WARP_ENUM)
 #undef ZH_WARP_ENUM
   kNumWarps
@@ -323,8 +378,9 @@ enum {
 __device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint32_t ty, const Luts* L,
                                               const float* rep) {
   switch (v) {
-#define ZH_WARP_CASE(name, ...) \
-  case WI_##name: warp_tile<V_##name>(d, tx, ty, L, rep); break;
+#define ZH_WARP_CASE(name, ...)                                   \
+  case WI_##name: warp_tile<V_##name>(d, tx, ty, L, rep); break; \
+  case WIS_##name: warp_tile<VS_##name>(d, tx, ty, L, rep); break;
     ZH_WARP_LIST(ZH_WARP_CASE)
 #undef ZH_WARP_CASE
     default: break;
@@ -362,8 +418,9 @@ struct WarpEntry {
   void (*fn)(const StageDesc*);
 };
 
-#define ZH_WARP_ENTRY(name, src, f, mat, ...) \
-  {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>},
+#define ZH_WARP_ENTRY(name, src, f, mat, ...)                                                            \
+  {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>},        \
+  {#name "_sep", src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<VS_##name>},
 static const WarpEntry kWarps[] = {ZH_WARP_LIST(ZH_WARP_ENTRY)};
 #undef ZH_WARP_ENTRY
 
@@ -373,7 +430,7 @@ static const WarpEntry kWarps[] = {ZH_WARP_LIST(ZH_WARP_ENTRY)};
 extern "C" {
 
 int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n) {
-  for (int i = 0; i < zh::spec::kNumWarps; ++i) {
+  for (int i = 0; i < zh::spec::kNumWarps; i += 2) {
     const auto& e = zh::spec::kWarps[i];
     if (e.src == src && e.fmt == fmt && e.mat == mat && e.n == n &&
         std::memcmp(e.post, post, sizeof(uint16_t) * (size_t)n) == 0)
@@ -386,20 +443,26 @@ const char* zh_warp_name(int index) {
   return index >= 0 && index < zh::spec::kNumWarps ? zh::spec::kWarps[index].name : "";
 }
 
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, hipStream_t stream) {
+// per_cu: workgroups per compute unit of the persistent grid (0: as many as
+// are resident; fewer keeps more of each workgroup's taps in the L1).
+hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
+                          hipStream_t stream) {
   if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
   static uint32_t resident[zh::spec::kNumWarps] = {};
   if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kWarps[index].fn, 1);
-  const uint32_t blocks = std::min(tiles, resident[index] * cus);
+  const uint32_t bpc = per_cu ? std::min(per_cu, resident[index]) : resident[index];
+  const uint32_t blocks = std::min(tiles, bpc * cus);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::spec::kWarps[index].fn, dim3(blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();
 }
 
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, hipStream_t stream) {
+hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, uint32_t per_cu,
+                                hipStream_t stream) {
   static uint32_t resident = 0;
   if (!resident) resident = zh::resident_blocks((const void*)zh::spec::k_warp_batch, 1);
-  const uint32_t blocks = std::min(total_tiles, resident * cus);
+  const uint32_t bpc = per_cu ? std::min(per_cu, resident) : resident;
+  const uint32_t blocks = std::min(total_tiles, bpc * cus);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::spec::k_warp_batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
   return hipGetLastError();
