@@ -218,6 +218,11 @@ zh_status zh_program_describe(zh_program* prog, char* buf, size_t len, size_t* n
  * (each input read once, each output written once). */
 zh_status zh_program_stats(zh_program* prog, uint32_t* launches, uint64_t* algorithmic_bytes);
 
+/* Algorithmic HBM bytes of each kernel launch of the program, in launch
+ * order (the sum is zh_program_stats' figure); for measurement, no
+ * counterpart in the reference. */
+zh_status zh_program_launch_bytes(zh_program* prog, uint64_t* bytes, uint32_t max_launches, uint32_t* num_launches);
+
 /* Knobs (Environment::knob, run.rs:1292-1306): override the parameter blob of
  * every op built with this knob id; the length must equal the original. */
 zh_status zh_program_set_knob(zh_program* prog, int32_t knob, const void* data, uint32_t len);

@@ -109,11 +109,16 @@ def test_plan_c1_c5_resample_layers():
     assert "post[unpremul mat3 cbrt mat3 cube mat3 clamp01 store#8]" in t5
 
 
-def test_plan_c3_batches_into_one_tile_queue():
+def test_plan_c3_batches_into_one_tile_queue_per_form():
+    """Independent resample stages batch by kernel form: one persistent
+    tile-queue launch per form, every stage in exactly one batch."""
     e = plan(W.c3(12, max_side=300))
     text = e.describe()
-    assert "batch: stages 0..11 as one tile queue" in text
-    assert e.stats()[0] == 1
+    forms = set(re.findall(r"\[kernel (\w+)\]", text))
+    batches = re.findall(r"batch: stages ([\d,]+) as one tile queue", text)
+    ids = sorted(int(i) for b in batches for i in b.split(","))
+    assert ids == list(range(12)) and len(batches) == len(forms) <= 2
+    assert e.stats()[0] == len(forms)
 
 
 def test_plan_strict_registers_quantises():

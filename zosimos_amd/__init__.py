@@ -388,6 +388,13 @@ class Executable:
         _check(self._lib.zh_program_time(self._h, arr, n, iterations, out, 64, ctypes.byref(cnt)))
         return [out[i] for i in range(cnt.value)]
 
+    def launch_bytes(self) -> List[int]:
+        """Algorithmic HBM bytes of each kernel launch, in launch order."""
+        out = (ctypes.c_uint64 * 256)()
+        n = ctypes.c_uint32()
+        _check(self._lib.zh_program_launch_bytes(self._h, out, 256, ctypes.byref(n)))
+        return [out[i] for i in range(min(n.value, 256))]
+
     def autotune(self, inputs, outputs, iterations: int = 5):
         """Time every tile shape of the program's warp kernels on these images
         and keep the fastest (``zh_program_autotune``)."""

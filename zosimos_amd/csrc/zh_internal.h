@@ -94,8 +94,9 @@ struct zh_image {
 // ---- programs (zh_runtime.cpp, zh_surrogate.cpp) --------------------------
 struct Unit {
   bool batch = false;
-  bool warp = false;             // batch of warp-form stages (k_warp_batch)
-  size_t first = 0, count = 1;   // stage range
+  bool warp = false;             // batch of one warp form (k_warp_batch<form>)
+  std::vector<uint32_t> ids;     // stages of the unit (one unless batched)
+  uint32_t* d_ids = nullptr;     // device copy of ids (batches)
   zh::BatchDesc* d_batch = nullptr;  // device
   uint32_t* d_counter = nullptr;
   uint32_t* d_prefix = nullptr;

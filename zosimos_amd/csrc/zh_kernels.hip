@@ -658,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* bb) {
       const uint32_t mid = (lo + hi) / 2;
       if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
     }
-    const CDesc d = as_const(b->stages + lo);
+    const CDesc d = as_const(b->stages + b->stage_ids[lo]);
     const uint32_t t = j - b->tile_prefix[lo];
     process_tile<TILE_STAGED>(d, (int)(t % d->tiles_x), (int)(t / d->tiles_x), tile, lut);
   }

@@ -9,6 +9,7 @@
 // are one graph launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -25,8 +26,8 @@ hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width,
                           uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
                           hipStream_t stream);
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, uint32_t per_cu,
-                                hipStream_t stream);
+hipError_t zh_launch_warp_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus,
+                                uint32_t per_cu, hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -291,6 +292,7 @@ static void free_device(zh_program* p) {
     if (u.d_batch) hipFree(u.d_batch);
     if (u.d_counter) hipFree(u.d_counter);
     if (u.d_prefix) hipFree(u.d_prefix);
+    if (u.d_ids) hipFree(u.d_ids);
   }
   p->units.clear();
   for (void* b : p->d_intermediates) hipFree(b);
@@ -301,30 +303,27 @@ static void free_device(zh_program* p) {
   p->h_stages = nullptr;
 }
 
-// Group stages into launch units: maximal runs of independent staged
-// resample stages become one batch (tile queue), everything else launches
-// alone.  A stage depends on an earlier one when it reads what that wrote.
+// Group stages into launch units.  A run of independent resample stages
+// (none reads what an earlier one of the run writes) is batched: warp-form
+// stages by form, one persistent tile-queue launch per form, and LDS-staged
+// interpreter stages together; everything else launches alone.
 static void make_units(zh_program* p) {
   p->units.clear();
   const auto& st = p->plan.stages;
+  auto family = [&](const StagePlan& sp) {
+    if (sp.desc.kind != STAGE_RESAMPLE) return 0;
+    if (sp.desc.variant >= 0) return 1;
+    return sp.tile_mode == 0 ? 2 : 0;
+  };
   size_t i = 0;
   while (i < st.size()) {
-    Unit u;
-    u.first = i;
-    u.count = 1;
-    // Batches: runs of independent resample stages that are all warp forms,
-    // or all LDS-staged interpreter stages.
-    const bool warp = st[i].desc.kind == STAGE_RESAMPLE && st[i].desc.variant >= 0;
-    auto same_family = [&](const StagePlan& sp) {
-      if (sp.desc.kind != STAGE_RESAMPLE) return false;
-      return warp ? sp.desc.variant >= 0 : (sp.desc.variant < 0 && sp.tile_mode == 0);
-    };
-    if (same_family(st[i])) {
+    const int fam = family(st[i]);
+    size_t j = i + 1;
+    if (fam) {
       std::vector<int64_t> written;
       for (size_t k = 0; k < st[i].slot_ref.size(); ++k)
         if (st[i].slot_ref[k] < 0 || st[i].slot_ref[k] >= kIntermediateBase) written.push_back(st[i].slot_ref[k]);
-      size_t j = i + 1;
-      while (j < st.size() && same_family(st[j])) {
+      while (j < st.size() && family(st[j]) == fam) {
         bool dep = false;
         for (int64_t r : st[j].slot_ref)
           for (int64_t w : written)
@@ -334,14 +333,23 @@ static void make_units(zh_program* p) {
           if (r < 0 || r >= kIntermediateBase) written.push_back(r);
         ++j;
       }
-      if (j - i >= 2) {
-        u.batch = true;
-        u.warp = warp;
-        u.count = j - i;
-      }
     }
-    p->units.push_back(u);
-    i += u.count;
+    // Partition [i, j): by warp form, or as one group.
+    std::vector<std::pair<int, std::vector<uint32_t>>> groups;
+    for (size_t k = i; k < j; ++k) {
+      const int key = fam == 1 ? st[k].desc.variant : -1;
+      auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
+      if (it == groups.end()) groups.push_back({key, {(uint32_t)k}});
+      else it->second.push_back((uint32_t)k);
+    }
+    for (auto& g : groups) {
+      Unit u;
+      u.ids = g.second;
+      u.batch = fam && u.ids.size() >= 2;
+      u.warp = u.batch && fam == 1;
+      p->units.push_back(u);
+    }
+    i = j;
   }
 }
 
@@ -361,21 +369,24 @@ static zh_status alloc_device(zh_program* p) {
   for (Unit& u : p->units) {
     if (!u.batch) continue;
     BatchDesc b{};
-    std::vector<uint32_t> prefix(u.count + 1, 0);
-    for (size_t k = 0; k < u.count; ++k) {
-      const StageDesc& d = p->plan.stages[u.first + k].desc;
+    std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
+    for (size_t k = 0; k < u.ids.size(); ++k) {
+      const StageDesc& d = p->plan.stages[u.ids[k]].desc;
       prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
-      u.lds = std::max(u.lds, p->plan.stages[u.first + k].lds_bytes);
+      u.lds = std::max(u.lds, p->plan.stages[u.ids[k]].lds_bytes);
     }
     ZH_HIP(hipMalloc(&u.d_batch, sizeof(BatchDesc)));
     ZH_HIP(hipMalloc(&u.d_counter, sizeof(uint32_t)));
     ZH_HIP(hipMalloc(&u.d_prefix, prefix.size() * sizeof(uint32_t)));
+    ZH_HIP(hipMalloc(&u.d_ids, u.ids.size() * sizeof(uint32_t)));
     ZH_HIP(hipMemcpy(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    b.n_stages = (uint32_t)u.count;
+    ZH_HIP(hipMemcpy(u.d_ids, u.ids.data(), u.ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    b.n_stages = (uint32_t)u.ids.size();
     b.total_tiles = prefix.back();
     u.total_tiles = b.total_tiles;
     b.counter = u.d_counter;
-    b.stages = p->d_stages + u.first;
+    b.stages = p->d_stages;
+    b.stage_ids = u.d_ids;
     b.tile_prefix = u.d_prefix;
     ZH_HIP(hipMemcpy(u.d_batch, &b, sizeof(b), hipMemcpyHostToDevice));
     // Persistent grid: enough workgroups to fill every CU at the kernel's
@@ -432,14 +443,28 @@ zh_status zh_program_destroy(zh_program* p) {
   return ZH_OK;
 }
 
+zh_status zh_program_launch_bytes(zh_program* p, uint64_t* bytes, uint32_t max_launches, uint32_t* num_launches) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (p->units.empty() && !p->plan.stages.empty()) make_units(p);
+  if (num_launches) *num_launches = (uint32_t)p->units.size();
+  for (size_t i = 0; i < p->units.size() && i < max_launches && bytes; ++i) {
+    uint64_t b = 0;
+    for (uint32_t id : p->units[i].ids) b += p->plan.stages[id].alg_bytes;
+    bytes[i] = b;
+  }
+  return ZH_OK;
+}
+
 zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* needed) {
   if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
   std::ostringstream o;
   for (size_t i = 0; i < p->plan.stages.size(); ++i) o << "stage " << i << ": " << p->plan.stages[i].text << "\n";
   for (size_t i = 0; i < p->units.size(); ++i) {
-    if (p->units[i].batch)
-      o << "batch: stages " << p->units[i].first << ".." << p->units[i].first + p->units[i].count - 1
-        << " as one tile queue\n";
+    if (p->units[i].batch) {
+      o << "batch: stages ";
+      for (size_t k = 0; k < p->units[i].ids.size(); ++k) o << (k ? "," : "") << p->units[i].ids[k];
+      o << " as one tile queue\n";
+    }
     if (p->units[i].blocks_per_cu)
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
   }
@@ -639,33 +664,35 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
   hipStream_t s = p->ctx->stream;
   if (u.batch) {
     bool all_warp = u.warp;
-    for (size_t k = 0; k < u.count && all_warp; ++k) all_warp = p->variant[u.first + k] == 3;
+    for (uint32_t id : u.ids) all_warp = all_warp && p->variant[id] == 3;
     if (all_warp) {
-      ZH_HIP(zh_launch_warp_batch(u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
+      ZH_HIP(zh_launch_warp_batch(p->plan.stages[u.ids[0]].desc.variant, u.d_batch, u.total_tiles,
+                                  (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
       return ZH_OK;
     }
     bool all_staged = true;
-    for (size_t k = 0; k < u.count; ++k) all_staged = all_staged && p->plan.stages[u.first + k].tile_mode == 0;
+    for (uint32_t id : u.ids) all_staged = all_staged && p->plan.stages[id].tile_mode == 0;
     if (all_staged) {
       ZH_HIP(zh_launch_batch(u.d_batch, u.d_counter, u.grid, u.lds, s));
       return ZH_OK;
     }
     // Mixed fallback kernels: one launch per stage.
-    for (size_t k = 0; k < u.count; ++k) {
+    for (uint32_t id : u.ids) {
       Unit one;
-      one.first = u.first + k;
+      one.ids = {id};
       zh_status st = launch_unit(p, one);
       if (st != ZH_OK) return st;
     }
     return ZH_OK;
   }
-  const StagePlan& sp = p->plan.stages[u.first];
-  const StageDesc* d = p->d_stages + u.first;
+  const uint32_t first = u.ids[0];
+  const StagePlan& sp = p->plan.stages[first];
+  const StageDesc* d = p->d_stages + first;
   switch (sp.desc.kind) {
     case STAGE_POINTWISE: {
-      const int v = u.first < p->variant.size() ? p->variant[u.first] : 0;
+      const int v = first < p->variant.size() ? p->variant[first] : 0;
       if (v == 5)
-        ZH_HIP(zh_mlp_infer_stage(p->d_surrogate[u.first], d, (uint64_t)sp.desc.width * sp.desc.height,
+        ZH_HIP(zh_mlp_infer_stage(p->d_surrogate[first], d, (uint64_t)sp.desc.width * sp.desc.height,
                                   (uint32_t)p->ctx->cus, s));
       else if (v == 2)
         ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus, s));
@@ -675,7 +702,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
       break;
     }
     case STAGE_RESAMPLE:
-      if (u.first < p->variant.size() && p->variant[u.first] == 3)
+      if (first < p->variant.size() && p->variant[first] == 3)
         ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus,
                               u.blocks_per_cu, s));
       else
@@ -741,8 +768,7 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
 static const uint32_t kWarpTiles[][2] = {{64, 16}, {32, 32}, {128, 8}, {16, 64}};
 
 static bool warp_unit(const zh_program* p, const Unit& u) {
-  for (size_t k = 0; k < u.count; ++k) {
-    const size_t s = u.first + k;
+  for (uint32_t s : u.ids) {
     if (p->plan.stages[s].desc.kind != STAGE_RESAMPLE || s >= p->variant.size() || p->variant[s] != 3) return false;
   }
   return true;
@@ -752,9 +778,9 @@ static bool warp_unit(const zh_program* p, const Unit& u) {
 // copies and, for a batch, the tile prefix).  The interpreter fallback of a
 // re-tiled stage gathers its taps directly (TILE_DIRECT accepts any tile).
 static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th) {
-  std::vector<uint32_t> prefix(u.count + 1, 0);
-  for (size_t k = 0; k < u.count; ++k) {
-    const size_t s = u.first + k;
+  std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
+  for (size_t k = 0; k < u.ids.size(); ++k) {
+    const size_t s = u.ids[k];
     StagePlan& sp = p->plan.stages[s];
     StageDesc& d = sp.desc;
     if (d.tile_w != tw || d.tile_h != th) {
@@ -777,8 +803,8 @@ static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th)
     prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
   }
   hipStream_t st = p->ctx->stream;
-  ZH_HIP(hipMemcpyAsync(p->d_stages + u.first, p->h_stages + u.first, u.count * sizeof(StageDesc),
-                        hipMemcpyHostToDevice, st));
+  for (uint32_t s : u.ids)
+    ZH_HIP(hipMemcpyAsync(p->d_stages + s, p->h_stages + s, sizeof(StageDesc), hipMemcpyHostToDevice, st));
   if (u.batch) {
     ZH_HIP(hipMemcpyAsync(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     u.total_tiles = prefix.back();
@@ -802,7 +828,7 @@ zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_
   bool changed = false;
   for (Unit& u : p->units) {
     if (!warp_unit(p, u)) continue;
-    const uint32_t tw0 = p->plan.stages[u.first].desc.tile_w, th0 = p->plan.stages[u.first].desc.tile_h;
+    const uint32_t tw0 = p->plan.stages[u.ids[0]].desc.tile_w, th0 = p->plan.stages[u.ids[0]].desc.tile_h;
     float best = 1e30f;
     uint32_t bw = tw0, bh = th0, bb = u.blocks_per_cu;
     static const uint32_t kPerCu[] = {0, 4, 3, 2, 1};

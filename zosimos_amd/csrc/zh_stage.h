@@ -137,7 +137,8 @@ struct BatchDesc {
   uint32_t n_stages;
   uint32_t total_tiles;
   uint32_t* counter;            // reset by a one-lane kernel before each launch
-  const StageDesc* stages;      // n_stages descriptors
+  const StageDesc* stages;      // the program's descriptors
+  const uint32_t* stage_ids;    // n_stages indices into `stages`
   const uint32_t* tile_prefix;  // n_stages + 1 exclusive prefix of tile counts
 };
 

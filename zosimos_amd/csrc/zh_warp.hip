@@ -373,30 +373,19 @@ enum {
   kNumWarps
 };
 
-// Every stage of a batch, whichever registered form it has: the form is
-// uniform per tile, so the switch costs one scalar branch per tile.
-__device__ __forceinline__ void warp_dispatch(int v, CDesc d, uint32_t tx, uint32_t ty, const Luts* L,
-                                              const float* rep) {
-  switch (v) {
-#define ZH_WARP_CASE(name, ...)                                   \
-  case WI_##name: warp_tile<V_##name>(d, tx, ty, L, rep); break; \
-  case WIS_##name: warp_tile<VS_##name>(d, tx, ty, L, rep); break;
-    ZH_WARP_LIST(ZH_WARP_CASE)
-#undef ZH_WARP_CASE
-    default: break;
-  }
-}
-
-// Persistent grid over the tiles of a batch of stages (images of uneven sizes
-// and formats): tile j belongs to the last stage s with prefix[s] <= j.
+// Persistent grid over the tiles of a batch of stages of one form (images of
+// uneven sizes): tile j belongs to the last stage s with prefix[s] <= j.
+template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
   __shared__ Luts luts;
-  __shared__ float rep[kRepDec];
+  __shared__ float rep[rep_words<V>()];
   load_lut(as_const(b->stages), &luts);
   __syncthreads();
-  load_rep(&luts, rep);
-  __syncthreads();
+  if constexpr (rep_words<V>() > 1) {
+    load_rep(&luts, rep);
+    __syncthreads();
+  }
   const uint32_t total = b->total_tiles;
   for (uint32_t j = virtual_block(); j < total; j += gridDim.x) {
     uint32_t lo = 0, hi = b->n_stages;
@@ -404,9 +393,9 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
       const uint32_t mid = (lo + hi) / 2;
       if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
     }
-    const CDesc d = as_const(b->stages + lo);
+    const CDesc d = as_const(b->stages + b->stage_ids[lo]);
     const uint32_t t = j - b->tile_prefix[lo];
-    warp_dispatch(d->variant, d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
+    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
   }
 }
 
@@ -416,11 +405,14 @@ struct WarpEntry {
   const uint16_t* post;
   int n;
   void (*fn)(const StageDesc*);
+  void (*batch)(const BatchDesc*);
 };
 
-#define ZH_WARP_ENTRY(name, src, f, mat, ...)                                                            \
-  {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>},        \
-  {#name "_sep", src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<VS_##name>},
+#define ZH_WARP_ENTRY(name, src, f, mat, ...)                                                          \
+  {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>,         \
+   k_warp_batch<V_##name>},                                                                              \
+  {#name "_sep", src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<VS_##name>, \
+   k_warp_batch<VS_##name>},
 static const WarpEntry kWarps[] = {ZH_WARP_LIST(ZH_WARP_ENTRY)};
 #undef ZH_WARP_ENTRY
 
@@ -457,14 +449,15 @@ hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles,
   return hipGetLastError();
 }
 
-hipError_t zh_launch_warp_batch(const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus, uint32_t per_cu,
-                                hipStream_t stream) {
-  static uint32_t resident = 0;
-  if (!resident) resident = zh::resident_blocks((const void*)zh::spec::k_warp_batch, 1);
-  const uint32_t bpc = per_cu ? std::min(per_cu, resident) : resident;
+hipError_t zh_launch_warp_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus,
+                                uint32_t per_cu, hipStream_t stream) {
+  if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
+  static uint32_t resident[zh::spec::kNumWarps] = {};
+  if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kWarps[index].batch, 1);
+  const uint32_t bpc = per_cu ? std::min(per_cu, resident[index]) : resident[index];
   const uint32_t blocks = std::min(total_tiles, bpc * cus);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(zh::spec::k_warp_batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
+  hipLaunchKernelGGL(zh::spec::kWarps[index].batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
   return hipGetLastError();
 }
 
