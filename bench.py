@@ -28,8 +28,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # A timed region of tens of milliseconds after a warm-up of similar length:
+    # shorter runs measure the GPU's clock ramp rather than the kernels.
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
                     help="copy: the byte-copy calibration workload of the HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -222,7 +224,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.name, "pixels_per_step": wl.pixels, "launches_per_step": launches,
-                   "algorithmic_bytes_per_step": alg_bytes},
+                   "algorithmic_bytes_per_step": alg_bytes, "algorithmic_bytes_per_launch": exe.launch_bytes()},
         "hbm_gbps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
         "hbm_frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -374,7 +374,12 @@ enum {
 };
 
 // Persistent grid over the tiles of a batch of stages of one form (images of
-// uneven sizes): tile j belongs to the last stage s with prefix[s] <= j.
+// uneven sizes): tile j belongs to the last stage s with prefix[s] <= j.  A
+// workgroup takes kChunk consecutive tiles at a time, so one stage lookup and
+// the (scalar-cache warm) descriptor reads of neighbouring tiles of the same
+// image are shared instead of paid per 1024-pixel tile.
+constexpr uint32_t kChunk = 8;
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
@@ -387,15 +392,20 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
     __syncthreads();
   }
   const uint32_t total = b->total_tiles;
-  for (uint32_t j = virtual_block(); j < total; j += gridDim.x) {
+  const uint32_t chunks = (total + kChunk - 1) / kChunk;
+  for (uint32_t c = virtual_block(); c < chunks; c += gridDim.x) {
+    const uint32_t j0 = c * kChunk, j1 = min(j0 + kChunk, total);
     uint32_t lo = 0, hi = b->n_stages;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
-      if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
+      if (b->tile_prefix[mid] <= j0) lo = mid; else hi = mid;
     }
-    const CDesc d = as_const(b->stages + b->stage_ids[lo]);
-    const uint32_t t = j - b->tile_prefix[lo];
-    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
+    for (uint32_t j = j0; j < j1; ++j) {
+      while (b->tile_prefix[lo + 1] <= j) ++lo;  // uniform; the chunk may span stages
+      const CDesc d = as_const(b->stages + b->stage_ids[lo]);
+      const uint32_t t = j - b->tile_prefix[lo];
+      warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
+    }
   }
 }
 
