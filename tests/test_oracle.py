@@ -246,3 +246,35 @@ def test_mlp_forward_layout_and_adam_first_step():
     q, m, v = M.adam_step(p, g, np.zeros_like(p), np.zeros_like(p), 1, 0.01)
     nz = np.abs(g) > 1e-12
     assert np.allclose((q - p)[nz], -0.01 * np.sign(g[nz]), rtol=1e-6)
+
+
+# ---- perceptual hash of the reference's tests (oracle/blockhash.py) --------------
+def _golden_rgba(name):
+    import os
+    from PIL import Image
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".png")
+    return np.asarray(Image.open(here).convert("RGBA"))
+
+
+def test_blockhash_known_answers():
+    """The restated blockhash256 gives the reference's blessed hash of its
+    red/green-swapped background for the unmodified background (a channel
+    swap keeps every r + g + b sum), and all-ones for a uniform bright image
+    (every block equals its band's median and exceeds half the block
+    maximum)."""
+    import json, os
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    bg = _golden_rgba("background")
+    assert bg.shape == (512, 512, 4)
+    assert B.blockhash256(bg) in hashes["swapped"]
+    swapped = bg[..., [1, 0, 2, 3]]
+    assert B.blockhash256(swapped) == B.blockhash256(bg)
+    assert B.blockhash256(np.full((64, 48, 4), 200, np.uint8)) == "f" * 64
+    # transparent pixels count as white; the precise method spreads a
+    # 17-pixel-wide image over 16 block columns without losing weight
+    odd = np.random.default_rng(0).integers(0, 256, (33, 17, 4), dtype=np.uint8)
+    ay, ax = B._axis_weights(33, 16), B._axis_weights(17, 16)
+    assert np.allclose(ay.sum(1), 1) and np.allclose(ax.sum(1), 1) and np.isclose(ay.sum(), 33)
+    assert len(B.blockhash256(odd)) == 64
