@@ -307,7 +307,7 @@ zh_status zh_mlp_gradient(zh_context* ctx, const float* params, uint32_t n, cons
 zh_status zh_mlp_evaluate(zh_context* ctx, const float* params, uint32_t n, const float* x, float* y);
 
 /* Host-only: the 8-bit sRGB tables the kernels stage in LDS (decode values,
- * bucket codes and code thresholds; 6,160 bytes), for inspection and tests.
+ * bucket codes and code thresholds; 6,164 bytes), for inspection and tests.
  * No counterpart in the reference, whose sRGB codec is the texture unit's. */
 zh_status zh_srgb_tables(void* out, size_t len);
 

@@ -154,12 +154,12 @@ def test_launch_without_device_fails_loudly():
 
 def srgb_tables():
     import numpy as np
-    buf = ctypes.create_string_buffer(6160)
-    assert _abi.lib().zh_srgb_tables(buf, 6160) == 0
+    buf = ctypes.create_string_buffer(6164)
+    assert _abi.lib().zh_srgb_tables(buf, 6164) == 0
     raw = np.frombuffer(buf.raw, dtype=np.uint8)
     dec = raw[:1024].view(np.float32)
     thr = raw[1024:1024 + 1040].view(np.float32)
-    idx = raw[2064:2064 + 4096]
+    idx = raw[2064:2064 + 4097]
     return dec, thr, idx
 
 
@@ -167,7 +167,7 @@ def table_encode(x, thr, idx):
     """The kernels' lookup: bucket code, then one threshold comparison."""
     import numpy as np
     c = np.minimum(np.maximum(np.nan_to_num(x.astype(np.float32), nan=0.0), np.float32(0)), np.float32(1))
-    i = np.minimum((c * np.float32(4096)).astype(np.int64), 4095)
+    i = (c * np.float32(4096)).astype(np.int64)  # 0 .. 4096
     k = idx[i].astype(np.int64)
     return (k + (c >= thr[k + 1])).astype(np.uint8)
 

@@ -22,7 +22,7 @@ namespace zh {
 // ---------------------------------------------------------------------------
 // 8-bit sRGB tables, built on the host in FP64 and staged in LDS per block.
 //   dec[k]   linear value of code k (IEC 61966-2-1, rounded once to f32)
-//   idx[i]   code of the linear value i / 4096
+//   idx[i]   code of the linear value i / 4096 (0 <= i <= 4096)
 //   thr[k]   smallest f32 x whose code is >= k (thr[256] = +inf)
 // Adjacent codes are at least 1 / (255 * 12.92) = 3.0e-4 apart in linear
 // light, more than a 1/4096 bucket, so a bucket holds at most one threshold:
@@ -34,9 +34,15 @@ __device__ __forceinline__ float clamp01_(float v) { return fminf(fmaxf(v, 0.0f)
 
 __device__ __forceinline__ uint32_t srgb_encode8(float v, const Luts* L) {
   const float c = clamp01_(v);  // NaN -> 0
-  const int i = min((int)(c * 4096.0f), 4095);
-  const uint32_t k = L->idx[i];
+  const uint32_t k = L->idx[(uint32_t)(c * 4096.0f)];  // bucket 4096 holds 1.0 alone
   return k + (c >= L->thr[k + 1] ? 1u : 0u);
+}
+
+// Round-to-nearest-even 8-bit unorm of v * 255 written into byte `pos` of w
+// (v_cvt_pk_u8_f32: saturates to [0, 255] and maps NaN to 0, so no clamp is
+// needed first; equal to rint(clamp01(v) * 255) for every f32 v).
+__device__ __forceinline__ uint32_t put_unorm8(float v, uint32_t pos, uint32_t w) {
+  return __builtin_amdgcn_cvt_pk_u8_f32(v * 255.0f, pos, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,11 +244,6 @@ __device__ __forceinline__ uint32_t f2h(float v) {
   return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
 }
 
-// Round-to-nearest unorm8 store of a value already clamped to [0, 1].
-__device__ __forceinline__ uint32_t unorm8(float c) {
-  return (uint32_t)__builtin_rintf(c * 255.0f);
-}
-
 // ---------------------------------------------------------------------------
 // whole-texel decode / encode.  A raw texel is up to 16 bytes in a uint4.
 // ---------------------------------------------------------------------------
@@ -308,19 +309,14 @@ __device__ __forceinline__ uint4 encode_texel(const Slot& s, float4 v, const Lut
   const uint32_t bits = s.bits;
   if (bits == 10 && s.native) {
     // clamp01 maps NaN to 0 (fmaxf returns the non-NaN operand).
-    uint32_t r, g, b;
+    const float lo_v = s.parts == 10 ? v.z : v.x, hi_v = s.parts == 10 ? v.x : v.z;
+    uint32_t w;
     if (s.transfer == 5) {
-      r = srgb_encode8(v.x, lut);
-      g = srgb_encode8(v.y, lut);
-      b = srgb_encode8(v.z, lut);
+      w = srgb_encode8(lo_v, lut) | (srgb_encode8(v.y, lut) << 8) | (srgb_encode8(hi_v, lut) << 16);
     } else {
-      r = unorm8(clamp01(v.x));
-      g = unorm8(clamp01(v.y));
-      b = unorm8(clamp01(v.z));
+      w = put_unorm8(hi_v, 2u, put_unorm8(v.y, 1u, put_unorm8(lo_v, 0u, 0u)));
     }
-    const uint32_t a = unorm8(clamp01(v.w));
-    const uint32_t lo = s.parts == 10 ? b : r, hi = s.parts == 10 ? r : b;
-    return make_uint4(lo | (g << 8) | (hi << 16) | (a << 24), 0u, 0u, 0u);
+    return make_uint4(put_unorm8(v.w, 3u, w), 0u, 0u, 0u);
   }
   if (bits == 18 && parts_identity(s.parts) && s.transfer == 4)
     return make_uint4(f2h(v.x) | (f2h(v.y) << 16), f2h(v.z) | (f2h(v.w) << 16), 0u, 0u);

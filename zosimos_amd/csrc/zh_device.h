@@ -28,6 +28,17 @@ __device__ __forceinline__ T ldc(const __attribute__((address_space(4))) T* p) {
   return r;
 }
 
+// Word loads through the scalar unit.  The descriptor tables are uniform and
+// read-only; reading them as dwords from the constant address space keeps
+// them on s_load (a byte field or a generic pointer would become a vector
+// load, a full memory round trip on each tile's critical path).
+typedef const __attribute__((address_space(4))) uint32_t* CWords;
+__device__ __forceinline__ uint32_t cword(const uint32_t* p, uint32_t i) { return ((CWords)(uintptr_t)p)[i]; }
+// Slot operand (the `a` byte) of instruction i of a stage.
+__device__ __forceinline__ uint32_t instr_slot(CDesc d, uint32_t i) {
+  return (((CWords)&d->code[i])[0] >> 8) & 255u;
+}
+
 typedef float4 Px[kPix];
 
 // Image memory is addressed in the global address space (addrspace 1): the
@@ -82,19 +93,13 @@ template <uint32_t F>
 __device__ __forceinline__ uint32_t enc_u8(float4 v, const Luts* L) {
   constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
   constexpr bool srgb = F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA;
-  uint32_t r, g, b;
-  if (srgb) {
-    r = srgb_encode8(v.x, L);
-    g = srgb_encode8(v.y, L);
-    b = srgb_encode8(v.z, L);
-  } else {
-    r = unorm8(clamp01(v.x));
-    g = unorm8(clamp01(v.y));
-    b = unorm8(clamp01(v.z));
-  }
-  const uint32_t a = unorm8(clamp01(v.w));
-  const uint32_t lo = bgra ? b : r, hi = bgra ? r : b;
-  return lo | (g << 8) | (hi << 16) | (a << 24);
+  const float lo = bgra ? v.z : v.x, hi = bgra ? v.x : v.z;
+  uint32_t w;
+  if (srgb)
+    w = srgb_encode8(lo, L) | (srgb_encode8(v.y, L) << 8) | (srgb_encode8(hi, L) << 16);
+  else
+    w = put_unorm8(hi, 2u, put_unorm8(v.y, 1u, put_unorm8(lo, 0u, 0u)));
+  return put_unorm8(v.w, 3u, w);
 }
 
 __device__ __forceinline__ float4 dec_f16(uint32_t lo, uint32_t hi) {

@@ -650,16 +650,16 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* bb) {
     __syncthreads();
     if (threadIdx.x == 0) job = atomicAdd(b->counter, 1u);
     __syncthreads();
-    const uint32_t j = job;
+    const uint32_t j = __builtin_amdgcn_readfirstlane(job);
     if (j >= b->total_tiles) break;
     // Stage of tile j: the last s with prefix[s] <= j (uniform binary search).
     uint32_t lo = 0, hi = b->n_stages;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
-      if (b->tile_prefix[mid] <= j) lo = mid; else hi = mid;
+      if (cword(b->tile_prefix, mid) <= j) lo = mid; else hi = mid;
     }
-    const CDesc d = as_const(b->stages + b->stage_ids[lo]);
-    const uint32_t t = j - b->tile_prefix[lo];
+    const CDesc d = as_const(b->stages + cword(b->stage_ids, lo));
+    const uint32_t t = j - cword(b->tile_prefix, lo);
     process_tile<TILE_STAGED>(d, (int)(t % d->tiles_x), (int)(t / d->tiles_x), tile, lut);
   }
 }

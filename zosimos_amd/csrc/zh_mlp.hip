@@ -343,8 +343,9 @@ __global__ __launch_bounds__(kBlock) void k_infer(const uint16_t* ops, const voi
 // without touching a captured graph.
 __global__ __launch_bounds__(kBlock) void k_infer_stage(const uint16_t* ops, const zh::StageDesc* d) {
   const uint64_t n = (uint64_t)d->width * d->height;
-  const void* src = (const void*)d->slot[d->code[0].a].ptr;
-  void* dst = (void*)d->slot[d->code[d->n_instr - 1].a].ptr;
+  const zh::CDesc c = zh::as_const(d);
+  const void* src = (const void*)c->slot[zh::instr_slot(c, 0)].ptr;
+  void* dst = (void*)c->slot[zh::instr_slot(c, c->n_instr - 1)].ptr;
   infer_body<1, 1>(ops, src, dst, n);
 }
 

@@ -70,7 +70,7 @@ static uint32_t srgb_code(double l) {
 static void srgb_tables(Luts* L) {
   std::memset(L, 0, sizeof(*L));
   for (int k = 0; k < 256; ++k) L->dec[k] = (float)srgb_eotf(k / 255.0);
-  for (int i = 0; i < 4096; ++i) L->idx[i] = (uint8_t)srgb_code(i / 4096.0);
+  for (int i = 0; i <= 4096; ++i) L->idx[i] = (uint8_t)srgb_code(i / 4096.0);
   // thr[k]: smallest non-negative f32 whose code is >= k, by bisection over
   // the ordered bit patterns of non-negative floats.
   for (int k = 1; k < 256; ++k) {

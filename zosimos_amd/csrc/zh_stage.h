@@ -53,12 +53,12 @@ struct Instr {
 };
 
 // 8-bit sRGB tables (built once per context on the host in FP64):
-//   dec[k]  linear value of code k, idx[i] code of i / 4096, thr[k] smallest
+//   dec[k]  linear value of code k, idx[i] code of i / 4096 (i <= 4096), thr[k] smallest
 //   f32 whose code is >= k (thr[256..259] = +inf).
 struct Luts {
   float dec[256];
   float thr[260];
-  uint8_t idx[4096];
+  uint8_t idx[4100];  // 4097 buckets (i / 4096 for i <= 4096), padded to a word
 };
 
 struct Slot {

@@ -68,11 +68,11 @@ template <uint32_t SRC, uint32_t F, bool MAT>
 __device__ __forceinline__ LayerArgs layer_args(CDesc d, const Layer& L) {
   LayerArgs a;
   const uint32_t pb = L.prog_begin;
-  const uint32_t s0 = d->code[pb].a;
+  const uint32_t s0 = instr_slot(d, pb);
   a.src0 = (const gbyte*)d->slot[s0].ptr;
   a.stride = (uint32_t)d->slot[s0].row_stride;
   a.src1 = a.src0;
-  if constexpr (SRC == WSRC_OVER2) a.src1 = (const gbyte*)d->slot[d->code[pb + 2].a].ptr;
+  if constexpr (SRC == WSRC_OVER2) a.src1 = (const gbyte*)d->slot[instr_slot(d, pb + 2)].ptr;
   if constexpr (MAT) {
     const uint32_t q = d->code[pb + 1].arg;
 #pragma unroll
@@ -319,7 +319,7 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
   // Post program on the composite; its last element stores the lane's pixels.
   Uni<Post> u;
   resolve<Post>(d, u, d->post_begin);
-  const uint32_t so = d->code[d->post_end - 1].a;
+  const uint32_t so = instr_slot(d, d->post_end - 1);
   const uint64_t stride_tx = d->slot[so].row_stride / fmt_bpp(el_fmt(Post::el[Post::n - 1]));
   const uint64_t xi = (uint64_t)y * stride_tx + (uint64_t)x0;
   Px Y, Z;
@@ -374,12 +374,12 @@ enum {
 };
 
 // Persistent grid over the tiles of a batch of stages of one form (images of
-// uneven sizes): tile j belongs to the last stage s with prefix[s] <= j.  A
-// workgroup takes kChunk consecutive tiles at a time, so one stage lookup and
-// the (scalar-cache warm) descriptor reads of neighbouring tiles of the same
-// image are shared instead of paid per 1024-pixel tile.
-constexpr uint32_t kChunk = 8;
-
+// uneven sizes): tile j belongs to the last stage s with prefix[s] <= j.  The
+// tiles are dealt exactly as one stage's tiles are (grid-stride, XCD-aware),
+// so at any moment the grid works on one contiguous band of tiles and their
+// source rows stay in the L2 and the TLBs.  A workgroup's tiles only move
+// forward, so after one binary search its stage is found by walking the
+// prefix, less than one scalar load per tile on average.
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
@@ -392,20 +392,18 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
     __syncthreads();
   }
   const uint32_t total = b->total_tiles;
-  const uint32_t chunks = (total + kChunk - 1) / kChunk;
-  for (uint32_t c = virtual_block(); c < chunks; c += gridDim.x) {
-    const uint32_t j0 = c * kChunk, j1 = min(j0 + kChunk, total);
-    uint32_t lo = 0, hi = b->n_stages;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (b->tile_prefix[mid] <= j0) lo = mid; else hi = mid;
-    }
-    for (uint32_t j = j0; j < j1; ++j) {
-      while (b->tile_prefix[lo + 1] <= j) ++lo;  // uniform; the chunk may span stages
-      const CDesc d = as_const(b->stages + b->stage_ids[lo]);
-      const uint32_t t = j - b->tile_prefix[lo];
-      warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
-    }
+  const uint32_t j0 = virtual_block();
+  if (j0 >= total) return;
+  uint32_t lo = 0, hi = b->n_stages;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (cword(b->tile_prefix, mid) <= j0) lo = mid; else hi = mid;
+  }
+  for (uint32_t j = j0; j < total; j += gridDim.x) {
+    while (cword(b->tile_prefix, lo + 1) <= j) ++lo;  // uniform
+    const CDesc d = as_const(b->stages + cword(b->stage_ids, lo));
+    const uint32_t t = j - cword(b->tile_prefix, lo);
+    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
   }
 }
 
