@@ -252,6 +252,58 @@ def test_c4_chain_small(ctx):
         assert mr <= 1 and ma == 0 and exact > 0.99
 
 
+def test_pointwise_batch_ragged(ctx):
+    """Independent images of one pointwise signature run as one launch of the
+    batched kernel, in chunks of 16,384 pixels: sizes below, at and above a
+    chunk, with partial groups of 4 pixels.  The same executable then runs
+    with one image at an address that is not 16-byte aligned (each stage
+    launches on its own), and aligned again (bit-identical to the first)."""
+    sizes = [(1, 1), (3, 1), (7, 5), (128, 128), (129, 127), (300, 61), (16384, 1), (16385, 2)]
+    cmd = z.CommandBuffer()
+    ins, outs, regs = {}, {}, []
+    for w, h in sizes:
+        d = z.Descriptor.rgba8_srgb(w, h)
+        r = cmd.input(d)
+        o, od = cmd.output(W.color_chain(cmd, r))
+        ins[r], outs[o] = d, od
+        regs.append((r, o, w, h))
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    assert "as one launch" in exe.describe() and exe.stats()[0] == 1
+    host = {r: rand_u8((h, w * 4), 100 + k) for k, (r, o, w, h) in enumerate(regs)}
+    imgs = {}
+    for r, d in ins.items():
+        imgs[r] = ctx.image(d)
+        imgs[r].upload(host[r])
+    out_imgs = {o: ctx.image(d) for o, d in outs.items()}
+
+    def check(tag):
+        for r, o, w, h in regs:
+            _, want = PL.c4(PL.u8_image(host[r], w, h))
+            mr, ma, _ = compare_u8(f"{tag}_{w}x{h}", out_imgs[o].download(), want)
+            assert mr <= 1 and ma == 0
+
+    exe.launch(imgs, out_imgs)
+    ctx.sync()
+    check("pointwise_batch")
+    first = {o: img.download() for o, img in out_imgs.items()}
+
+    r0, _, w0, h0 = regs[4]
+    big = ctx.image(z.Descriptor.rgba8_srgb(w0 * h0 + 1, 1))
+    mis = ctx.wrap(ins[r0], big.device_ptr + 4, w0 * h0 * 4)
+    mis.upload(host[r0])
+    for img in out_imgs.values():
+        img.upload(np.zeros_like(img.download()))
+    exe.launch({**imgs, r0: mis}, out_imgs)
+    ctx.sync()
+    check("pointwise_batch_unaligned")
+    exe.launch(imgs, out_imgs)
+    ctx.sync()
+    for o, img in out_imgs.items():
+        assert np.array_equal(img.download(), first[o])
+    mis.free()
+    big.free()
+
+
 def test_linear_f32_output_within_1e5(ctx):
     """A linear RGBA32F chain (P3 conversion, Oklab round trip with chroma and
     hue) against FP64: <= 1e-5 relative (plus 1e-6 absolute near zero)."""

@@ -95,6 +95,7 @@ struct zh_image {
 struct Unit {
   bool batch = false;
   bool warp = false;             // batch of one warp form (k_warp_batch<form>)
+  bool spec = false;             // batch of one pointwise signature (k_spec_batch<signature>)
   std::vector<uint32_t> ids;     // stages of the unit (one unless batched)
   uint32_t* d_ids = nullptr;     // device copy of ids (batches)
   zh::BatchDesc* d_batch = nullptr;  // device

@@ -24,6 +24,8 @@ hipError_t zh_launch_pointwise(const zh::StageDesc* d_dev, uint32_t width, uint3
                                uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width, uint32_t height,
                           uint32_t cus, hipStream_t stream);
+hipError_t zh_launch_spec_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_chunks, uint32_t cus,
+                                hipStream_t stream);
 hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
                           hipStream_t stream);
 hipError_t zh_launch_warp_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus,
@@ -310,7 +312,10 @@ static void free_device(zh_program* p) {
 static void make_units(zh_program* p) {
   p->units.clear();
   const auto& st = p->plan.stages;
+  // 1: warp stages (partitioned by form), 2: LDS-staged resample stages,
+  // 3: flat pointwise stages with a specialised kernel (by signature).
   auto family = [&](const StagePlan& sp) {
+    if (sp.desc.kind == STAGE_POINTWISE) return sp.spec >= 0 ? 3 : 0;
     if (sp.desc.kind != STAGE_RESAMPLE) return 0;
     if (sp.desc.variant >= 0) return 1;
     return sp.tile_mode == 0 ? 2 : 0;
@@ -337,7 +342,7 @@ static void make_units(zh_program* p) {
     // Partition [i, j): by warp form, or as one group.
     std::vector<std::pair<int, std::vector<uint32_t>>> groups;
     for (size_t k = i; k < j; ++k) {
-      const int key = fam == 1 ? st[k].desc.variant : -1;
+      const int key = fam == 1 ? st[k].desc.variant : fam == 3 ? st[k].spec : -1;
       auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
       if (it == groups.end()) groups.push_back({key, {(uint32_t)k}});
       else it->second.push_back((uint32_t)k);
@@ -347,6 +352,7 @@ static void make_units(zh_program* p) {
       u.ids = g.second;
       u.batch = fam && u.ids.size() >= 2;
       u.warp = u.batch && fam == 1;
+      u.spec = u.batch && fam == 3;
       p->units.push_back(u);
     }
     i = j;
@@ -372,7 +378,8 @@ static zh_status alloc_device(zh_program* p) {
     std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
     for (size_t k = 0; k < u.ids.size(); ++k) {
       const StageDesc& d = p->plan.stages[u.ids[k]].desc;
-      prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
+      const uint64_t chunks = ((uint64_t)d.width * d.height + kSpecChunkPixels - 1) / kSpecChunkPixels;
+      prefix[k + 1] = prefix[k] + (u.spec ? (uint32_t)chunks : d.tiles_x * d.tiles_y);
       u.lds = std::max(u.lds, p->plan.stages[u.ids[k]].lds_bytes);
     }
     ZH_HIP(hipMalloc(&u.d_batch, sizeof(BatchDesc)));
@@ -463,7 +470,7 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
     if (p->units[i].batch) {
       o << "batch: stages ";
       for (size_t k = 0; k < p->units[i].ids.size(); ++k) o << (k ? "," : "") << p->units[i].ids[k];
-      o << " as one tile queue\n";
+      o << (p->units[i].spec ? " as one launch\n" : " as one tile queue\n");
     }
     if (p->units[i].blocks_per_cu)
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
@@ -663,6 +670,12 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
 static zh_status launch_unit(zh_program* p, const Unit& u) {
   hipStream_t s = p->ctx->stream;
   if (u.batch) {
+    bool all_spec = u.spec;
+    for (uint32_t id : u.ids) all_spec = all_spec && p->variant[id] == 2;
+    if (all_spec) {
+      ZH_HIP(zh_launch_spec_batch(p->plan.stages[u.ids[0]].spec, u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, s));
+      return ZH_OK;
+    }
     bool all_warp = u.warp;
     for (uint32_t id : u.ids) all_warp = all_warp && p->variant[id] == 3;
     if (all_warp) {
@@ -670,7 +683,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
                                   (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
       return ZH_OK;
     }
-    bool all_staged = true;
+    bool all_staged = !u.spec;  // pointwise stages have no staged form
     for (uint32_t id : u.ids) all_staged = all_staged && p->plan.stages[id].tile_mode == 0;
     if (all_staged) {
       ZH_HIP(zh_launch_batch(u.d_batch, u.d_counter, u.grid, u.lds, s));

@@ -25,6 +25,10 @@ constexpr int kMaxInstr = 64;
 constexpr int kMaxParams = 256;  // floats
 constexpr int kMaxLayers = 8;
 constexpr int kBlock = 256;      // threads per workgroup
+// A batch of flat pointwise stages is dealt in chunks of kSpecChunkGroups
+// groups of kPix pixels per lane of one workgroup (16,384 pixels).
+constexpr uint32_t kSpecChunkGroups = 16;
+constexpr uint64_t kSpecChunkPixels = (uint64_t)kBlock * kPix * kSpecChunkGroups;
 // LDS bytes available to one resample tile (float4 texels).
 constexpr int kTileLdsBytes = 64 * 1024;
 
