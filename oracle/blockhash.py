@@ -95,3 +95,15 @@ def _axis_weights(n: int, bits: int):
 
 def blockhash256(rgba_u8) -> str:
     return blockhash(rgba_u8, 16)
+
+
+def lumaa16_to_rgba8(raw_bytes, width: int, height: int):
+    """An (H, W, 4) RGBA8 view of LumaA16 texels (little-endian u16 luma,
+    u16 alpha), as the reference's test hashes a LumaA16 image
+    (``blend.rs:340-375``): grey replicated to r, g, b, each 16-bit channel
+    rounded to 8 bits, round(v * 255 / 65535).  The ``image`` crate (0.25) that
+    converts it is un-vendored; the rule is pinned by the reference's blessed
+    ``transmute`` hash (``test_oracle.py``)."""
+    v = np.frombuffer(np.ascontiguousarray(raw_bytes).tobytes(), dtype="<u2").reshape(height, width, 2)
+    c = ((v.astype(np.uint32) * 255 + 32767) // 65535).astype(np.uint8)
+    return np.stack([c[..., 0], c[..., 0], c[..., 0], c[..., 1]], -1)

@@ -6,8 +6,9 @@ with a blessed line (``tests/util.rs:14-60``).  Same inputs (the reference's
 by ``make_reference_fixtures.py``), same programs, same check.
 
 Covered: ``run_blending`` (inscribe, ``composed``), ``run_affine`` (nearest
-affine rotated 45 degrees about the centres, ``affine``) and
-``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``).
+affine rotated 45 degrees about the centres, ``affine``),
+``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``) and
+``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``).
 """
 import json
 import math
@@ -83,3 +84,18 @@ def test_convert_bt709_matches_reference_hash(ctx):
     i = cmd.input(bt)
     o, od = cmd.output(cmd.color_convert(i, srgb.color, srgb.texel))
     check("convert_bt709", run_program(ctx, cmd, {i: (bt, bg)}, o, od))
+
+
+def test_transmute_matches_reference_hash(ctx):
+    """blend.rs:340-375: the transmuted bytes equal the input, and the LumaA16
+    image hashes to the blessed line."""
+    bg = rgba("background")
+    h, w = bg.shape[:2]
+    src = z.Descriptor.rgba8_srgb(w, h)
+    dst = z.Descriptor(w, h, z.Texel(z.SampleBits.UInt16x2, z.SampleParts.LumaA), z.Color.SRGB)
+    cmd = z.CommandBuffer()
+    i = cmd.input(src)
+    o, od = cmd.output(cmd.transmute(i, dst))
+    out = run_program(ctx, cmd, {i: (src, bg)}, o, od)
+    assert np.array_equal(out.reshape(-1), bg.reshape(-1))
+    check("transmute", B.lumaa16_to_rgba8(out, w, h))

@@ -278,3 +278,19 @@ def test_blockhash_known_answers():
     ay, ax = B._axis_weights(33, 16), B._axis_weights(17, 16)
     assert np.allclose(ay.sum(1), 1) and np.allclose(ax.sum(1), 1) and np.isclose(ay.sum(), 33)
     assert len(B.blockhash256(odd)) == 64
+
+
+def test_blockhash_of_lumaa16_matches_transmute_hash():
+    """The reference transmutes its RGBA8 background to LumaA16 (a byte copy,
+    ``blend.rs:340-375``) and hashes the LumaA16 image: the background's
+    bytes read as LumaA16, rounded to 8 bits per channel, give the blessed
+    ``transmute`` hash; truncating instead of rounding does not."""
+    import json, os
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    bg = _golden_rgba("background")
+    assert B.blockhash256(B.lumaa16_to_rgba8(bg, 512, 512)) in hashes["transmute"]
+    v = np.frombuffer(bg.tobytes(), dtype="<u2").reshape(512, 512, 2)
+    t = (v // 257).astype(np.uint8)
+    assert B.blockhash256(np.stack([t[..., 0]] * 3 + [t[..., 1]], -1)) not in hashes["transmute"]

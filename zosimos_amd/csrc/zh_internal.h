@@ -104,6 +104,7 @@ struct Unit {
   uint32_t grid = 0, lds = 0;
   uint32_t total_tiles = 0;
   uint32_t blocks_per_cu = 0;    // persistent warp grids: 0 = as many as are resident
+  uint32_t strip_w = 0;          // warp units: tiles dealt in strips this many tiles wide (0: by rows)
 };
 
 struct zh_program {

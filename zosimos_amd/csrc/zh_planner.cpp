@@ -698,7 +698,7 @@ static int warp_variant(const StageDesc& d) {
       if (c[i].op == OP_LOAD) {
         const Slot& sl = d.slot[c[i].a];
         if (sl.width != L.src_w || sl.height != L.src_h || sl.row_stride != d.slot[c[0].a].row_stride ||
-            sl.row_stride * sl.height >= (1ull << 32) || sl.row_stride >= (1u << 24) || sl.height >= (1u << 24))
+            sl.row_stride * sl.height >= (1ull << 32) || sl.row_stride >= (1u << 23) || sl.height >= (1u << 23))
           return -1;
       }
     if (src == ~0u) {

@@ -474,6 +474,8 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
     }
     if (p->units[i].blocks_per_cu)
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
+    if (p->units[i].strip_w)
+      o << "launch " << i << ": tiles in strips " << p->units[i].strip_w << " tiles wide\n";
   }
   for (size_t i = 0; i < p->plan.intermediates.size(); ++i) {
     const zh_descriptor& d = p->plan.intermediates[i];
@@ -828,6 +830,21 @@ static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th)
   return ZH_OK;
 }
 
+// Deal every stage's tiles of a warp unit in vertical strips sw tiles wide
+// (0: by rows).
+static zh_status set_warp_strip(zh_program* p, Unit& u, uint32_t sw) {
+  hipStream_t st = p->ctx->stream;
+  for (uint32_t s : u.ids) {
+    p->plan.stages[s].desc.strip_w = sw;
+    p->h_stages[s].strip_w = sw;
+    ZH_HIP(hipMemcpyAsync((char*)(p->d_stages + s) + offsetof(StageDesc, strip_w), &p->h_stages[s].strip_w,
+                          sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  }
+  ZH_HIP(hipStreamSynchronize(st));
+  u.strip_w = sw;
+  return ZH_OK;
+}
+
 zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_t num_bindings, uint32_t iterations) {
   if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
   if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
@@ -872,6 +889,27 @@ zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_
     if (st == ZH_OK) st = set_warp_tile(p, u, bw, bh);
     if (st != ZH_OK) break;
     changed = changed || bw != tw0 || bh != th0;
+    // Then the dealing order for that tile and grid.
+    static const uint32_t kStrips[] = {0, 2, 4, 8, 16};
+    uint32_t bs = u.strip_w;
+    for (uint32_t sw : kStrips) {
+      st = set_warp_strip(p, u, sw);
+      if (st == ZH_OK) st = launch_unit(p, u);  // warm-up
+      if (st != ZH_OK) break;
+      hipEventRecord(e0, p->ctx->stream);
+      for (uint32_t i = 0; i < iterations && st == ZH_OK; ++i) st = launch_unit(p, u);
+      hipEventRecord(e1, p->ctx->stream);
+      if (st != ZH_OK) break;
+      hipEventSynchronize(e1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bs = sw;
+      }
+    }
+    if (st == ZH_OK) st = set_warp_strip(p, u, bs);
+    if (st != ZH_OK) break;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);

@@ -124,7 +124,7 @@ struct StageDesc {
   uint32_t band_h;            // resample: rows per LDS fill (divides tile_h)
   int32_t pre_slot[2];        // pointwise (flat): slots whose raw texels are prefetched, -1 = none
   int32_t variant;            // resample: warp kernel index (zh_spec.h), -1 = none
-  uint32_t pad0;
+  uint32_t strip_w;           // warp kernels: tiles dealt in vertical strips strip_w tiles wide (0: by rows)
   // copy stage
   uint64_t copy_src, copy_dst, copy_src_stride, copy_dst_stride, copy_row_bytes;
   uint32_t copy_rows, copy_pad;

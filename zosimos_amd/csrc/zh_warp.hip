@@ -15,10 +15,12 @@
 //     8K); float outputs map every pixel in FP64, since a bilinear weight
 //     error of ~4e-6 reaches a few f16 ulps after un-premultiplying a small
 //     alpha;
-//   * the four taps are read straight from global memory (the L1/L2 caches
-//     serve the overlap between neighbouring pixels), decoded in-line,
-//     premultiplied and weighted; taps outside the source get weight 0
-//     (transparent border) and read a clamped in-bounds texel;
+//   * the four taps are read straight from memory (the L1/L2 caches serve
+//     the overlap between neighbouring pixels) through buffer loads whose
+//     range is the source image, decoded in-line, premultiplied and
+//     weighted; taps outside the source get weight 0 (transparent border),
+//     and a tap offset past the end of the source reads 0 in hardware, so no
+//     coordinate is clamped;
 //   * a 3x3 matrix in the source program is linear, so it is applied once to
 //     the interpolated premultiplied value rather than to every tap.
 #include <hip/hip_runtime.h>
@@ -53,26 +55,64 @@ __device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const float*
 __device__ __forceinline__ float4 premul4(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
 
 // Uniform data of one layer, read per tile.  Both sources of WSRC_OVER2 have
-// the same row stride (the planner checks), so one offset serves both.  Row
-// offsets use 24-bit multiplies: the planner admits only sources under 4 GiB
-// with fewer than 2^24 rows of fewer than 2^24 bytes.
+// the same row stride and size (the planner checks), so one offset serves
+// both.  Each source is a buffer resource whose range is its rows (stride x
+// height bytes): a load at an offset past the range returns 0 without a
+// memory access.  Row offsets use signed 24-bit multiplies: the planner
+// admits only sources under 4 GiB with fewer than 2^23 rows of fewer than
+// 2^23 bytes, so the offset of every row from -1 to the last is exact, and
+// the next row's offset is the row's plus the stride modulo 2^32 (row -1's
+// is negative: out of range).  Offsets of taps farther outside the source
+// (weight 0) may wrap and land anywhere, in range or not.
 struct LayerArgs {
-  const gbyte* src0;
-  const gbyte* src1;  // WSRC_OVER2: the upper source
+  __amdgpu_buffer_rsrc_t src0;
+  __amdgpu_buffer_rsrc_t src1;  // WSRC_OVER2: the upper source
   uint32_t stride;
   int sw, sh;
   float m[9];  // WSRC_ONE with mat3
 };
+
+// Fourth word of a raw buffer resource on gfx9-family GPUs (32-bit data
+// format; offsets in bytes, range check against the byte count).
+constexpr int kBufferFlags = 0x00020000;
+
+// A tap offset kept in its own register.  Left visible, the constant step
+// between taps would be folded into the load's immediate offset; a tap at
+// texel column 0 would then be addressed as a negative register offset (the
+// column -1 tap's) plus the step, and whether the hardware's range check
+// sees that sum wrapped to 32 bits is not something to rely on.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <class R>
+__device__ __forceinline__ R bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ uint32_t bload<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+template <>
+__device__ __forceinline__ u32x2 bload<u32x2>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+}
+template <>
+__device__ __forceinline__ u32x4 bload<u32x4>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
 
 template <uint32_t SRC, uint32_t F, bool MAT>
 __device__ __forceinline__ LayerArgs layer_args(CDesc d, const Layer& L) {
   LayerArgs a;
   const uint32_t pb = L.prog_begin;
   const uint32_t s0 = instr_slot(d, pb);
-  a.src0 = (const gbyte*)d->slot[s0].ptr;
   a.stride = (uint32_t)d->slot[s0].row_stride;
+  const int bytes = (int)(a.stride * L.src_h);  // < 4 GiB: the 32-bit range field
+  a.src0 = __builtin_amdgcn_make_buffer_rsrc((void*)d->slot[s0].ptr, (short)0, bytes, kBufferFlags);
   a.src1 = a.src0;
-  if constexpr (SRC == WSRC_OVER2) a.src1 = (const gbyte*)d->slot[instr_slot(d, pb + 2)].ptr;
+  if constexpr (SRC == WSRC_OVER2)
+    a.src1 = __builtin_amdgcn_make_buffer_rsrc((void*)d->slot[instr_slot(d, pb + 2)].ptr, (short)0, bytes,
+                                               kBufferFlags);
   if constexpr (MAT) {
     const uint32_t q = d->code[pb + 1].arg;
 #pragma unroll
@@ -119,8 +159,9 @@ struct LaneMap {
 };
 
 // Composite one layer's bilinear samples over the lane's kPix pixels.
-// Taps of all pixels are addressed first (clamped in-bounds offsets, weight 0
-// for taps outside the source or missing pixels), then loaded, then blended:
+// Taps of all pixels are addressed first (weight 0 for taps outside the
+// source or missing pixels, whose loads read 0 or any in-range texel), then
+// loaded, then blended:
 // 4 * kPix loads in flight per lane.  Straight-alpha sources are weighted by
 // w * alpha (premultiplying inside the weighted sum).
 template <class V, bool SEP, bool P64>
@@ -150,8 +191,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
     ybs = (unsigned)(j + 1) < (unsigned)a.sh;
     wy0s = yas ? 1.0f - ay : 0.0f;
     wy1s = ybs ? ay : 0.0f;
-    r0s = __umul24((uint32_t)min(max(j, 0), a.sh - 1), a.stride);
-    r1s = __umul24((uint32_t)min(max(j + 1, 0), a.sh - 1), a.stride);
+    r0s = (uint32_t)__mul24(j, (int)a.stride);
+    r1s = r0s + a.stride;
   }
 #pragma unroll
   for (int k = 0; k < kPix; ++k) {
@@ -198,8 +239,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
       yb = (unsigned)(j + 1) < (unsigned)a.sh;
       wy0 = ya ? 1.0f - ay : 0.0f;
       wy1 = yb ? ay : 0.0f;
-      r0 = __umul24((uint32_t)min(max(j, 0), a.sh - 1), a.stride);
-      r1 = __umul24((uint32_t)min(max(j + 1, 0), a.sh - 1), a.stride);
+      r0 = (uint32_t)__mul24(j, (int)a.stride);
+      r1 = r0 + a.stride;
     }
     const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
     w[k][0] = wx0 * wy0;
@@ -207,11 +248,11 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
     w[k][2] = wx0 * wy1;
     w[k][3] = wx1 * wy1;
     any = any || ((xa || xb) && (ya || yb));
-    const uint32_t i0 = (uint32_t)min(max(i, 0), a.sw - 1) * B, i1 = (uint32_t)min(max(i + 1, 0), a.sw - 1) * B;
-    off[k][0] = r0 + i0;
-    off[k][1] = r0 + i1;
-    off[k][2] = r1 + i0;
-    off[k][3] = r1 + i1;
+    const uint32_t c0 = (uint32_t)i * B;
+    off[k][0] = opaque(r0 + c0);
+    off[k][1] = opaque(off[k][0] + B);
+    off[k][2] = opaque(r1 + c0);
+    off[k][3] = opaque(off[k][2] + B);
   }
   if (!any) return;
   R t0[kPix][4], t1[kPix][4];
@@ -219,8 +260,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
   for (int k = 0; k < kPix; ++k)
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      t0[k][t] = gld<R>(a.src0 + off[k][t]);
-      if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = gld<R>(a.src1 + off[k][t]);
+      t0[k][t] = bload<R>(a.src0, off[k][t]);
+      if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = bload<R>(a.src1, off[k][t]);
     }
   // Keep every load above this point: left free, the scheduler may trade
   // the loads in flight for registers and wait on each tap in turn.
@@ -335,6 +376,25 @@ __device__ __forceinline__ uint32_t virtual_block() {
   return (b % 8u) * (g / 8u) + b / 8u;
 }
 
+// Tile t of a stage in dealing order -> (tx, ty).  With strip_w > 0 the tiles
+// are taken in vertical strips strip_w tiles wide, by rows within a strip, so
+// a run of consecutive tiles (what one XCD works on at once) is a compact
+// block: a rotated or scaled source footprint with a shorter perimeter, fewer
+// texels fetched twice by neighbouring XCDs, and fewer partly used lines.
+__device__ __forceinline__ void tile_xy(CDesc d, uint32_t t, uint32_t& tx, uint32_t& ty) {
+  const uint32_t nx = d->tiles_x, sw = d->strip_w;
+  if (sw == 0 || sw >= nx) {
+    tx = t % nx;
+    ty = t / nx;
+    return;
+  }
+  const uint32_t per = sw * d->tiles_y;
+  const uint32_t s = t / per, r = t - s * per;
+  const uint32_t w = min(sw, nx - s * sw);
+  ty = r / w;
+  tx = s * sw + (r - ty * w);
+}
+
 // Replicated decode table only for 8-bit sRGB sources.
 template <class V>
 constexpr int rep_words() {
@@ -353,8 +413,11 @@ __global__ __launch_bounds__(kBlock) void k_warp(const StageDesc* dd) {
     __syncthreads();
   }
   const uint32_t tiles = d->tiles_x * d->tiles_y;
-  for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x)
-    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
+  for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x) {
+    uint32_t tx, ty;
+    tile_xy(d, t, tx, ty);
+    warp_tile<V>(d, tx, ty, &luts, rep);
+  }
 }
 
 // ---- registry ------------------------------------------------------------------
@@ -402,8 +465,9 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   for (uint32_t j = j0; j < total; j += gridDim.x) {
     while (cword(b->tile_prefix, lo + 1) <= j) ++lo;  // uniform
     const CDesc d = as_const(b->stages + cword(b->stage_ids, lo));
-    const uint32_t t = j - cword(b->tile_prefix, lo);
-    warp_tile<V>(d, t % d->tiles_x, t / d->tiles_x, &luts, rep);
+    uint32_t tx, ty;
+    tile_xy(d, j - cword(b->tile_prefix, lo), tx, ty);
+    warp_tile<V>(d, tx, ty, &luts, rep);
   }
 }
 
