@@ -54,6 +54,11 @@ zh_status zs_command_blend(zs_command* cmd, uint32_t below, zh_rect rect, uint32
                            uint32_t* reg);
 /* CommandBuffer::solid_rgba, command.rs:1524-1551 */
 zh_status zs_command_solid_rgba(zs_command* cmd, const zh_descriptor* desc, const float color[4], uint32_t* reg);
+/* CommandBuffer::bilinear, command.rs:1615-1633: a generated image, per pixel
+ * mix(u_min, u_max, u) + mix(v_min, v_max, v) + mix(uv_min, uv_max, u v) with
+ * (u, v) the pixel centre over the image size (bilinear.frag).  params: the
+ * six vec4 u_min, u_max, v_min, v_max, uv_min, uv_max (shaders/bilinear.rs). */
+zh_status zs_command_bilinear(zs_command* cmd, const zh_descriptor* desc, const float params[24], uint32_t* reg);
 /* Oklab chroma scale and hue rotation (radians) on an Oklab register; no
  * counterpart in the reference. */
 zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg);

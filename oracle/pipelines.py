@@ -179,3 +179,40 @@ def c5(layers_f16, matrices, out_hw):
         acc = C.over(C.bilinear_affine(C.premultiply(lay), mat_f32(m), out_hw), acc)
     res = chain(C.unpremultiply(acc))
     return res, C.encode_u8(res, 5)
+
+
+def bilinear_grid(w, h, params):
+    """``bilinear.frag`` over a w x h image (``command.rs:1615-1633``): at the
+    pixel centre (u, v) = ((x + 0.5) / w, (y + 0.5) / h), per channel
+    ``mix(u_min, u_max, u) + mix(v_min, v_max, v) + mix(uv_min, uv_max, u v)``
+    with the vec4 parameters in ``shaders/bilinear.rs`` order (u_min, u_max,
+    v_min, v_max, uv_min, uv_max).  FP64, shape (h, w, 4)."""
+    p = np.asarray(params, dtype=np.float64).reshape(6, 4)
+    u = ((np.arange(w) + 0.5) / w)[None, :, None]
+    v = ((np.arange(h) + 0.5) / h)[:, None, None]
+    uv = u * v
+
+    def mix(a, b, t):  # GLSL mix, as a + (b - a) t (exact where a = b)
+        return a + (b - a) * t
+
+    return mix(p[0], p[1], u) + mix(p[2], p[3], v) + mix(p[4], p[5], uv)
+
+
+def staged_u8_store(values):
+    """A non-native 8-bit texel store (``stage.frag`` encode: clamp to [0, 1],
+    then the truncating ``uint(c * 255)`` of ``mux_uint``, ``stage.frag:
+    533-641``), e.g. RGBA8 with a Scalars colour."""
+    v = np.nan_to_num(np.asarray(values, dtype=np.float64), nan=0.0)
+    return np.floor(np.clip(v, 0.0, 1.0) * 255.0).astype(np.uint8)
+
+
+def oklab_lch8_to_srgb(lch_u8):
+    """8-bit LchA Oklab texels (``program.rs:873-899``: unorm, then the LabLch
+    transfer, polar to Lab, ``stage.frag:415-425``) converted to sRGB
+    (``oklab.frag:50-64`` with its clamp, Rec.709 primaries, D65).  Returns
+    (linear RGBA, 8-bit sRGB RGBA)."""
+    lab = C.decode_u8(lch_u8, 0x100)
+    out = lab.copy()
+    out[..., :3] = C.oklab_to_rgb(lab[..., :3], primaries=3, clamp=True)
+    return out, C.encode_u8(out, 5)
+

@@ -304,6 +304,38 @@ def test_pointwise_batch_ragged(ctx):
     big.free()
 
 
+def test_bilinear_generator(ctx):
+    """command.rs:1615-1633 / bilinear.frag at the pixel centres against the
+    FP64 oracle: RGBA32F linear within 1e-5 relative (plus 1e-6), and a
+    staged RGBA8 store (Scalars colour: truncating, stage.frag mux) equal to
+    the oracle's code, or one off where the value sits within rounding of a
+    code edge."""
+    w, h = 37, 23
+    params = np.random.default_rng(21).uniform(-0.5, 1.0, 24)
+    b = z.Bilinear(*(tuple(params[4 * k:4 * k + 4]) for k in range(6)))
+    want = PL.bilinear_grid(w, h, params)
+    cmd = z.CommandBuffer()
+    f32 = z.Descriptor.rgba32f_linear(w, h)
+    o, od = cmd.output(cmd.bilinear(f32, b))
+    out, exe = run(ctx, cmd, {}, {}, {o: od})
+    assert "grid" in exe.describe()
+    got = PL.f32_image(out[o], w, h)
+    err = np.abs(got - want)
+    report("bilinear_f32", max_abs=f"{err.max():.2e}")
+    assert np.all(err <= 1e-5 * np.abs(want) + 1e-6)
+
+    scal = z.Descriptor.rgba8_srgb(w, h).with_(color=z.Color(model=z.ColorModel.Scalars, transfer=z.Transfer.Linear))
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.bilinear(scal, b))
+    out, _ = run(ctx, cmd, {}, {}, {o: od})
+    got8 = PL.u8_image(out[o], w, h).astype(np.int32)
+    want8 = PL.staged_u8_store(want).astype(np.int32)
+    d = want8 - got8
+    edge = np.abs(np.clip(want, 0, 1) * 255 - np.rint(np.clip(want, 0, 1) * 255)) < 1e-3
+    report("bilinear_u8_staged", exact=f"{float(np.mean(d == 0)):.5f}", max_diff=int(np.abs(d).max()))
+    assert np.all((d == 0) | ((np.abs(d) == 1) & edge))
+
+
 def test_linear_f32_output_within_1e5(ctx):
     """A linear RGBA32F chain (P3 conversion, Oklab round trip with chroma and
     hue) against FP64: <= 1e-5 relative (plus 1e-6 absolute near zero)."""

@@ -7,8 +7,10 @@ by ``make_reference_fixtures.py``), same programs, same check.
 
 Covered: ``run_blending`` (inscribe, ``composed``), ``run_affine`` (nearest
 affine rotated 45 degrees about the centres, ``affine``),
-``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``) and
-``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``).
+``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``),
+``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``) and
+``run_oklab`` (a bilinear grid read as Oklab LCh and converted to sRGB,
+``oklab``).
 """
 import json
 import math
@@ -19,6 +21,7 @@ import pytest
 
 import zosimos_amd as z
 from oracle import blockhash as B
+from oracle import pipelines as PL
 
 from gpu_util import report
 
@@ -99,3 +102,29 @@ def test_transmute_matches_reference_hash(ctx):
     out = run_program(ctx, cmd, {i: (src, bg)}, o, od)
     assert np.array_equal(out.reshape(-1), bg.reshape(-1))
     check("transmute", B.lumaa16_to_rgba8(out, w, h))
+
+
+def test_oklab_matches_reference_hash(ctx):
+    """blend.rs:449-508: lightness 0.4 + 0.4, chroma 0 to 1 down, hue 0 to 1
+    across, generated into RGBA8 with a Scalars colour, transmuted to an LchA
+    Oklab texel and converted to sRGB.  Against the FP64 oracle within 1 code
+    where the oracle's value is not within rounding of a code edge, and the
+    blessed hash."""
+    w = h = 400
+    srgb = z.Descriptor.rgba8_srgb(w, h)
+    scal = srgb.with_(color=z.Color(model=z.ColorModel.Scalars, transfer=z.Transfer.Linear))
+    lch = z.Descriptor(w, h, z.Texel(z.SampleBits.UInt8x4, z.SampleParts.LchA), z.Color.OKLAB)
+    plane = z.Bilinear(u_min=(0.4, 0.0, 0.0, 1.0), u_max=(0.4, 0.0, 1.0, 1.0),
+                       v_min=(0.4, 0.0, 0.0, 1.0), v_max=(0.4, 1.0, 0.0, 1.0))
+    cmd = z.CommandBuffer()
+    g = cmd.bilinear(scal, plane)
+    t = cmd.transmute(g, lch)
+    o, od = cmd.output(cmd.color_convert(t, srgb.color, srgb.texel))
+    out = run_program(ctx, cmd, {}, o, od)
+    staged = PL.staged_u8_store(PL.bilinear_grid(w, h, plane.params()))
+    _, want = PL.oklab_lch8_to_srgb(staged)
+    d = np.abs(out.astype(np.int32) - want.astype(np.int32))
+    report("oklab_plane", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
+    assert d[..., 3].max() == 0 and d.max() <= 1
+    check("oklab", out)
+

@@ -294,3 +294,57 @@ def test_blockhash_of_lumaa16_matches_transmute_hash():
     v = np.frombuffer(bg.tobytes(), dtype="<u2").reshape(512, 512, 2)
     t = (v // 257).astype(np.uint8)
     assert B.blockhash256(np.stack([t[..., 0]] * 3 + [t[..., 1]], -1)) not in hashes["transmute"]
+
+
+def test_bilinear_grid_known_answers():
+    """bilinear.frag: the reference's mgrid parameters (shaders/bilinear.rs:
+    22-31) give each pixel centre's coordinates; constant corners give the
+    constant; the uv term alone is (u v) scaled at the pixel centres."""
+    from oracle import pipelines as PL
+    g = PL.bilinear_grid(5, 3, [0, 0, 0, 0, 5, 0, 0, 0, 0, 0, 0, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0])
+    xs, ys = np.meshgrid(np.arange(5) + 0.5, np.arange(3) + 0.5)
+    assert np.array_equal(g[..., 0], xs) and np.array_equal(g[..., 1], ys) and not g[..., 2:].any()
+    c = PL.bilinear_grid(4, 4, [0.25] * 4 * 4 + [0.0] * 8)
+    assert np.allclose(c, 0.5)  # mix(0.25, 0.25, u) + mix(0.25, 0.25, v)
+    uv = PL.bilinear_grid(2, 2, [0.0] * 20 + [1.0, 2.0, 3.0, 4.0])
+    w = np.array([[0.0625, 0.1875], [0.1875, 0.5625]])
+    assert np.allclose(uv, w[..., None] * np.array([1.0, 2.0, 3.0, 4.0]))
+
+
+def test_oklab_lch_decode_known_answers():
+    """8-bit LchA Oklab: zero chroma gives the same grey whatever the hue
+    (neutral to 1e-3: Ottosson's matrices against the D65 sRGB matrix derived
+    from chromaticities); L = 1, C = 0 decodes to white; hue 128/255 puts the
+    chroma near +a."""
+    from oracle import pipelines as PL
+    grey = np.array([[[204, 0, h, 255] for h in (0, 64, 128, 255)]], np.uint8)
+    lin, enc = PL.oklab_lch8_to_srgb(grey)
+    assert np.all(lin == lin[:, :1]) and np.all(enc == enc[:, :1])
+    assert np.ptp(lin[..., :3], axis=-1).max() < 1e-3
+    white, _ = PL.oklab_lch8_to_srgb(np.array([[[255, 0, 0, 255]]], np.uint8))
+    assert np.allclose(white[..., :3], 1.0, atol=1e-3)
+    from oracle import color as C
+    lab = C.decode_u8(np.array([[[128, 51, 128, 255]]], np.uint8), 0x100)
+    assert np.isclose(lab[0, 0, 1], 0.2 * np.cos(np.radians(360.0 * (128 / 255 - 0.5))))
+
+
+def test_oklab_hue_plane_reproduces_blessed_hash():
+    """blend.rs:449-508 (run_oklab): a bilinear grid of constant lightness 0.8,
+    chroma 0 to 1 down and hue 0 to 1 across, stored to RGBA8 with a Scalars
+    colour (a staged format: truncating store), reinterpreted as LchA Oklab
+    and converted to sRGB.  The restated pipeline hashes to a blessed line;
+    rounding the staged store instead does not."""
+    import json, os
+    from oracle import blockhash as B
+    from oracle import pipelines as PL
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    g = PL.bilinear_grid(400, 400, OKLAB_PLANE)
+    _, enc = PL.oklab_lch8_to_srgb(PL.staged_u8_store(g))
+    assert B.blockhash256(enc) in hashes["oklab"]
+    _, rounded = PL.oklab_lch8_to_srgb(np.rint(np.clip(g, 0, 1) * 255).astype(np.uint8))
+    assert B.blockhash256(rounded) not in hashes["oklab"]
+
+
+# blend.rs:466-483: u_min, u_max, v_min, v_max, uv_min, uv_max
+OKLAB_PLANE = [0.4, 0.0, 0.0, 1.0, 0.4, 0.0, 1.0, 1.0, 0.4, 0.0, 0.0, 1.0, 0.4, 1.0, 0.0, 1.0] + [0.0] * 8

@@ -265,6 +265,12 @@ class CommandBuffer:
         c = (ctypes.c_float * 4)(*color)
         return self._reg(self._lib.zs_command_solid_rgba, ctypes.byref(desc.c()), c)
 
+    def bilinear(self, desc: Descriptor, distribution: "Bilinear") -> int:
+        """CommandBuffer::bilinear (command.rs:1615-1633): a generated image
+        of bilinear.frag over the pixel centres' normalised coordinates."""
+        p = (ctypes.c_float * 24)(*distribution.params())
+        return self._reg(self._lib.zs_command_bilinear, ctypes.byref(desc.c()), p)
+
     def oklab_adjust(self, src: int, chroma: float = 1.0, hue_rad: float = 0.0) -> int:
         return self._reg(self._lib.zs_command_oklab_adjust, src, chroma, hue_rad)
 
@@ -277,6 +283,32 @@ class CommandBuffer:
         d = DescriptorC()
         _check(self._lib.zs_command_describe(self._h, reg, ctypes.byref(d)))
         return Descriptor.from_c(d)
+
+
+@dataclass
+class Bilinear:
+    """shaders/bilinear.rs ShaderData: at the pixel centre (u, v) in [0, 1]^2
+    the value is mix(u_min, u_max, u) + mix(v_min, v_max, v)
+    + mix(uv_min, uv_max, u v), per channel."""
+    u_min: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    u_max: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    v_min: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    v_max: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    uv_min: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    uv_max: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+
+    @staticmethod
+    def mgrid(width: float, height: float) -> "Bilinear":
+        """shaders/bilinear.rs:22-31: the pixel centre's coordinates in r, g."""
+        return Bilinear(u_max=(width, 0.0, 0.0, 0.0), v_max=(0.0, height, 0.0, 0.0))
+
+    def params(self) -> List[float]:
+        out: List[float] = []
+        for v in (self.u_min, self.u_max, self.v_min, self.v_max, self.uv_min, self.uv_max):
+            if len(v) != 4:
+                raise ValueError("each Bilinear corner is a vec4")
+            out += [float(x) for x in v]
+        return out
 
 
 class Linker:

@@ -15,7 +15,9 @@ using namespace zh;
 
 namespace {
 
-enum OpKind { K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST };
+enum OpKind {
+  K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR
+};
 
 struct CmdOp {
   OpKind kind;
@@ -335,6 +337,23 @@ zh_status zs_command_solid_rgba(zs_command* cmd, const zh_descriptor* desc, cons
   return ZH_OK;
 }
 
+// CommandBuffer::bilinear (command.rs:1615-1633): a generated image, one
+// bilinear function of the pixel centre's normalised coordinates
+// (bilinear.frag); params = u_min, u_max, v_min, v_max, uv_min, uv_max
+// (shaders/bilinear.rs:33-45).
+zh_status zs_command_bilinear(zs_command* cmd, const zh_descriptor* desc, const float params[24], uint32_t* reg) {
+  if (!cmd || !desc || !params || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent descriptor for bilinear");
+  CmdOp op;
+  op.kind = K_BILINEAR;
+  op.desc = normalized(*desc);
+  op.blob.resize(96);
+  std::memcpy(op.blob.data(), params, 96);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg) {
   if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   zh_descriptor s;
@@ -447,6 +466,9 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_SOLID:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_SOLID_RGB, {}, c.blob, c.knob);
+        break;
+      case K_BILINEAR:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_BILINEAR, {}, c.blob, c.knob);
         break;
       case K_TRANSMUTE: {
         zh_op op;

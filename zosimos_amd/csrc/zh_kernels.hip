@@ -265,6 +265,28 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         }
         break;
       }
+      case OP_GRID: {
+        // bilinear.frag at the pixel centres: the stage runs 2-D (not flat),
+        // so (x0 + i, y) are image coordinates.  mix(a, b, t) is evaluated as
+        // a + (b - a) t, exact where a = b.
+        if (in.b) {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
+        }
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+        const float v = ((float)c.y + 0.5f) / (float)d->height;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float u = ((float)(c.x0 + i) + 0.5f) / (float)d->width, uv = u * v;
+          float r[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            r[k] = fmaf(q[4 + k] - q[k], u, q[k]) + fmaf(q[12 + k] - q[8 + k], v, q[8 + k]) +
+                   fmaf(q[20 + k] - q[16 + k], uv, q[16 + k]);
+          X[i] = make_float4(r[0], r[1], r[2], r[3]);
+        }
+        break;
+      }
       case OP_MAT3: {
         const __attribute__((address_space(4))) float* m = d->params + in.arg;
 #pragma unroll

@@ -110,7 +110,9 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // ---------------------------------------------------------------------------
 // value graph
 // ---------------------------------------------------------------------------
-enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES };
+// N_GRID: the bilinear generator (always materialised by a stage of its own,
+// which runs on the 2-D interpreter: it needs pixel coordinates).
+enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID };
 enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL };
 
 struct Micro {
@@ -124,6 +126,7 @@ struct Node {
   uint32_t w = 0, h = 0;
   int input_slot = -1;
   float color[4] = {0, 0, 0, 0};
+  float grid[24] = {};  // N_GRID: u_min, u_max, v_min, v_max, uv_min, uv_max
   std::vector<Micro> micro;
   int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a
   uint32_t mode = 0;
@@ -277,7 +280,7 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
         n.h = td.height;
         if (op.initializer == ZH_INIT_PAINT_FULLSCREEN) {
           const int src = op.num_operands > 0 ? B.cur[op.operands[0]] : -1;
-          if (op.shader != ZH_SHADER_SOLID_RGB) {
+          if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR) {
             if (src < 0) return fail(ZH_ERR_INVALID_ARGUMENT, "full-screen shader without operand" + at);
             const Node& sn = B.nodes[src];
             if (sn.w != td.width || sn.h != td.height)
@@ -292,6 +295,18 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               float c[4];
               std::memcpy(c, blob.data(), 16);
               B.cur[op.dst] = solid(B, td, c[0], c[1], c[2], c[3], "solid");
+              continue;
+            }
+            case ZH_SHADER_BILINEAR: {
+              if (!need_blob(96)) return fail(ZH_ERR_INVALID_ARGUMENT, "bilinear needs 96 bytes" + at);
+              Node g;
+              g.kind = N_GRID;
+              g.desc = td;
+              g.w = td.width;
+              g.h = td.height;
+              std::memcpy(g.grid, blob.data(), 96);
+              g.label = "bilinear";
+              B.cur[op.dst] = B.add(g);
               continue;
             }
             case ZH_SHADER_LINEAR_COLOR_MATRIX:
@@ -579,6 +594,12 @@ struct Emitter {
         depth++;
         return false;
       }
+      case N_GRID: {
+        pending_mat_valid = false;
+        ins(OP_GRID, 0, depth > 0 ? 1 : 0, param(n.grid, 24));
+        depth++;
+        return false;
+      }
       case N_UNARY: {
         bool pm = value(n.a, root);
         for (const Micro& m : n.micro) {
@@ -628,8 +649,8 @@ struct Emitter {
 };
 
 static const char* op_name(uint8_t o) {
-  static const char* n[] = {"nop", "load", "solid", "mat3", "cbrt", "cube", "clamp01", "premul",
-                            "unpremul", "over", "swap", "eotf", "oetf", "quant", "store", "offset"};
+  static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -792,6 +813,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     if (!live[id]) continue;
     Node& n = B.nodes[id];
     if (!n.out_slots.empty()) mark((int)id);
+    if (n.kind == N_GRID) mark((int)id);
     if (n.kind == N_BYTES && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.uses > 1 && n.kind != N_BYTES) mark((int)id);
   }
@@ -981,8 +1003,11 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       emit_stores(pm);
       if (!E.err.empty()) return fail(ZH_ERR_UNSUPPORTED, "stage " + std::to_string(plan.stages.size()) + ": " + E.err);
       E.finish();
-      // Flat (1-D) indexing when every slot is tight and the same size.
+      // Flat (1-D) indexing when every slot is tight and the same size, and
+      // no generator needs the pixel coordinates.
       bool flat = true;
+      for (uint32_t i = 0; i < sp.desc.n_instr; ++i)
+        if (sp.desc.code[i].op == OP_GRID) flat = false;
       for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
         const Slot& s = sp.desc.slot[k];
         if (s.width != n.w || s.height != n.h || s.row_stride != (uint64_t)s.width * s.bpp) flat = false;

@@ -49,6 +49,9 @@ enum Opcode : uint8_t {
   OP_QUANT,     // a = slot whose format X is rounded through (encode+decode)
   OP_STORE,     // a = slot; encode X at the pixel
   OP_OFFSET,    // a = 1: following loads read at (x + dx, y + dy); arg = params (dx, dy); a = 0 resets
+  OP_GRID,      // arg = params of 6 vec4 {u_min, u_max, v_min, v_max, uv_min, uv_max}; b = push.
+                //   X <- mix(u_min, u_max, u) + mix(v_min, v_max, v) + mix(uv_min, uv_max, u v) at the
+                //   pixel centre, (u, v) = ((x + 0.5) / width, (y + 0.5) / height) (bilinear.frag)
 };
 
 struct Instr {
