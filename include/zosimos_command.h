@@ -25,6 +25,11 @@ typedef struct zs_linked zs_linked;   /* linked Program, program.rs:24-39 */
 #define ZS_AFFINE_BILINEAR 1
 /* Blend, command.rs:287-291 */
 #define ZS_BLEND_ALPHA 0
+/* ChromaticAdaptationMethod, command.rs:422-460 (declaration order). */
+#define ZS_ADAPT_XYZ 0
+#define ZS_ADAPT_VON_KRIES 1
+#define ZS_ADAPT_BRADFORD_VON_KRIES 2
+#define ZS_ADAPT_BRADFORD_NONLINEAR 3 /* not implemented, as in the reference (command.rs:3328-3332) */
 
 zh_status zs_command_create(zs_command** out);
 zh_status zs_command_destroy(zs_command* cmd);
@@ -62,6 +67,11 @@ zh_status zs_command_bilinear(zs_command* cmd, const zh_descriptor* desc, const 
 /* Oklab chroma scale and hue rotation (radians) on an Oklab register; no
  * counterpart in the reference. */
 zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg);
+/* CommandBuffer::chromatic_adaptation, command.rs:1112-1174: an RGB register
+ * adapted to the target white point (ZH_WHITEPOINT_*) by a ZS_ADAPT_* method;
+ * one linear colour matrix (command.rs:2527-2548). */
+zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_t method, uint32_t target,
+                                          uint32_t* reg);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
  * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob. */
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);

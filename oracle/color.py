@@ -224,6 +224,35 @@ def transfer_encode(rgba, transfer: int):
 
 
 # --------------------------------------------------------------------------
+# Chromatic adaptation (command.rs:1112-1174, 2527-2548, 3279-3340)
+# --------------------------------------------------------------------------
+# Cone-space matrices of the methods, as the reference documents them at
+# command.rs:422-460 (the ``palette`` crate 0.7.6 computes with them; it is
+# not vendored): XYZ scaling, von Kries (Hunt-Pointer-Estevez), Bradford.
+ADAPT_CONES = {
+    0: np.eye(3),
+    1: np.array([[0.40024, 0.70760, -0.08081], [-0.22630, 1.16532, 0.04570], [0.0, 0.0, 0.91822]]),
+    2: np.array([[0.8951, 0.2664, -0.1614], [-0.7502, 1.7135, 0.0367], [0.0389, -0.0685, 1.0296]]),
+}
+
+
+def adaptation_matrix(method: int, src_wp: int, dst_wp: int) -> np.ndarray:
+    """von Kries transform in the method's cone space: M^-1 diag(M w_dst /
+    M w_src) M (XYZ -> XYZ)."""
+    m = ADAPT_CONES[method]
+    ratio = (m @ whitepoint_xyz(dst_wp)) / (m @ whitepoint_xyz(src_wp))
+    return np.linalg.inv(m) @ np.diag(ratio) @ m
+
+
+def adapt_rgb_matrix(primaries: int, src_wp: int, dst_wp: int, method: int) -> np.ndarray:
+    """The linear RGB matrix of ``chromatic_adaptation`` (command.rs:2527-2531):
+    RGB relative to dst_wp <- XYZ <- adaptation <- XYZ <- RGB relative to
+    src_wp."""
+    return np.linalg.inv(rgb_to_xyz_matrix(primaries, dst_wp)) @ adaptation_matrix(method, src_wp, dst_wp) \
+        @ rgb_to_xyz_matrix(primaries, src_wp)
+
+
+# --------------------------------------------------------------------------
 # Oklab (lib/std/src/oklab.frag:14-64; Ottosson 2020)
 # --------------------------------------------------------------------------
 # Row-major forms of the column-major GLSL constants at oklab.frag:14-24.

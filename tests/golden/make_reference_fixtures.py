@@ -12,7 +12,7 @@ import shutil
 
 REF = "/root/reference/lib/zosimos/tests"
 HERE = os.path.dirname(os.path.abspath(__file__))
-NAMES = ["composed", "affine", "convert_bt709", "swapped", "transmute", "oklab"]
+NAMES = ["composed", "affine", "convert_bt709", "swapped", "transmute", "oklab", "adapted"]
 
 
 def main():

@@ -8,9 +8,10 @@ by ``make_reference_fixtures.py``), same programs, same check.
 Covered: ``run_blending`` (inscribe, ``composed``), ``run_affine`` (nearest
 affine rotated 45 degrees about the centres, ``affine``),
 ``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``),
-``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``) and
+``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``),
 ``run_oklab`` (a bilinear grid read as Oklab LCh and converted to sRGB,
-``oklab``).
+``oklab``) and ``run_adaptation`` (von Kries adaptation from D65 to D50,
+``adapted``).
 """
 import json
 import math
@@ -21,6 +22,7 @@ import pytest
 
 import zosimos_amd as z
 from oracle import blockhash as B
+from oracle import color as C
 from oracle import pipelines as PL
 
 from gpu_util import report
@@ -127,4 +129,25 @@ def test_oklab_matches_reference_hash(ctx):
     report("oklab_plane", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
     assert d[..., 3].max() == 0 and d.max() <= 1
     check("oklab", out)
+
+
+def test_adapted_matches_reference_hash(ctx):
+    """blend.rs:169-197: the background adapted from D65 to D50 by von Kries,
+    within 1 LSB of the FP64 oracle and hashing to the blessed line."""
+    bg = rgba("background")
+    h, w = bg.shape[:2]
+    src = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    i = cmd.input(src)
+    a = cmd.chromatic_adaptation(i, z.ChromaticAdaptationMethod.VonKries, z.Whitepoint.D50)
+    o, od = cmd.output(a)
+    assert od.color.whitepoint == z.Whitepoint.D50
+    out = run_program(ctx, cmd, {i: (src, bg)}, o, od)
+    lin = C.decode_u8(bg, 5)
+    want = lin.copy()
+    want[..., :3] = np.einsum("ij,...j->...i", C.adapt_rgb_matrix(3, 5, 3, 1), lin[..., :3])
+    d = np.abs(out.astype(np.int32) - C.encode_u8(want, 5).astype(np.int32))
+    report("adapted", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
+    assert d.max() <= 1
+    check("adapted", out)
 

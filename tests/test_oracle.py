@@ -348,3 +348,27 @@ def test_oklab_hue_plane_reproduces_blessed_hash():
 
 # blend.rs:466-483: u_min, u_max, v_min, v_max, uv_min, uv_max
 OKLAB_PLANE = [0.4, 0.0, 0.0, 1.0, 0.4, 0.0, 1.0, 1.0, 0.4, 0.0, 0.0, 1.0, 0.4, 1.0, 0.0, 1.0] + [0.0] * 8
+
+
+def test_chromatic_adaptation_matrix_and_blessed_hash():
+    """Adapting white to its own white point is the identity for every
+    method; the source white maps to the target white in XYZ; and the
+    reference's run_adaptation (blend.rs:169-197: the sRGB background adapted
+    from D65 to D50 by von Kries) hashes to its blessed line, which the
+    Bradford and XYZ-scaling matrices miss."""
+    import json, os
+    from oracle import blockhash as B
+    from oracle import color as C
+    for m in (0, 1, 2):
+        assert np.allclose(C.adaptation_matrix(m, 5, 5), np.eye(3))
+        assert np.allclose(C.adaptation_matrix(m, 5, 3) @ C.whitepoint_xyz(5), C.whitepoint_xyz(3))
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    lin = C.decode_u8(_golden_rgba("background"), 5)
+    got = {}
+    for m in (0, 1, 2):
+        out = lin.copy()
+        out[..., :3] = np.einsum("ij,...j->...i", C.adapt_rgb_matrix(3, 5, 3, m), lin[..., :3])
+        got[m] = B.blockhash256(C.encode_u8(out, 5))
+    assert got[1] in hashes["adapted"]
+    assert got[0] not in hashes["adapted"] and got[2] not in hashes["adapted"]

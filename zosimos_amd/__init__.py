@@ -51,6 +51,11 @@ class Primaries:
     Xyz, Bt601_525, Bt601_625, Bt709, Smpte240, Bt2020, Bt2100, DisplayP3 = range(8)
 
 
+class ChromaticAdaptationMethod:
+    """command.rs:422-460, in declaration order."""
+    Xyz, VonKries, BradfordVonKries, BradfordNonLinear = range(4)
+
+
 class Whitepoint:
     A, B, C, D50, D55, D65, D75, E, F2, F7, F11 = range(11)
 
@@ -273,6 +278,12 @@ class CommandBuffer:
 
     def oklab_adjust(self, src: int, chroma: float = 1.0, hue_rad: float = 0.0) -> int:
         return self._reg(self._lib.zs_command_oklab_adjust, src, chroma, hue_rad)
+
+    def chromatic_adaptation(self, src: int, method: int, target: int) -> int:
+        """CommandBuffer::chromatic_adaptation (command.rs:1112-1174): the RGB
+        register adapted to white point ``target`` (``Whitepoint``) by a
+        ``ChromaticAdaptationMethod``."""
+        return self._reg(self._lib.zs_command_chromatic_adaptation, src, method, target)
 
     def knob(self, reg: int) -> int:
         k = ctypes.c_int32()

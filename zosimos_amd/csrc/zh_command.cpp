@@ -31,6 +31,21 @@ struct CmdOp {
   int32_t knob = -1;
 };
 
+// CIE 1931 2-degree chromaticities of the white points A .. F11 (the order of
+// image-canvas Whitepoint); D65 and D50 use the 4-digit values of BT.709 /
+// IEC 61966-2-1 and ICC.
+const double kWhite[11][2] = {{0.44757, 0.40745}, {0.34840, 0.35160}, {0.31006, 0.31616}, {0.3457, 0.3585},
+                              {0.33242, 0.34743}, {0.3127, 0.3290},   {0.29902, 0.31485}, {1.0 / 3, 1.0 / 3},
+                              {0.37208, 0.37529}, {0.31292, 0.32933}, {0.38052, 0.37713}};
+
+// XYZ of a white point at Y = 1.
+void white_xyz(uint32_t wp, double W[3]) {
+  const double wx = kWhite[wp][0], wy = kWhite[wp][1];
+  W[0] = wx / wy;
+  W[1] = 1.0;
+  W[2] = (1.0 - wx - wy) / wy;
+}
+
 // RGB -> XYZ from chromaticities and white point (textbook derivation, as
 // image-canvas Primaries::to_xyz_row_matrix provides it to command.rs:1023).
 bool rgb_to_xyz(uint32_t primaries, uint32_t wp, Mat3* out) {
@@ -44,10 +59,6 @@ bool rgb_to_xyz(uint32_t primaries, uint32_t wp, Mat3* out) {
       {{0.708, 0.292}, {0.170, 0.797}, {0.131, 0.046}},  // BT.2100
       {{0.680, 0.320}, {0.265, 0.690}, {0.150, 0.060}},  // Display P3 (SMPTE EG 432-1)
   };
-  static const double white[11][2] = {{0.44757, 0.40745}, {0.34840, 0.35160}, {0.31006, 0.31616},
-                                      {0.3457, 0.3585},   {0.33242, 0.34743}, {0.3127, 0.3290},
-                                      {0.29902, 0.31485}, {1.0 / 3, 1.0 / 3}, {0.37208, 0.37529},
-                                      {0.31292, 0.32933}, {0.38052, 0.37713}};
   if (primaries > 7 || wp > 10) return false;
   if (primaries == 0) { *out = mat_identity(); return true; }
   Mat3 P;
@@ -57,8 +68,8 @@ bool rgb_to_xyz(uint32_t primaries, uint32_t wp, Mat3* out) {
     P[1 * 3 + c] = 1.0;
     P[2 * 3 + c] = (1.0 - x - y) / y;
   }
-  const double wx = white[wp][0], wy = white[wp][1];
-  const double W[3] = {wx / wy, 1.0, (1.0 - wx - wy) / wy};
+  double W[3];
+  white_xyz(wp, W);
   Mat3 Pi;
   if (!mat_inv(P, &Pi)) return false;
   double s[3];
@@ -350,6 +361,56 @@ zh_status zs_command_bilinear(zs_command* cmd, const zh_descriptor* desc, const 
   op.desc = normalized(*desc);
   op.blob.resize(96);
   std::memcpy(op.blob.data(), params, 96);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+// CommandBuffer::chromatic_adaptation (command.rs:1112-1174), lowered as one
+// linear colour matrix (command.rs:2527-2548): RGB relative to the target
+// white <- XYZ <- adaptation <- XYZ <- RGB relative to the source white.  The
+// adaptation is the von Kries transform M^-1 diag(M w_dst / M w_src) M in the
+// method's cone space: identity (XYZ scaling), Hunt-Pointer-Estevez (von
+// Kries) or Bradford, the matrices the reference documents at
+// command.rs:422-460.
+zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_t method, uint32_t target,
+                                          uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor s;
+  zh_status st = reg_desc(cmd, src, &s);
+  if (st != ZH_OK) return st;
+  if (s.color.model != ZH_COLOR_RGB) return fail(ZH_ERR_TYPE, "non-rgb chromatic adaptation");
+  if (target > 10 || s.color.whitepoint > 10) return fail(ZH_ERR_TYPE, "unknown white point");
+  Mat3 M;
+  switch (method) {
+    case ZS_ADAPT_XYZ: M = mat_identity(); break;
+    case ZS_ADAPT_VON_KRIES: M = {0.40024, 0.70760, -0.08081, -0.22630, 1.16532, 0.04570, 0.0, 0.0, 0.91822}; break;
+    case ZS_ADAPT_BRADFORD_VON_KRIES: M = {0.8951, 0.2664, -0.1614, -0.7502, 1.7135, 0.0367, 0.0389, -0.0685, 1.0296}; break;
+    case ZS_ADAPT_BRADFORD_NONLINEAR: return fail(ZH_ERR_UNIMPLEMENTED, "Bradford's non-linear adaptation");
+    default: return fail(ZH_ERR_INVALID_ARGUMENT, "unknown adaptation method");
+  }
+  double ws[3], wd[3];
+  white_xyz(s.color.whitepoint, ws);
+  white_xyz(target, wd);
+  Mat3 Mi, to_xyz, from_xyz, xyz_to_rgb;
+  if (!mat_inv(M, &Mi) || !rgb_to_xyz(s.color.primaries, s.color.whitepoint, &to_xyz) ||
+      !rgb_to_xyz(s.color.primaries, target, &from_xyz) || !mat_inv(from_xyz, &xyz_to_rgb))
+    return fail(ZH_ERR_TYPE, "unknown primaries");
+  Mat3 D = mat_identity();
+  for (int i = 0; i < 3; ++i) {
+    const double cs = M[i * 3] * ws[0] + M[i * 3 + 1] * ws[1] + M[i * 3 + 2] * ws[2];
+    const double cd = M[i * 3] * wd[0] + M[i * 3 + 1] * wd[1] + M[i * 3 + 2] * wd[2];
+    D[i * 4] = cd / cs;
+  }
+  const Mat3 adapt = mat_mul(Mi, mat_mul(D, M));
+  CmdOp op;
+  op.kind = K_CONVERT;
+  op.src = src;
+  op.shader = ZH_SHADER_LINEAR_COLOR_MATRIX;
+  op.blob = std140(mat_mul(xyz_to_rgb, mat_mul(adapt, to_xyz)));
+  zh_descriptor d = s;
+  d.color.whitepoint = target;
+  d.row_stride = 0;
+  op.desc = normalized(d);
   *reg = push(cmd, op);
   return ZH_OK;
 }
