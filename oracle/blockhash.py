@@ -12,7 +12,8 @@ blockhash.io reference implementation):
 * the image is cut into bits x bits blocks; a pixel's value is r + g + b, or
   765 (white) when its alpha is 0; each block sums its pixels (the "quick"
   method when the sides divide evenly, otherwise each pixel is spread over the
-  blocks it overlaps in proportion to the overlap -- the "precise" method);
+  blocks it overlaps in proportion to the overlap -- the "precise" method,
+  which also covers images narrower than the grid);
 * the blocks are taken in four horizontal bands; in each band a block's bit
   is 1 when its sum exceeds the band's median (or equals it within 1 while
   the median is above half of the block's maximum);
@@ -76,21 +77,14 @@ def blockhash(rgba_u8, bits: int = 16) -> str:
 
 
 def _axis_weights(n: int, bits: int):
-    size = n / bits
-    out = np.zeros((n, bits), dtype=np.float64)
-    for i in range(n):
-        if n % bits == 0:
-            out[i, int(i // size)] = 1.0
-            continue
-        frac, whole = np.modf((i + 1) % size)
-        lo = int(i // size)
-        if whole > 0 or i + 1 == n:
-            hi = lo
-        else:
-            hi = int(-(-i // size))
-        out[i, lo] += 1.0 - frac
-        out[i, min(hi, bits - 1)] += frac
-    return out
+    """Share of pixel i (the interval [i, i + 1)) in block b (the interval
+    [b n / bits, (b + 1) n / bits)): the length of their overlap.  A pixel
+    that straddles one block boundary splits in proportion, as the
+    published algorithm does; a pixel wider than a block (n < bits) covers
+    several blocks, each with the part it holds."""
+    edges = np.arange(bits + 1, dtype=np.float64) * (n / bits)
+    lo = np.arange(n, dtype=np.float64)[:, None]
+    return np.clip(np.minimum(lo + 1.0, edges[None, 1:]) - np.maximum(lo, edges[None, :-1]), 0.0, None)
 
 
 def blockhash256(rgba_u8) -> str:

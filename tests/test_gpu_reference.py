@@ -10,8 +10,13 @@ affine rotated 45 degrees about the centres, ``affine``),
 ``run_conversion`` (BT.709 transfer to sRGB, ``convert_bt709``),
 ``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``),
 ``run_oklab`` (a bilinear grid read as Oklab LCh and converted to sRGB,
-``oklab``) and ``run_adaptation`` (von Kries adaptation from D65 to D50,
-``adapted``).
+``oklab``), ``run_adaptation`` (von Kries adaptation from D65 to D50,
+``adapted``) and ``run_solid`` (``solid``); from ``knobs.rs`` one bilinear
+executable re-parameterised through a knob five times
+(``bilinear-knob-0`` .. ``4``); from ``buffer.rs`` ``run_bilinear``
+(``bilinear_from_buffer``, its six parameters given directly rather than
+through a buffer register) and ``run_from_buffer`` (an 8x8 Luma8 image read
+from a buffer with 256-byte rows, ``from_buffer``).
 """
 import json
 import math
@@ -150,4 +155,79 @@ def test_adapted_matches_reference_hash(ctx):
     report("adapted", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
     assert d.max() <= 1
     check("adapted", out)
+
+
+def test_solid_matches_reference_hash(ctx):
+    """blend.rs:620-643: a 400x400 solid (0.5, 0.5, 1, 1) in RGBA8 sRGB."""
+    d = z.Descriptor.rgba8_srgb(400, 400)
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.solid_rgba(d, (0.5, 0.5, 1.0, 1.0)))
+    out = run_program(ctx, cmd, {}, o, od)
+    want = C.encode_u8(np.array([0.5, 0.5, 1.0, 1.0]), 5)
+    assert np.all(out == want)
+    check("solid", out)
+
+
+KNOB_PLANES = [((0.0, 0.0, 0.5, 0.5), (0.5, 0.5, 0.5, 0.5)), ((0.2, 0.0, 0.5, 0.5), (0.5, 0.5, 0.5, 0.5)),
+               ((0.4, 0.0, 0.5, 0.5), (0.5, 0.5, 0.5, 0.5)), ((0.5, 0.0, 0.4, 0.5), (0.5, 0.5, 0.5, 0.5)),
+               ((0.5, 0.0, 0.2, 0.5), (0.5, 0.5, 0.5, 0.5))]
+
+
+def test_bilinear_knob_matches_reference_hashes(ctx):
+    """knobs.rs:46-167: one executable of a 512x512 bilinear (RGBA8 sRGB), its
+    parameters overridden through a knob for each of five launches; each
+    output within 1 LSB of the FP64 oracle and hashing to its blessed line."""
+    d = z.Descriptor.rgba8_srgb(512, 512)
+    cmd = z.CommandBuffer()
+    g = cmd.bilinear(d, z.Bilinear(u_min=(0, 0, 1, 1), u_max=(1, 1, 1, 1), v_min=(0, 0, 1, 1), v_max=(1, 1, 1, 1)))
+    k = cmd.knob(g)
+    o, od = cmd.output(g)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    for idx, (lo, hi) in enumerate(KNOB_PLANES):
+        b = z.Bilinear(u_min=lo, u_max=hi, v_min=lo, v_max=hi)
+        exe.knob(k, np.asarray(b.params(), np.float32).tobytes())
+        exe.launch({}, {o: out})
+        ctx.sync()
+        got = out.download().reshape(512, 512, 4)
+        want = C.encode_u8(PL.bilinear_grid(512, 512, b.params()), 5)
+        assert np.abs(got.astype(np.int32) - want.astype(np.int32)).max() <= 1
+        check(f"bilinear-knob-{idx}", got)
+
+
+def test_bilinear_from_buffer_matches_reference_hash(ctx):
+    """buffer.rs:121-149: a 256x256 bilinear (RGBA8 sRGB) with the buffer's six
+    vec4 as its parameters."""
+    p = [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.7, 1.0, 0.0, 0.0, 0.3, 1.0, 0.0, 1.0, 0.3, 1.0] + [0.0, 0.0, 0.0, 1.0] * 2
+    b = z.Bilinear(*(tuple(p[4 * i:4 * i + 4]) for i in range(6)))
+    d = z.Descriptor.rgba8_srgb(256, 256)
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.bilinear(d, b))
+    out = run_program(ctx, cmd, {}, o, od)
+    want = C.encode_u8(PL.bilinear_grid(256, 256, p), 5)
+    assert np.abs(out.astype(np.int32) - want.astype(np.int32)).max() <= 1
+    check("bilinear_from_buffer", out)
+
+
+def test_from_buffer_matches_reference_hash(ctx):
+    """buffer.rs:39-65: 2,048 buffer bytes read as an 8x8 Luma8 sRGB image
+    with 256-byte rows (row 1 black, the rest white), output tight; the bytes
+    of the output equal the buffer's and the image hashes to the blessed
+    line."""
+    a = np.full(8 * 256, 0xFF, np.uint8)
+    a[256:264] = 0
+    buf = ctx.image(z.Descriptor.rgba8_srgb(512, 1))  # 2,048 bytes of device memory
+    buf.upload(a)
+    luma = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)
+    img = ctx.wrap(luma.with_(row_stride=256), buf.device_ptr, a.nbytes)
+    cmd = z.CommandBuffer()
+    i = cmd.input(luma.with_(row_stride=256))
+    o, od = cmd.output(i)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    exe.launch({i: img}, {o: out})
+    ctx.sync()
+    got = out.download().reshape(8, 8)
+    assert np.array_equal(got, a.reshape(8, 256)[:, :8])
+    check("from_buffer", np.stack([got, got, got, np.full_like(got, 255)], -1))
 

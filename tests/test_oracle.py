@@ -372,3 +372,18 @@ def test_chromatic_adaptation_matrix_and_blessed_hash():
         got[m] = B.blockhash256(C.encode_u8(out, 5))
     assert got[1] in hashes["adapted"]
     assert got[0] not in hashes["adapted"] and got[2] not in hashes["adapted"]
+
+
+def test_blockhash_of_an_image_narrower_than_the_grid():
+    """buffer.rs:39-65 hashes an 8x8 image on the 16x16 grid: each pixel
+    covers two block rows and columns, and the restated precise method
+    reproduces the blessed from_buffer hash (row 1 black, the rest white)."""
+    import json, os
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    lum = np.full((8, 8), 255, np.uint8)
+    lum[1] = 0
+    assert B.blockhash256(np.stack([lum, lum, lum, np.full_like(lum, 255)], -1)) in hashes["from_buffer"]
+    w = B._axis_weights(8, 16)
+    assert np.allclose(w.sum(0), 0.5) and np.allclose(w.sum(1), 1.0)
