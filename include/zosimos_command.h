@@ -25,6 +25,18 @@ typedef struct zs_linked zs_linked;   /* linked Program, program.rs:24-39 */
 #define ZS_AFFINE_BILINEAR 1
 /* Blend, command.rs:287-291 */
 #define ZS_BLEND_ALPHA 0
+/* DerivativeMethod and Direction, command.rs:547-566 (declaration order). */
+#define ZS_DERIVATIVE_ROBERTS 0
+#define ZS_DERIVATIVE_PREWITT 1
+#define ZS_DERIVATIVE_SOBEL 2
+#define ZS_DERIVATIVE_SCHARR3 3
+#define ZS_DERIVATIVE_SCHARR4 4
+#define ZS_DERIVATIVE_SCHARR5 5
+#define ZS_DERIVATIVE_SCHARR3_TO_4BIT 6
+#define ZS_DERIVATIVE_SCHARR5_TAB 7
+#define ZS_DERIVATIVE_SCHARR3_TO_8BIT 8
+#define ZS_DIRECTION_HEIGHT 0
+#define ZS_DIRECTION_WIDTH 1
 /* ChromaticAdaptationMethod, command.rs:422-460 (declaration order). */
 #define ZS_ADAPT_XYZ 0
 #define ZS_ADAPT_VON_KRIES 1
@@ -72,6 +84,11 @@ zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, f
  * one linear colour matrix (command.rs:2527-2548). */
 zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_t method, uint32_t target,
                                           uint32_t* reg);
+/* CommandBuffer::derivative, command.rs:1487-1507: a 3x3 derivative filter
+ * (ZS_DERIVATIVE_*) along a ZS_DIRECTION_*, lowered to box3.frag
+ * (command.rs:2598-2609, 3344-3409).  Roberts and the Scharr 4/5 variants are
+ * not implemented, as in the reference. */
+zh_status zs_command_derivative(zs_command* cmd, uint32_t src, uint32_t method, uint32_t direction, uint32_t* reg);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
  * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob. */
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);

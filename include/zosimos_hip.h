@@ -162,6 +162,9 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *  SOLID_RGB             vec4, 16 B (solid_rgb.frag)
  *  BILINEAR              6 vec4 {u_min, u_max, v_min, v_max, uv_min, uv_max},
  *                        96 B (shaders/bilinear.rs:33-45, bilinear.frag)
+ *  BOX3                  mat3 std140, 48 B (shaders/box3.rs:25-27): row-major
+ *                        weights, row = y offset + 1, column = x offset + 1;
+ *                        operand = the sampled image (box3.frag)
  *  OKLAB_ADJUST          vec4 {chroma scale, hue rotation (rad), 0, 0}
  *  BLEND_OVER            none; operands (below, above)
  *  AFFINE_BILINEAR       mat3 std140 of the row-major Affine matrix, 48 B;
@@ -174,6 +177,7 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_OKLAB_DECODE 3
 #define ZH_SHADER_SOLID_RGB 4
 #define ZH_SHADER_BILINEAR 5
+#define ZH_SHADER_BOX3 6
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

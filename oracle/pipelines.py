@@ -216,3 +216,36 @@ def oklab_lch8_to_srgb(lch_u8):
     out[..., :3] = C.oklab_to_rgb(lab[..., :3], primaries=3, clamp=True)
     return out, C.encode_u8(out, 5)
 
+
+# Smoothing vectors of the derivative methods (command.rs:3344-3400); the
+# difference vector is (0.5, 0, -0.5).
+DERIVATIVE_SMOOTH = {
+    1: (1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0),        # Prewitt
+    2: (0.25, 0.5, 0.25),                        # Sobel
+    3: (46.84 / 256.0, 162.32 / 256.0, 46.84 / 256.0),  # Scharr3
+    6: (3.0 / 16.0, 10.0 / 16.0, 3.0 / 16.0),    # Scharr3To4Bit
+    8: (47.0 / 256.0, 162.0 / 256.0, 47.0 / 256.0),  # Scharr3To8Bit
+}
+
+
+def derivative_weights(method: int, width: bool = True):
+    """The 3x3 weights, row = y offset + 1, column = x offset + 1: the outer
+    product smooth x difference, transposed for the height direction."""
+    w = np.outer(DERIVATIVE_SMOOTH[method], (0.5, 0.0, -0.5))
+    return w if width else w.T
+
+
+def box3(rgba, weights):
+    """box3.frag on linear RGBA (H, W, 4): per colour channel the weighted sum
+    of the 3x3 neighbourhood, texels at (x + dx, y + dy) with weight
+    weights[dy + 1][dx + 1], coordinates clamped to the image; alpha 1."""
+    img = np.asarray(rgba, dtype=np.float64)
+    h, w = img.shape[:2]
+    out = np.zeros_like(img)
+    for dy in range(3):
+        ys = np.clip(np.arange(h) + dy - 1, 0, h - 1)
+        for dx in range(3):
+            xs = np.clip(np.arange(w) + dx - 1, 0, w - 1)
+            out[..., :3] += weights[dy][dx] * img[ys][:, xs, :3]
+    out[..., 3] = 1.0
+    return out

@@ -336,6 +336,31 @@ def test_bilinear_generator(ctx):
     assert np.all((d == 0) | ((np.abs(d) == 1) & edge))
 
 
+def test_derivative_height_f32_ragged(ctx):
+    """box3.frag along the height (transposed weights) of a ragged RGBA32F
+    image (partial groups of 4, clamp-to-edge at every border) after a
+    colour matrix (its own stage), against the FP64 oracle within 1e-5
+    relative plus 1e-6."""
+    w, h = 45, 13
+    d = z.Descriptor.rgba32f_linear(w, h)
+    rng = np.random.default_rng(33)
+    host = rng.uniform(-0.2, 1.2, (h, w, 4)).astype(np.float32)
+    cmd = z.CommandBuffer()
+    i = cmd.input(d)
+    p3 = cmd.color_convert(i, z.Color(primaries=z.Primaries.DisplayP3, transfer=z.Transfer.Linear), d.texel)
+    o, od = cmd.output(cmd.derivative(p3, z.DerivativeMethod.Sobel, z.Direction.Height))
+    out, exe = run(ctx, cmd, {i: d}, {i: host}, {o: od})
+    assert "box3" in exe.describe()
+    m = C.convert_matrix(3, 7, 5)
+    lin = host.astype(np.float64).copy()
+    lin[..., :3] = np.einsum("ij,...j->...i", m, lin[..., :3])
+    want = PL.box3(lin, PL.derivative_weights(z.DerivativeMethod.Sobel, width=False))
+    got = PL.f32_image(out[o], w, h)
+    err = np.abs(got - want)
+    report("derivative_height_f32", max_abs=f"{err.max():.2e}")
+    assert np.all(err <= 1e-5 * np.abs(want) + 1e-6)
+
+
 def test_linear_f32_output_within_1e5(ctx):
     """A linear RGBA32F chain (P3 conversion, Oklab round trip with chroma and
     hue) against FP64: <= 1e-5 relative (plus 1e-6 absolute near zero)."""

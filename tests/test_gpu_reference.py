@@ -11,7 +11,8 @@ affine rotated 45 degrees about the centres, ``affine``),
 ``run_transmute`` (RGBA8 to LumaA16 byte copy, ``transmute``),
 ``run_oklab`` (a bilinear grid read as Oklab LCh and converted to sRGB,
 ``oklab``), ``run_adaptation`` (von Kries adaptation from D65 to D50,
-``adapted``) and ``run_solid`` (``solid``); from ``knobs.rs`` one bilinear
+``adapted``), ``run_solid`` (``solid``) and ``run_derivative`` (five 3x3
+derivative filters, ``derived_<method>``); from ``knobs.rs`` one bilinear
 executable re-parameterised through a knob five times
 (``bilinear-knob-0`` .. ``4``); from ``buffer.rs`` ``run_bilinear``
 (``bilinear_from_buffer``, its six parameters given directly rather than
@@ -230,4 +231,26 @@ def test_from_buffer_matches_reference_hash(ctx):
     got = out.download().reshape(8, 8)
     assert np.array_equal(got, a.reshape(8, 256)[:, :8])
     check("from_buffer", np.stack([got, got, got, np.full_like(got, 255)], -1))
+
+
+@pytest.mark.parametrize("method,name", [(z.DerivativeMethod.Scharr3, "Scharr3"),
+                                         (z.DerivativeMethod.Scharr3To4Bit, "Scharr3To4Bit"),
+                                         (z.DerivativeMethod.Scharr3To8Bit, "Scharr3To8Bit"),
+                                         (z.DerivativeMethod.Prewitt, "Prewitt"),
+                                         (z.DerivativeMethod.Sobel, "Sobel")])
+def test_derivative_matches_reference_hash(ctx, method, name):
+    """blend.rs:577-618: the background's derivative along the width, within
+    1 LSB of the FP64 oracle and hashing to the method's blessed line."""
+    bg = rgba("background")
+    h, w = bg.shape[:2]
+    src = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    i = cmd.input(src)
+    o, od = cmd.output(cmd.derivative(i, method, z.Direction.Width))
+    out = run_program(ctx, cmd, {i: (src, bg)}, o, od)
+    want = C.encode_u8(PL.box3(C.decode_u8(bg, 5), PL.derivative_weights(method)), 5)
+    d = np.abs(out.astype(np.int32) - want.astype(np.int32))
+    report(f"derived_{name}", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
+    assert d.max() <= 1
+    check(f"derived_{name}", out)
 

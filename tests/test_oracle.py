@@ -387,3 +387,22 @@ def test_blockhash_of_an_image_narrower_than_the_grid():
     assert B.blockhash256(np.stack([lum, lum, lum, np.full_like(lum, 255)], -1)) in hashes["from_buffer"]
     w = B._axis_weights(8, 16)
     assert np.allclose(w.sum(0), 0.5) and np.allclose(w.sum(1), 1.0)
+
+
+def test_derivatives_reproduce_blessed_hashes():
+    """blend.rs:577-618 (run_derivative, width direction): box3 of the linear
+    background with each method's weights, encoded to sRGB, hashes to the
+    blessed line of that method; weights applied transposed (x and y offsets
+    swapped) do not.  A constant image has zero derivative."""
+    import json, os
+    from oracle import blockhash as B
+    from oracle import color as C
+    from oracle import pipelines as PL
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    lin = C.decode_u8(_golden_rgba("background"), 5)
+    for m, name in ((1, "Prewitt"), (2, "Sobel"), (3, "Scharr3"), (6, "Scharr3To4Bit"), (8, "Scharr3To8Bit")):
+        w = PL.derivative_weights(m)
+        assert B.blockhash256(C.encode_u8(PL.box3(lin, w), 5)) in hashes["derived_" + name]
+        assert B.blockhash256(C.encode_u8(PL.box3(lin, w.T), 5)) not in hashes["derived_" + name]
+        assert np.allclose(PL.box3(np.full((5, 6, 4), 0.3), w)[..., :3], 0.0)

@@ -51,6 +51,16 @@ class Primaries:
     Xyz, Bt601_525, Bt601_625, Bt709, Smpte240, Bt2020, Bt2100, DisplayP3 = range(8)
 
 
+class DerivativeMethod:
+    """command.rs:556-566, in declaration order."""
+    Roberts, Prewitt, Sobel, Scharr3, Scharr4, Scharr5, Scharr3To4Bit, Scharr5Tab, Scharr3To8Bit = range(9)
+
+
+class Direction:
+    """command.rs:547-552."""
+    Height, Width = range(2)
+
+
 class ChromaticAdaptationMethod:
     """command.rs:422-460, in declaration order."""
     Xyz, VonKries, BradfordVonKries, BradfordNonLinear = range(4)
@@ -284,6 +294,11 @@ class CommandBuffer:
         register adapted to white point ``target`` (``Whitepoint``) by a
         ``ChromaticAdaptationMethod``."""
         return self._reg(self._lib.zs_command_chromatic_adaptation, src, method, target)
+
+    def derivative(self, src: int, method: int, direction: int) -> int:
+        """CommandBuffer::derivative (command.rs:1487-1507): a 3x3 derivative
+        filter (``DerivativeMethod``) along a ``Direction`` (box3.frag)."""
+        return self._reg(self._lib.zs_command_derivative, src, method, direction)
 
     def knob(self, reg: int) -> int:
         k = ctypes.c_int32()

@@ -16,7 +16,8 @@ using namespace zh;
 namespace {
 
 enum OpKind {
-  K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR
+  K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
+  K_DERIVATIVE
 };
 
 struct CmdOp {
@@ -415,6 +416,49 @@ zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_
   return ZH_OK;
 }
 
+// CommandBuffer::derivative (command.rs:1487-1507): box3.frag with the
+// outer product of a smoothing and a central-difference vector
+// (command.rs:3344-3409), transposed for the height direction.  The output
+// keeps the source's descriptor.
+zh_status zs_command_derivative(zs_command* cmd, uint32_t src, uint32_t method, uint32_t direction, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor s;
+  zh_status st = reg_desc(cmd, src, &s);
+  if (st != ZH_OK) return st;
+  if (direction != ZS_DIRECTION_WIDTH && direction != ZS_DIRECTION_HEIGHT)
+    return fail(ZH_ERR_INVALID_ARGUMENT, "unknown direction");
+  double smooth[3];
+  switch (method) {
+    case ZS_DERIVATIVE_PREWITT: smooth[0] = smooth[1] = smooth[2] = 1.0 / 3.0; break;
+    case ZS_DERIVATIVE_SOBEL: smooth[0] = 0.25; smooth[1] = 0.5; smooth[2] = 0.25; break;
+    case ZS_DERIVATIVE_SCHARR3: smooth[0] = smooth[2] = 46.84 / 256.0; smooth[1] = 162.32 / 256.0; break;
+    case ZS_DERIVATIVE_SCHARR3_TO_4BIT: smooth[0] = smooth[2] = 3.0 / 16.0; smooth[1] = 10.0 / 16.0; break;
+    case ZS_DERIVATIVE_SCHARR3_TO_8BIT: smooth[0] = smooth[2] = 47.0 / 256.0; smooth[1] = 162.0 / 256.0; break;
+    case ZS_DERIVATIVE_ROBERTS:
+    case ZS_DERIVATIVE_SCHARR4:
+    case ZS_DERIVATIVE_SCHARR5:
+    case ZS_DERIVATIVE_SCHARR5_TAB: return fail(ZH_ERR_UNIMPLEMENTED, "derivative method not implemented");
+    default: return fail(ZH_ERR_INVALID_ARGUMENT, "unknown derivative method");
+  }
+  const double diff[3] = {0.5, 0.0, -0.5};
+  Mat3 m;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r * 3 + c] = smooth[r] * diff[c];
+  if (direction == ZS_DIRECTION_HEIGHT) {
+    Mat3 t;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) t[r * 3 + c] = m[c * 3 + r];
+    m = t;
+  }
+  CmdOp op;
+  op.kind = K_DERIVATIVE;
+  op.src = src;
+  op.blob = std140(m);
+  op.desc = s;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg) {
   if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   zh_descriptor s;
@@ -530,6 +574,9 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_BILINEAR:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_BILINEAR, {}, c.blob, c.knob);
+        break;
+      case K_DERIVATIVE:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_BOX3, {c.src}, c.blob, c.knob);
         break;
       case K_TRANSMUTE: {
         zh_op op;

@@ -287,6 +287,37 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         }
         break;
       }
+      case OP_BOX3: {
+        // box3.frag: the weighted 3x3 neighbourhood of the slot's texels,
+        // coordinates clamped to the slot (clamp-to-edge sampling); alpha 1.
+        // The stage runs 2-D, so (x0 + i, y) are image coordinates.
+        const Slot sl = ldc(&d->slot[in.a]);
+        if (in.b) {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
+        }
+        const __attribute__((address_space(4))) float* w = d->params + in.arg;
+        const int W = (int)sl.width - 1, H = (int)sl.height - 1;
+        for (int i = 0; i < kPix; ++i) {
+          float r = 0.0f, g = 0.0f, b = 0.0f;
+          if (i < c.n) {
+            for (int dy = 0; dy < 3; ++dy) {
+              const int yy = min(max(c.y + dy - 1, 0), H);
+#pragma unroll
+              for (int dx = 0; dx < 3; ++dx) {
+                const int xx = min(max(c.x0 + i + dx - 1, 0), W);
+                const float4 t = decode_texel<FAST>(sl, fetch_one(sl, xx, yy), lut);
+                const float k = w[dy * 3 + dx];
+                r = fmaf(k, t.x, r);
+                g = fmaf(k, t.y, g);
+                b = fmaf(k, t.z, b);
+              }
+            }
+          }
+          X[i] = make_float4(r, g, b, 1.0f);
+        }
+        break;
+      }
       case OP_MAT3: {
         const __attribute__((address_space(4))) float* m = d->params + in.arg;
 #pragma unroll

@@ -110,9 +110,11 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // ---------------------------------------------------------------------------
 // value graph
 // ---------------------------------------------------------------------------
-// N_GRID: the bilinear generator (always materialised by a stage of its own,
-// which runs on the 2-D interpreter: it needs pixel coordinates).
-enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID };
+// N_GRID: the bilinear generator; N_STENCIL: box3.frag over a physical
+// texture (its operand is materialised).  Both are always the root of a stage
+// of their own, which runs on the 2-D interpreter: they need pixel
+// coordinates.
+enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL };
 enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL };
 
 struct Micro {
@@ -126,7 +128,7 @@ struct Node {
   uint32_t w = 0, h = 0;
   int input_slot = -1;
   float color[4] = {0, 0, 0, 0};
-  float grid[24] = {};  // N_GRID: u_min, u_max, v_min, v_max, uv_min, uv_max
+  float grid[24] = {};  // N_GRID: u_min, u_max, v_min, v_max, uv_min, uv_max; N_STENCIL: 9 weights
   std::vector<Micro> micro;
   int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a
   uint32_t mode = 0;
@@ -340,6 +342,16 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               n.micro = {{M_MAT3, {1, 0, 0, 0, k * c, -k * s, 0, k * s, k * c}}};
               n.label = "oklab-adjust";
               break;
+            }
+            case ZH_SHADER_BOX3: {
+              if (!need_blob(48)) return fail(ZH_ERR_INVALID_ARGUMENT, "box3 blob needs 48 bytes" + at);
+              const Mat3 m = mat_from_std140(blob.data());
+              n.kind = N_STENCIL;
+              n.a = src;
+              for (int k = 0; k < 9; ++k) n.grid[k] = (float)m[k];
+              n.label = "box3";
+              B.cur[op.dst] = B.add(n);
+              continue;
             }
             case ZH_SHADER_PREMULTIPLY:
               n.kind = N_UNARY;
@@ -600,6 +612,13 @@ struct Emitter {
         depth++;
         return false;
       }
+      case N_STENCIL: {
+        const int s = slot(phys_ref(n.a), phys_desc(n.a));
+        pending_mat_valid = false;
+        ins(OP_BOX3, (uint8_t)s, depth > 0 ? 1 : 0, param(n.grid, 9));
+        depth++;
+        return false;
+      }
       case N_UNARY: {
         bool pm = value(n.a, root);
         for (const Micro& m : n.micro) {
@@ -650,7 +669,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -659,7 +678,8 @@ static std::string describe_code(const StageDesc& d, uint32_t b, uint32_t e) {
   for (uint32_t i = b; i < e; ++i) {
     if (i > b) o << ' ';
     o << op_name(d.code[i].op);
-    if (d.code[i].op == OP_LOAD || d.code[i].op == OP_STORE || d.code[i].op == OP_QUANT) o << '#' << (int)d.code[i].a;
+    if (d.code[i].op == OP_LOAD || d.code[i].op == OP_STORE || d.code[i].op == OP_QUANT || d.code[i].op == OP_BOX3)
+      o << '#' << (int)d.code[i].a;
   }
   return o.str();
 }
@@ -813,7 +833,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     if (!live[id]) continue;
     Node& n = B.nodes[id];
     if (!n.out_slots.empty()) mark((int)id);
-    if (n.kind == N_GRID) mark((int)id);
+    if (n.kind == N_GRID || n.kind == N_STENCIL) mark((int)id);
+    if (n.kind == N_STENCIL && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.kind == N_BYTES && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.uses > 1 && n.kind != N_BYTES) mark((int)id);
   }
@@ -1007,7 +1028,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       // no generator needs the pixel coordinates.
       bool flat = true;
       for (uint32_t i = 0; i < sp.desc.n_instr; ++i)
-        if (sp.desc.code[i].op == OP_GRID) flat = false;
+        if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3) flat = false;
       for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
         const Slot& s = sp.desc.slot[k];
         if (s.width != n.w || s.height != n.h || s.row_stride != (uint64_t)s.width * s.bpp) flat = false;
@@ -1058,7 +1079,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         }
       for (uint32_t i = 0; i < sp.desc.n_instr; ++i) {
         const Instr& in = sp.desc.code[i];
-        if (in.op == OP_LOAD || in.op == OP_STORE) sp.alg_bytes += slot_bytes(sp.desc.slot[in.a]);
+        if (in.op == OP_LOAD || in.op == OP_STORE || in.op == OP_BOX3) sp.alg_bytes += slot_bytes(sp.desc.slot[in.a]);
       }
       std::ostringstream t;
       t << "pointwise " << n.w << "x" << n.h << (flat ? " flat" : "") << ": "

@@ -52,6 +52,9 @@ enum Opcode : uint8_t {
   OP_GRID,      // arg = params of 6 vec4 {u_min, u_max, v_min, v_max, uv_min, uv_max}; b = push.
                 //   X <- mix(u_min, u_max, u) + mix(v_min, v_max, v) + mix(uv_min, uv_max, u v) at the
                 //   pixel centre, (u, v) = ((x + 0.5) / width, (y + 0.5) / height) (bilinear.frag)
+  OP_BOX3,      // a = slot; arg = 9 weights, row-major by y offset; b = push.  X.rgb <- sum of
+                //   w[dy + 1][dx + 1] * texel (x + dx, y + dy), coordinates clamped to the slot; X.a <- 1
+                //   (box3.frag)
 };
 
 struct Instr {
