@@ -89,17 +89,19 @@ __device__ __forceinline__ float4 dec_u8(uint32_t w, const Luts* L) {
   return make_float4((float)r * (1.0f / 255.0f), (float)b1 * (1.0f / 255.0f), (float)b * (1.0f / 255.0f), a);
 }
 
+// The colour bytes of an 8-bit texel (alpha byte 0).
 template <uint32_t F>
-__device__ __forceinline__ uint32_t enc_u8(float4 v, const Luts* L) {
+__device__ __forceinline__ uint32_t enc_u8_rgb(float4 v, const Luts* L) {
   constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
   constexpr bool srgb = F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA;
   const float lo = bgra ? v.z : v.x, hi = bgra ? v.x : v.z;
-  uint32_t w;
-  if (srgb)
-    w = srgb_encode8(lo, L) | (srgb_encode8(v.y, L) << 8) | (srgb_encode8(hi, L) << 16);
-  else
-    w = put_unorm8(hi, 2u, put_unorm8(v.y, 1u, put_unorm8(lo, 0u, 0u)));
-  return put_unorm8(v.w, 3u, w);
+  if (srgb) return srgb_encode8(lo, L) | (srgb_encode8(v.y, L) << 8) | (srgb_encode8(hi, L) << 16);
+  return put_unorm8(hi, 2u, put_unorm8(v.y, 1u, put_unorm8(lo, 0u, 0u)));
+}
+
+template <uint32_t F>
+__device__ __forceinline__ uint32_t enc_u8(float4 v, const Luts* L) {
+  return put_unorm8(v.w, 3u, enc_u8_rgb<F>(v, L));
 }
 
 __device__ __forceinline__ float4 dec_f16(uint32_t lo, uint32_t hi) {

@@ -44,6 +44,19 @@ struct Prog {
     return o;
   }
   static constexpr int nparams = poff(n) > 0 ? poff(n) : 1;
+  // The program's only load (element 0) and its store (the last element) are
+  // 8-bit, and nothing between them writes alpha (the colour ops touch rgb
+  // only): the stored alpha byte is the loaded one, since
+  // rint((a / 255) * 255) == a for every byte a in FP32.
+  static constexpr bool alpha_byte_passes() {
+    if (n < 2 || el_op(el[0]) != OP_LOAD || el_op(el[n - 1]) != OP_STORE) return false;
+    if (fmt_bpp(el_fmt(el[0])) != 4 || fmt_bpp(el_fmt(el[n - 1])) != 4) return false;
+    for (int k = 1; k < n - 1; ++k) {
+      const uint32_t o = el_op(el[k]);
+      if (o != OP_MAT3 && o != OP_CBRT && o != OP_CUBE && o != OP_CLAMP01) return false;
+    }
+    return true;
+  }
   // Elements i .. i + 4 composite a loaded layer over the value with straight
   // alpha in and out: premul, load (push), premul, over, unpremul.
   static constexpr bool straight_over(int i) {
@@ -57,10 +70,13 @@ struct Prog {
 // convex combination up * (a_u / a) + dn * (a_d (1 - a_u) / a), 0 where a = 0.
 // Equal to premul -> over -> unpremul up to FP32 rounding, in fewer
 // instructions, and a weighted mean cannot amplify the inputs' rounding.
+// Where a = 0 both weights are 0 whatever finite r is, so the reciprocal is
+// taken of max(a, FLT_MIN) (a is never positive and below FLT_MIN for 8- and
+// 16-bit alphas; an FP32 alpha that small is treated as FLT_MIN).
 __device__ __forceinline__ float4 over_straight(float4 up, float4 dn) {
   const float k = 1.0f - up.w;
   const float a = fmaf(dn.w, k, up.w);
-  const float r = a > 0.0f ? __builtin_amdgcn_rcpf(a) : 0.0f;
+  const float r = __builtin_amdgcn_rcpf(fmaxf(a, 1.17549435e-38f));
   const float wu = up.w * r, wd = dn.w * k * r;
   return make_float4(fmaf(up.x, wu, dn.x * wd), fmaf(up.y, wu, dn.y * wd), fmaf(up.z, wu, dn.z * wd), a);
 }
@@ -143,8 +159,11 @@ __device__ __forceinline__ void decode(const uint4* r, Px& X, const Luts* L) {
 // Store the lane's pixels: FULL writes kPix consecutive texels with 16-byte
 // stores; otherwise texels [0, n) are written one by one, pstride texels
 // apart.
+// alpha_src: when given (8-bit formats), the 4 words whose top bytes are the
+// alpha bytes to store, instead of encoding X[i].w.
 template <uint32_t F, bool FULL>
-__device__ __forceinline__ void encode(gbyte* q, int n, const Px& X, const Luts* L, int pstride = 1) {
+__device__ __forceinline__ void encode(gbyte* q, int n, const Px& X, const Luts* L, int pstride = 1,
+                                       const uint4* alpha_src = nullptr) {
   constexpr int W = fmt_words(F);
   uint4 o[W];
   if constexpr (F == FMT_F16) {
@@ -157,6 +176,10 @@ __device__ __forceinline__ void encode(gbyte* q, int n, const Px& X, const Luts*
     for (int i = 0; i < kPix; ++i)
       o[i] = make_uint4(__float_as_uint(X[i].x), __float_as_uint(X[i].y), __float_as_uint(X[i].z),
                         __float_as_uint(X[i].w));
+  } else if (alpha_src) {
+    const uint4 a = alpha_src[0];
+    o[0] = make_uint4(enc_u8_rgb<F>(X[0], L) | (a.x & 0xff000000u), enc_u8_rgb<F>(X[1], L) | (a.y & 0xff000000u),
+                      enc_u8_rgb<F>(X[2], L) | (a.z & 0xff000000u), enc_u8_rgb<F>(X[3], L) | (a.w & 0xff000000u));
   } else {
     o[0] = make_uint4(enc_u8<F>(X[0], L), enc_u8<F>(X[1], L), enc_u8<F>(X[2], L), enc_u8<F>(X[3], L));
   }
@@ -237,7 +260,12 @@ __device__ __forceinline__ void exec(const Uni<P>& u, uint64_t x0, int n, const 
       }
     } else if constexpr (op == OP_STORE) {
       constexpr uint32_t F = el_fmt(e);
-      encode<F, FULL>((gbyte*)u.base[I] + x0 * fmt_bpp(F), n, X, L, pstride);
+      if constexpr (P::alpha_byte_passes()) {
+        // Alpha bytes straight from the loaded texels (raw[0]: the 4 words).
+        encode<F, FULL>((gbyte*)u.base[I] + x0 * fmt_bpp(F), n, X, L, pstride, raw);
+      } else {
+        encode<F, FULL>((gbyte*)u.base[I] + x0 * fmt_bpp(F), n, X, L, pstride);
+      }
     } else {
       static_assert(op == OP_NOP, "opcode without a specialised body");
     }
