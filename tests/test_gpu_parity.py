@@ -360,6 +360,15 @@ def test_derivative_height_f32_ragged(ctx):
     report("derivative_height_f32", max_abs=f"{err.max():.2e}")
     assert np.all(err <= 1e-5 * np.abs(want) + 1e-6)
 
+    # Of a constant image (materialised for the stencil): zero colour, alpha 1.
+    d8 = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    s = cmd.solid_rgba(d8, (0.2, 0.4, 0.6, 0.5))
+    o, od = cmd.output(cmd.derivative(s, z.DerivativeMethod.Scharr3, z.Direction.Width))
+    out, _ = run(ctx, cmd, {}, {}, {o: od})
+    px = PL.u8_image(out[o], w, h)
+    assert np.all(px[..., :3] == 0) and np.all(px[..., 3] == 255)
+
 
 def test_linear_f32_output_within_1e5(ctx):
     """A linear RGBA32F chain (P3 conversion, Oklab round trip with chroma and

@@ -454,7 +454,8 @@ zh_status zs_command_derivative(zs_command* cmd, uint32_t src, uint32_t method, 
   op.kind = K_DERIVATIVE;
   op.src = src;
   op.blob = std140(m);
-  op.desc = s;
+  s.row_stride = 0;
+  op.desc = normalized(s);
   *reg = push(cmd, op);
   return ZH_OK;
 }

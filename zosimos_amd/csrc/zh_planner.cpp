@@ -140,6 +140,7 @@ struct Node {
   int uses = 0;
   std::vector<int> out_slots;
   bool mat = false;     // materialised by its own stage
+  bool sampled = false; // read as a texture by a stencil: stored even when constant
   int buffer = -1;      // intermediate buffer id
 };
 
@@ -823,8 +824,9 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
   // paints that cannot be the core of a resample stage.
   auto mark = [&](int id) {
     Node& n = B.nodes[id];
-    // Inputs and constants are read in place unless an output needs them.
-    if ((n.kind == N_INPUT || n.kind == N_SOLID) && n.out_slots.empty()) return false;
+    // Inputs and constants are read in place unless an output or a stencil
+    // needs their texels.
+    if ((n.kind == N_INPUT || (n.kind == N_SOLID && !n.sampled)) && n.out_slots.empty()) return false;
     if (n.mat) return false;
     n.mat = true;
     return true;
@@ -834,7 +836,10 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     Node& n = B.nodes[id];
     if (!n.out_slots.empty()) mark((int)id);
     if (n.kind == N_GRID || n.kind == N_STENCIL) mark((int)id);
-    if (n.kind == N_STENCIL && B.nodes[n.a].kind != N_INPUT) mark(n.a);
+    if (n.kind == N_STENCIL && B.nodes[n.a].kind != N_INPUT) {
+      B.nodes[n.a].sampled = true;
+      mark(n.a);
+    }
     if (n.kind == N_BYTES && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.uses > 1 && n.kind != N_BYTES) mark((int)id);
   }
@@ -901,7 +906,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
   // Intermediate buffers for materialised values read by other stages.
   for (size_t id = 0; id < B.nodes.size(); ++id) {
     Node& n = B.nodes[id];
-    if (!live[id] || !n.mat || n.kind == N_INPUT || n.kind == N_SOLID) continue;
+    if (!live[id] || !n.mat || n.kind == N_INPUT || (n.kind == N_SOLID && !n.sampled)) continue;
     const bool internal = n.uses > 0;
     if (!internal) continue;
     zh_descriptor d = n.desc;
