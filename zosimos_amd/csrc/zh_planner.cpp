@@ -735,11 +735,12 @@ static int warp_variant(const StageDesc& d) {
     f_ = d.slot[c[0].a].fmt;
     if (f_ == FMT_GENERIC) return -1;
     // The kernel addresses every source over the whole layer grid with
-    // 32-bit byte offsets, one row stride for both sources of WSRC_OVER2.
+    // 32-bit byte offsets, one row stride for both sources of WSRC_OVER2, and
+    // fetches horizontal tap pairs (sources at least 2 texels wide).
     for (uint32_t i = 0; i < n; ++i)
       if (c[i].op == OP_LOAD) {
         const Slot& sl = d.slot[c[i].a];
-        if (sl.width != L.src_w || sl.height != L.src_h || sl.row_stride != d.slot[c[0].a].row_stride ||
+        if (sl.width != L.src_w || sl.height != L.src_h || sl.width < 2 || sl.row_stride != d.slot[c[0].a].row_stride ||
             sl.row_stride * sl.height >= (1ull << 32) || sl.row_stride >= (1u << 23) || sl.height >= (1u << 23))
           return -1;
       }

@@ -164,10 +164,32 @@ struct LaneMap {
 // loaded, then blended:
 // 4 * kPix loads in flight per lane.  Straight-alpha sources are weighted by
 // w * alpha (premultiplying inside the weighted sum).
+// Two horizontally adjacent texels of 4 or 8 bytes, fetched by one load.
+template <uint32_t B> struct PairT { typedef u32x4 type; };
+template <> struct PairT<4> { typedef u32x2 type; };
+template <uint32_t F>
+__device__ __forceinline__ typename RawT<F>::type pair_half(const typename PairT<fmt_bpp(F)>::type& p, int h) {
+  if constexpr (fmt_bpp(F) == 8) {
+    u32x2 r;
+    r.x = h ? p.z : p.x;
+    r.y = h ? p.w : p.y;
+    return r;
+  } else {
+    return h ? p.y : p.x;
+  }
+}
+
 template <class V, bool SEP, bool P64>
 __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp, int n, Px& X, const float* rep) {
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
+  // 4- and 8-byte texels: the two taps of a row are one load.  The left tap
+  // is clamped to [0, sw - 2] (the planner admits sources at least 2 texels
+  // wide), so the pair lies inside its row, and each texel of the pair gets
+  // the tent weight max(0, 1 - |sample - texel|): the bilinear weight for
+  // the sample's own taps, 0 for a texel the clamp substituted at a border.
+  constexpr bool PAIR = B <= 8;
+  typedef typename PairT<B <= 8 ? B : 8>::type PR;
   uint32_t off[kPix][4];
   float w[kPix][4];
   bool any = false;
@@ -242,27 +264,60 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
       r0 = (uint32_t)__mul24(j, (int)a.stride);
       r1 = r0 + a.stride;
     }
-    const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
-    w[k][0] = wx0 * wy0;
-    w[k][1] = wx1 * wy0;
-    w[k][2] = wx0 * wy1;
-    w[k][3] = wx1 * wy1;
     any = any || ((xa || xb) && (ya || yb));
-    const uint32_t c0 = (uint32_t)i * B;
-    off[k][0] = opaque(r0 + c0);
-    off[k][1] = opaque(off[k][0] + B);
-    off[k][2] = opaque(r1 + c0);
-    off[k][3] = opaque(off[k][2] + B);
+    if constexpr (PAIR) {
+      const int ic = min(max(i, 0), a.sw - 2);
+      const float sx = (float)(i - ic) + ax;
+      const float wl = __builtin_amdgcn_fmed3f(1.0f - fabsf(sx), 0.0f, 1.0f);
+      const float wh = __builtin_amdgcn_fmed3f(1.0f - fabsf(sx - 1.0f), 0.0f, 1.0f);
+      w[k][0] = wl * wy0;
+      w[k][1] = wh * wy0;
+      w[k][2] = wl * wy1;
+      w[k][3] = wh * wy1;
+      const uint32_t c0 = (uint32_t)ic * B;
+      off[k][0] = opaque(r0 + c0);
+      off[k][1] = opaque(r1 + c0);
+    } else {
+      const float wx0 = xa ? 1.0f - ax : 0.0f, wx1 = xb ? ax : 0.0f;
+      w[k][0] = wx0 * wy0;
+      w[k][1] = wx1 * wy0;
+      w[k][2] = wx0 * wy1;
+      w[k][3] = wx1 * wy1;
+      const uint32_t c0 = (uint32_t)i * B;
+      off[k][0] = opaque(r0 + c0);
+      off[k][1] = opaque(off[k][0] + B);
+      off[k][2] = opaque(r1 + c0);
+      off[k][3] = opaque(off[k][2] + B);
+    }
   }
   if (!any) return;
   R t0[kPix][4], t1[kPix][4];
+  if constexpr (PAIR) {
+    PR p0[kPix][2], p1[kPix][2];
 #pragma unroll
-  for (int k = 0; k < kPix; ++k)
+    for (int k = 0; k < kPix; ++k)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      t0[k][t] = bload<R>(a.src0, off[k][t]);
-      if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = bload<R>(a.src1, off[k][t]);
-    }
+      for (int r = 0; r < 2; ++r) {
+        p0[k][r] = bload<PR>(a.src0, off[k][r]);
+        if constexpr (V::SRC == WSRC_OVER2) p1[k][r] = bload<PR>(a.src1, off[k][r]);
+      }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < kPix; ++k)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        t0[k][t] = pair_half<V::F>(p0[k][t >> 1], t & 1);
+        if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = pair_half<V::F>(p1[k][t >> 1], t & 1);
+      }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPix; ++k)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        t0[k][t] = bload<R>(a.src0, off[k][t]);
+        if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = bload<R>(a.src1, off[k][t]);
+      }
+  }
   // Keep every load above this point: left free, the scheduler may trade
   // the loads in flight for registers and wait on each tap in turn.
   __builtin_amdgcn_sched_barrier(0);
