@@ -292,22 +292,14 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
   }
   if (!any) return;
   R t0[kPix][4], t1[kPix][4];
+  PR p0[kPix][2], p1[kPix][2];
   if constexpr (PAIR) {
-    PR p0[kPix][2], p1[kPix][2];
 #pragma unroll
     for (int k = 0; k < kPix; ++k)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         p0[k][r] = bload<PR>(a.src0, off[k][r]);
         if constexpr (V::SRC == WSRC_OVER2) p1[k][r] = bload<PR>(a.src1, off[k][r]);
-      }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < kPix; ++k)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        t0[k][t] = pair_half<V::F>(p0[k][t >> 1], t & 1);
-        if constexpr (V::SRC == WSRC_OVER2) t1[k][t] = pair_half<V::F>(p1[k][t >> 1], t & 1);
       }
   } else {
 #pragma unroll
@@ -326,11 +318,11 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const float4 c = raw_dec<V::F>(t0[k][t], rep);
+      const float4 c = raw_dec<V::F>(PAIR ? pair_half<V::F>(p0[k][t >> 1], t & 1) : t0[k][t], rep);
       if constexpr (V::SRC == WSRC_OVER2) {
         // Premultiplied 'over' of the two sources, then weighted.
         const float4 lo = premul4(c);
-        const float4 h = premul4(raw_dec<V::F>(t1[k][t], rep));
+        const float4 h = premul4(raw_dec<V::F>(PAIR ? pair_half<V::F>(p1[k][t >> 1], t & 1) : t1[k][t], rep));
         const float q = 1.0f - h.w, wt = w[k][t];
         const float4 m = make_float4(fmaf(lo.x, q, h.x), fmaf(lo.y, q, h.y), fmaf(lo.z, q, h.z), fmaf(lo.w, q, h.w));
         s = make_float4(fmaf(m.x, wt, s.x), fmaf(m.y, wt, s.y), fmaf(m.z, wt, s.z), fmaf(m.w, wt, s.w));
