@@ -1196,6 +1196,22 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       ly.src_h = L.src_h;
       for (int k = 0; k < 9; ++k) ly.inv[k] = L.inv[k];
       for (int k = 0; k < 4; ++k) ly.cover[k] = L.cover[k];
+      // An affine map takes the output's pixel-centre hull to a
+      // parallelogram; when its corners (minus the half texel) lie inside
+      // (-1, w) x (-1, h), so does every tile's footprint, and the kernels'
+      // per-tile test would always pass.
+      ly.covers_all = 0;
+      if (L.inv[6] == 0.0 && L.inv[7] == 0.0 && L.inv[8] == 1.0) {
+        bool inside = true;
+        const double xs[2] = {0.5, n.w - 0.5}, ys[2] = {0.5, n.h - 0.5};
+        for (double x : xs)
+          for (double y : ys) {
+            const double u = L.inv[0] * x + L.inv[1] * y + L.inv[2] - 0.5;
+            const double v = L.inv[3] * x + L.inv[4] * y + L.inv[5] - 0.5;
+            inside = inside && u > -1.0 && v > -1.0 && u < (double)L.src_w && v < (double)L.src_h;
+          }
+        ly.covers_all = inside ? 1u : 0u;
+      }
       ly.prog_begin = (uint32_t)E.code.size();
       E.depth = 0;
       E.pending_mat_valid = false;

@@ -105,7 +105,7 @@ struct Layer {
   uint32_t mode;
   uint32_t prog_begin, prog_end;  // source program (evaluated on the source grid)
   uint32_t src_w, src_h;          // source grid
-  uint32_t pad;
+  uint32_t covers_all;            // affine map: every output pixel's taps meet the source (no tile test)
   // Output pixel centre (x + 0.5, y + 0.5) -> source texel space, in FP64
   // (at 8K an FP32 ulp is 5e-4 px):
   //   u = (m0 px + m1 py + m2) / (m6 px + m7 py + m8), v likewise with m3..m5.

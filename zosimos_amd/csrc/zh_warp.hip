@@ -374,7 +374,7 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
   const uint32_t nl = d->n_layers;
   for (uint32_t l = 0; l < nl; ++l) {
     const Layer Ly = ldc(&d->layer[l]);
-    if (!layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;  // uniform
+    if (!Ly.covers_all && !layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;  // uniform
     const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
     // Map of the tile's first pixel centre: integer texel + FP32 remainder
     // (8-bit outputs), or the lane's first pixel in FP64 (float outputs).
