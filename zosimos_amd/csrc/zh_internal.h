@@ -118,6 +118,8 @@ struct zh_program {
   zh::StageDesc* d_stages = nullptr;       // device copies, one per stage
   zh::StageDesc* h_stages = nullptr;       // pinned host staging
   std::vector<void*> d_intermediates;
+  std::vector<uint32_t*> d_tile_layers;  // per stage: the tile-layer masks (warp stages)
+  std::vector<uint32_t> tile_layer_words;  // their capacities in words
   std::vector<void*> bound_inputs, bound_outputs;
   bool uploaded = false;
   hipGraph_t graph = nullptr;

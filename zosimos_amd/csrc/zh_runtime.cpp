@@ -299,6 +299,10 @@ static void free_device(zh_program* p) {
   p->units.clear();
   for (void* b : p->d_intermediates) hipFree(b);
   p->d_intermediates.clear();
+  for (uint32_t* m : p->d_tile_layers)
+    if (m) hipFree(m);
+  p->d_tile_layers.clear();
+  p->tile_layer_words.clear();
   if (p->d_stages) hipFree(p->d_stages);
   if (p->h_stages) hipHostFree(p->h_stages);
   p->d_stages = nullptr;
@@ -359,6 +363,62 @@ static void make_units(zh_program* p) {
   }
 }
 
+// The kernels' per-tile layer test (warp_tile in zh_warp.hip) on the host:
+// does any bilinear tap of the tile's pixel centres land in the layer's
+// source?  Widened by 1e-6 texel so that rounding never drops a tile the
+// kernel's own test would keep (a kept tile whose taps all miss adds 0).
+static bool tile_meets_layer(const Layer& L, double ex0, double ey0, double ex1, double ey1) {
+  if (L.covers_all) return true;
+  const double* m = L.inv;
+  const double u[4] = {m[0] * ex0 + m[1] * ey0 + m[2], m[0] * ex1 + m[1] * ey0 + m[2], m[0] * ex0 + m[1] * ey1 + m[2],
+                       m[0] * ex1 + m[1] * ey1 + m[2]};
+  const double v[4] = {m[3] * ex0 + m[4] * ey0 + m[5], m[3] * ex1 + m[4] * ey0 + m[5], m[3] * ex0 + m[4] * ey1 + m[5],
+                       m[3] * ex1 + m[4] * ey1 + m[5]};
+  const double umin = std::min(std::min(u[0], u[1]), std::min(u[2], u[3])) - 0.5;
+  const double umax = std::max(std::max(u[0], u[1]), std::max(u[2], u[3])) - 0.5;
+  const double vmin = std::min(std::min(v[0], v[1]), std::min(v[2], v[3])) - 0.5;
+  const double vmax = std::max(std::max(v[0], v[1]), std::max(v[2], v[3])) - 0.5;
+  const double e = 1e-6;
+  return umax > -1.0 - e && vmax > -1.0 - e && umin < (double)L.src_w + e && vmin < (double)L.src_h + e;
+}
+
+// The tile-layer masks of warp stage s for its current tile shape (host
+// computation, uploaded once; the kernels then read one byte per tile).
+static zh_status tile_layer_masks(zh_program* p, size_t s) {
+  StageDesc& d = p->plan.stages[s].desc;
+  if (d.kind != STAGE_RESAMPLE || d.variant < 0 || d.n_layers == 0 || d.n_layers > 8) return ZH_OK;
+  if (p->d_tile_layers.size() < p->plan.stages.size()) {
+    p->d_tile_layers.resize(p->plan.stages.size(), nullptr);
+    p->tile_layer_words.resize(p->plan.stages.size(), 0);
+  }
+  const uint32_t tiles = d.tiles_x * d.tiles_y, words = (tiles + 3) / 4;
+  std::vector<uint32_t> m(std::max(1u, words), 0u);
+  const int TW = (int)d.tile_w, TH = (int)d.tile_h, W = (int)d.width, H = (int)d.height;
+  for (uint32_t ty = 0; ty < d.tiles_y; ++ty)
+    for (uint32_t tx = 0; tx < d.tiles_x; ++tx) {
+      const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
+      const double ex1 = std::min((int)tx * TW + TW, W) - 0.5, ey1 = std::min((int)ty * TH + TH, H) - 0.5;
+      uint32_t bits = 0;
+      for (uint32_t l = 0; l < d.n_layers; ++l)
+        if (tile_meets_layer(d.layer[l], ex0, ey0, ex1, ey1)) bits |= 1u << l;
+      const uint32_t t = ty * d.tiles_x + tx;
+      m[t >> 2] |= bits << ((t & 3u) * 8u);
+    }
+  // Launches still in flight may read the old masks.
+  ZH_HIP(hipStreamSynchronize(p->ctx->stream));
+  if (p->tile_layer_words[s] < m.size()) {
+    if (p->d_tile_layers[s]) hipFree(p->d_tile_layers[s]);
+    p->d_tile_layers[s] = nullptr;
+    p->tile_layer_words[s] = 0;
+    ZH_HIP(hipMalloc(&p->d_tile_layers[s], m.size() * sizeof(uint32_t)));
+    p->tile_layer_words[s] = (uint32_t)m.size();
+  }
+  ZH_HIP(hipMemcpy(p->d_tile_layers[s], m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  d.tile_layers = p->d_tile_layers[s];
+  if (p->h_stages) p->h_stages[s].tile_layers = d.tile_layers;
+  return ZH_OK;
+}
+
 static zh_status alloc_device(zh_program* p) {
   zh_context* ctx = p->ctx;
   ZH_HIP(hipSetDevice(ctx->device));
@@ -372,6 +432,10 @@ static zh_status alloc_device(zh_program* p) {
     p->d_intermediates.push_back(b);
   }
   make_units(p);
+  for (size_t s = 0; s < n; ++s) {
+    zh_status st = tile_layer_masks(p, s);
+    if (st != ZH_OK) return st;
+  }
   for (Unit& u : p->units) {
     if (!u.batch) continue;
     BatchDesc b{};
@@ -540,6 +604,12 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     for (size_t i = 0; i < np.stages.size(); ++i) np.stages[i].lds_bytes = p->plan.stages[i].lds_bytes;
     p->plan = std::move(np);
     p->uploaded = false;
+    // The new maps move the layers' footprints.
+    if (p->ctx)
+      for (size_t s = 0; s < p->plan.stages.size(); ++s) {
+        zh_status ms = tile_layer_masks(p, s);
+        if (ms != ZH_OK) return ms;
+      }
     return ZH_OK;
   }
   if (p->ctx) {
@@ -809,12 +879,15 @@ static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th)
       const size_t a = sp.text.find(" tile "), b = a == std::string::npos ? a : sp.text.find(' ', a + 6);
       if (b != std::string::npos) sp.text.replace(a + 6, b - (a + 6), std::to_string(tw) + "x" + std::to_string(th));
     }
+    zh_status ms = tile_layer_masks(p, s);
+    if (ms != ZH_OK) return ms;
     StageDesc& h = p->h_stages[s];
     h.tile_w = d.tile_w;
     h.tile_h = d.tile_h;
     h.band_h = d.band_h;
     h.tiles_x = d.tiles_x;
     h.tiles_y = d.tiles_y;
+    h.tile_layers = d.tile_layers;
     prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
   }
   hipStream_t st = p->ctx->stream;

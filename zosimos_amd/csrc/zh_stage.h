@@ -135,6 +135,10 @@ struct StageDesc {
   uint64_t copy_src, copy_dst, copy_src_stride, copy_dst_stride, copy_row_bytes;
   uint32_t copy_rows, copy_pad;
   const void* srgb_lut;       // the context's 8-bit sRGB tables (Luts of zh_codec.h)
+  // Warp stages: per tile (row-major) a byte of the layers whose footprint
+  // meets the tile, four tiles per word (bit l: layer l); null: the kernel
+  // tests each layer itself.
+  const uint32_t* tile_layers;
   uint32_t* error_flag;       // set to 1 by a kernel that met an impossible case
   Layer layer[kMaxLayers];
   Slot slot[kMaxSlots];

@@ -372,9 +372,14 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
   const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
   const double ex1 = min((int)tx * TW + TW, W) - 0.5, ey1 = min((int)ty * TH + TH, H) - 0.5;
   const uint32_t nl = d->n_layers;
+  // Layers meeting this tile: the host's per-tile mask when there is one.
+  const uint32_t* tl = d->tile_layers;
+  const uint32_t t = ty * d->tiles_x + tx;
+  const uint32_t lmask = tl ? (cword(tl, t >> 2) >> ((t & 3u) * 8u)) & 255u : 255u;
   for (uint32_t l = 0; l < nl; ++l) {
+    if (!((lmask >> l) & 1u)) continue;  // uniform
     const Layer Ly = ldc(&d->layer[l]);
-    if (!Ly.covers_all && !layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;  // uniform
+    if (!tl && !Ly.covers_all && !layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;
     const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
     // Map of the tile's first pixel centre: integer texel + FP32 remainder
     // (8-bit outputs), or the lane's first pixel in FP64 (float outputs).
