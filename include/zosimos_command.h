@@ -89,8 +89,23 @@ zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_
  * (command.rs:2598-2609, 3344-3409).  Roberts and the Scharr 4/5 variants are
  * not implemented, as in the reference. */
 zh_status zs_command_derivative(zs_command* cmd, uint32_t src, uint32_t method, uint32_t direction, uint32_t* reg);
+/* CommandBuffer::buffer_init, command.rs:1777-1790: a buffer register holding
+ * a copy of `len` bytes. */
+zh_status zs_command_buffer_init(zs_command* cmd, const void* data, uint64_t len, uint32_t* reg);
+/* CommandBuffer::buffer_zero, command.rs:1793-1801: `len` zero bytes. */
+zh_status zs_command_buffer_zero(zs_command* cmd, uint64_t len, uint32_t* reg);
+/* Byte size of a buffer register (GenericBuffer::size, command.rs:93-96). */
+zh_status zs_command_buffer_size(zs_command* cmd, uint32_t buffer, uint64_t* len);
+/* CommandBuffer::from_buffer, command.rs:937-968: the image `desc` read from a
+ * buffer register whose rows are padded to 256 bytes (Descriptor::to_aligned,
+ * buffer.rs:121-134).  ZH_ERR_TYPE unless `buffer` is a buffer;
+ * ZH_ERR_INVALID_ARGUMENT (CommandError::INVALID_CALL) when the buffer is
+ * shorter than height padded rows.  The register's own descriptor is tight. */
+zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_descriptor* desc, uint32_t* reg);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
- * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob. */
+ * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob.  A
+ * buffer register's knob takes the buffer's bytes, of the same length
+ * (WithKnob::buffer_init, command.rs:1937-1940). */
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);
 /* Descriptor of a register (RegisterDescription::Texture). */
 zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc);

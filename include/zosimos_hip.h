@@ -145,6 +145,13 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_HIGH_DRAW 2    /* High::PushOperand* + High::DrawInto */
 #define ZH_HIGH_COPY 3    /* High::Copy: byte copy `src` -> `dst` */
 #define ZH_HIGH_UNINIT 4  /* High::Uninit */
+/* A buffer register's bytes read as texture `dst` (Op::BufferInit and
+ * ConstructOp::FromBuffer, command.rs:937-968, 1777-1801): `params` holds the
+ * bytes, laid out by the texture's descriptor (its row_stride; 256-byte rows
+ * for a device buffer, buffer.rs:121-134), at least row_stride x height of
+ * them.  The program keeps them in device memory; a knob replaces them
+ * (Environment::knob of a WithBuffer register, run.rs:1292-1306). */
+#define ZH_HIGH_BUFFER 5
 
 /* Target, program.rs:142-148. */
 #define ZH_TARGET_DISCARD 0 /* cleared first (to opaque blue, program.rs:1495-1499) */

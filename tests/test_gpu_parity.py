@@ -497,6 +497,78 @@ def test_knob_matches_rebuilt_program(ctx):
         assert np.array_equal(got, ref[o2])
 
 
+def _pitched(data, pitch, rows):
+    """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
+    buf = np.zeros(pitch * rows, np.uint8)
+    buf.reshape(rows, pitch)[:, :data.shape[1]] = data
+    return buf
+
+
+@pytest.mark.parametrize("desc", [z.Descriptor.rgba8_srgb(37, 5), z.Descriptor.rgba16f_linear(40, 3),
+                                  z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma),
+                                               z.Color.SRGB)])
+def test_from_buffer_bit_exact(ctx, desc):
+    """command.rs:937-968: from_buffer reads rows padded to 256 bytes; the
+    tight output holds each row's bytes."""
+    row = desc.width * desc.texel.bytes
+    pitch = (row + 255) // 256 * 256
+    data = rand_u8((desc.height, row), 31)
+    cmd = z.CommandBuffer()
+    buf = cmd.buffer_init(_pitched(data, pitch, desc.height).tobytes() + b"\x5a" * 13)
+    o, od = cmd.output(cmd.from_buffer(buf, desc))
+    out, exe = run(ctx, cmd, {}, {}, {o: od})
+    report("from_buffer", shape=(desc.width, desc.height), pitch=pitch)
+    assert "copy" in exe.describe()
+    assert np.array_equal(out[o].reshape(desc.height, row), data)
+
+
+def test_from_buffer_feeds_stages_like_an_input(ctx):
+    """A buffer image read in place by a colour conversion and an affine
+    resample equals the same program over a bound input of those texels."""
+    d = z.Descriptor.rgba8_srgb(45, 30)
+    data = rand_u8((30, 180), 32)
+    p3 = z.Color(primaries=z.Primaries.DisplayP3, transfer=z.Transfer.Srgb)
+
+    def body(cmd, src):
+        c = cmd.color_convert(src, p3, RGBA8)
+        canvas = cmd.solid_rgba(z.Descriptor(45, 30, RGBA8, p3), (0, 0, 0, 0))
+        m = z.workloads.about_centre(45, 30, 0.9, 0.4).transformation
+        return cmd.output(cmd.affine(canvas, z.Affine(tuple(m), z.AffineSample.BiLinear), c))
+
+    cmd = z.CommandBuffer()
+    o, od = body(cmd, cmd.from_buffer(cmd.buffer_init(_pitched(data, 256, 30).tobytes()), d))
+    got, _ = run(ctx, cmd, {}, {}, {o: od})
+    cmd2 = z.CommandBuffer()
+    a = cmd2.input(d)
+    o2, od2 = body(cmd2, a)
+    want, _ = run(ctx, cmd2, {a: d}, {a: data}, {o2: od2})
+    assert np.array_equal(got[o], want[o2])
+
+
+def test_buffer_knob_replaces_bytes(ctx):
+    """A knob on a buffer register (WithKnob::buffer_init) swaps the bytes a
+    from_buffer image reads, launch after launch, through the captured graph."""
+    d = z.Descriptor.rgba8_srgb(20, 6)
+    cmd = z.CommandBuffer()
+    buf = cmd.buffer_zero(256 * 6)
+    k = cmd.knob(buf)
+    o, od = cmd.output(cmd.color_convert(cmd.from_buffer(buf, d), z.Color.SRGB, RGBA8))
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    exe.launch({}, {o: out})
+    ctx.sync()
+    assert not out.download().any()
+    for seed in (40, 41, 42):
+        data = rand_u8((6, 80), seed)
+        exe.knob(k, _pitched(data, 256, 6).tobytes())
+        for _ in range(2):
+            exe.launch({}, {o: out})
+        ctx.sync()
+        assert np.array_equal(out.download().reshape(6, 80), data)
+    with pytest.raises(z.ZosimosError):
+        exe.knob(k, b"\0" * 100)
+
+
 def test_bind_errors(ctx):
     d = z.Descriptor.rgba8_srgb(16, 16)
     cmd = z.CommandBuffer()

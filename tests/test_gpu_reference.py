@@ -16,8 +16,11 @@ derivative filters, ``derived_<method>``); from ``knobs.rs`` one bilinear
 executable re-parameterised through a knob five times
 (``bilinear-knob-0`` .. ``4``); from ``buffer.rs`` ``run_bilinear``
 (``bilinear_from_buffer``, its six parameters given directly rather than
-through a buffer register) and ``run_from_buffer`` (an 8x8 Luma8 image read
-from a buffer with 256-byte rows, ``from_buffer``).
+through a buffer register), ``run_from_buffer`` (an 8x8 Luma8 image read
+from a buffer with 256-byte rows, ``from_buffer``, both as a bound input of
+that layout and through ``buffer_init``/``from_buffer``) and
+``run_from_buffer_knob`` (the same through a buffer knob,
+``from_buffer-with-knob``).
 """
 import json
 import math
@@ -231,6 +234,49 @@ def test_from_buffer_matches_reference_hash(ctx):
     got = out.download().reshape(8, 8)
     assert np.array_equal(got, a.reshape(8, 256)[:, :8])
     check("from_buffer", np.stack([got, got, got, np.full_like(got, 255)], -1))
+
+
+LUMA8 = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)
+
+
+def test_buffer_init_from_buffer_matches_reference_hash(ctx):
+    """buffer.rs:39-65 through the buffer register itself: buffer_init of the
+    2,048 bytes, from_buffer as an 8x8 Luma8 image, output tight."""
+    a = np.full(8 * 256, 0xFF, np.uint8)
+    a[256:264] = 0
+    cmd = z.CommandBuffer()
+    buf = cmd.buffer_init(a.tobytes())
+    o, od = cmd.output(cmd.from_buffer(buf, LUMA8))
+    assert od.row_stride in (0, 8)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    exe.launch({}, {o: out})
+    ctx.sync()
+    got = out.download().reshape(8, 8)
+    assert np.array_equal(got, a.reshape(8, 256)[:, :8])
+    check("from_buffer", np.stack([got, got, got, np.full_like(got, 255)], -1))
+
+
+def test_from_buffer_knob_matches_reference_hash(ctx):
+    """buffer.rs:66-119: a knobbed buffer of 0xff bytes read as an 8x8 Luma8
+    image; the launch patches row 1 to black through the knob."""
+    a = np.full(8 * 256, 0xFF, np.uint8)
+    cmd = z.CommandBuffer()
+    buf = cmd.buffer_init(a.tobytes())
+    k = cmd.knob(buf)
+    o, od = cmd.output(cmd.from_buffer(buf, LUMA8))
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    exe.launch({}, {o: out})
+    ctx.sync()
+    assert np.all(out.download() == 0xFF)
+    a[256:264] = 0
+    exe.knob(k, a.tobytes())
+    exe.launch({}, {o: out})
+    ctx.sync()
+    got = out.download().reshape(8, 8)
+    assert np.array_equal(got, a.reshape(8, 256)[:, :8])
+    check("from_buffer-with-knob", np.stack([got, got, got, np.full_like(got, 255)], -1))
 
 
 @pytest.mark.parametrize("method,name", [(z.DerivativeMethod.Scharr3, "Scharr3"),

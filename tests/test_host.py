@@ -145,6 +145,35 @@ def test_knob_length_and_replan_without_device():
     expect(z.Status.INVALID_ARGUMENT, e.knob, k + 7, b"\0" * 48)
 
 
+def test_buffer_commands():
+    """buffer_init / buffer_zero / from_buffer (command.rs:937-968,
+    1777-1801): type checks, the 256-byte row requirement, and the plan."""
+    cmd = z.CommandBuffer()
+    a = cmd.input(rgba8(8, 8))
+    luma = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)
+    buf = cmd.buffer_init(b"\xff" * (8 * 256))
+    assert cmd.buffer_size(buf) == 2048
+    assert cmd.buffer_size(cmd.buffer_zero(17)) == 17
+    # not a buffer / not an image
+    expect(z.Status.TYPE, cmd.from_buffer, a, luma)
+    expect(z.Status.TYPE, cmd.describe, buf)
+    expect(z.Status.TYPE, cmd.color_convert, buf, z.Color.SRGB, z.Texel(10, 8))
+    expect(z.Status.TYPE, cmd.buffer_size, a)
+    # rows padded to 256 bytes: nine rows do not fit, eight do
+    expect(z.Status.INVALID_ARGUMENT, cmd.from_buffer, buf, luma.with_(height=9))
+    expect(z.Status.INVALID_ARGUMENT, cmd.from_buffer, buf, rgba8(65, 8))
+    img = cmd.from_buffer(buf, luma)
+    assert cmd.describe(img).row_stride in (0, 8)
+    expect(z.Status.TYPE, cmd.knob, img)
+    k = cmd.knob(buf)
+    o, od = cmd.output(img)
+    assert (od.width, od.height) == (8, 8)
+    e = plan(cmd)
+    assert "copy 64 bytes" in e.describe()
+    expect(z.Status.KNOB_LENGTH, e.knob, k, b"\0" * 2047)
+    e.knob(k, b"\0" * 2048)
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:

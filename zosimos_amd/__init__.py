@@ -300,6 +300,27 @@ class CommandBuffer:
         filter (``DerivativeMethod``) along a ``Direction`` (box3.frag)."""
         return self._reg(self._lib.zs_command_derivative, src, method, direction)
 
+    def buffer_init(self, data: bytes) -> int:
+        """CommandBuffer::buffer_init (command.rs:1777-1790): a buffer register
+        holding a copy of ``data``."""
+        data = bytes(data)
+        buf = ctypes.create_string_buffer(data, len(data)) if data else None
+        return self._reg(self._lib.zs_command_buffer_init, buf, len(data))
+
+    def buffer_zero(self, length: int) -> int:
+        """CommandBuffer::buffer_zero (command.rs:1793-1801)."""
+        return self._reg(self._lib.zs_command_buffer_zero, length)
+
+    def buffer_size(self, reg: int) -> int:
+        n = ctypes.c_uint64()
+        _check(self._lib.zs_command_buffer_size(self._h, reg, ctypes.byref(n)))
+        return n.value
+
+    def from_buffer(self, buffer: int, desc: Descriptor) -> int:
+        """CommandBuffer::from_buffer (command.rs:937-968): the image ``desc``
+        read from a buffer whose rows are padded to 256 bytes."""
+        return self._reg(self._lib.zs_command_from_buffer, buffer, ctypes.byref(desc.c()))
+
     def knob(self, reg: int) -> int:
         k = ctypes.c_int32()
         _check(self._lib.zs_command_knob(self._h, reg, ctypes.byref(k)))

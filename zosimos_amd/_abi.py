@@ -104,6 +104,10 @@ _SIGNATURES = {
     "zs_command_oklab_adjust": (i32, [vp, u32, f32, f32, ctypes.POINTER(u32)]),
     "zs_command_chromatic_adaptation": (i32, [vp, u32, u32, u32, ctypes.POINTER(u32)]),
     "zs_command_derivative": (i32, [vp, u32, u32, u32, ctypes.POINTER(u32)]),
+    "zs_command_buffer_init": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
+    "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
+    "zs_command_buffer_size": (i32, [vp, u32, ctypes.POINTER(ctypes.c_uint64)]),
+    "zs_command_from_buffer": (i32, [vp, u32, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
     "zs_command_knob": (i32, [vp, u32, ctypes.POINTER(i32)]),
     "zs_command_describe": (i32, [vp, u32, ctypes.POINTER(DescriptorC)]),
     "zs_command_compile": (i32, [vp, ctypes.POINTER(vp)]),

@@ -17,7 +17,7 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER
 };
 
 struct CmdOp {
@@ -26,7 +26,7 @@ struct CmdOp {
   uint32_t src = 0, above = 0;
   zh_rect rect{};
   uint32_t shader = 0;                  // colour conversion shader key
-  std::vector<uint8_t> blob;            // std140 parameters
+  std::vector<uint8_t> blob;            // std140 parameters (buffers: their bytes)
   float matrix[9] = {0};
   uint32_t sampling = 0;
   int32_t knob = -1;
@@ -140,6 +140,7 @@ struct zs_linked {
 static zh_status reg_desc(zs_command* c, uint32_t r, zh_descriptor* d) {
   if (!c || r >= c->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
   if (c->ops[r].kind == K_OUTPUT) return fail(ZH_ERR_TYPE, "register is an output, not a texture");
+  if (c->ops[r].kind == K_BUFFER) return fail(ZH_ERR_TYPE, "register is a buffer, not a texture");
   *d = c->ops[r].desc;
   return ZH_OK;
 }
@@ -477,11 +478,75 @@ zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, f
   return ZH_OK;
 }
 
+// A buffer register stands for its bytes; it has no texture of its own, so
+// its slot in the texture table holds a one-byte placeholder.
+static zh_descriptor buffer_placeholder() {
+  zh_descriptor d;
+  std::memset(&d, 0, sizeof(d));
+  d.width = d.height = 1;
+  d.texel.bits = ZH_BITS_INT8;
+  d.texel_stride = 1;
+  d.color.model = ZH_COLOR_SCALARS;
+  return d;
+}
+
+zh_status zs_command_buffer_init(zs_command* cmd, const void* data, uint64_t len, uint32_t* reg) {
+  if (!cmd || !reg || (len && !data)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (len > 0xffffffffull) return fail(ZH_ERR_UNSUPPORTED, "buffers are limited to 4 GiB");
+  CmdOp op;
+  op.kind = K_BUFFER;
+  op.desc = buffer_placeholder();
+  const uint8_t* b = static_cast<const uint8_t*>(data);
+  op.blob.assign(b, b + len);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_buffer_zero(zs_command* cmd, uint64_t len, uint32_t* reg) {
+  if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (len > 0xffffffffull) return fail(ZH_ERR_UNSUPPORTED, "buffers are limited to 4 GiB");
+  CmdOp op;
+  op.kind = K_BUFFER;
+  op.desc = buffer_placeholder();
+  op.blob.assign((size_t)len, 0);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_buffer_size(zs_command* cmd, uint32_t buffer, uint64_t* len) {
+  if (!cmd || !len || buffer >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
+  if (cmd->ops[buffer].kind != K_BUFFER) return fail(ZH_ERR_TYPE, "register is not a buffer");
+  *len = cmd->ops[buffer].blob.size();
+  return ZH_OK;
+}
+
+zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_descriptor* desc, uint32_t* reg) {
+  if (!cmd || !desc || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (buffer >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
+  if (cmd->ops[buffer].kind != K_BUFFER) return fail(ZH_ERR_TYPE, "register is not a buffer");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent image descriptor: " + why);
+  // Descriptor::to_aligned (buffer.rs:121-134): rows padded to 256 bytes; the
+  // buffer must hold height such rows (command.rs:946-962).
+  const uint64_t pitch = (tight_stride(*desc) + 255) / 256 * 256;
+  if (pitch * desc->height > cmd->ops[buffer].blob.size())
+    return fail(ZH_ERR_INVALID_ARGUMENT, "buffer of " + std::to_string(cmd->ops[buffer].blob.size()) +
+                                             " bytes is smaller than the image's " +
+                                             std::to_string(pitch * desc->height));
+  CmdOp op;
+  op.kind = K_FROM_BUFFER;
+  op.desc = normalized(*desc);
+  op.desc.row_stride = 0;
+  op.src = buffer;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob) {
   if (!cmd || !knob || reg >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
   CmdOp& op = cmd->ops[reg];
   if (op.kind == K_INPUT || op.kind == K_OUTPUT || op.kind == K_TRANSMUTE || op.kind == K_CROP ||
-      op.kind == K_INSCRIBE || op.kind == K_BLEND)
+      op.kind == K_INSCRIBE || op.kind == K_BLEND || op.kind == K_FROM_BUFFER)
     return fail(ZH_ERR_TYPE, "operation has no parameter data to override");
   if (op.knob < 0) op.knob = cmd->next_knob++;
   *knob = op.knob;
@@ -579,6 +644,26 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
       case K_DERIVATIVE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_BOX3, {c.src}, c.blob, c.knob);
         break;
+      case K_BUFFER:
+        break;  // read through the images constructed from it
+      case K_FROM_BUFFER: {
+        // The buffer's bytes as a texture of 256-byte rows, then a full copy
+        // into the register's (tight) texture.
+        const CmdOp& b = cmd->ops[c.src];
+        zh_descriptor pitched = c.desc;
+        pitched.row_stride = (tight_stride(c.desc) + 255) / 256 * 256;
+        const uint32_t t = (uint32_t)L->textures.size();
+        L->textures.push_back(pitched);
+        zh_op op;
+        std::memset(&op, 0, sizeof(op));
+        op.high = ZH_HIGH_BUFFER;
+        op.dst = t;
+        op.knob = b.knob;
+        L->ops.push_back(op);
+        L->blobs.push_back(b.blob);
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_PAINT_COPY, {t}, {}, -1);
+        break;
+      }
       case K_TRANSMUTE: {
         zh_op op;
         std::memset(&op, 0, sizeof(op));

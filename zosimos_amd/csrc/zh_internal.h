@@ -60,6 +60,10 @@ struct Plan {
   std::vector<zh_descriptor> input_desc;     // per input slot (declared texture desc)
   std::vector<zh_descriptor> output_desc;    // per output slot
   std::vector<bool> input_used, output_used;
+  // Input slots from user_inputs on are the program's own buffers
+  // (ZH_HIGH_BUFFER): slot user_inputs + k holds the bytes of op buffer_ops[k].
+  uint32_t user_inputs = 0;
+  std::vector<uint32_t> buffer_ops;
   bool batched = false;                      // resample stages run as one tile queue
   std::string text;
 };
@@ -118,6 +122,7 @@ struct zh_program {
   zh::StageDesc* d_stages = nullptr;       // device copies, one per stage
   zh::StageDesc* h_stages = nullptr;       // pinned host staging
   std::vector<void*> d_intermediates;
+  std::vector<zh_image> buffer_images;   // the program's buffers (Plan::buffer_ops), device memory owned here
   std::vector<uint32_t*> d_tile_layers;  // per stage: the tile-layer masks (warp stages)
   std::vector<uint32_t> tile_layer_words;  // their capacities in words
   std::vector<void*> bound_inputs, bound_outputs;

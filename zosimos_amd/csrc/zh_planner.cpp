@@ -213,6 +213,7 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
                        std::vector<std::pair<int, int>>& outputs) {
   B.cur.assign(tex.size(), -1);
   auto tex_ok = [&](uint32_t t) { return t < tex.size(); };
+  std::vector<std::pair<int, uint32_t>> buffers;  // (node, op) of each ZH_HIGH_BUFFER
   for (size_t oi = 0; oi < ops.size(); ++oi) {
     const zh_op& op = ops[oi];
     const std::vector<uint8_t>& blob = params[oi];
@@ -234,6 +235,25 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
         plan.input_desc[op.src] = n.desc;
         plan.input_used[op.src] = true;
         B.cur[op.dst] = B.add(n);
+        break;
+      }
+      case ZH_HIGH_BUFFER: {
+        // Read in place like an input; its slot is numbered after the
+        // caller's inputs once they are all known.
+        if (!tex_ok(op.dst)) return fail(ZH_ERR_INVALID_ARGUMENT, "buffer texture id out of range" + at);
+        const zh_descriptor& d = tex[op.dst];
+        if ((uint64_t)blob.size() < row_stride_of(d) * d.height)
+          return fail(ZH_ERR_INVALID_ARGUMENT, "buffer of " + std::to_string(blob.size()) +
+                                                   " bytes is shorter than the image it holds" + at);
+        Node n;
+        n.kind = N_INPUT;
+        n.desc = d;
+        n.w = d.width;
+        n.h = d.height;
+        n.label = "buffer";
+        const int id = B.add(n);
+        buffers.push_back({id, (uint32_t)oi});
+        B.cur[op.dst] = id;
         break;
       }
       case ZH_HIGH_OUTPUT: {
@@ -445,6 +465,13 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
       default:
         return fail(ZH_ERR_INVALID_ARGUMENT, "unknown High kind" + at);
     }
+  }
+  plan.user_inputs = (uint32_t)plan.input_desc.size();
+  for (const auto& b : buffers) {
+    B.nodes[b.first].input_slot = (int)plan.input_desc.size();
+    plan.input_desc.push_back(B.nodes[b.first].desc);
+    plan.input_used.push_back(true);
+    plan.buffer_ops.push_back(b.second);
   }
   return ZH_OK;
 }
@@ -991,15 +1018,18 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     }
 
     if (n.kind == N_INPUT) {
-      // Passthrough of an input (tests/color_support.rs): byte copies when the
-      // layouts agree, a decode/encode stage otherwise.
+      // Passthrough of an input (tests/color_support.rs): row copies when the
+      // texels agree (the row pitches may differ), a decode/encode stage
+      // otherwise.
       for (int slot_out : n.out_slots) {
         const zh_descriptor& od = plan.output_desc[slot_out];
         StagePlan cp;
         std::memset(&cp.desc, 0, sizeof(cp.desc));
         cp.desc.pre_slot[0] = cp.desc.pre_slot[1] = -1;
         cp.desc.variant = -1;
-        if (same_layout(n.desc, od)) {
+        zh_descriptor pitched = n.desc;
+        pitched.row_stride = od.row_stride;
+        if (same_layout(pitched, od)) {
           cp.desc.kind = STAGE_COPY;
           cp.slot_ref = {n.input_slot, ~(int64_t)slot_out};
           cp.desc.copy_rows = n.desc.height;

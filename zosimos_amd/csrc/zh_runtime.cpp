@@ -299,6 +299,9 @@ static void free_device(zh_program* p) {
   p->units.clear();
   for (void* b : p->d_intermediates) hipFree(b);
   p->d_intermediates.clear();
+  for (zh_image& b : p->buffer_images)
+    if (b.ptr) hipFree(b.ptr);
+  p->buffer_images.clear();
   for (uint32_t* m : p->d_tile_layers)
     if (m) hipFree(m);
   p->d_tile_layers.clear();
@@ -419,6 +422,17 @@ static zh_status tile_layer_masks(zh_program* p, size_t s) {
   return ZH_OK;
 }
 
+// Copy the bytes of the program's buffers to their device images (the
+// stream is idle: at set-up, or after a knob change synchronised it).
+static zh_status upload_buffers(zh_program* p) {
+  const Plan& plan = p->plan;
+  for (size_t k = 0; k < plan.buffer_ops.size(); ++k) {
+    const zh_image& b = p->buffer_images[k];
+    ZH_HIP(hipMemcpy(b.ptr, p->params[plan.buffer_ops[k]].data(), b.bytes, hipMemcpyHostToDevice));
+  }
+  return ZH_OK;
+}
+
 static zh_status alloc_device(zh_program* p) {
   zh_context* ctx = p->ctx;
   ZH_HIP(hipSetDevice(ctx->device));
@@ -434,6 +448,18 @@ static zh_status alloc_device(zh_program* p) {
   make_units(p);
   for (size_t s = 0; s < n; ++s) {
     zh_status st = tile_layer_masks(p, s);
+    if (st != ZH_OK) return st;
+  }
+  for (size_t k = 0; k < p->plan.buffer_ops.size(); ++k) {
+    zh_image b;
+    b.ctx = ctx;
+    b.desc = p->plan.input_desc[p->plan.user_inputs + k];
+    b.bytes = image_bytes(b.desc);
+    ZH_HIP(hipMalloc(&b.ptr, b.bytes));
+    p->buffer_images.push_back(b);
+  }
+  {
+    zh_status st = upload_buffers(p);
     if (st != ZH_OK) return st;
   }
   for (Unit& u : p->units) {
@@ -604,12 +630,15 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     for (size_t i = 0; i < np.stages.size(); ++i) np.stages[i].lds_bytes = p->plan.stages[i].lds_bytes;
     p->plan = std::move(np);
     p->uploaded = false;
-    // The new maps move the layers' footprints.
-    if (p->ctx)
+    // The new maps move the layers' footprints; buffers take the new bytes.
+    if (p->ctx) {
       for (size_t s = 0; s < p->plan.stages.size(); ++s) {
         zh_status ms = tile_layer_masks(p, s);
         if (ms != ZH_OK) return ms;
       }
+      zh_status bs = upload_buffers(p);
+      if (bs != ZH_OK) return bs;
+    }
     return ZH_OK;
   }
   if (p->ctx) {
@@ -634,7 +663,8 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
     if (b[i].image->ctx != p->ctx) return fail(ZH_ERR_INVALID_ARGUMENT, "image belongs to another context");
     auto& vec = b[i].is_output ? outs : in;
     const auto& descs = b[i].is_output ? plan.output_desc : plan.input_desc;
-    if (b[i].slot >= vec.size()) return fail(ZH_ERR_INVALID_ARGUMENT, "binding slot out of range");
+    if (b[i].slot >= (b[i].is_output ? vec.size() : plan.user_inputs))
+      return fail(ZH_ERR_INVALID_ARGUMENT, "binding slot out of range");
     const zh_descriptor& want = descs[b[i].slot];
     const zh_descriptor& have = b[i].image->desc;
     // Environment::bind requires the pool image's descriptor to equal the
@@ -644,6 +674,7 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
                                                     std::to_string(b[i].slot) + " bound to a mismatched image");
     vec[b[i].slot] = b[i].image;
   }
+  for (size_t k = 0; k < p->buffer_images.size(); ++k) in[plan.user_inputs + k] = &p->buffer_images[k];
   for (size_t i = 0; i < in.size(); ++i)
     if (plan.input_used[i] && !in[i]) return fail(ZH_ERR_MISSING_KEY, "input slot " + std::to_string(i) + " unbound");
   for (size_t i = 0; i < outs.size(); ++i)
