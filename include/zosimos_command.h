@@ -77,7 +77,8 @@ zh_status zs_command_solid_rgba(zs_command* cmd, const zh_descriptor* desc, cons
  * six vec4 u_min, u_max, v_min, v_max, uv_min, uv_max (shaders/bilinear.rs). */
 zh_status zs_command_bilinear(zs_command* cmd, const zh_descriptor* desc, const float params[24], uint32_t* reg);
 /* Oklab chroma scale and hue rotation (radians) on an Oklab register; no
- * counterpart in the reference. */
+ * counterpart in the reference, where such an adjustment needs a user shader
+ * (ShaderCommand, command/dynamic.rs:7-28). */
 zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, float hue_rad, uint32_t* reg);
 /* CommandBuffer::chromatic_adaptation, command.rs:1112-1174: an RGB register
  * adapted to the target white point (ZH_WHITEPOINT_*) by a ZS_ADAPT_* method;
@@ -89,6 +90,18 @@ zh_status zs_command_chromatic_adaptation(zs_command* cmd, uint32_t src, uint32_
  * (command.rs:2598-2609, 3344-3409).  Roberts and the Scharr 4/5 variants are
  * not implemented, as in the reference. */
 zh_status zs_command_derivative(zs_command* cmd, uint32_t src, uint32_t method, uint32_t direction, uint32_t* reg);
+/* DistributionNormal2d (shaders/distribution_normal2d.rs ShaderData): the 8
+ * floats {expectation (2), inverse covariance (4, row-major),
+ * pseudo-determinant, 0} of a centred normal distribution with variances
+ * (var0, var1) on the axes (with_diagonal), or of a 1-D one along the
+ * direction (x, y), whose squared length is the variance (with_direction). */
+zh_status zs_normal2d_with_diagonal(float var0, float var1, float out[8]);
+zh_status zs_normal2d_with_direction(float x, float y, float out[8]);
+/* CommandBuffer::distribution_normal2d, command.rs:1559-1583: the density at
+ * the pixel centres, in screen coordinates [-1, 1]^2, as a grey image;
+ * `desc` must have Luma or LumaA parts (ZH_ERR_TYPE otherwise). */
+zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor* desc, const float params[8],
+                                           uint32_t* reg);
 /* CommandBuffer::buffer_init, command.rs:1777-1790: a buffer register holding
  * a copy of `len` bytes. */
 zh_status zs_command_buffer_init(zs_command* cmd, const void* data, uint64_t len, uint32_t* reg);

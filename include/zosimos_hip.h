@@ -172,6 +172,9 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *  BOX3                  mat3 std140, 48 B (shaders/box3.rs:25-27): row-major
  *                        weights, row = y offset + 1, column = x offset + 1;
  *                        operand = the sampled image (box3.frag)
+ *  DISTRIBUTION_NORMAL2D 8 floats {expectation (2), inverse covariance (4,
+ *                        row-major), pseudo-determinant, 0}, 32 B
+ *                        (shaders/distribution_normal2d.rs binary_data)
  *  OKLAB_ADJUST          vec4 {chroma scale, hue rotation (rad), 0, 0}
  *  BLEND_OVER            none; operands (below, above)
  *  AFFINE_BILINEAR       mat3 std140 of the row-major Affine matrix, 48 B;
@@ -185,6 +188,7 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_SOLID_RGB 4
 #define ZH_SHADER_BILINEAR 5
 #define ZH_SHADER_BOX3 6
+#define ZH_SHADER_DISTRIBUTION_NORMAL2D 7
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

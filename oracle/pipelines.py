@@ -198,6 +198,23 @@ def bilinear_grid(w, h, params):
     return mix(p[0], p[1], u) + mix(p[2], p[3], v) + mix(p[4], p[5], uv)
 
 
+def normal2d(w, h, params):
+    """``distribution_normal2d.frag`` over a w x h image (``command.rs:
+    1559-1583``): at the pixel centre, in screen coordinates
+    p = 2 ((x + 0.5) / w, (y + 0.5) / h) - 1 - mu, the grey value
+    exp(-p.C p / 2) / sqrt(det) with C the parameters' inverse covariance
+    (``shaders/distribution_normal2d.rs`` order: mu (2), C row-major (4),
+    pseudo-determinant).  FP64, shape (h, w, 4), alpha 1."""
+    p = np.asarray(params, dtype=np.float64)
+    px = (2.0 * ((np.arange(w) + 0.5) / w) - 1.0 - p[0])[None, :]
+    py = (2.0 * ((np.arange(h) + 0.5) / h) - 1.0 - p[1])[:, None]
+    quad = p[2] * px * px + (p[3] + p[4]) * px * py + p[5] * py * py
+    g = np.exp(-0.5 * quad) / np.sqrt(p[6])
+    out = np.ones((h, w, 4))
+    out[..., :3] = g[..., None]
+    return out
+
+
 def staged_u8_store(values):
     """A non-native 8-bit texel store (``stage.frag`` encode: clamp to [0, 1],
     then the truncating ``uint(c * 255)`` of ``mux_uint``, ``stage.frag:

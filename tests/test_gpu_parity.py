@@ -497,6 +497,29 @@ def test_knob_matches_rebuilt_program(ctx):
         assert np.array_equal(got, ref[o2])
 
 
+@pytest.mark.parametrize("bits,parts,w,h,tol", [(z.SampleBits.UInt8, z.SampleParts.Luma, 400, 400, 1),
+                                                  (z.SampleBits.UInt8, z.SampleParts.Luma, 37, 23, 1),
+                                                  (z.SampleBits.UInt16x2, z.SampleParts.LumaA, 61, 17, 2)])
+def test_distribution_normal2d(ctx, bits, parts, w, h, tol):
+    """distribution_normal2d.frag into staged grey formats (blend.rs:285-311
+    for the 400x400 Luma8 case) against the FP64 oracle through the FP32
+    stage.frag encode: within `tol` codes."""
+    d = z.Descriptor(w, h, z.Texel(bits, parts), z.Color.SRGB)
+    for p in (z.DistributionNormal2d.with_direction([0.04998, 0.0501]),
+              z.DistributionNormal2d.with_diagonal(0.05, 0.3)):
+        cmd = z.CommandBuffer()
+        o, od = cmd.output(cmd.distribution_normal2d(d, p))
+        out, _ = run(ctx, cmd, {}, {}, {o: od})
+        want = S.stage_encode(PL.normal2d(w, h, p.params()).reshape(-1, 4).astype(np.float32), bits, parts, 5)
+        if parts == z.SampleParts.Luma:
+            got, want = out[o].reshape(-1).astype(np.int64), want.astype(np.int64)
+        else:
+            got = np.frombuffer(out[o].tobytes(), "<u2").reshape(-1, 2).astype(np.int64)
+            want = np.stack([want & 0xFFFF, want >> 16], -1).astype(np.int64)
+        report("normal2d", size=(w, h), bits=bits, max_code_diff=int(np.abs(got - want).max()))
+        assert np.abs(got - want).max() <= tol
+
+
 def _pitched(data, pitch, rows):
     """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
     buf = np.zeros(pitch * rows, np.uint8)

@@ -20,7 +20,11 @@ through a buffer register), ``run_from_buffer`` (an 8x8 Luma8 image read
 from a buffer with 256-byte rows, ``from_buffer``, both as a bound input of
 that layout and through ``buffer_init``/``from_buffer``) and
 ``run_from_buffer_knob`` (the same through a buffer knob,
-``from_buffer-with-knob``).
+``from_buffer-with-knob``); ``run_distribution`` and
+``run_distribution_normal1d`` (``distribution_normal2d``, ``_normal1d``).
+``run_distribution_u8`` is checked against the oracle only
+(``test_gpu_parity.py``): its blessed hash needs a rounding 8-bit store,
+where stage.frag truncates.
 """
 import json
 import math
@@ -33,6 +37,7 @@ import zosimos_amd as z
 from oracle import blockhash as B
 from oracle import color as C
 from oracle import pipelines as PL
+from oracle import stage as S
 
 from gpu_util import report
 
@@ -234,6 +239,30 @@ def test_from_buffer_matches_reference_hash(ctx):
     got = out.download().reshape(8, 8)
     assert np.array_equal(got, a.reshape(8, 256)[:, :8])
     check("from_buffer", np.stack([got, got, got, np.full_like(got, 255)], -1))
+
+
+@pytest.mark.parametrize("name,dist", [
+    ("distribution_normal2d", lambda: z.DistributionNormal2d.with_diagonal(0.2, 0.2)),
+    ("distribution_normal1d", lambda: z.DistributionNormal2d.with_direction([0.04998, 0.0501]))])
+def test_distribution_normal2d_matches_reference_hash(ctx, name, dist):
+    """blend.rs:226-282: normal distributions generated into a 400x400 sRGB
+    LumaA16 image (staged: sRGB transfer, truncating 16-bit store); within 2
+    codes of the oracle and hashing to the blessed line."""
+    d = z.Descriptor(400, 400, z.Texel(z.SampleBits.UInt16x2, z.SampleParts.LumaA), z.Color.SRGB)
+    p = dist()
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.distribution_normal2d(d, p))
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    out = ctx.image(od)
+    exe.launch({}, {o: out})
+    ctx.sync()
+    raw = out.download()
+    want = S.stage_encode(PL.normal2d(400, 400, p.params()).reshape(-1, 4).astype(np.float32), 11, 5, 5)
+    got = np.frombuffer(raw.tobytes(), "<u2").reshape(-1, 2)
+    w16 = np.stack([want & 0xFFFF, want >> 16], -1)
+    report(name, max_code_diff=int(np.abs(got.astype(np.int64) - w16).max()))
+    assert np.abs(got.astype(np.int64) - w16).max() <= 2
+    check(name, B.lumaa16_to_rgba8(raw, 400, 400))
 
 
 LUMA8 = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)

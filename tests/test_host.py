@@ -6,6 +6,7 @@ import math
 import os
 import re
 
+import numpy as np
 import pytest
 
 import zosimos_amd as z
@@ -172,6 +173,34 @@ def test_buffer_commands():
     assert "copy 64 bytes" in e.describe()
     expect(z.Status.KNOB_LENGTH, e.knob, k, b"\0" * 2047)
     e.knob(k, b"\0" * 2048)
+
+
+def test_normal2d_parameters():
+    """DistributionNormal2d::with_diagonal / with_direction
+    (shaders/distribution_normal2d.rs) against the linear algebra they stand
+    for: the inverse (pseudo-inverse) covariance and (2 pi)^k times its
+    (pseudo-)determinant, in f32."""
+    p = z.DistributionNormal2d.with_diagonal(0.2, 0.3).params()
+    assert np.allclose(p, [0, 0, 5.0, 0, 0, 1 / 0.3, (2 * np.pi) ** 2 * 0.06, 0], rtol=1e-6)
+    p = z.DistributionNormal2d.with_diagonal(0.0, 0.5).params()  # a zero variance drops its axis
+    assert np.allclose(p, [0, 0, 0, 0, 0, 2.0, 2 * np.pi * 0.5, 0], rtol=1e-6)
+    d = np.array([0.3, -0.3])
+    p = z.DistributionNormal2d.with_direction(d).params()
+    assert np.allclose(np.reshape(p[2:6], (2, 2)), np.linalg.pinv(np.outer(d, d)), rtol=1e-5)
+    assert np.isclose(p[6], 2 * np.pi * (d @ d), rtol=1e-6)
+    # |x| != |y|: the off-diagonal is x y / |d|^4; the diagonal is evaluated
+    # as 1 / (4 x^2), 1 / (4 y^2), as the reference evaluates it
+    x, y = 0.04998, 0.0501
+    p = z.DistributionNormal2d.with_direction([x, y]).params()
+    assert np.allclose(p[3:5], x * y / (x * x + y * y) ** 2, rtol=1e-5)
+    assert np.allclose([p[2], p[5]], [1 / (4 * x * x), 1 / (4 * y * y)], rtol=1e-5)
+    expect(z.Status.INVALID_ARGUMENT, z.DistributionNormal2d.with_direction, [float("inf"), 0.0])
+    cmd = z.CommandBuffer()
+    expect(z.Status.TYPE, cmd.distribution_normal2d, rgba8(8, 8), z.DistributionNormal2d.with_diagonal(1, 1))
+    luma = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)
+    g = cmd.distribution_normal2d(luma, z.DistributionNormal2d.with_diagonal(1, 1))
+    cmd.output(g)
+    assert "normal2d store" in plan(cmd).describe()
 
 
 def test_launch_without_device_fails_loudly():

@@ -406,3 +406,79 @@ def test_derivatives_reproduce_blessed_hashes():
         assert B.blockhash256(C.encode_u8(PL.box3(lin, w), 5)) in hashes["derived_" + name]
         assert B.blockhash256(C.encode_u8(PL.box3(lin, w.T), 5)) not in hashes["derived_" + name]
         assert np.allclose(PL.box3(np.full((5, 6, 4), 0.3), w)[..., :3], 0.0)
+
+
+# ---- distribution_normal2d.frag -------------------------------------------------------
+def _diag_params(v0, v1):
+    """Inverse covariance diag(1/v0, 1/v1) and determinant (2 pi)^2 v0 v1."""
+    return [0.0, 0.0, 1.0 / v0, 0.0, 0.0, 1.0 / v1, (2 * math.pi) ** 2 * v0 * v1, 0.0]
+
+
+def test_normal2d_is_a_density():
+    """Summed over the pixels of the screen square [-1, 1]^2 (pixel area
+    (2/w)(2/h)) the density integrates to 1 when its mass lies inside; its
+    peak at an exact pixel centre is 1 / sqrt(det); the marginal along x has
+    the variance v0."""
+    w, h = 301, 201
+    v0, v1 = 0.02, 0.035
+    g = PL.normal2d(w, h, _diag_params(v0, v1))[..., 0]
+    assert abs(g.sum() * (2.0 / w) * (2.0 / h) - 1.0) < 1e-6
+    assert np.isclose(g[h // 2, w // 2], 1.0 / math.sqrt(_diag_params(v0, v1)[6]), rtol=1e-12)
+    x = 2.0 * (np.arange(w) + 0.5) / w - 1.0
+    mx = g.sum(axis=0)
+    assert np.isclose((mx * x * x).sum() / mx.sum(), v0, rtol=1e-5)
+    # a shifted expectation moves the peak: mu = (0.5, 0) puts it 3/4 across
+    mu = _diag_params(v0, v1)
+    mu[0] = 0.5
+    g2 = PL.normal2d(w, h, mu)[..., 0]
+    assert np.unravel_index(np.argmax(g2), g2.shape) == (h // 2, int(0.75 * w))
+
+
+def test_normal2d_one_dimensional_pseudo_inverse():
+    """A 1-D distribution along d: with C = pinv(d d^T) the value is constant
+    along the perpendicular of d through the centre."""
+    d = np.array([0.12, 0.12])
+    c = np.linalg.pinv(np.outer(d, d))
+    p = [0.0, 0.0, c[0, 0], c[0, 1], c[1, 0], c[1, 1], 2 * math.pi * d @ d, 0.0]
+    g = PL.normal2d(101, 101, p)[..., 0]
+    anti = g[np.arange(101), 100 - np.arange(101)]  # the perpendicular diagonal
+    assert np.allclose(anti, anti[50], rtol=1e-12)
+    assert np.isclose(g.max(), 1.0 / math.sqrt(p[6]))
+
+
+def _lumaa16_hash_image(values):
+    from oracle import blockhash as B
+    n = S.stage_encode(values.reshape(-1, 4).astype(np.float32), 11, 5, 5)
+    return B.lumaa16_to_rgba8(n.astype("<u4").tobytes(), values.shape[1], values.shape[0])
+
+
+def test_normal2d_reproduces_blessed_hashes():
+    """blend.rs:226-311: the normal distributions stored to 400x400 sRGB
+    LumaA16 (a staged format: sRGB transfer, truncating 16-bit store) hash to
+    the blessed ``distribution_normal2d`` and ``distribution_normal1d`` lines;
+    without the transfer the 1-D one does not.  The Luma8 image of
+    ``run_distribution_u8`` hashes to its blessed line only when the 8-bit
+    store rounds, not with the truncating store of stage.frag's ``mux_uint``
+    (a documented divergence of the blessed data from the shader source)."""
+    import json, os
+    import zosimos_amd as z
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    p2 = z.DistributionNormal2d.with_diagonal(0.2, 0.2).params()
+    p1 = z.DistributionNormal2d.with_direction([0.04998, 0.0501]).params()
+    v2 = PL.normal2d(400, 400, p2)
+    v1 = PL.normal2d(400, 400, p1)
+    assert B.blockhash256(_lumaa16_hash_image(v2)) in hashes["distribution_normal2d"]
+    assert B.blockhash256(_lumaa16_hash_image(v1)) in hashes["distribution_normal1d"]
+    lin = np.floor(np.clip(v1[..., 0], 0, 1) * 65535).astype(np.uint32)
+    raw = np.stack([lin, np.full_like(lin, 65535)], -1).astype("<u2").tobytes()
+    assert B.blockhash256(B.lumaa16_to_rgba8(raw, 400, 400)) not in hashes["distribution_normal1d"]
+    enc = C.oe_srgb(np.clip(v1[..., 0], 0, 1))
+    for store, blessed in ((np.floor, False), (np.rint, True)):
+        c = store(enc * 255).astype(np.uint8)
+        got = B.blockhash256(np.stack([c, c, c, np.full_like(c, 255)], -1))
+        assert (got in hashes["distribution_u8"]) == blessed
+    trunc = S.stage_encode(v1.reshape(-1, 4).astype(np.float32), 0, 4, 5).astype(np.uint8).reshape(400, 400)
+    assert np.array_equal(trunc, np.floor(enc * 255).astype(np.uint8)) or \
+        np.abs(trunc.astype(int) - np.floor(enc * 255)).max() <= 1

@@ -286,6 +286,12 @@ class CommandBuffer:
         p = (ctypes.c_float * 24)(*distribution.params())
         return self._reg(self._lib.zs_command_bilinear, ctypes.byref(desc.c()), p)
 
+    def distribution_normal2d(self, desc: Descriptor, distribution: "DistributionNormal2d") -> int:
+        """CommandBuffer::distribution_normal2d (command.rs:1559-1583): the
+        density at the pixel centres as a grey (Luma or LumaA) image."""
+        p = (ctypes.c_float * 8)(*distribution.params())
+        return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
+
     def oklab_adjust(self, src: int, chroma: float = 1.0, hue_rad: float = 0.0) -> int:
         return self._reg(self._lib.zs_command_oklab_adjust, src, chroma, hue_rad)
 
@@ -356,6 +362,36 @@ class Bilinear:
                 raise ValueError("each Bilinear corner is a vec4")
             out += [float(x) for x in v]
         return out
+
+
+class DistributionNormal2d:
+    """shaders/distribution_normal2d.rs ShaderData: expectation, inverse
+    covariance (row-major) and pseudo-determinant of a normal distribution in
+    screen coordinates [-1, 1]^2."""
+
+    def __init__(self, params: Sequence[float]):
+        if len(params) != 8:
+            raise ValueError("a normal distribution has 8 parameter floats")
+        self._p = [float(x) for x in params]
+
+    @staticmethod
+    def _make(fn, a: float, b: float) -> "DistributionNormal2d":
+        out = (ctypes.c_float * 8)()
+        _check(fn(a, b, out))
+        return DistributionNormal2d(list(out))
+
+    @staticmethod
+    def with_diagonal(var0: float, var1: float) -> "DistributionNormal2d":
+        """Centred, with variances ``var0``, ``var1`` along the axes."""
+        return DistributionNormal2d._make(_abi.lib().zs_normal2d_with_diagonal, var0, var1)
+
+    @staticmethod
+    def with_direction(direction: Sequence[float]) -> "DistributionNormal2d":
+        """Centred, 1-D along ``direction``, whose squared length is the variance."""
+        return DistributionNormal2d._make(_abi.lib().zs_normal2d_with_direction, direction[0], direction[1])
+
+    def params(self) -> List[float]:
+        return list(self._p)
 
 
 class Linker:

@@ -104,6 +104,10 @@ _SIGNATURES = {
     "zs_command_oklab_adjust": (i32, [vp, u32, f32, f32, ctypes.POINTER(u32)]),
     "zs_command_chromatic_adaptation": (i32, [vp, u32, u32, u32, ctypes.POINTER(u32)]),
     "zs_command_derivative": (i32, [vp, u32, u32, u32, ctypes.POINTER(u32)]),
+    "zs_normal2d_with_diagonal": (i32, [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "zs_normal2d_with_direction": (i32, [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "zs_command_distribution_normal2d": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(ctypes.c_float),
+                                               ctypes.POINTER(u32)]),
     "zs_command_buffer_init": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_size": (i32, [vp, u32, ctypes.POINTER(ctypes.c_uint64)]),

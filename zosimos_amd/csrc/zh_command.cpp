@@ -3,6 +3,7 @@
 // Each constructor validates its operands the way lib/zosimos/src/command.rs
 // does and records an SSA op; zs_command_compile emits the High list the way
 // Linker::link_in does (command.rs:2345-2880), one texture per register.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -17,7 +18,7 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D
 };
 
 struct CmdOp {
@@ -478,6 +479,60 @@ zh_status zs_command_oklab_adjust(zs_command* cmd, uint32_t src, float chroma, f
   return ZH_OK;
 }
 
+zh_status zs_normal2d_with_diagonal(float var0, float var1, float out[8]) {
+  if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  // ShaderData::with_diagonal (shaders/distribution_normal2d.rs): a zero
+  // variance drops its axis from the inverse and the determinant.
+  const float two_pi = 2.0f * 3.14159265358979323846f;
+  const float d0 = var0 == 0.0f ? 0.0f : 1.0f / var0, d1 = var1 == 0.0f ? 0.0f : 1.0f / var1;
+  const float f0 = var0 == 0.0f ? 1.0f : two_pi * var0, f1 = var1 == 0.0f ? 1.0f : two_pi * var1;
+  const float v[8] = {0.0f, 0.0f, d0, 0.0f, 0.0f, d1, f0 * f1, 0.0f};
+  std::memcpy(out, v, sizeof(v));
+  return ZH_OK;
+}
+
+zh_status zs_normal2d_with_direction(float x, float y, float out[8]) {
+  if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  // ShaderData::with_direction: the covariance d d^T of direction d has the
+  // single eigenvalue |d|^2 (the pseudo-determinant, times 2 pi) and the
+  // pseudo-inverse d d^T / |d|^4, whose entries are evaluated in f32 in the
+  // same rearranged forms (hypot first) for stability.  The diagonal is
+  // evaluated as the reference does, diag(x, x) and diag(y, y): 1 / (4 x^2)
+  // and 1 / (4 y^2), which equal the pseudo-inverse's only when |x| = |y|.
+  const double lsq = (double)x * x + (double)y * y;
+  const float length_sq = (float)lsq;
+  if (!std::isfinite(length_sq)) return fail(ZH_ERR_INVALID_ARGUMENT, "direction length is not finite");
+  auto diag = [](float a, float b) {  // a a / (a a + b b)^2
+    const float h = std::hypot(a, b);
+    return ((1.0f / h) * (a / h)) / (a + b * (b / a));
+  };
+  auto off = [](float a, float b) {  // a b / (a a + b b)^2
+    const float lo = std::min(a, b), hi = std::max(a, b);
+    const float h = std::hypot(lo, hi);
+    return ((lo / h) / std::fma(lo, lo / hi, hi)) / h;
+  };
+  const float two_pi = 2.0f * 3.14159265358979323846f;
+  const float v[8] = {0.0f, 0.0f, diag(x, x), off(x, y), off(y, x), diag(y, y), two_pi * length_sq, 0.0f};
+  std::memcpy(out, v, sizeof(v));
+  return ZH_OK;
+}
+
+zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor* desc, const float params[8],
+                                           uint32_t* reg) {
+  if (!cmd || !desc || !params || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent normal2d: " + why);
+  // command.rs:1559-1583: one grey channel (Luma or LumaA parts).
+  if (desc->texel.parts != 4 && desc->texel.parts != 5) return fail(ZH_ERR_TYPE, "normal2d for non-LumA texel");
+  CmdOp op;
+  op.kind = K_NORMAL2D;
+  op.desc = normalized(*desc);
+  op.blob.resize(32);
+  std::memcpy(op.blob.data(), params, 32);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 // A buffer register stands for its bytes; it has no texture of its own, so
 // its slot in the texture table holds a one-byte placeholder.
 static zh_descriptor buffer_placeholder() {
@@ -643,6 +698,9 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_DERIVATIVE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_BOX3, {c.src}, c.blob, c.knob);
+        break;
+      case K_NORMAL2D:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_DISTRIBUTION_NORMAL2D, {}, c.blob, c.knob);
         break;
       case K_BUFFER:
         break;  // read through the images constructed from it

@@ -287,6 +287,27 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         }
         break;
       }
+      case OP_NORMAL2D: {
+        // distribution_normal2d.frag at the pixel centres (2-D stage): the
+        // quadratic form of the inverse covariance, read as the shader's
+        // column-major mat2 of the row-major blob (the form is symmetric in
+        // the off-diagonal pair).
+        if (in.b) {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
+        }
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+        const float py = 2.0f * (((float)c.y + 0.5f) / (float)d->height - 0.5f) - q[1];
+        const float root = sqrtf(q[6]);
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float px = 2.0f * (((float)(c.x0 + i) + 0.5f) / (float)d->width - 0.5f) - q[0];
+          const float e = 0.5f * (px * (q[2] * px + q[4] * py) + py * (q[3] * px + q[5] * py));
+          const float g = expf(-e) / root;
+          X[i] = make_float4(g, g, g, 1.0f);
+        }
+        break;
+      }
       case OP_BOX3: {
         // box3.frag: the weighted 3x3 neighbourhood of the slot's texels,
         // coordinates clamped to the slot (clamp-to-edge sampling); alpha 1.

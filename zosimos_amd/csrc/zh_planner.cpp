@@ -110,7 +110,8 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // ---------------------------------------------------------------------------
 // value graph
 // ---------------------------------------------------------------------------
-// N_GRID: the bilinear generator; N_STENCIL: box3.frag over a physical
+// N_GRID: a generator of pixel coordinates (bilinear.frag,
+// distribution_normal2d.frag: micro-op gen_op); N_STENCIL: box3.frag over a physical
 // texture (its operand is materialised).  Both are always the root of a stage
 // of their own, which runs on the 2-D interpreter: they need pixel
 // coordinates.
@@ -128,7 +129,9 @@ struct Node {
   uint32_t w = 0, h = 0;
   int input_slot = -1;
   float color[4] = {0, 0, 0, 0};
-  float grid[24] = {};  // N_GRID: u_min, u_max, v_min, v_max, uv_min, uv_max; N_STENCIL: 9 weights
+  float grid[24] = {};  // N_GRID: the generator's parameters; N_STENCIL: 9 weights
+  uint8_t gen_op = OP_GRID;
+  uint32_t gen_params = 24;
   std::vector<Micro> micro;
   int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a
   uint32_t mode = 0;
@@ -303,7 +306,8 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
         n.h = td.height;
         if (op.initializer == ZH_INIT_PAINT_FULLSCREEN) {
           const int src = op.num_operands > 0 ? B.cur[op.operands[0]] : -1;
-          if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR) {
+          if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR &&
+              op.shader != ZH_SHADER_DISTRIBUTION_NORMAL2D) {
             if (src < 0) return fail(ZH_ERR_INVALID_ARGUMENT, "full-screen shader without operand" + at);
             const Node& sn = B.nodes[src];
             if (sn.w != td.width || sn.h != td.height)
@@ -329,6 +333,20 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               g.h = td.height;
               std::memcpy(g.grid, blob.data(), 96);
               g.label = "bilinear";
+              B.cur[op.dst] = B.add(g);
+              continue;
+            }
+            case ZH_SHADER_DISTRIBUTION_NORMAL2D: {
+              if (!need_blob(28)) return fail(ZH_ERR_INVALID_ARGUMENT, "normal2d needs 28 bytes" + at);
+              Node g;
+              g.kind = N_GRID;
+              g.desc = td;
+              g.w = td.width;
+              g.h = td.height;
+              std::memcpy(g.grid, blob.data(), 28);
+              g.gen_op = OP_NORMAL2D;
+              g.gen_params = 7;
+              g.label = "normal2d";
               B.cur[op.dst] = B.add(g);
               continue;
             }
@@ -636,7 +654,7 @@ struct Emitter {
       }
       case N_GRID: {
         pending_mat_valid = false;
-        ins(OP_GRID, 0, depth > 0 ? 1 : 0, param(n.grid, 24));
+        ins(n.gen_op, 0, depth > 0 ? 1 : 0, param(n.grid, n.gen_params));
         depth++;
         return false;
       }
@@ -697,7 +715,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -1064,7 +1082,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       // no generator needs the pixel coordinates.
       bool flat = true;
       for (uint32_t i = 0; i < sp.desc.n_instr; ++i)
-        if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3) flat = false;
+        if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3 || sp.desc.code[i].op == OP_NORMAL2D)
+          flat = false;
       for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
         const Slot& s = sp.desc.slot[k];
         if (s.width != n.w || s.height != n.h || s.row_stride != (uint64_t)s.width * s.bpp) flat = false;

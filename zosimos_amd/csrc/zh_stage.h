@@ -55,6 +55,9 @@ enum Opcode : uint8_t {
   OP_BOX3,      // a = slot; arg = 9 weights, row-major by y offset; b = push.  X.rgb <- sum of
                 //   w[dy + 1][dx + 1] * texel (x + dx, y + dy), coordinates clamped to the slot; X.a <- 1
                 //   (box3.frag)
+  OP_NORMAL2D,  // arg = params {mu.x, mu.y, C00, C01, C10, C11, det}; b = push.  At the pixel centre,
+                //   p = 2 (u, v) - 1 - mu, X <- (g, g, g, 1) with g = exp(-p.C p / 2) / sqrt(det)
+                //   (distribution_normal2d.frag)
 };
 
 struct Instr {
