@@ -102,6 +102,24 @@ zh_status zs_normal2d_with_direction(float x, float y, float out[8]);
  * `desc` must have Luma or LumaA parts (ZH_ERR_TYPE otherwise). */
 zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor* desc, const float params[8],
                                            uint32_t* reg);
+/* FractalNoise (shaders/fractal_noise.rs ShaderData). */
+typedef struct zs_fractal_noise {
+  uint32_t num_octaves;
+  float initial_amplitude;
+  float amplitude_damping;
+  float grid_scale[2];
+} zs_fractal_noise;
+/* ShaderData::with_octaves: amplitude 1 / octaves, damping 1, cells of
+ * 1/100 of the image. */
+zh_status zs_fractal_noise_with_octaves(uint32_t num_octaves, zs_fractal_noise* out);
+/* ShaderData::set_damping: the damping, and the initial amplitude that makes
+ * the octaves' amplitudes sum to 1. */
+zh_status zs_fractal_noise_set_damping(zs_fractal_noise* noise, float damping);
+/* CommandBuffer::distribution_fractal_noise, command.rs:1589-1607: fractal
+ * value noise (fractal_noise.frag) at the pixel centres, all four channels;
+ * at most 4096 octaves (ZH_ERR_UNSUPPORTED beyond). */
+zh_status zs_command_distribution_fractal_noise(zs_command* cmd, const zh_descriptor* desc,
+                                                const zs_fractal_noise* noise, uint32_t* reg);
 /* CommandBuffer::buffer_init, command.rs:1777-1790: a buffer register holding
  * a copy of `len` bytes. */
 zh_status zs_command_buffer_init(zs_command* cmd, const void* data, uint64_t len, uint32_t* reg);

@@ -175,6 +175,10 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *  DISTRIBUTION_NORMAL2D 8 floats {expectation (2), inverse covariance (4,
  *                        row-major), pseudo-determinant, 0}, 32 B
  *                        (shaders/distribution_normal2d.rs binary_data)
+ *  FRACTAL_NOISE         {grid_scale (2 f32), initial amplitude f32, amplitude
+ *                        damping f32, octaves u32}, 20 B padded to 24
+ *                        (shaders/fractal_noise.rs binary_data); at most 4096
+ *                        octaves
  *  OKLAB_ADJUST          vec4 {chroma scale, hue rotation (rad), 0, 0}
  *  BLEND_OVER            none; operands (below, above)
  *  AFFINE_BILINEAR       mat3 std140 of the row-major Affine matrix, 48 B;
@@ -189,6 +193,7 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_BILINEAR 5
 #define ZH_SHADER_BOX3 6
 #define ZH_SHADER_DISTRIBUTION_NORMAL2D 7
+#define ZH_SHADER_FRACTAL_NOISE 8
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

@@ -215,6 +215,72 @@ def normal2d(w, h, params):
     return out
 
 
+def pcg4d(x, y, z, w):
+    """``pcg4d`` of ``fractal_noise.frag`` (Jarzynski and Olano, JCGT 9(3),
+    2020, the published 4-D PCG hash): per lane v = v * 1664525 + 1013904223,
+    then x += y w, y += z x, z += x y, w += y z (in that order, each using the
+    values already updated), v ^= v >> 16, and the cross step again; uint32
+    arithmetic modulo 2^32."""
+    m = np.uint32
+    v = [np.asarray(a, dtype=np.uint32) for a in (x, y, z, w)]
+    with np.errstate(over="ignore"):
+        v = [a * m(1664525) + m(1013904223) for a in v]
+
+        def cross(v):
+            x, y, z, w = v
+            x = x + y * w
+            y = y + z * x
+            z = z + x * y
+            w = w + y * z
+            return [x, y, z, w]
+
+        v = cross(v)
+        v = [a ^ (a >> m(16)) for a in v]
+        v = cross(v)
+    return v
+
+
+def fractal_noise(w, h, num_octaves, amplitude, damping, scale=(100.0, 100.0)):
+    """``fractal_noise.frag`` (``command.rs:1589-1607``) over a w x h image in
+    IEEE f32, every product and sum rounded on its own in the shader's order:
+    at the pixel centre (u, v), per octave the cell corners of
+    p = (u, v) * scale are hashed (pcg4d of (floor p, 0, 0) over 2^32 - 1),
+    blended with the smoothstep of fract p as
+    mix(a, b, sx) + (c - a) sy (1 - sx) + (d - b) sx sy (mix(a, b, t) =
+    a (1 - t) + b t), scaled by the amplitude and summed; then (u, v) is
+    rotated by 0.5 rad and doubled, and the amplitude damped.  Returns
+    (h, w, 4) as float64 of the f32 values."""
+    f = np.float32
+    x = np.broadcast_to(((np.arange(w, dtype=f) + f(0.5)) / f(w))[None, :], (h, w)).astype(f)
+    y = np.broadcast_to(((np.arange(h, dtype=f) + f(0.5)) / f(h))[:, None], (h, w)).astype(f)
+    kc, ks = f(2.0 * np.cos(0.5)), f(2.0 * np.sin(0.5))
+    big = f(4294967295.0)
+    amp = f(amplitude)
+    acc = [np.zeros((h, w), f) for _ in range(4)]
+
+    def corner(cx, cy):
+        hx = pcg4d(cx, cy, np.zeros_like(cx), np.zeros_like(cx))
+        return [a.astype(f) / big for a in hx]
+
+    for _ in range(int(num_octaves)):
+        px, py = x * f(scale[0]), y * f(scale[1])
+        flx, fly = np.floor(px), np.floor(py)
+        fx, fy = px - flx, py - fly
+        cx = np.clip(flx, -2147483648.0, 2147483520.0).astype(np.int64).astype(np.uint32)
+        cy = np.clip(fly, -2147483648.0, 2147483520.0).astype(np.int64).astype(np.uint32)
+        with np.errstate(over="ignore"):
+            a, b = corner(cx, cy), corner(cx + np.uint32(1), cy)
+            c, d = corner(cx, cy + np.uint32(1)), corner(cx + np.uint32(1), cy + np.uint32(1))
+        ux = fx * fx * (f(3.0) - f(2.0) * fx)
+        uy = fy * fy * (f(3.0) - f(2.0) * fy)
+        vx = f(1.0) - ux
+        for k in range(4):
+            acc[k] = acc[k] + amp * (a[k] * vx + b[k] * ux + (c[k] - a[k]) * uy * vx + (d[k] - b[k]) * ux * uy)
+        x, y = kc * x + (-ks) * y, ks * x + kc * y
+        amp = f(amp * f(damping))
+    return np.stack(acc, -1).astype(np.float64)
+
+
 def staged_u8_store(values):
     """A non-native 8-bit texel store (``stage.frag`` encode: clamp to [0, 1],
     then the truncating ``uint(c * 255)`` of ``mux_uint``, ``stage.frag:

@@ -520,6 +520,27 @@ def test_distribution_normal2d(ctx, bits, parts, w, h, tol):
         assert np.abs(got - want).max() <= tol
 
 
+@pytest.mark.parametrize("w,h,octaves,damping,scale", [(61, 37, 4, 1.0, (100.0, 100.0)),
+                                                         (256, 19, 9, 0.6, (13.5, 40.0)),
+                                                         (5, 3, 1, 1.0, (5.0, 3.0))])
+def test_fractal_noise_f32_bit_exact(ctx, w, h, octaves, damping, scale):
+    """fractal_noise.frag into RGBA32F: the kernel rounds every product and
+    sum on its own in the shader's order, as the f32 oracle does, so the
+    values agree bit for bit (negative cells after the rotations included)."""
+    d = z.Descriptor.rgba32f_linear(w, h)
+    n = z.FractalNoise.with_octaves(octaves).set_damping(damping)
+    n.grid_scale = scale
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.distribution_fractal_noise(d, n))
+    out, _ = run(ctx, cmd, {}, {}, {o: od})
+    got = out[o].view(np.float32).reshape(h, w, 4)
+    want = PL.fractal_noise(w, h, octaves, n.initial_amplitude, n.amplitude_damping, scale).astype(np.float32)
+    diff = np.abs(got.astype(np.float64) - want)
+    report("fractal_noise_f32", size=(w, h), octaves=octaves, max_abs=float(diff.max()),
+           unequal=int((got != want).sum()))
+    assert np.array_equal(got, want)
+
+
 def _pitched(data, pitch, rows):
     """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
     buf = np.zeros(pitch * rows, np.uint8)

@@ -21,7 +21,8 @@ from a buffer with 256-byte rows, ``from_buffer``, both as a bound input of
 that layout and through ``buffer_init``/``from_buffer``) and
 ``run_from_buffer_knob`` (the same through a buffer knob,
 ``from_buffer-with-knob``); ``run_distribution`` and
-``run_distribution_normal1d`` (``distribution_normal2d``, ``_normal1d``).
+``run_distribution_normal1d`` (``distribution_normal2d``, ``_normal1d``),
+``run_fractal_noise`` (``distribution_fractal2d``).
 ``run_distribution_u8`` is checked against the oracle only
 (``test_gpu_parity.py``): its blessed hash needs a rounding 8-bit store,
 where stage.frag truncates.
@@ -263,6 +264,19 @@ def test_distribution_normal2d_matches_reference_hash(ctx, name, dist):
     report(name, max_code_diff=int(np.abs(got.astype(np.int64) - w16).max()))
     assert np.abs(got.astype(np.int64) - w16).max() <= 2
     check(name, B.lumaa16_to_rgba8(raw, 400, 400))
+
+
+def test_fractal_noise_matches_reference_hash(ctx):
+    """blend.rs:314-338: four octaves of fractal noise into 400x400 RGBA8
+    sRGB; within 1 code of the oracle and hashing to the blessed line."""
+    d = z.Descriptor.rgba8_srgb(400, 400)
+    cmd = z.CommandBuffer()
+    o, od = cmd.output(cmd.distribution_fractal_noise(d, z.FractalNoise.with_octaves(4)))
+    out = run_program(ctx, cmd, {}, o, od)
+    want = C.encode_u8(PL.fractal_noise(400, 400, 4, 0.25, 1.0), 5)
+    report("fractal_noise", max_code_diff=int(np.abs(out.astype(int) - want.astype(int)).max()))
+    assert np.abs(out.astype(int) - want.astype(int)).max() <= 1
+    check("distribution_fractal2d", out)
 
 
 LUMA8 = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)

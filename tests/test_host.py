@@ -203,6 +203,27 @@ def test_normal2d_parameters():
     assert "normal2d store" in plan(cmd).describe()
 
 
+def test_fractal_noise_parameters():
+    """shaders/fractal_noise.rs: with_octaves and set_damping, and the
+    reference's own test_set_damping (8 octaves damped by 0.5: the
+    amplitudes sum to 1)."""
+    n = z.FractalNoise.with_octaves(4)
+    assert (n.num_octaves, n.initial_amplitude, n.amplitude_damping, n.grid_scale) == (4, 0.25, 1.0, (100.0, 100.0))
+    n = z.FractalNoise.with_octaves(8).set_damping(0.5)
+    amp, total = n.initial_amplitude, 0.0
+    for _ in range(8):
+        total += amp
+        amp *= n.amplitude_damping
+    assert abs(1.0 - total) < 1e-6
+    assert z.FractalNoise.with_octaves(3).set_damping(1.0).initial_amplitude == 1.0
+    cmd = z.CommandBuffer()
+    cmd.output(cmd.distribution_fractal_noise(rgba8(16, 16), z.FractalNoise.with_octaves(4)))
+    assert "noise store" in plan(cmd).describe()
+    cmd = z.CommandBuffer()
+    cmd.output(cmd.distribution_fractal_noise(rgba8(16, 16), z.FractalNoise.with_octaves(5000)))
+    expect(z.Status.UNSUPPORTED, plan, cmd)
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:

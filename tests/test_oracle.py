@@ -482,3 +482,54 @@ def test_normal2d_reproduces_blessed_hashes():
     trunc = S.stage_encode(v1.reshape(-1, 4).astype(np.float32), 0, 4, 5).astype(np.uint8).reshape(400, 400)
     assert np.array_equal(trunc, np.floor(enc * 255).astype(np.uint8)) or \
         np.abs(trunc.astype(int) - np.floor(enc * 255)).max() <= 1
+
+
+# ---- fractal_noise.frag ---------------------------------------------------------------
+def test_pcg4d_is_a_bijection_of_its_inputs():
+    """pcg4d is a composition of invertible steps (LCG, cross multiply-adds,
+    xorshift): undoing them in reverse order returns the inputs."""
+    rng = np.random.default_rng(7)
+    v = [rng.integers(0, 2 ** 32, 1000, dtype=np.uint64).astype(np.uint32) for _ in range(4)]
+    h = PL.pcg4d(*v)
+    m = np.uint32
+    with np.errstate(over="ignore"):
+        def uncross(u):
+            x, y, z, w = u
+            w = w - y * z
+            z = z - x * y
+            y = y - z * x
+            x = x - y * w
+            return [x, y, z, w]
+
+        def unshift(a):  # inverse of a ^ (a >> 16)
+            return a ^ (a >> m(16))
+
+        u = uncross(h)
+        u = [unshift(a) for a in u]
+        u = uncross(u)
+        inv = m(pow(1664525, -1, 2 ** 32))
+        u = [(a - m(1013904223)) * inv for a in u]
+    for a, b in zip(u, v):
+        assert np.array_equal(a, b)
+
+
+def test_fractal_noise_range_and_blessed_hash():
+    """blend.rs:314-338 (run_fractal_noise): four octaves into 400x400 RGBA8
+    sRGB hash to the blessed ``distribution_fractal2d`` line (the hash pins
+    the pcg4d constants and the cell blend); with amplitudes summing to 1 the
+    noise is a convex combination of corner values in [0, 1]."""
+    import json, os
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    v = PL.fractal_noise(400, 400, 4, 0.25, 1.0)
+    assert v.min() >= 0.0 and v.max() <= 1.0
+    assert B.blockhash256(C.encode_u8(v, 5)) in hashes["distribution_fractal2d"]
+    # One octave with one cell per pixel samples each cell at its centre,
+    # where the smoothstep weights are 1/2: the value is the mean of the
+    # four corner hashes.
+    w = PL.fractal_noise(8, 6, 1, 1.0, 1.0, scale=(8.0, 6.0))
+    for x, y in ((0, 0), (3, 4), (7, 5)):
+        hs = [np.stack(PL.pcg4d([x + i], [y + j], [0], [0]))[:, 0].astype(np.float64) / 2.0 ** 32
+              for i, j in ((0, 0), (1, 0), (0, 1), (1, 1))]
+        assert np.allclose(w[y, x], np.mean(hs, axis=0), atol=1e-6)

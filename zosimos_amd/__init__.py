@@ -292,6 +292,12 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
+    def distribution_fractal_noise(self, desc: Descriptor, noise: "FractalNoise") -> int:
+        """CommandBuffer::distribution_fractal_noise (command.rs:1589-1607):
+        fractal value noise in all four channels."""
+        return self._reg(self._lib.zs_command_distribution_fractal_noise, ctypes.byref(desc.c()),
+                         ctypes.byref(noise.c()))
+
     def oklab_adjust(self, src: int, chroma: float = 1.0, hue_rad: float = 0.0) -> int:
         return self._reg(self._lib.zs_command_oklab_adjust, src, chroma, hue_rad)
 
@@ -392,6 +398,44 @@ class DistributionNormal2d:
 
     def params(self) -> List[float]:
         return list(self._p)
+
+
+class FractalNoise:
+    """shaders/fractal_noise.rs ShaderData: octaves, initial amplitude,
+    amplitude damping and the grid scale (cells per unit of uv)."""
+
+    def __init__(self, num_octaves: int, initial_amplitude: float, amplitude_damping: float,
+                 grid_scale: Sequence[float] = (100.0, 100.0)):
+        self.num_octaves = int(num_octaves)
+        self.initial_amplitude = float(initial_amplitude)
+        self.amplitude_damping = float(amplitude_damping)
+        self.grid_scale = (float(grid_scale[0]), float(grid_scale[1]))
+
+    def c(self) -> "_abi.FractalNoiseC":
+        n = _abi.FractalNoiseC()
+        n.num_octaves = self.num_octaves
+        n.initial_amplitude = self.initial_amplitude
+        n.amplitude_damping = self.amplitude_damping
+        n.grid_scale[0], n.grid_scale[1] = self.grid_scale
+        return n
+
+    @staticmethod
+    def _from_c(n) -> "FractalNoise":
+        return FractalNoise(n.num_octaves, n.initial_amplitude, n.amplitude_damping, tuple(n.grid_scale))
+
+    @staticmethod
+    def with_octaves(num_octaves: int) -> "FractalNoise":
+        n = _abi.FractalNoiseC()
+        _check(_abi.lib().zs_fractal_noise_with_octaves(num_octaves, ctypes.byref(n)))
+        return FractalNoise._from_c(n)
+
+    def set_damping(self, damping: float) -> "FractalNoise":
+        """The damping, and the initial amplitude that makes the octaves'
+        amplitudes sum to 1."""
+        n = self.c()
+        _check(_abi.lib().zs_fractal_noise_set_damping(ctypes.byref(n), damping))
+        self.initial_amplitude, self.amplitude_damping = n.initial_amplitude, n.amplitude_damping
+        return self
 
 
 class Linker:

@@ -28,6 +28,10 @@ class DescriptorC(ctypes.Structure):
                 ("color", ColorC)]
 
 
+class FractalNoiseC(ctypes.Structure):
+    _fields_ = [("num_octaves", u32), ("initial_amplitude", f32), ("amplitude_damping", f32), ("grid_scale", f32 * 2)]
+
+
 class RectC(ctypes.Structure):
     _fields_ = [("x", u32), ("y", u32), ("max_x", u32), ("max_y", u32)]
 
@@ -108,6 +112,10 @@ _SIGNATURES = {
     "zs_normal2d_with_direction": (i32, [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "zs_command_distribution_normal2d": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(ctypes.c_float),
                                                ctypes.POINTER(u32)]),
+    "zs_fractal_noise_with_octaves": (i32, [u32, ctypes.POINTER(FractalNoiseC)]),
+    "zs_fractal_noise_set_damping": (i32, [ctypes.POINTER(FractalNoiseC), ctypes.c_float]),
+    "zs_command_distribution_fractal_noise": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(FractalNoiseC),
+                                                    ctypes.POINTER(u32)]),
     "zs_command_buffer_init": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_size": (i32, [vp, u32, ctypes.POINTER(ctypes.c_uint64)]),

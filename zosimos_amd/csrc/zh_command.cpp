@@ -18,7 +18,7 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE
 };
 
 struct CmdOp {
@@ -533,6 +533,43 @@ zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor*
   return ZH_OK;
 }
 
+zh_status zs_fractal_noise_with_octaves(uint32_t num_octaves, zs_fractal_noise* out) {
+  if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  out->num_octaves = num_octaves;
+  out->initial_amplitude = (float)(1.0 / (double)num_octaves);
+  out->amplitude_damping = 1.0f;
+  out->grid_scale[0] = out->grid_scale[1] = 100.0f;
+  return ZH_OK;
+}
+
+zh_status zs_fractal_noise_set_damping(zs_fractal_noise* noise, float damping) {
+  if (!noise) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  // Geometric series: sum_{i<n} d^i = (1 - d^n) / (1 - d); a vanishing
+  // 1 - d^n (d = 1) leaves amplitude 1, as the reference does.
+  const float total = 1.0f - std::pow(damping, (float)noise->num_octaves);
+  noise->initial_amplitude = std::fabs(total) < 1e-7f ? 1.0f : (1.0f - damping) / total;
+  noise->amplitude_damping = damping;
+  return ZH_OK;
+}
+
+zh_status zs_command_distribution_fractal_noise(zs_command* cmd, const zh_descriptor* desc,
+                                                const zs_fractal_noise* noise, uint32_t* reg) {
+  if (!cmd || !desc || !noise || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::string why;
+  if (!descriptor_valid(*desc, &why)) return fail(ZH_ERR_TYPE, "inconsistent descriptor for fractal noise: " + why);
+  CmdOp op;
+  op.kind = K_NOISE;
+  op.desc = normalized(*desc);
+  // binary_data: grid scale, amplitude, damping, octaves; padded to 8 bytes.
+  op.blob.assign(24, 0);
+  std::memcpy(op.blob.data(), noise->grid_scale, 8);
+  std::memcpy(op.blob.data() + 8, &noise->initial_amplitude, 4);
+  std::memcpy(op.blob.data() + 12, &noise->amplitude_damping, 4);
+  std::memcpy(op.blob.data() + 16, &noise->num_octaves, 4);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 // A buffer register stands for its bytes; it has no texture of its own, so
 // its slot in the texture table holds a one-byte placeholder.
 static zh_descriptor buffer_placeholder() {
@@ -701,6 +738,9 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_NORMAL2D:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_DISTRIBUTION_NORMAL2D, {}, c.blob, c.knob);
+        break;
+      case K_NOISE:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_FRACTAL_NOISE, {}, c.blob, c.knob);
         break;
       case K_BUFFER:
         break;  // read through the images constructed from it

@@ -224,6 +224,68 @@ struct Pre {
 
 // Evaluate instructions [begin, end) of the stage program.  FAST programs
 // (see decode_texel) take their contiguous loads from the prefetched words.
+// ---- fractal_noise.frag ---------------------------------------------------------
+// pcg4d (Jarzynski and Olano, "Hash Functions for GPU Rendering", JCGT 9(3),
+// 2020): an LCG step per lane, then two rounds of cross-lane multiply-adds
+// around a 16-bit xorshift; wrapping 32-bit arithmetic.
+__device__ __forceinline__ uint4 pcg4d(uint4 v) {
+  v.x = v.x * 1664525u + 1013904223u;
+  v.y = v.y * 1664525u + 1013904223u;
+  v.z = v.z * 1664525u + 1013904223u;
+  v.w = v.w * 1664525u + 1013904223u;
+  v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+  v.x ^= v.x >> 16; v.y ^= v.y >> 16; v.z ^= v.z >> 16; v.w ^= v.w >> 16;
+  v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+  return v;
+}
+
+// The corner value of a cell: pcg4d of (cell, 0, 0) over 2^32 - 1 (as f32,
+// that is 2^32), each lane a channel.
+__device__ __forceinline__ float4 cell_value(uint32_t sx, uint32_t sy) {
+  const uint4 h = pcg4d(make_uint4(sx, sy, 0u, 0u));
+  const float k = 4294967295.0f;
+  return make_float4((float)h.x / k, (float)h.y / k, (float)h.z / k, (float)h.w / k);
+}
+
+// f32 floor as the shader's ivec2(floor(.)) cell index, saturated to int32.
+__device__ __forceinline__ uint32_t cell_index(float f) {
+  return (uint32_t)(int32_t)fminf(fmaxf(f, -2147483648.0f), 2147483520.0f);
+}
+
+// Octave-summed value noise at (x, y) (uv of the pixel centre); every
+// product and sum rounded on its own (no contraction), in the shader's order,
+// so the result is a fixed function of the parameters.
+__device__ __forceinline__ float4 fractal_noise(float x, float y, const __attribute__((address_space(4))) float* q) {
+#pragma clang fp contract(off)
+  const float sx = q[0], sy = q[1], damping = q[3];
+  const uint32_t octaves = (uint32_t)q[4];
+  // 2 mat3(cos .5, sin .5, 1, -sin .5, cos .5, 1, 0, 0, 1): only x and y
+  // reach the noise.
+  const float kc = 1.75516512378075f, ks = 0.958851077208406f;
+  float amp = q[2];
+  float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (uint32_t o = 0; o < octaves; ++o) {  // uniform
+    const float px = x * sx, py = y * sy;
+    const float flx = floorf(px), fly = floorf(py);
+    const float fx = px - flx, fy = py - fly;
+    const uint32_t cx = cell_index(flx), cy = cell_index(fly);
+    const float4 a = cell_value(cx, cy), b = cell_value(cx + 1u, cy);
+    const float4 c = cell_value(cx, cy + 1u), e = cell_value(cx + 1u, cy + 1u);
+    const float ux = fx * fx * (3.0f - 2.0f * fx), uy = fy * fy * (3.0f - 2.0f * fy);
+    const float vx = 1.0f - ux;
+    // mix(a, b, ux) + (c - a) uy (1 - ux) + (d - b) ux uy, mix as a (1 - t) + b t.
+    v.x = v.x + amp * (a.x * vx + b.x * ux + (c.x - a.x) * uy * vx + (e.x - b.x) * ux * uy);
+    v.y = v.y + amp * (a.y * vx + b.y * ux + (c.y - a.y) * uy * vx + (e.y - b.y) * ux * uy);
+    v.z = v.z + amp * (a.z * vx + b.z * ux + (c.z - a.z) * uy * vx + (e.z - b.z) * ux * uy);
+    v.w = v.w + amp * (a.w * vx + b.w * ux + (c.w - a.w) * uy * vx + (e.w - b.w) * ux * uy);
+    const float nx = kc * x + -ks * y, ny = ks * x + kc * y;
+    x = nx;
+    y = ny;
+    amp = amp * damping;
+  }
+  return v;
+}
+
 template <bool FAST = false>
 __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& c, Px& X, Px& Y, Px& Z,
                                     const Luts* lut, Pre pre = Pre{}, bool has_pre = false, bool full = false) {
@@ -306,6 +368,18 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
           const float g = expf(-e) / root;
           X[i] = make_float4(g, g, g, 1.0f);
         }
+        break;
+      }
+      case OP_NOISE: {
+        // fractal_noise.frag at the pixel centres (2-D stage).
+        if (in.b) {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) { Z[i] = Y[i]; Y[i] = X[i]; }
+        }
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+        const float v = ((float)c.y + 0.5f) / (float)d->height;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = fractal_noise(((float)(c.x0 + i) + 0.5f) / (float)d->width, v, q);
         break;
       }
       case OP_BOX3: {

@@ -307,7 +307,7 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
         if (op.initializer == ZH_INIT_PAINT_FULLSCREEN) {
           const int src = op.num_operands > 0 ? B.cur[op.operands[0]] : -1;
           if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR &&
-              op.shader != ZH_SHADER_DISTRIBUTION_NORMAL2D) {
+              op.shader != ZH_SHADER_DISTRIBUTION_NORMAL2D && op.shader != ZH_SHADER_FRACTAL_NOISE) {
             if (src < 0) return fail(ZH_ERR_INVALID_ARGUMENT, "full-screen shader without operand" + at);
             const Node& sn = B.nodes[src];
             if (sn.w != td.width || sn.h != td.height)
@@ -347,6 +347,25 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               g.gen_op = OP_NORMAL2D;
               g.gen_params = 7;
               g.label = "normal2d";
+              B.cur[op.dst] = B.add(g);
+              continue;
+            }
+            case ZH_SHADER_FRACTAL_NOISE: {
+              if (!need_blob(20)) return fail(ZH_ERR_INVALID_ARGUMENT, "fractal noise needs 20 bytes" + at);
+              uint32_t octaves;
+              std::memcpy(&octaves, blob.data() + 16, 4);
+              // Each octave is a uniform loop trip of the kernel; bound the launch.
+              if (octaves > 4096) return fail(ZH_ERR_UNSUPPORTED, "more than 4096 noise octaves" + at);
+              Node g;
+              g.kind = N_GRID;
+              g.desc = td;
+              g.w = td.width;
+              g.h = td.height;
+              std::memcpy(g.grid, blob.data(), 16);
+              g.grid[4] = (float)octaves;
+              g.gen_op = OP_NOISE;
+              g.gen_params = 5;
+              g.label = "noise";
               B.cur[op.dst] = B.add(g);
               continue;
             }
@@ -715,7 +734,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "noise"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -1082,7 +1101,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       // no generator needs the pixel coordinates.
       bool flat = true;
       for (uint32_t i = 0; i < sp.desc.n_instr; ++i)
-        if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3 || sp.desc.code[i].op == OP_NORMAL2D)
+        if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3 || sp.desc.code[i].op == OP_NORMAL2D ||
+            sp.desc.code[i].op == OP_NOISE)
           flat = false;
       for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
         const Slot& s = sp.desc.slot[k];
