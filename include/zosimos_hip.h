@@ -64,6 +64,7 @@ typedef int32_t zh_status;
 #define ZH_COLOR_RGB 0
 #define ZH_COLOR_OKLAB 1
 #define ZH_COLOR_SCALARS 2
+#define ZH_COLOR_SRLAB2 3 /* Color::SrLab2 { whitepoint } (LabA / LchA texels) */
 
 /* Primaries: image-canvas order (serde_bindings.rs:65-75) + Display-P3. */
 #define ZH_PRIMARIES_XYZ 0
@@ -175,6 +176,10 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *  DISTRIBUTION_NORMAL2D 8 floats {expectation (2), inverse covariance (4,
  *                        row-major), pseudo-determinant, 0}, 32 B
  *                        (shaders/distribution_normal2d.rs binary_data)
+ *  SRLAB2_ENCODE/DECODE  {white point XYZ vec4, mat3 std140}, 64 B
+ *                        (shaders/srlab2.rs binary_data): encode takes the
+ *                        RGB -> XYZ matrix, decode the XYZ -> RGB one
+ *                        (srlab2.frag)
  *  FRACTAL_NOISE         {grid_scale (2 f32), initial amplitude f32, amplitude
  *                        damping f32, octaves u32}, 20 B padded to 24
  *                        (shaders/fractal_noise.rs binary_data); at most 4096
@@ -194,6 +199,8 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_BOX3 6
 #define ZH_SHADER_DISTRIBUTION_NORMAL2D 7
 #define ZH_SHADER_FRACTAL_NOISE 8
+#define ZH_SHADER_SRLAB2_ENCODE 9
+#define ZH_SHADER_SRLAB2_DECODE 10
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

@@ -292,9 +292,54 @@ def oklab_to_rgb(lab, primaries: int = 3, clamp: bool = True):
     return np.clip(rgb, 0.0, 1.0) if clamp else rgb
 
 
+# CIECAM02's CAT02 and Hunt-Pointer-Estevez matrices, row-major
+# (srlab2.frag:13-23, written column by column there).
+CAT02 = np.array([[0.7328, 0.4296, -0.1624], [-0.7036, 1.6975, 0.0061], [0.0030, 0.0136, 0.9834]])
+HPE = np.array([[0.38971, 0.68898, -0.07868], [-0.22981, 1.18340, 0.04641], [0.0, 0.0, 1.0]])
+# SrLab2's difference scheme: (L, a, b) = (y, (x - y) 5 / 1.16, (z - y) 2 / 1.16).
+SRLAB_DIFF = np.array([[0.0, 1.0, 0.0], [5.0 / 1.16, -5.0 / 1.16, 0.0], [0.0, -2.0 / 1.16, 2.0 / 1.16]])
+
+
+def srlab_nl(v):
+    """srlab2.frag:87-94: linear below 216/24389 in magnitude, 1.16 cbrt - 0.16
+    above (the cube root taken with its sign for negative values, where the
+    shader's pow is undefined)."""
+    v = np.asarray(v, dtype=np.float64)
+    return np.where(np.abs(v) < 216.0 / 24389.0, v * 24389.0 / 2700.0, 1.16 * np.cbrt(v) - 0.16)
+
+
+def srlab_nl_inv(v):
+    """srlab2.frag:104-111."""
+    v = np.asarray(v, dtype=np.float64)
+    return np.where(np.abs(v) < 0.08, v * 2700.0 / 24389.0, ((v + 0.16) / 1.16) ** 3)
+
+
+def rgb_to_srlab2(rgb, primaries: int = 3, rgb_wp: int = 5):
+    """srlab2.frag:35-59 (encode): XYZ of the linear RGB, into HPE cone
+    space (the CAT02 round trip of the shader cancels), the non-linearity,
+    back to XYZ' and the difference scheme.  The SrLab2 white point plays no
+    part (its division is commented out in the shader)."""
+    xyz = np.asarray(rgb, dtype=np.float64) @ rgb_to_xyz_matrix(primaries, rgb_wp).T
+    xyz_e = srlab_nl(xyz @ HPE.T) @ np.linalg.inv(HPE).T
+    return xyz_e @ SRLAB_DIFF.T
+
+
+def srlab2_to_rgb(lab, primaries: int = 3, rgb_wp: int = 5, lab_wp: int = 5, clamp: bool = True):
+    """srlab2.frag:62-85 (decode): undo the difference scheme and the
+    non-linearity, into CAT02 cone space, scale by the CAT02 response of the
+    SrLab2 white point (which the encode does not divide out), back to XYZ
+    and RGB, clamped to [0, 1]."""
+    xyz_e = np.asarray(lab, dtype=np.float64) @ np.linalg.inv(SRLAB_DIFF).T
+    rgb_w = srlab_nl_inv(xyz_e @ HPE.T) @ (CAT02 @ np.linalg.inv(HPE)).T
+    xyz = (rgb_w * (CAT02 @ whitepoint_xyz(lab_wp))) @ np.linalg.inv(CAT02).T
+    rgb = xyz @ np.linalg.inv(rgb_to_xyz_matrix(primaries, rgb_wp)).T
+    return np.clip(rgb, 0.0, 1.0) if clamp else rgb
+
+
 def lch_adjust(lab, chroma: float = 1.0, hue_deg: float = 0.0):
     """Scale Oklab chroma and rotate hue: ``(a, b) <- k R(theta) (a, b)``.
-    No reference implementation exists; this is the polar form of
+    No reference implementation exists (the reference needs a user shader for
+    it, command/dynamic.rs:7-28); this is the polar form of
     stage.frag:415-425 applied as one 2x2 map on (a, b)."""
     t = np.radians(hue_deg)
     c, s = np.cos(t), np.sin(t)

@@ -541,6 +541,36 @@ def test_fractal_noise_f32_bit_exact(ctx, w, h, octaves, damping, scale):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("primaries,wp", [(z.Primaries.Bt709, z.Whitepoint.D65),
+                                           (z.Primaries.DisplayP3, z.Whitepoint.D50)])
+def test_srlab2_encode_decode(ctx, primaries, wp):
+    """ColorConversion::XyzToSrLab2 / SrLab2ToXyz (srlab2.frag) on ragged
+    linear RGBA32F images: encode within 2e-5 of the FP64 oracle; decode
+    (white point scaling and clamp included) within 2e-5."""
+    w, h = 53, 11
+    rgb_c = z.Color(primaries=primaries, transfer=z.Transfer.Linear, whitepoint=wp)
+    src = z.Descriptor(w, h, z.Texel(z.SampleBits.Float32x4, z.SampleParts.RgbA), rgb_c)
+    lab_c = z.Color(model=z.ColorModel.SrLab2, primaries=0, transfer=z.Transfer.Linear, whitepoint=z.Whitepoint.D65)
+    lab_t = z.Texel(z.SampleBits.Float32x4, z.SampleParts.LabA)
+    rng = np.random.default_rng(12)
+    data = rng.uniform(0.0, 1.0, (h, w, 4)).astype(np.float32)
+    cmd = z.CommandBuffer()
+    a = cmd.input(src)
+    lab = cmd.color_convert(a, lab_c, lab_t)
+    ol, old_ = cmd.output(lab)
+    ob, obd = cmd.output(cmd.color_convert(lab, rgb_c, src.texel))
+    out, _ = run(ctx, cmd, {a: src}, {a: data.view(np.uint8).reshape(h, -1)}, {ol: old_, ob: obd})
+    got_lab = out[ol].view(np.float32).reshape(h, w, 4)
+    got_rgb = out[ob].view(np.float32).reshape(h, w, 4)
+    want_lab = C.rgb_to_srlab2(data[..., :3].astype(np.float64), primaries, wp)
+    want_rgb = C.srlab2_to_rgb(got_lab[..., :3].astype(np.float64), primaries, wp, z.Whitepoint.D65)
+    e1 = np.abs(got_lab[..., :3] - want_lab).max()
+    e2 = np.abs(got_rgb[..., :3] - want_rgb).max()
+    report("srlab2", primaries=primaries, wp=wp, encode_err=float(e1), decode_err=float(e2))
+    assert e1 <= 2e-5 and e2 <= 2e-5
+    assert np.array_equal(got_lab[..., 3], data[..., 3]) and np.array_equal(got_rgb[..., 3], data[..., 3])
+
+
 def _pitched(data, pitch, rows):
     """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
     buf = np.zeros(pitch * rows, np.uint8)

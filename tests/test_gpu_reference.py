@@ -22,7 +22,8 @@ that layout and through ``buffer_init``/``from_buffer``) and
 ``run_from_buffer_knob`` (the same through a buffer knob,
 ``from_buffer-with-knob``); ``run_distribution`` and
 ``run_distribution_normal1d`` (``distribution_normal2d``, ``_normal1d``),
-``run_fractal_noise`` (``distribution_fractal2d``).
+``run_fractal_noise`` (``distribution_fractal2d``), ``run_srlab2``
+(``srlab2``).
 ``run_distribution_u8`` is checked against the oracle only
 (``test_gpu_parity.py``): its blessed hash needs a rounding 8-bit store,
 where stage.frag truncates.
@@ -144,6 +145,30 @@ def test_oklab_matches_reference_hash(ctx):
     report("oklab_plane", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
     assert d[..., 3].max() == 0 and d.max() <= 1
     check("oklab", out)
+
+
+def test_srlab2_matches_reference_hash(ctx):
+    """blend.rs:510-575: the grid of run_oklab transmuted to an LchA SrLab2
+    (D65) texel and converted to sRGB; within 1 code of the FP64 oracle
+    where it is not within rounding of a code edge, and the blessed hash."""
+    w = h = 400
+    srgb = z.Descriptor.rgba8_srgb(w, h)
+    scal = srgb.with_(color=z.Color(model=z.ColorModel.Scalars, transfer=z.Transfer.Linear))
+    lch = z.Descriptor(w, h, z.Texel(z.SampleBits.UInt8x4, z.SampleParts.LchA), z.Color.SRLAB2_D65)
+    plane = z.Bilinear(u_min=(0.4, 0.0, 0.0, 1.0), u_max=(0.4, 0.0, 1.0, 1.0),
+                       v_min=(0.4, 0.0, 0.0, 1.0), v_max=(0.4, 1.0, 0.0, 1.0))
+    cmd = z.CommandBuffer()
+    t = cmd.transmute(cmd.bilinear(scal, plane), lch)
+    o, od = cmd.output(cmd.color_convert(t, srgb.color, srgb.texel))
+    out = run_program(ctx, cmd, {}, o, od)
+    lab = C.decode_u8(PL.staged_u8_store(PL.bilinear_grid(w, h, plane.params())), 0x100)
+    lin = lab.copy()
+    lin[..., :3] = C.srlab2_to_rgb(lab[..., :3])
+    want = C.encode_u8(lin, 5)
+    d = np.abs(out.astype(np.int32) - want.astype(np.int32))
+    report("srlab2_plane", max_code=int(d.max()), exact=f"{float(np.mean(d == 0)):.5f}")
+    assert d[..., 3].max() == 0 and d.max() <= 1
+    check("srlab2", out)
 
 
 def test_adapted_matches_reference_hash(ctx):

@@ -101,7 +101,7 @@ def test_srgb_u8_encode_decode_identity():
 
 # ---- stage.frag restatement (FP32 C) -----------------------------------------------
 def test_stage_u8_u16_exhaustive_round_trip():
-    # SURVEY hard part 2: correctly rounded division round-trips every code.
+    # Correctly rounded division round-trips every code.
     v8 = np.arange(256, dtype=np.uint32)
     assert np.array_equal(S.stage_encode(S.stage_decode(v8, 0, 4, 4), 0, 4, 4), v8)
     v16 = np.arange(65536, dtype=np.uint32)
@@ -533,3 +533,48 @@ def test_fractal_noise_range_and_blessed_hash():
         hs = [np.stack(PL.pcg4d([x + i], [y + j], [0], [0]))[:, 0].astype(np.float64) / 2.0 ** 32
               for i, j in ((0, 0), (1, 0), (0, 1), (1, 1))]
         assert np.allclose(w[y, x], np.mean(hs, axis=0), atol=1e-6)
+
+
+# ---- srlab2.frag ----------------------------------------------------------------------
+def test_srlab2_non_linearity_is_continuous_and_inverted():
+    """The two branches meet at 216/24389 -> 0.08 (6^3/29^3 and 1.16 * 6/29 -
+    0.16), and the inverse undoes the forward map on both sides."""
+    k = 216.0 / 24389.0
+    assert np.isclose(k * 24389.0 / 2700.0, 0.08) and np.isclose(1.16 * np.cbrt(k) - 0.16, 0.08)
+    v = np.concatenate([np.linspace(-0.5, 1.5, 2001), [k * (1 - 1e-9), k * (1 + 1e-9)]])
+    assert np.allclose(C.srlab_nl_inv(C.srlab_nl(v[v > -k])), v[v > -k], atol=1e-12)
+
+
+def test_srlab2_decode_of_encode_is_the_white_point_scaling():
+    """decode(encode(rgb)) is the linear map T^-1 CAT02^-1 diag(CAT02 w)
+    CAT02 T (T: RGB -> XYZ, w: the SrLab2 white point): the decode scales the
+    cone responses by the white's and the encode does not divide them out
+    (srlab2.frag:37, 64)."""
+    rng = np.random.default_rng(11)
+    rgb = rng.uniform(0.05, 0.6, (200, 3))
+    T = C.rgb_to_xyz_matrix(3, 5)
+    M = np.linalg.inv(T) @ np.linalg.inv(C.CAT02) @ np.diag(C.CAT02 @ C.whitepoint_xyz(5)) @ C.CAT02 @ T
+    got = C.srlab2_to_rgb(C.rgb_to_srlab2(rgb), clamp=False)
+    assert np.allclose(got, rgb @ M.T, atol=1e-12)
+    # L is the XYZ' luminance: a scaled grey stays on the grey axis of XYZ'
+    lab = C.rgb_to_srlab2(np.array([[0.2, 0.2, 0.2], [0.7, 0.7, 0.7]]))
+    assert lab[1, 0] > lab[0, 0]
+
+
+def test_srlab2_lch_plane_reproduces_blessed_hash():
+    """blend.rs:510-575 (run_srlab2): the bilinear grid of run_oklab stored to
+    RGBA8 with a Scalars colour (truncating staged store), reinterpreted as
+    SrLab2 LchA and converted to sRGB hashes to the blessed line; the decode
+    without the white point's cone scaling does not."""
+    import json, os
+    from oracle import blockhash as B
+    hashes = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "reference_hashes.json")))
+    lab = C.decode_u8(PL.staged_u8_store(PL.bilinear_grid(400, 400, OKLAB_PLANE)), 0x100)
+    out = lab.copy()
+    out[..., :3] = C.srlab2_to_rgb(lab[..., :3])
+    assert B.blockhash256(C.encode_u8(out, 5)) in hashes["srlab2"]
+    xyz_e = lab[..., :3] @ np.linalg.inv(C.SRLAB_DIFF).T
+    xyz = C.srlab_nl_inv(xyz_e @ C.HPE.T) @ np.linalg.inv(C.HPE).T
+    out[..., :3] = np.clip(xyz @ np.linalg.inv(C.rgb_to_xyz_matrix(3, 5)).T, 0, 1)
+    assert B.blockhash256(C.encode_u8(out, 5)) not in hashes["srlab2"]

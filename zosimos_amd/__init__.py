@@ -71,7 +71,7 @@ class Whitepoint:
 
 
 class ColorModel:
-    Rgb, Oklab, Scalars = 0, 1, 2
+    Rgb, Oklab, Scalars, SrLab2 = 0, 1, 2, 3
 
 
 class Status:
@@ -111,6 +111,7 @@ class Color:
 Color.SRGB = Color()
 Color.BT709_RGB = Color(transfer=Transfer.Bt709)
 Color.OKLAB = Color(model=ColorModel.Oklab, primaries=0, transfer=Transfer.Linear, whitepoint=Whitepoint.D65)
+Color.SRLAB2_D65 = Color(model=ColorModel.SrLab2, primaries=0, transfer=Transfer.Linear, whitepoint=Whitepoint.D65)
 
 
 @dataclass(frozen=True)

@@ -92,11 +92,12 @@ std::vector<uint8_t> std140(const Mat3& m) {
 }
 
 bool chroma_equal(const zh_descriptor& a, const zh_descriptor& b) {
-  return a.texel.bits == b.texel.bits && a.texel.parts == b.texel.parts && a.color.model == b.color.model &&
-         (a.color.model == ZH_COLOR_OKLAB ||
-          (a.color.transfer == b.color.transfer &&
-           (a.color.model != ZH_COLOR_RGB ||
-            (a.color.primaries == b.color.primaries && a.color.whitepoint == b.color.whitepoint))));
+  if (a.texel.bits != b.texel.bits || a.texel.parts != b.texel.parts || a.color.model != b.color.model) return false;
+  if (a.color.model == ZH_COLOR_OKLAB) return true;
+  if (a.color.model == ZH_COLOR_SRLAB2) return a.color.whitepoint == b.color.whitepoint;
+  return a.color.transfer == b.color.transfer &&
+         (a.color.model != ZH_COLOR_RGB ||
+          (a.color.primaries == b.color.primaries && a.color.whitepoint == b.color.whitepoint));
 }
 
 zh_rect with_size(uint32_t w, uint32_t h) { return zh_rect{0, 0, w, h}; }
@@ -238,6 +239,25 @@ zh_status zs_command_color_convert(zs_command* cmd, uint32_t src, const zh_color
       return fail(ZH_ERR_TYPE, "unknown primaries");
     op.shader = ZH_SHADER_OKLAB_DECODE;
     op.blob = std140(pdi);
+  } else if ((a.model == ZH_COLOR_RGB && b.model == ZH_COLOR_SRLAB2) ||
+             (a.model == ZH_COLOR_SRLAB2 && b.model == ZH_COLOR_RGB)) {
+    // ColorConversion::XyzToSrLab2 / SrLab2ToXyz (command.rs:1052-1075,
+    // 3252-3274): the RGB side's XYZ matrix at its own white point (inverted
+    // for the decode) and the SrLab2 white point's XYZ.
+    const bool enc = a.model == ZH_COLOR_RGB;
+    const zh_color& rgb = enc ? a : b;
+    const zh_color& lab = enc ? b : a;
+    Mat3 m, mi;
+    if (lab.whitepoint > 10 || !rgb_to_xyz(rgb.primaries, rgb.whitepoint, &m) || !mat_inv(m, &mi))
+      return fail(ZH_ERR_TYPE, "unknown primaries or white point");
+    double W[3];
+    white_xyz(lab.whitepoint, W);
+    const float wp[4] = {(float)W[0], (float)W[1], (float)W[2], 0.0f};
+    op.shader = enc ? ZH_SHADER_SRLAB2_ENCODE : ZH_SHADER_SRLAB2_DECODE;
+    op.blob.resize(16);
+    std::memcpy(op.blob.data(), wp, 16);
+    const std::vector<uint8_t> mb = std140(enc ? m : mi);
+    op.blob.insert(op.blob.end(), mb.begin(), mb.end());
   } else {
     return fail(ZH_ERR_TYPE, "No conversion");
   }

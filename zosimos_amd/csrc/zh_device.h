@@ -118,6 +118,17 @@ __device__ __forceinline__ float3 mat3(const __attribute__((address_space(4))) f
 // (relative error ~1e-7 over the colour range; cbrt(0) = 0 since
 // exp2(-inf) = 0).
 __device__ __forceinline__ float cbrt_s(float x) { return copysignf(pow_pos(fabsf(x), 1.0f / 3.0f), x); }
+// SrLab2's cone non-linearity and its inverse (srlab2.frag:87-120).  The
+// shader's pow of a negative base is undefined; below -216/24389 the cube
+// root is taken with its sign.
+__device__ __forceinline__ float srlab_nl(float v) {
+  return fabsf(v) < 216.0f / 24389.0f ? v * 24389.0f / 2700.0f : 1.16f * cbrt_s(v) - 0.16f;
+}
+__device__ __forceinline__ float srlab_nl_inv(float v) {
+  if (fabsf(v) < 0.08f) return v * 2700.0f / 24389.0f;
+  const float p = (v + 0.16f) / 1.16f;
+  return p * p * p;
+}
 
 // 8-bit sRGB decode table replicated once per LDS bank: entry e of copy c
 // sits at word e * 32 + c, so lane l reads copy l % 32 from its own bank and

@@ -431,6 +431,15 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         for (int i = 0; i < kPix; ++i)
           X[i] = make_float4(X[i].x * X[i].x * X[i].x, X[i].y * X[i].y * X[i].y, X[i].z * X[i].z * X[i].z, X[i].w);
         break;
+      case OP_SRLAB:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) X[i] = make_float4(srlab_nl(X[i].x), srlab_nl(X[i].y), srlab_nl(X[i].z), X[i].w);
+        break;
+      case OP_SRLAB_INV:
+#pragma unroll
+        for (int i = 0; i < kPix; ++i)
+          X[i] = make_float4(srlab_nl_inv(X[i].x), srlab_nl_inv(X[i].y), srlab_nl_inv(X[i].z), X[i].w);
+        break;
       case OP_CLAMP01:
 #pragma unroll
         for (int i = 0; i < kPix; ++i) X[i] = make_float4(clamp01(X[i].x), clamp01(X[i].y), clamp01(X[i].z), X[i].w);

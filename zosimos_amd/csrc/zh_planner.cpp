@@ -40,13 +40,15 @@ bool is_native(const zh_descriptor& d) {
 }
 
 uint32_t stage_transfer(const zh_descriptor& d) {
-  if (d.color.model == ZH_COLOR_OKLAB) return d.texel.parts == ZH_PARTS_LCHA ? ZH_TRANSFER_LABLCH : ZH_TRANSFER_LINEAR;
+  if (d.color.model == ZH_COLOR_OKLAB || d.color.model == ZH_COLOR_SRLAB2)
+    return d.texel.parts == ZH_PARTS_LCHA ? ZH_TRANSFER_LABLCH : ZH_TRANSFER_LINEAR;
   return d.color.transfer;
 }
 
 uint32_t stage_parts(const zh_descriptor& d) {
-  // Oklab texels stage as LchA (program.rs:873-899), which passes components.
-  if (d.color.model == ZH_COLOR_OKLAB) return ZH_PARTS_LCHA;
+  // Oklab and SrLab2 texels stage as LchA (program.rs:873-930), which passes
+  // components.
+  if (d.color.model == ZH_COLOR_OKLAB || d.color.model == ZH_COLOR_SRLAB2) return ZH_PARTS_LCHA;
   return d.texel.parts;
 }
 
@@ -60,9 +62,10 @@ bool descriptor_valid(const zh_descriptor& d, std::string* why) {
   if (d.width == 0 || d.height == 0) { if (why) *why = "empty image"; return false; }
   if (d.row_stride && d.row_stride < tight_stride(d)) { if (why) *why = "row_stride shorter than a row"; return false; }
   if (d.texel.parts > 20) { if (why) *why = "unknown sample parts id"; return false; }
-  if (d.color.model > ZH_COLOR_SCALARS) { if (why) *why = "unknown color model"; return false; }
-  if (d.color.model == ZH_COLOR_OKLAB && d.texel.parts != ZH_PARTS_LCHA && d.texel.parts != 18) {
-    if (why) *why = "Oklab needs LabA or LchA parts";
+  if (d.color.model > ZH_COLOR_SRLAB2) { if (why) *why = "unknown color model"; return false; }
+  if ((d.color.model == ZH_COLOR_OKLAB || d.color.model == ZH_COLOR_SRLAB2) && d.texel.parts != ZH_PARTS_LCHA &&
+      d.texel.parts != 18) {
+    if (why) *why = "Oklab and SrLab2 need LabA or LchA parts";
     return false;
   }
   return true;
@@ -104,6 +107,10 @@ Mat3 mat_from_std140(const void* blob) {
 // Oklab constants (oklab.frag:14-24, row-major).
 static const Mat3 kM1 = {0.8189330101, 0.3618667424, -0.1288597137, 0.0329845436, 0.9293118715,
                          0.0361456387, 0.0482003018, 0.2643662691, 0.6338517070};
+// CIECAM02's CAT02 and Hunt-Pointer-Estevez matrices, row-major
+// (srlab2.frag:13-23 writes them column by column).
+static const Mat3 kCAT02 = {0.7328, 0.4296, -0.1624, -0.7036, 1.6975, 0.0061, 0.0030, 0.0136, 0.9834};
+static const Mat3 kHPE = {0.38971, 0.68898, -0.07868, -0.22981, 1.18340, 0.04641, 0.0, 0.0, 1.0};
 static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951, -2.4285922050,
                          0.4505937099, 0.0259040371, 0.7827717662, -0.8086757660};
 
@@ -116,7 +123,7 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // of their own, which runs on the 2-D interpreter: they need pixel
 // coordinates.
 enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL };
-enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL };
+enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL, M_SRLAB, M_SRLAB_INV };
 
 struct Micro {
   MicroKind k;
@@ -389,6 +396,38 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               n.kind = N_UNARY;
               n.micro = {{M_MAT3, m2i}, {M_CUBE, {}}, {M_MAT3, mat_mul(mat_from_std140(blob.data()), m1i)}, {M_CLAMP, {}}};
               n.label = "oklab-decode";
+              break;
+            }
+            case ZH_SHADER_SRLAB2_ENCODE:
+            case ZH_SHADER_SRLAB2_DECODE: {
+              // srlab2.frag: XYZ into Hunt-Pointer-Estevez cone space, the
+              // non-linearity, and back through the difference scheme
+              // (L, a, b) = (y, (x - y) 5 / 1.16, (z - y) 2 / 1.16).  The
+              // encode's CAT02 round trip cancels; the decode scales the CAT02
+              // cone responses by those of the white point, which the encode
+              // does not divide out (srlab2.frag:35-81).
+              if (!need_blob(64)) return fail(ZH_ERR_INVALID_ARGUMENT, "srlab2 blob needs 64 bytes" + at);
+              float wpf[4];
+              std::memcpy(wpf, blob.data(), 16);
+              const Mat3 m = mat_from_std140(blob.data() + 16);
+              const Mat3 E = {0, 1, 0, 5 / 1.16, -5 / 1.16, 0, 0, -2 / 1.16, 2 / 1.16};
+              Mat3 Ei, Hi, Ci;
+              mat_inv(E, &Ei);
+              mat_inv(kHPE, &Hi);
+              mat_inv(kCAT02, &Ci);
+              n.kind = N_UNARY;
+              if (op.shader == ZH_SHADER_SRLAB2_ENCODE) {
+                n.micro = {{M_MAT3, mat_mul(kHPE, m)}, {M_SRLAB, {}}, {M_MAT3, mat_mul(E, Hi)}};
+                n.label = "srlab2-encode";
+              } else {
+                double w[3];
+                for (int r = 0; r < 3; ++r)
+                  w[r] = kCAT02[r * 3] * wpf[0] + kCAT02[r * 3 + 1] * wpf[1] + kCAT02[r * 3 + 2] * wpf[2];
+                const Mat3 D = {w[0], 0, 0, 0, w[1], 0, 0, 0, w[2]};
+                const Mat3 back = mat_mul(m, mat_mul(Ci, mat_mul(D, mat_mul(kCAT02, Hi))));
+                n.micro = {{M_MAT3, mat_mul(kHPE, Ei)}, {M_SRLAB_INV, {}}, {M_MAT3, back}, {M_CLAMP, {}}};
+                n.label = "srlab2-decode";
+              }
               break;
             }
             case ZH_SHADER_OKLAB_ADJUST: {
@@ -695,6 +734,8 @@ struct Emitter {
             case M_CLAMP: op(OP_CLAMP01); break;
             case M_PREMUL: op(OP_PREMUL); break;
             case M_UNPREMUL: op(OP_UNPREMUL); break;
+            case M_SRLAB: op(OP_SRLAB); break;
+            case M_SRLAB_INV: op(OP_SRLAB_INV); break;
           }
         }
         quantise(n, pm);
@@ -734,7 +775,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "noise"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "srlab", "srlab_inv", "noise"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -1311,6 +1352,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
           case M_CLAMP: E.op(OP_CLAMP01); break;
           case M_PREMUL: E.op(OP_PREMUL); break;
           case M_UNPREMUL: E.op(OP_UNPREMUL); break;
+          case M_SRLAB: E.op(OP_SRLAB); break;
+          case M_SRLAB_INV: E.op(OP_SRLAB_INV); break;
         }
       }
       E.quantise(u, pm);

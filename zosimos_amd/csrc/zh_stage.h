@@ -58,6 +58,9 @@ enum Opcode : uint8_t {
   OP_NORMAL2D,  // arg = params {mu.x, mu.y, C00, C01, C10, C11, det}; b = push.  At the pixel centre,
                 //   p = 2 (u, v) - 1 - mu, X <- (g, g, g, 1) with g = exp(-p.C p / 2) / sqrt(det)
                 //   (distribution_normal2d.frag)
+  OP_SRLAB,     // X.rgb <- the SrLab2 non-linearity per channel: 24389/2700 x below 216/24389 in
+                //   magnitude, 1.16 x^(1/3) - 0.16 above (srlab2.frag)
+  OP_SRLAB_INV, // X.rgb <- its inverse: 2700/24389 x below 0.08 in magnitude, ((x + 0.16) / 1.16)^3 above
   OP_NOISE,     // arg = params {scale.x, scale.y, amplitude, damping, octaves}; b = push.  X <- fractal
                 //   (octave-summed) value noise of pcg4d-hashed cell corners at the pixel centre
                 //   (fractal_noise.frag)
