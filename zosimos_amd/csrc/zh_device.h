@@ -130,22 +130,6 @@ __device__ __forceinline__ float srlab_nl_inv(float v) {
   return p * p * p;
 }
 
-// 8-bit sRGB decode table replicated once per LDS bank: entry e of copy c
-// sits at word e * 32 + c, so lane l reads copy l % 32 from its own bank and
-// a wave's data-dependent lookups never conflict (32 KiB).
-constexpr int kRepDec = 256 * 32;
-__device__ __forceinline__ void load_rep(const Luts* L, float* rep) {
-  for (int i = (int)threadIdx.x; i < kRepDec; i += kBlock) rep[i] = L->dec[i >> 5];
-}
-template <uint32_t F>
-__device__ __forceinline__ float4 dec_u8_rep(uint32_t w, const float* rep) {
-  constexpr bool bgra = F == FMT_U8_SRGB_BGRA || F == FMT_U8_LIN_BGRA;
-  const uint32_t c = threadIdx.x & 31u;
-  const uint32_t b0 = w & 255u, b1 = (w >> 8) & 255u, b2 = (w >> 16) & 255u, b3 = w >> 24;
-  const uint32_t r = bgra ? b2 : b0, b = bgra ? b0 : b2;
-  return make_float4(rep[(r << 5) | c], rep[(b1 << 5) | c], rep[(b << 5) | c], (float)b3 * (1.0f / 255.0f));
-}
-
 // Workgroups of `kernel` (kBlock lanes) resident on a device of `cus` compute
 // units: the grid of a persistent kernel, which then has no second partial
 // round of workgroups.

@@ -40,15 +40,13 @@ template <> struct RawT<FMT_F16> { typedef u32x2 type; };
 template <> struct RawT<FMT_F32> { typedef u32x4 type; };
 
 template <uint32_t F>
-__device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const float* rep) {
+__device__ __forceinline__ float4 raw_dec(typename RawT<F>::type r, const Luts* L) {
   if constexpr (F == FMT_F16) {
     return dec_f16(r.x, r.y);
   } else if constexpr (F == FMT_F32) {
     return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w));
-  } else if constexpr (F == FMT_U8_SRGB_RGBA || F == FMT_U8_SRGB_BGRA) {
-    return dec_u8_rep<F>(r, rep);
   } else {
-    return dec_u8<F>(r, nullptr);
+    return dec_u8<F>(r, L);
   }
 }
 
@@ -180,7 +178,7 @@ __device__ __forceinline__ typename RawT<F>::type pair_half(const typename PairT
 }
 
 template <class V, bool SEP, bool P64>
-__device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp, int n, Px& X, const float* rep) {
+__device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp, int n, Px& X, const Luts* L) {
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
   // 4- and 8-byte texels: the two taps of a row are one load.  The left tap
@@ -318,11 +316,11 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const float4 c = raw_dec<V::F>(PAIR ? pair_half<V::F>(p0[k][t >> 1], t & 1) : t0[k][t], rep);
+      const float4 c = raw_dec<V::F>(PAIR ? pair_half<V::F>(p0[k][t >> 1], t & 1) : t0[k][t], L);
       if constexpr (V::SRC == WSRC_OVER2) {
         // Premultiplied 'over' of the two sources, then weighted.
         const float4 lo = premul4(c);
-        const float4 h = premul4(raw_dec<V::F>(PAIR ? pair_half<V::F>(p1[k][t >> 1], t & 1) : t1[k][t], rep));
+        const float4 h = premul4(raw_dec<V::F>(PAIR ? pair_half<V::F>(p1[k][t >> 1], t & 1) : t1[k][t], L));
         const float q = 1.0f - h.w, wt = w[k][t];
         const float4 m = make_float4(fmaf(lo.x, q, h.x), fmaf(lo.y, q, h.y), fmaf(lo.z, q, h.z), fmaf(lo.w, q, h.w));
         s = make_float4(fmaf(m.x, wt, s.x), fmaf(m.y, wt, s.y), fmaf(m.z, wt, s.z), fmaf(m.w, wt, s.w));
@@ -347,7 +345,7 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
 // row ty * tile_h + ly, so every load and store instruction covers runs of
 // lanes_x adjacent pixels.
 template <class V>
-__device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, const Luts* L, const float* rep) {
+__device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, const Luts* L) {
   typedef typename V::Post Post;
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
@@ -405,7 +403,7 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
     // Axis-aligned maps (pure scale and shift; the planner picks the SEP
     // variant only when every layer has one) are separable: the lane's
     // pixels share one source row pair, computed once.
-    layer_taps<V, V::SEP, P64>(a, mp, n, X, rep);
+    layer_taps<V, V::SEP, P64>(a, mp, n, X, L);
   }
 
   if (n == 0) return;
@@ -447,28 +445,33 @@ __device__ __forceinline__ void tile_xy(CDesc d, uint32_t t, uint32_t& tx, uint3
   tx = s * sw + (r - ty * w);
 }
 
-// Replicated decode table only for 8-bit sRGB sources.
+// The 8-bit sRGB tables are staged in LDS only where a source or the post
+// program codes 8-bit sRGB texels.  (Decoding straight from the 256-entry
+// table beats a bank-replicated copy here: a tile is too short to repay the
+// copy's fill.)
 template <class V>
-constexpr int rep_words() {
-  return (V::F == FMT_U8_SRGB_RGBA || V::F == FMT_U8_SRGB_BGRA) ? kRepDec : 1;
+constexpr bool needs_lut() {
+  if (V::F == FMT_U8_SRGB_RGBA || V::F == FMT_U8_SRGB_BGRA) return true;
+  for (int k = 0; k < V::Post::n; ++k) {
+    const uint32_t f = el_fmt(V::Post::el[k]);
+    if (f == FMT_U8_SRGB_RGBA || f == FMT_U8_SRGB_BGRA) return true;
+  }
+  return false;
 }
 
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp(const StageDesc* dd) {
   const CDesc d = as_const(dd);
   __shared__ Luts luts;
-  __shared__ float rep[rep_words<V>()];
-  load_lut(d, &luts);
-  __syncthreads();
-  if constexpr (rep_words<V>() > 1) {
-    load_rep(&luts, rep);
+  if constexpr (needs_lut<V>()) {
+    load_lut(d, &luts);
     __syncthreads();
   }
   const uint32_t tiles = d->tiles_x * d->tiles_y;
   for (uint32_t t = virtual_block(); t < tiles; t += gridDim.x) {
     uint32_t tx, ty;
     tile_xy(d, t, tx, ty);
-    warp_tile<V>(d, tx, ty, &luts, rep);
+    warp_tile<V>(d, tx, ty, &luts);
   }
 }
 
@@ -499,11 +502,8 @@ template <class V>
 __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   const CBatch b = (CBatch)(uintptr_t)bb;
   __shared__ Luts luts;
-  __shared__ float rep[rep_words<V>()];
-  load_lut(as_const(b->stages), &luts);
-  __syncthreads();
-  if constexpr (rep_words<V>() > 1) {
-    load_rep(&luts, rep);
+  if constexpr (needs_lut<V>()) {
+    load_lut(as_const(b->stages), &luts);
     __syncthreads();
   }
   const uint32_t total = b->total_tiles;
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
     const CDesc d = as_const(b->stages + cword(b->stage_ids, lo));
     uint32_t tx, ty;
     tile_xy(d, j - cword(b->tile_prefix, lo), tx, ty);
-    warp_tile<V>(d, tx, ty, &luts, rep);
+    warp_tile<V>(d, tx, ty, &luts);
   }
 }
 
