@@ -102,6 +102,25 @@ zh_status zs_normal2d_with_direction(float x, float y, float out[8]);
  * `desc` must have Luma or LumaA parts (ZH_ERR_TYPE otherwise). */
 zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor* desc, const float params[8],
                                            uint32_t* reg);
+/* Colour channels that can index a palette (image-canvas ColorChannel; the
+ * shader takes R, G or B, shaders/palette.rs:30-38). */
+#define ZS_CHANNEL_NONE (-1)
+#define ZS_CHANNEL_R 0
+#define ZS_CHANNEL_G 1
+#define ZS_CHANNEL_B 2
+#define ZS_CHANNEL_A 3
+/* Palette, command.rs:527-537. */
+typedef struct zs_palette {
+  int32_t width;   /* ZS_CHANNEL_*: the coordinate along the palette's width */
+  int32_t height;  /* ZS_CHANNEL_*: the coordinate along its height */
+  int32_t width_base, height_base; /* carried; the shader does not read them */
+} zs_palette;
+/* CommandBuffer::palette, command.rs:1442-1485: per pixel of `indices`, the
+ * `palette` texel at the coordinates its channels select (nearest, clamped
+ * to the edge, palette.frag).  The result has the palette's texel and colour
+ * and the indices' size.  ZH_ERR_TYPE for a channel other than R, G, B. */
+zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette* config, uint32_t indices,
+                             uint32_t* reg);
 /* FractalNoise (shaders/fractal_noise.rs ShaderData). */
 typedef struct zs_fractal_noise {
   uint32_t num_octaves;

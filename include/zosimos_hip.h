@@ -180,6 +180,12 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *                        (shaders/srlab2.rs binary_data): encode takes the
  *                        RGB -> XYZ matrix, decode the XYZ -> RGB one
  *                        (srlab2.frag)
+ *  PALETTE               mat4x2 of 8 floats {x0, y0, x1, y1, x2, y2, x3, y3}
+ *                        (shaders/palette.rs binary_data); operands (palette,
+ *                        indices): the palette texel nearest to
+ *                        (sum x_k c_k, sum y_k c_k) + 0.5 / indices' size, c
+ *                        the indices' colour, clamped to the palette's edge
+ *                        (palette.frag)
  *  FRACTAL_NOISE         {grid_scale (2 f32), initial amplitude f32, amplitude
  *                        damping f32, octaves u32}, 20 B padded to 24
  *                        (shaders/fractal_noise.rs binary_data); at most 4096
@@ -201,6 +207,7 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_FRACTAL_NOISE 8
 #define ZH_SHADER_SRLAB2_ENCODE 9
 #define ZH_SHADER_SRLAB2_DECODE 10
+#define ZH_SHADER_PALETTE 11
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

@@ -281,6 +281,25 @@ def fractal_noise(w, h, num_octaves, amplitude, damping, scale=(100.0, 100.0)):
     return np.stack(acc, -1).astype(np.float64)
 
 
+def palette_lookup(palette, values, mu, mv, return_index=False):
+    """``palette.frag`` (``command.rs:1442-1485``): per pixel the texel of
+    ``palette`` (H_p, W_p, ...) nearest to (mu . c + 0.5 / w, mv . c + 0.5 / h),
+    c the (h, w, 4) colour of the indices image, in IEEE f32, clamped to the
+    palette's edge (the sampler is nearest with ClampToEdge,
+    ``encoder.rs:1533-1537``).  Returns the gathered texels (and their
+    (ty, tx) indices with ``return_index``)."""
+    f = np.float32
+    c = np.asarray(values, dtype=f)
+    h, w = c.shape[:2]
+    hp, wp = palette.shape[:2]
+    with np.errstate(invalid="ignore", over="ignore"):  # NaN and infinite colours clamp like any other
+        u = (c @ np.asarray(mu, f)).astype(f) + f(0.5) / f(w)
+        v = (c @ np.asarray(mv, f)).astype(f) + f(0.5) / f(h)
+        tx = np.clip(np.nan_to_num(np.floor(u * f(wp)), nan=0.0), 0, wp - 1).astype(np.int64)
+        ty = np.clip(np.nan_to_num(np.floor(v * f(hp)), nan=0.0), 0, hp - 1).astype(np.int64)
+    return (palette[ty, tx], (ty, tx)) if return_index else palette[ty, tx]
+
+
 def staged_u8_store(values):
     """A non-native 8-bit texel store (``stage.frag`` encode: clamp to [0, 1],
     then the truncating ``uint(c * 255)`` of ``mux_uint``, ``stage.frag:

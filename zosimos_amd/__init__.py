@@ -293,6 +293,11 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
+    def palette(self, palette: int, config: "Palette", indices: int) -> int:
+        """CommandBuffer::palette (command.rs:1442-1485): per pixel of
+        ``indices`` the ``palette`` texel its channels select (nearest)."""
+        return self._reg(self._lib.zs_command_palette, palette, ctypes.byref(config.c()), indices)
+
     def distribution_fractal_noise(self, desc: Descriptor, noise: "FractalNoise") -> int:
         """CommandBuffer::distribution_fractal_noise (command.rs:1589-1607):
         fractal value noise in all four channels."""
@@ -399,6 +404,23 @@ class DistributionNormal2d:
 
     def params(self) -> List[float]:
         return list(self._p)
+
+
+class ColorChannel:
+    """Channels that can index a palette (shaders/palette.rs:30-38)."""
+    NONE, R, G, B, A = -1, 0, 1, 2, 3
+
+
+@dataclass
+class Palette:
+    """command.rs:527-537: which channels give the palette coordinates."""
+    width: int = ColorChannel.NONE
+    height: int = ColorChannel.NONE
+    width_base: int = 0
+    height_base: int = 0
+
+    def c(self) -> "_abi.PaletteC":
+        return _abi.PaletteC(self.width, self.height, self.width_base, self.height_base)
 
 
 class FractalNoise:

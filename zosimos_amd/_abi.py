@@ -32,6 +32,10 @@ class FractalNoiseC(ctypes.Structure):
     _fields_ = [("num_octaves", u32), ("initial_amplitude", f32), ("amplitude_damping", f32), ("grid_scale", f32 * 2)]
 
 
+class PaletteC(ctypes.Structure):
+    _fields_ = [("width", i32), ("height", i32), ("width_base", i32), ("height_base", i32)]
+
+
 class RectC(ctypes.Structure):
     _fields_ = [("x", u32), ("y", u32), ("max_x", u32), ("max_y", u32)]
 
@@ -116,6 +120,7 @@ _SIGNATURES = {
     "zs_fractal_noise_set_damping": (i32, [ctypes.POINTER(FractalNoiseC), ctypes.c_float]),
     "zs_command_distribution_fractal_noise": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(FractalNoiseC),
                                                     ctypes.POINTER(u32)]),
+    "zs_command_palette": (i32, [vp, u32, ctypes.POINTER(PaletteC), u32, ctypes.POINTER(u32)]),
     "zs_command_buffer_init": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_size": (i32, [vp, u32, ctypes.POINTER(ctypes.c_uint64)]),

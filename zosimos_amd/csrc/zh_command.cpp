@@ -18,7 +18,7 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE
 };
 
 struct CmdOp {
@@ -553,6 +553,38 @@ zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor*
   return ZH_OK;
 }
 
+zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette* config, uint32_t indices,
+                             uint32_t* reg) {
+  if (!config || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor pd, id;
+  zh_status st = reg_desc(cmd, palette, &pd);
+  if (st == ZH_OK) st = reg_desc(cmd, indices, &id);
+  if (st != ZH_OK) return st;
+  // ChannelPosition::into_vec4: a one-hot row per axis, or zeros.
+  float x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
+  for (int axis = 0; axis < 2; ++axis) {
+    const int32_t ch = axis == 0 ? config->width : config->height;
+    if (ch == ZS_CHANNEL_NONE) continue;
+    if (ch < ZS_CHANNEL_R || ch > ZS_CHANNEL_B) return fail(ZH_ERR_TYPE, "palette channel must be R, G or B");
+    (axis == 0 ? x : y)[ch] = 1.0f;
+  }
+  CmdOp op;
+  op.kind = K_PALETTE;
+  op.src = palette;
+  op.above = indices;
+  // GenericDescriptor { chroma: palette's, ..indices' } (command.rs:1466-1470).
+  op.desc = pd;
+  op.desc.width = id.width;
+  op.desc.height = id.height;
+  op.desc.row_stride = 0;
+  // binary_data: the mat4x2's columns (x_k, y_k) (shaders/palette.rs:57-66).
+  const float m[8] = {x[0], y[0], x[1], y[1], x[2], y[2], x[3], y[3]};
+  op.blob.resize(32);
+  std::memcpy(op.blob.data(), m, 32);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 zh_status zs_fractal_noise_with_octaves(uint32_t num_octaves, zs_fractal_noise* out) {
   if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   out->num_octaves = num_octaves;
@@ -758,6 +790,9 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_NORMAL2D:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_DISTRIBUTION_NORMAL2D, {}, c.blob, c.knob);
+        break;
+      case K_PALETTE:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_PALETTE, {c.src, c.above}, c.blob, c.knob);
         break;
       case K_NOISE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_FRACTAL_NOISE, {}, c.blob, c.knob);

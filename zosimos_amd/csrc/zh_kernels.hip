@@ -370,6 +370,24 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         }
         break;
       }
+      case OP_PALETTE: {
+        // palette.frag: the texel of the slot nearest to the coordinates the
+        // pixel's colour selects, clamped to the edge (the index is clamped
+        // before the conversion, so any value, NaN included, stays inside).
+        const Slot sl = ldc(&d->slot[in.a]);
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+        const float w = (float)sl.width, h = (float)sl.height;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float4 v = X[i];
+          const float u = q[0] * v.x + q[1] * v.y + q[2] * v.z + q[3] * v.w + q[8];
+          const float t = q[4] * v.x + q[5] * v.y + q[6] * v.z + q[7] * v.w + q[9];
+          const int tx = (int)fminf(fmaxf(floorf(u * w), 0.0f), w - 1.0f);
+          const int ty = (int)fminf(fmaxf(floorf(t * h), 0.0f), h - 1.0f);
+          X[i] = decode_texel<FAST>(sl, fetch_one(sl, tx, ty), lut);
+        }
+        break;
+      }
       case OP_NOISE: {
         // fractal_noise.frag at the pixel centres (2-D stage).
         if (in.b) {

@@ -122,7 +122,7 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // texture (its operand is materialised).  Both are always the root of a stage
 // of their own, which runs on the 2-D interpreter: they need pixel
 // coordinates.
-enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL };
+enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL, N_PALETTE };
 enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL, M_SRLAB, M_SRLAB_INV };
 
 struct Micro {
@@ -140,7 +140,8 @@ struct Node {
   uint8_t gen_op = OP_GRID;
   uint32_t gen_params = 24;
   std::vector<Micro> micro;
-  int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a
+  int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a;
+                       // PALETTE: a palette (sampled), b indices
   uint32_t mode = 0;
   Mat3 inv{};
   double cover[4] = {0, 0, 0, 0};
@@ -316,7 +317,9 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
           if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR &&
               op.shader != ZH_SHADER_DISTRIBUTION_NORMAL2D && op.shader != ZH_SHADER_FRACTAL_NOISE) {
             if (src < 0) return fail(ZH_ERR_INVALID_ARGUMENT, "full-screen shader without operand" + at);
-            const Node& sn = B.nodes[src];
+            // A palette's first operand is sampled anywhere; its indices
+            // (second operand) are read at the pixel.
+            const Node& sn = B.nodes[op.shader == ZH_SHADER_PALETTE && op.num_operands > 1 ? B.cur[op.operands[1]] : src];
             if (sn.w != td.width || sn.h != td.height)
               return fail(ZH_ERR_UNSUPPORTED, "full-screen paint of a differently sized operand (resize)" + at);
           }
@@ -447,6 +450,25 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               n.a = src;
               for (int k = 0; k < 9; ++k) n.grid[k] = (float)m[k];
               n.label = "box3";
+              B.cur[op.dst] = B.add(n);
+              continue;
+            }
+            case ZH_SHADER_PALETTE: {
+              if (op.num_operands < 2) return fail(ZH_ERR_INVALID_ARGUMENT, "palette needs two operands" + at);
+              if (!need_blob(32)) return fail(ZH_ERR_INVALID_ARGUMENT, "palette blob needs 32 bytes" + at);
+              float m[8];
+              std::memcpy(m, blob.data(), 32);
+              n.kind = N_PALETTE;
+              n.a = src;
+              n.b = B.cur[op.operands[1]];
+              for (int k = 0; k < 4; ++k) {
+                n.grid[k] = m[2 * k];          // u row: x_k
+                n.grid[4 + k] = m[2 * k + 1];  // v row: y_k
+              }
+              // palette.frag's bias is half a texel of the indices' image.
+              n.grid[8] = 0.5f / (float)B.nodes[n.b].w;
+              n.grid[9] = 0.5f / (float)B.nodes[n.b].h;
+              n.label = "palette";
               B.cur[op.dst] = B.add(n);
               continue;
             }
@@ -708,20 +730,28 @@ struct Emitter {
         pending_mat_valid = false;
         ins(OP_SOLID, 0, depth > 0 ? 1 : 0, param(n.color, 4));
         depth++;
-        return false;
+        return quantise(n, false, root);
       }
       case N_GRID: {
         pending_mat_valid = false;
         ins(n.gen_op, 0, depth > 0 ? 1 : 0, param(n.grid, n.gen_params));
         depth++;
-        return false;
+        return quantise(n, false, root);
       }
       case N_STENCIL: {
         const int s = slot(phys_ref(n.a), phys_desc(n.a));
         pending_mat_valid = false;
         ins(OP_BOX3, (uint8_t)s, depth > 0 ? 1 : 0, param(n.grid, 9));
         depth++;
-        return false;
+        return quantise(n, false, root);
+      }
+      case N_PALETTE: {
+        // The indices' straight colour replaces itself with the palette's.
+        if (value(n.b, root)) op(OP_UNPREMUL);
+        const int s = slot(phys_ref(n.a), phys_desc(n.a));
+        pending_mat_valid = false;
+        ins(OP_PALETTE, (uint8_t)s, 0, param(n.grid, 10));
+        return quantise(n, false, root);
       }
       case N_UNARY: {
         bool pm = value(n.a, root);
@@ -738,8 +768,7 @@ struct Emitter {
             case M_SRLAB_INV: op(OP_SRLAB_INV); break;
           }
         }
-        quantise(n, pm);
-        return false;
+        return quantise(n, pm, root);
       }
       case N_BLEND: {
         if (!value(n.a, root)) op(OP_PREMUL);
@@ -747,8 +776,7 @@ struct Emitter {
         if (depth > 3) err = "expression deeper than the register stack";
         op(OP_OVER);
         depth--;
-        if (strict) { quantise(n, true); return false; }
-        return true;
+        return quantise(n, true, root);
       }
       case N_PAINT:
         err = "internal: paint node inside a per-pixel expression";
@@ -758,11 +786,15 @@ struct Emitter {
   }
 
   // Round the value through its register's declared texel format (strict mode).
-  void quantise(const Node& n, bool pm) {
-    if (!strict) return;
+  // Skipped at the stage's root, whose stores encode into the declared
+  // formats themselves.  Returns whether X is still premultiplied.
+  bool quantise(const Node& n, bool pm, int root) {
+    const int64_t id = (int64_t)(&n - &B.nodes[0]);
+    if (!strict || id == root) return pm;
     if (pm) op(OP_UNPREMUL);
-    const int s = slot(-1000000 - (int64_t)(&n - &B.nodes[0]), n.desc);
+    const int s = slot(-1000000 - id, n.desc);
     op(OP_QUANT, (uint8_t)s);
+    return false;
   }
 
   void finish() {
@@ -775,7 +807,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "srlab", "srlab_inv", "noise"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "srlab", "srlab_inv", "palette", "noise"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -942,10 +974,11 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     Node& n = B.nodes[id];
     if (!n.out_slots.empty()) mark((int)id);
     if (n.kind == N_GRID || n.kind == N_STENCIL) mark((int)id);
-    if (n.kind == N_STENCIL && B.nodes[n.a].kind != N_INPUT) {
+    if ((n.kind == N_STENCIL || n.kind == N_PALETTE) && B.nodes[n.a].kind != N_INPUT) {
       B.nodes[n.a].sampled = true;
       mark(n.a);
     }
+    if (n.kind == N_PALETTE) mark((int)id);
     if (n.kind == N_BYTES && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.uses > 1 && n.kind != N_BYTES) mark((int)id);
   }
@@ -1019,7 +1052,9 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     bool bytes_child = false;
     for (size_t j = 0; j < B.nodes.size(); ++j)
       if (live[j] && B.nodes[j].kind == N_BYTES && B.nodes[j].a == (int)id) bytes_child = true;
-    if (!bytes_child) {
+    // Strict registers keep their declared texels, so a later stage reads
+    // exactly what the reference's register would hold.
+    if (!bytes_child && !strict) {
       // FP16 where the declared register allows it, FP32 otherwise.
       const bool wide = d.texel.bits == ZH_BITS_FLOAT32X4 || d.texel.bits == ZH_BITS_INT16X4 ||
                         d.texel.bits == ZH_BITS_INT16 || d.texel.bits == 11 || d.texel.bits == 12;
@@ -1143,7 +1178,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       bool flat = true;
       for (uint32_t i = 0; i < sp.desc.n_instr; ++i)
         if (sp.desc.code[i].op == OP_GRID || sp.desc.code[i].op == OP_BOX3 || sp.desc.code[i].op == OP_NORMAL2D ||
-            sp.desc.code[i].op == OP_NOISE)
+            sp.desc.code[i].op == OP_NOISE || sp.desc.code[i].op == OP_PALETTE)
           flat = false;
       for (size_t k = 0; k < sp.slot_ref.size(); ++k) {
         const Slot& s = sp.desc.slot[k];
@@ -1356,7 +1391,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
           case M_SRLAB_INV: E.op(OP_SRLAB_INV); break;
         }
       }
-      E.quantise(u, pm);
+      pm = E.quantise(u, pm, (int)id);
     }
     emit_stores(pm);
     sp.desc.post_end = (uint32_t)E.code.size();

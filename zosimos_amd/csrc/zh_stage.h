@@ -61,6 +61,8 @@ enum Opcode : uint8_t {
   OP_SRLAB,     // X.rgb <- the SrLab2 non-linearity per channel: 24389/2700 x below 216/24389 in
                 //   magnitude, 1.16 x^(1/3) - 0.16 above (srlab2.frag)
   OP_SRLAB_INV, // X.rgb <- its inverse: 2700/24389 x below 0.08 in magnitude, ((x + 0.16) / 1.16)^3 above
+  OP_PALETTE,   // a = slot; arg = params {mu (4), mv (4), bias u, bias v}.  X <- decode(slot texel nearest to
+                //   (mu.X + bias u, mv.X + bias v) in [0, 1]^2, clamped to the edge) (palette.frag)
   OP_NOISE,     // arg = params {scale.x, scale.y, amplitude, damping, octaves}; b = push.  X <- fractal
                 //   (octave-summed) value noise of pcg4d-hashed cell corners at the pixel centre
                 //   (fractal_noise.frag)
