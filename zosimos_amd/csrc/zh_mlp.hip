@@ -232,13 +232,18 @@ __global__ void k_pack(const float* p, uint16_t* ops) {
   ops[e] = (uint16_t)bf16_bits(v);
 }
 
+// Four floats as bfloat16, rounded to nearest even: gfx950 converts a pair
+// per instruction (v_cvt_pk_bf16_f32), the same rounding as bf16_bits.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  bf16x2 r;
+  r.x = (__bf16)a;
+  r.y = (__bf16)b;
+  return __builtin_bit_cast(uint32_t, r);
+}
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-  s4 r;
-  r.x = bf16_bits(a);
-  r.y = bf16_bits(b);
-  r.z = bf16_bits(c);
-  r.w = bf16_bits(d);
-  return r;
+  const uint2 u = make_uint2(pack2(a, b), pack2(c, d));
+  return __builtin_bit_cast(s4, u);
 }
 
 // Activations of accumulator group g (rows 8g + 4(l/32) + 0..3) as the next
