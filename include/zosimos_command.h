@@ -121,6 +121,14 @@ typedef struct zs_palette {
  * and the indices' size.  ZH_ERR_TYPE for a channel other than R, G, B. */
 zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette* config, uint32_t indices,
                              uint32_t* reg);
+/* CommandBuffer::extract, command.rs:1221-1265: one colour channel (R, G or
+ * B; ZH_ERR_OTHER otherwise, and for texels without an 8- or 16-bit
+ * single-channel form) as a single-channel image of the same colour. */
+zh_status zs_command_extract(zs_command* cmd, uint32_t src, int32_t channel, uint32_t* reg);
+/* CommandBuffer::inject, command.rs:1360-1440: `below` with its `channel`
+ * replaced by the single channel of `above`, which must have the texel and
+ * colour `extract` of that channel would give (ZH_ERR_TYPE otherwise). */
+zh_status zs_command_inject(zs_command* cmd, uint32_t below, int32_t channel, uint32_t above, uint32_t* reg);
 /* FractalNoise (shaders/fractal_noise.rs ShaderData). */
 typedef struct zs_fractal_noise {
   uint32_t num_octaves;

@@ -186,6 +186,12 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *                        (sum x_k c_k, sum y_k c_k) + 0.5 / indices' size, c
  *                        the indices' colour, clamped to the palette's edge
  *                        (palette.frag)
+ *  EXTRACT               vec4 {channel position, 0, 0, 0} as u32, 16 B: one
+ *                        colour channel of the operand, decoded as a
+ *                        single-channel texel is (the channel at its
+ *                        position, 0 elsewhere, alpha 1)
+ *  INJECT                2 vec4 {select, color}, 32 B (inject.frag); operands
+ *                        (below, above): mix(below, dot(above, color), select)
  *  FRACTAL_NOISE         {grid_scale (2 f32), initial amplitude f32, amplitude
  *                        damping f32, octaves u32}, 20 B padded to 24
  *                        (shaders/fractal_noise.rs binary_data); at most 4096
@@ -208,6 +214,8 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_SRLAB2_ENCODE 9
 #define ZH_SHADER_SRLAB2_DECODE 10
 #define ZH_SHADER_PALETTE 11
+#define ZH_SHADER_EXTRACT 12
+#define ZH_SHADER_INJECT 13
 #define ZH_SHADER_OKLAB_ADJUST 16
 #define ZH_SHADER_BLEND_OVER 17
 #define ZH_SHADER_AFFINE_BILINEAR 18

@@ -597,6 +597,27 @@ def test_palette_lookup_bit_exact(ctx, pal_desc):
     assert np.array_equal(out[o].reshape(h, w, -1), want)
 
 
+@pytest.mark.parametrize("channel", [z.ColorChannel.R, z.ColorChannel.G, z.ColorChannel.B])
+def test_extract_to_single_channel(ctx, channel):
+    """extract (command.rs:1221-1265) stored as a single-channel sRGB texel:
+    within 1 code of the oracle's stage.frag encode of the decoded channel.
+    The staged 8-bit store truncates OETF(EOTF(k / 255)) * 255, which lies
+    within 1e-5 of k: an ulp of the power function decides whether it lands
+    on k or k - 1, so device and oracle agree on most codes, not all."""
+    w, h = 37, 9
+    d = z.Descriptor.rgba8_srgb(w, h)
+    data = rand_u8((h, w * 4), 40 + channel)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    o, od = cmd.output(cmd.extract(a, channel))
+    out, _ = run(ctx, cmd, {a: d}, {a: data}, {o: od})
+    lin = C.decode_u8(data.reshape(h, w, 4), 5).astype(np.float32)
+    want = S.stage_encode(lin.reshape(-1, 4), 0, 1 + channel, 5).astype(np.int64).reshape(h, w)
+    got = out[o].reshape(h, w).astype(np.int64)
+    report("extract", channel=channel, equal=f"{float(np.mean(got == want)):.3f}")
+    assert np.abs(got - want).max() <= 1
+
+
 def _pitched(data, pitch, rows):
     """Host bytes laid out with `pitch`-byte rows (Descriptor::to_aligned)."""
     buf = np.zeros(pitch * rows, np.uint8)

@@ -23,7 +23,7 @@ that layout and through ``buffer_init``/``from_buffer``) and
 ``from_buffer-with-knob``); ``run_distribution`` and
 ``run_distribution_normal1d`` (``distribution_normal2d``, ``_normal1d``),
 ``run_fractal_noise`` (``distribution_fractal2d``), ``run_srlab2``
-(``srlab2``).
+(``srlab2``), ``run_swap`` (``swapped``).
 ``run_distribution_u8`` is checked against the oracle only
 (``test_gpu_parity.py``): its blessed hash needs a rounding 8-bit store,
 where stage.frag truncates.
@@ -343,6 +343,38 @@ def test_palette_reference_program(ctx):
     dist = min(bin(int(got, 16) ^ int(b, 16)).count("1") for b in HASHES["palette"])
     report("palette_hash", hamming=dist)
     assert dist <= 8
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_swapped_matches_reference_hash(ctx, flags):
+    """blend.rs:423-447 (run_swap): R and G of the background extracted and
+    injected into each other's channel; the result is the background with R
+    and G swapped byte for byte (fused, and with strict registers, where the
+    channels pass through single-channel sRGB texels), and hashes to the
+    blessed line."""
+    bg = rgba("background")
+    h, w = bg.shape[:2]
+    d = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    i = cmd.input(d)
+    r = cmd.extract(i, z.ColorChannel.R)
+    g = cmd.extract(i, z.ColorChannel.G)
+    s = cmd.inject(cmd.inject(i, z.ColorChannel.G, r), z.ColorChannel.R, g)
+    o, od = cmd.output(s)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx, flags)
+    img = ctx.image(d)
+    img.upload(bg.reshape(-1))
+    out_img = ctx.image(od)
+    exe.launch({i: img}, {o: out_img})
+    ctx.sync()
+    out = out_img.download().reshape(h, w, 4)
+    d_ = np.abs(out.astype(int) - bg[..., [1, 0, 2, 3]].astype(int))
+    report("swapped", flags=flags, max_code=int(d_.max()), exact=f"{float(np.mean(d_ == 0)):.5f}")
+    if flags == 0:
+        assert d_.max() == 0
+    else:  # an 8-bit sRGB channel store truncates: OETF(EOTF(k)) may land a hair below k
+        assert d_.max() <= 1
+    check("swapped", out)
 
 
 LUMA8 = z.Descriptor(8, 8, z.Texel(z.SampleBits.UInt8, z.SampleParts.Luma), z.Color.SRGB)

@@ -240,6 +240,26 @@ def test_palette_command():
     assert "load#0 palette store#2" in text or "palette" in text
 
 
+def test_extract_inject_commands():
+    """extract / inject (command.rs:1221-1265, 1360-1440): single-channel
+    texels of the source's bits and colour; alpha and multi-channel images
+    are refused; the swap of run_swap plans as channel and inject micro-ops."""
+    cmd = z.CommandBuffer()
+    i = cmd.input(rgba8(16, 8))
+    r = cmd.extract(i, z.ColorChannel.R)
+    d = cmd.describe(r)
+    assert (d.texel.bits, d.texel.parts, d.width) == (z.SampleBits.UInt8, z.SampleParts.R, 16)
+    expect(z.Status.OTHER, cmd.extract, i, z.ColorChannel.A)
+    expect(z.Status.TYPE, cmd.inject, i, z.ColorChannel.G, i)
+    h = cmd.input(z.Descriptor(16, 8, z.Texel(z.SampleBits.UInt16, z.SampleParts.G), z.Color.SRGB))
+    expect(z.Status.TYPE, cmd.inject, i, z.ColorChannel.G, h)  # 16-bit channel into an 8-bit image
+    g = cmd.extract(i, z.ColorChannel.G)
+    s = cmd.inject(cmd.inject(i, z.ColorChannel.G, r), z.ColorChannel.R, g)
+    cmd.output(s)
+    text = plan(cmd).describe()
+    assert "channel" in text and "inject" in text
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:

@@ -578,3 +578,18 @@ def test_srlab2_lch_plane_reproduces_blessed_hash():
     xyz = C.srlab_nl_inv(xyz_e @ C.HPE.T) @ np.linalg.inv(C.HPE).T
     out[..., :3] = np.clip(xyz @ np.linalg.inv(C.rgb_to_xyz_matrix(3, 5)).T, 0, 1)
     assert B.blockhash256(C.encode_u8(out, 5)) not in hashes["srlab2"]
+
+
+def test_channel_swap_reproduces_blessed_hash():
+    """blend.rs:423-447 (run_swap): extracting R and G of the background and
+    injecting each into the other's channel swaps them; the swapped image
+    hashes to the blessed ``swapped`` line.  The hash sums r + g + b, so it
+    cannot tell the channel order: the swap itself is pinned byte for byte
+    on the device (test_gpu_reference.py)."""
+    import json, os
+    from PIL import Image
+    from oracle import blockhash as B
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    hashes = json.load(open(os.path.join(here, "reference_hashes.json")))
+    bg = np.asarray(Image.open(os.path.join(here, "background.png")).convert("RGBA"))
+    assert B.blockhash256(bg[..., [1, 0, 2, 3]]) in hashes["swapped"]

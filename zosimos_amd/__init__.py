@@ -293,6 +293,16 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
+    def extract(self, src: int, channel: int) -> int:
+        """CommandBuffer::extract (command.rs:1221-1265): one colour channel
+        (``ColorChannel.R``, ``G`` or ``B``) as a single-channel image."""
+        return self._reg(self._lib.zs_command_extract, src, channel)
+
+    def inject(self, below: int, channel: int, above: int) -> int:
+        """CommandBuffer::inject (command.rs:1360-1440): ``below`` with its
+        ``channel`` replaced by the single-channel image ``above``."""
+        return self._reg(self._lib.zs_command_inject, below, channel, above)
+
     def palette(self, palette: int, config: "Palette", indices: int) -> int:
         """CommandBuffer::palette (command.rs:1442-1485): per pixel of
         ``indices`` the ``palette`` texel its channels select (nearest)."""

@@ -18,7 +18,7 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE, K_EXTRACT, K_INJECT
 };
 
 struct CmdOp {
@@ -585,6 +585,83 @@ zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette
   return ZH_OK;
 }
 
+// Texel::channel_texel (image-canvas): the single-channel texel holding one
+// colour channel of `t`: 8- or 16-bit per channel, parts R (1), G (2), B (3).
+static bool channel_texel(const zh_texel& t, int32_t channel, zh_texel* out) {
+  if (channel < ZS_CHANNEL_R || channel > ZS_CHANNEL_B) return false;
+  uint32_t bits;
+  if (t.bits == ZH_BITS_INT8X4 || t.bits == 9 /* UInt8x3 */) bits = ZH_BITS_INT8;
+  else if (t.bits == ZH_BITS_INT16X4 || t.bits == 12 /* UInt16x3 */) bits = ZH_BITS_INT16;
+  else return false;
+  // Colour-ordered parts (Rgb, Bgr, Rgba, Bgra and their x/A variants).
+  if (t.parts < 6 || t.parts > 15) return false;
+  out->bits = bits;
+  out->parts = 1u + (uint32_t)channel;
+  return true;
+}
+
+zh_status zs_command_extract(zs_command* cmd, uint32_t src, int32_t channel, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor d;
+  zh_status st = reg_desc(cmd, src, &d);
+  if (st != ZH_OK) return st;
+  zh_texel t;
+  if (!channel_texel(d.texel, channel, &t)) return fail(ZH_ERR_OTHER, "no single-channel texel for that channel");
+  CmdOp op;
+  op.kind = K_EXTRACT;
+  op.src = src;
+  op.desc = d;
+  op.desc.texel = t;
+  op.desc.row_stride = 0;
+  op.desc = normalized(op.desc);
+  // The channel's position in the decoded colour (ChannelPosition).
+  op.blob.assign(16, 0);
+  const uint32_t pos = (uint32_t)channel;
+  std::memcpy(op.blob.data(), &pos, 4);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_inject(zs_command* cmd, uint32_t below, int32_t channel, uint32_t above, uint32_t* reg) {
+  if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor b, a;
+  zh_status st = reg_desc(cmd, below, &b);
+  if (st == ZH_OK) st = reg_desc(cmd, above, &a);
+  if (st != ZH_OK) return st;
+  zh_texel want;
+  if (!channel_texel(b.texel, channel, &want)) return fail(ZH_ERR_OTHER, "no single-channel texel for that channel");
+  // One component above (A, R, G, B or Luma), with the expected bits and the
+  // same colour; its own parts say where its value sits (channel_weight_vec4).
+  float color[4] = {0, 0, 0, 0};
+  switch (a.texel.parts) {
+    case 0: color[3] = 1.0f; break;                                     // A
+    case 1: case 4: color[0] = 1.0f; break;                             // R, Luma
+    case 2: color[1] = 1.0f; break;                                     // G
+    case 3: color[2] = 1.0f; break;                                     // B
+    default: return fail(ZH_ERR_TYPE, "inject from a texel with more than one channel");
+  }
+  zh_descriptor cmp = a;
+  cmp.texel.parts = want.parts;
+  if (cmp.texel.bits != want.bits || cmp.texel.parts != want.parts || !chroma_equal(cmp, [&] {
+        zh_descriptor e = b;
+        e.texel = want;
+        return e;
+      }()) || a.width != b.width || a.height != b.height)
+    return fail(ZH_ERR_TYPE, "inject of a mismatched channel image");
+  float select[4] = {0, 0, 0, 0};
+  select[channel] = 1.0f;
+  CmdOp op;
+  op.kind = K_INJECT;
+  op.src = below;
+  op.above = above;
+  op.desc = b;
+  op.blob.resize(32);
+  std::memcpy(op.blob.data(), select, 16);
+  std::memcpy(op.blob.data() + 16, color, 16);
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
 zh_status zs_fractal_noise_with_octaves(uint32_t num_octaves, zs_fractal_noise* out) {
   if (!out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   out->num_octaves = num_octaves;
@@ -793,6 +870,12 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         break;
       case K_PALETTE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_PALETTE, {c.src, c.above}, c.blob, c.knob);
+        break;
+      case K_EXTRACT:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_EXTRACT, {c.src}, c.blob, c.knob);
+        break;
+      case K_INJECT:
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_INJECT, {c.src, c.above}, c.blob, c.knob);
         break;
       case K_NOISE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_FRACTAL_NOISE, {}, c.blob, c.knob);

@@ -487,6 +487,27 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
 #pragma unroll
         for (int i = 0; i < kPix; ++i) { const float4 t = X[i]; X[i] = Y[i]; Y[i] = t; }
         break;
+      case OP_CHANNEL: {
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i)
+          X[i] = make_float4(fmaf(X[i].x, q[0], q[4]), fmaf(X[i].y, q[1], q[5]), fmaf(X[i].z, q[2], q[6]),
+                             fmaf(X[i].w, q[3], q[7]));
+        break;
+      }
+      case OP_INJECT: {
+        // inject.frag: mix(below, vec4(dot(above, color)), select) with
+        // below in Y and above in X; GLSL mix as a (1 - t) + b t.
+        const __attribute__((address_space(4))) float* q = d->params + in.arg;
+#pragma unroll
+        for (int i = 0; i < kPix; ++i) {
+          const float v = q[4] * X[i].x + q[5] * X[i].y + q[6] * X[i].z + q[7] * X[i].w;
+          X[i] = make_float4(Y[i].x * (1.0f - q[0]) + v * q[0], Y[i].y * (1.0f - q[1]) + v * q[1],
+                             Y[i].z * (1.0f - q[2]) + v * q[2], Y[i].w * (1.0f - q[3]) + v * q[3]);
+          Y[i] = Z[i];
+        }
+        break;
+      }
       case OP_EOTF:
         if constexpr (!FAST) {
 #pragma unroll

@@ -122,12 +122,12 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // texture (its operand is materialised).  Both are always the root of a stage
 // of their own, which runs on the 2-D interpreter: they need pixel
 // coordinates.
-enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL, N_PALETTE };
-enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL, M_SRLAB, M_SRLAB_INV };
+enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL, N_PALETTE, N_INJECT };
+enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL, M_SRLAB, M_SRLAB_INV, M_CHANNEL };
 
 struct Micro {
   MicroKind k;
-  Mat3 m;
+  Mat3 m;  // M_CHANNEL: m[0..3] the per-component factor, m[4..7] the addend
 };
 
 struct Node {
@@ -141,7 +141,7 @@ struct Node {
   uint32_t gen_params = 24;
   std::vector<Micro> micro;
   int a = -1, b = -1;  // UNARY: a; BLEND: a below, b above; PAINT: a base, b source; BYTES: a;
-                       // PALETTE: a palette (sampled), b indices
+                       // PALETTE: a palette (sampled), b indices; INJECT: a below, b above
   uint32_t mode = 0;
   Mat3 inv{};
   double cover[4] = {0, 0, 0, 0};
@@ -472,6 +472,35 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
               B.cur[op.dst] = B.add(n);
               continue;
             }
+            case ZH_SHADER_EXTRACT: {
+              if (!need_blob(4)) return fail(ZH_ERR_INVALID_ARGUMENT, "extract blob needs 4 bytes" + at);
+              uint32_t pos;
+              std::memcpy(&pos, blob.data(), 4);
+              if (pos > 2) return fail(ZH_ERR_INVALID_ARGUMENT, "extract of a channel other than R, G, B" + at);
+              // The single-channel texel decodes to its channel at its
+              // position, 0 elsewhere and alpha 1 (stage.frag parts_normalize).
+              Mat3 m{};
+              m[pos] = 1.0;
+              m[7] = 1.0;
+              n.kind = N_UNARY;
+              n.micro = {{M_CHANNEL, m}};
+              n.label = "extract";
+              break;
+            }
+            case ZH_SHADER_INJECT: {
+              if (op.num_operands < 2) return fail(ZH_ERR_INVALID_ARGUMENT, "inject needs two operands" + at);
+              if (!need_blob(32)) return fail(ZH_ERR_INVALID_ARGUMENT, "inject blob needs 32 bytes" + at);
+              const Node& above = B.nodes[B.cur[op.operands[1]]];
+              if (above.w != td.width || above.h != td.height)
+                return fail(ZH_ERR_UNSUPPORTED, "inject of a differently sized image" + at);
+              n.kind = N_INJECT;
+              n.a = src;
+              n.b = B.cur[op.operands[1]];
+              std::memcpy(n.grid, blob.data(), 32);
+              n.label = "inject";
+              B.cur[op.dst] = B.add(n);
+              continue;
+            }
             case ZH_SHADER_PREMULTIPLY:
               n.kind = N_UNARY;
               n.micro = {{M_PREMUL, {}}};
@@ -677,6 +706,14 @@ struct Emitter {
     ins(o, a, b, arg);
   }
 
+  // X <- X * mul + add per component (M_CHANNEL).
+  void channel(const Mat3& m) {
+    float f[8];
+    for (int k = 0; k < 8; ++k) f[k] = (float)m[k];
+    pending_mat_valid = false;
+    ins(OP_CHANNEL, 0, 0, param(f, 8));
+  }
+
   bool leaf_physical(int id) const {
     const Node& n = B.nodes[id];
     return n.kind == N_INPUT || n.mat;
@@ -766,9 +803,19 @@ struct Emitter {
             case M_UNPREMUL: op(OP_UNPREMUL); break;
             case M_SRLAB: op(OP_SRLAB); break;
             case M_SRLAB_INV: op(OP_SRLAB_INV); break;
+            case M_CHANNEL: channel(m.m); break;
           }
         }
         return quantise(n, pm, root);
+      }
+      case N_INJECT: {
+        if (value(n.a, root)) op(OP_UNPREMUL);
+        if (value(n.b, root)) op(OP_UNPREMUL);
+        if (depth > 3) err = "expression deeper than the register stack";
+        pending_mat_valid = false;
+        ins(OP_INJECT, 0, 0, param(n.grid, 8));
+        depth--;
+        return quantise(n, false, root);
       }
       case N_BLEND: {
         if (!value(n.a, root)) op(OP_PREMUL);
@@ -807,7 +854,7 @@ struct Emitter {
 
 static const char* op_name(uint8_t o) {
   static const char* n[] = {"nop",     "load", "solid", "mat3", "cbrt",  "cube",  "clamp01", "premul", "unpremul",
-                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "srlab", "srlab_inv", "palette", "noise"};
+                            "over",    "swap", "eotf",  "oetf", "quant", "store", "offset",  "grid",    "box3", "normal2d", "srlab", "srlab_inv", "palette", "channel", "inject", "noise"};
   return o < sizeof(n) / sizeof(n[0]) ? n[o] : "?";
 }
 
@@ -1389,6 +1436,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
           case M_UNPREMUL: E.op(OP_UNPREMUL); break;
           case M_SRLAB: E.op(OP_SRLAB); break;
           case M_SRLAB_INV: E.op(OP_SRLAB_INV); break;
+          case M_CHANNEL: E.channel(m.m); break;
         }
       }
       pm = E.quantise(u, pm, (int)id);

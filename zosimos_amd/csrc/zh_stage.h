@@ -63,6 +63,9 @@ enum Opcode : uint8_t {
   OP_SRLAB_INV, // X.rgb <- its inverse: 2700/24389 x below 0.08 in magnitude, ((x + 0.16) / 1.16)^3 above
   OP_PALETTE,   // a = slot; arg = params {mu (4), mv (4), bias u, bias v}.  X <- decode(slot texel nearest to
                 //   (mu.X + bias u, mv.X + bias v) in [0, 1]^2, clamped to the edge) (palette.frag)
+  OP_CHANNEL,   // arg = params {mul (4), add (4)}.  X <- X * mul + add per component (alpha included)
+  OP_INJECT,    // arg = params {select (4), color (4)}.  X <- mix(Y, dot(X, color), select); pop (Y <- Z)
+                //   (inject.frag: Y the image injected into, X the injected one)
   OP_NOISE,     // arg = params {scale.x, scale.y, amplitude, damping, octaves}; b = push.  X <- fractal
                 //   (octave-summed) value noise of pcg4d-hashed cell corners at the pixel centre
                 //   (fractal_noise.frag)
