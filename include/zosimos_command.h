@@ -121,6 +121,10 @@ typedef struct zs_palette {
  * and the indices' size.  ZH_ERR_TYPE for a channel other than R, G, B. */
 zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette* config, uint32_t indices,
                              uint32_t* reg);
+/* CommandBuffer::resize, command.rs:1675-1705: `below` sampled onto a
+ * width x height grid (a bilinear coordinate ramp in RGBA8 with a Scalars
+ * colour, then a palette lookup of R across and G down). */
+zh_status zs_command_resize(zs_command* cmd, uint32_t below, uint32_t width, uint32_t height, uint32_t* reg);
 /* CommandBuffer::extract, command.rs:1221-1265: one colour channel (R, G or
  * B; ZH_ERR_OTHER otherwise, and for texels without an 8- or 16-bit
  * single-channel form) as a single-channel image of the same colour. */

@@ -597,6 +597,31 @@ def test_palette_lookup_bit_exact(ctx, pal_desc):
     assert np.array_equal(out[o].reshape(h, w, -1), want)
 
 
+@pytest.mark.parametrize("out_wh", [(83, 41), (20, 11), (50, 30)])
+def test_resize_nearest(ctx, out_wh):
+    """CommandBuffer::resize (command.rs:1675-1705): the coordinate ramp's
+    pixel-centre values index the source as a palette, half a ramp texel
+    added (palette.frag's bias), so output pixel x reads source texel
+    floor((x + 1) * W_in / W_out) clamped: equal to the oracle byte for
+    byte (the ramp's FP32 values, as the fused program keeps them)."""
+    w, h = 50, 30
+    ow, oh = out_wh
+    d = z.Descriptor.rgba8_srgb(w, h)
+    data = rand_u8((h, w * 4), 50)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    o, od = cmd.output(cmd.resize(a, ow, oh))
+    assert (od.width, od.height) == (ow, oh)
+    out, _ = run(ctx, cmd, {a: d}, {a: data}, {o: od})
+    f = np.float32
+    ramp = np.zeros((oh, ow, 4), f)
+    ramp[..., 0] = ((np.arange(ow, dtype=f) + f(0.5)) / f(ow))[None, :]
+    ramp[..., 1] = ((np.arange(oh, dtype=f) + f(0.5)) / f(oh))[:, None]
+    ramp[..., 3] = 1.0
+    want = PL.palette_lookup(data.reshape(h, w, 4), ramp, [1, 0, 0, 0], [0, 1, 0, 0])
+    assert np.array_equal(out[o].reshape(oh, ow, 4), want)
+
+
 @pytest.mark.parametrize("channel", [z.ColorChannel.R, z.ColorChannel.G, z.ColorChannel.B])
 def test_extract_to_single_channel(ctx, channel):
     """extract (command.rs:1221-1265) stored as a single-channel sRGB texel:

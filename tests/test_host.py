@@ -238,6 +238,13 @@ def test_palette_command():
     cmd.output(r)
     text = plan(cmd).describe()
     assert "load#0 palette store#2" in text or "palette" in text
+    # resize: a coordinate ramp of the target size, then the palette lookup
+    cmd = z.CommandBuffer()
+    rs = cmd.resize(cmd.input(rgba8(30, 20)), 61, 7)
+    assert (cmd.describe(rs).width, cmd.describe(rs).height) == (61, 7)
+    cmd.output(rs)
+    text = plan(cmd).describe()
+    assert "grid" in text and "palette" in text
 
 
 def test_extract_inject_commands():

@@ -293,6 +293,11 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
+    def resize(self, below: int, width: int, height: int) -> int:
+        """CommandBuffer::resize (command.rs:1675-1705): a bilinear coordinate
+        ramp of the target size looked up in ``below`` as a palette."""
+        return self._reg(self._lib.zs_command_resize, below, width, height)
+
     def extract(self, src: int, channel: int) -> int:
         """CommandBuffer::extract (command.rs:1221-1265): one colour channel
         (``ColorChannel.R``, ``G`` or ``B``) as a single-channel image."""

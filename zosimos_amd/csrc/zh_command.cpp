@@ -585,6 +585,30 @@ zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette
   return ZH_OK;
 }
 
+zh_status zs_command_resize(zs_command* cmd, uint32_t below, uint32_t width, uint32_t height, uint32_t* reg) {
+  if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (width == 0 || height == 0) return fail(ZH_ERR_OTHER, "empty resize target");
+  // Descriptor::with_texel(Texel::new_u8(RgbA)): RGBA8, Scalars, linear.
+  zh_descriptor grid;
+  std::memset(&grid, 0, sizeof(grid));
+  grid.width = width;
+  grid.height = height;
+  grid.texel.bits = ZH_BITS_INT8X4;
+  grid.texel.parts = ZH_PARTS_RGBA;
+  grid.color.model = ZH_COLOR_SCALARS;
+  grid.color.transfer = ZH_TRANSFER_LINEAR;
+  // u_max = (1, 0, 0, 1), v_max = (0, 1, 0, 1), every alpha 1.
+  float p[24] = {0};
+  for (int k = 0; k < 6; ++k) p[4 * k + 3] = 1.0f;
+  p[4 * 1 + 0] = 1.0f;
+  p[4 * 3 + 1] = 1.0f;
+  uint32_t g;
+  zh_status st = zs_command_bilinear(cmd, &grid, p, &g);
+  if (st != ZH_OK) return st;
+  const zs_palette cfg = {ZS_CHANNEL_R, ZS_CHANNEL_G, 0, 0};
+  return zs_command_palette(cmd, below, &cfg, g, reg);
+}
+
 // Texel::channel_texel (image-canvas): the single-channel texel holding one
 // colour channel of `t`: 8- or 16-bit per channel, parts R (1), G (2), B (3).
 static bool channel_texel(const zh_texel& t, int32_t channel, zh_texel* out) {

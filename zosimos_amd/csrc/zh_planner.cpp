@@ -1020,7 +1020,16 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     if (!live[id]) continue;
     Node& n = B.nodes[id];
     if (!n.out_slots.empty()) mark((int)id);
-    if (n.kind == N_GRID || n.kind == N_STENCIL) mark((int)id);
+    // Generators run on the 2-D interpreter, so they get a stage of their
+    // own; a coordinate grid read only by a palette lookup (itself 2-D) is
+    // evaluated inside the lookup's stage instead, at full precision.
+    bool palette_only = n.kind == N_GRID && n.uses == 1 && n.out_slots.empty();
+    if (palette_only) {
+      palette_only = false;
+      for (size_t j = 0; j < B.nodes.size(); ++j)
+        if (live[j] && B.nodes[j].kind == N_PALETTE && B.nodes[j].b == (int)id) palette_only = true;
+    }
+    if ((n.kind == N_GRID && !palette_only) || n.kind == N_STENCIL) mark((int)id);
     if ((n.kind == N_STENCIL || n.kind == N_PALETTE) && B.nodes[n.a].kind != N_INPUT) {
       B.nodes[n.a].sampled = true;
       mark(n.a);
