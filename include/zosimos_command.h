@@ -164,6 +164,15 @@ zh_status zs_command_buffer_size(zs_command* cmd, uint32_t buffer, uint64_t* len
  * ZH_ERR_INVALID_ARGUMENT (CommandError::INVALID_CALL) when the buffer is
  * shorter than height padded rows.  The register's own descriptor is tight. */
 zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_descriptor* desc, uint32_t* reg);
+/* CommandBuffer::function + invoke, command.rs:856-920, folded into the
+ * caller (the reference links the function as its own High function,
+ * command.rs:2083-2150; inlining puts its stages into the caller's fused
+ * plan and graph).  The inputs of `body`, in declaration order, take the
+ * registers `args`, whose descriptors must equal them (ZH_ERR_TYPE
+ * otherwise); its outputs, in order, come back in `results`.  Knobs inside
+ * `body` are not carried over.  Generic inputs are not supported. */
+zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint32_t* args, uint32_t num_args,
+                            uint32_t* results, uint32_t num_results);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
  * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob.  A
  * buffer register's knob takes the buffer's bytes, of the same length

@@ -597,6 +597,34 @@ def test_palette_lookup_bit_exact(ctx, pal_desc):
     assert np.array_equal(out[o].reshape(h, w, -1), want)
 
 
+def test_invoke_matches_inline_program(ctx):
+    """A function (colour conversion, affine resample, over) invoked twice
+    gives the bytes of the same ops written inline."""
+    d = z.Descriptor.rgba8_srgb(48, 32)
+    m = tuple(z.workloads.about_centre(48, 32, 0.9, 0.3).transformation)
+
+    def body_ops(cmd, a, b):
+        c = cmd.color_convert(b, z.Color(primaries=z.Primaries.DisplayP3), RGBA8)
+        c = cmd.color_convert(c, z.Color.SRGB, RGBA8)
+        return cmd.affine(a, z.Affine(m, z.AffineSample.BiLinear), c)
+
+    body = z.CommandBuffer()
+    ba, bb = body.input(d), body.input(d)
+    body.output(body_ops(body, ba, bb))
+    cmd = z.CommandBuffer()
+    x, y = cmd.input(d), cmd.input(d)
+    (r,) = cmd.invoke(body, [x, y])
+    (r,) = cmd.invoke(body, [r, y])
+    o, od = cmd.output(r)
+    ref = z.CommandBuffer()
+    x2, y2 = ref.input(d), ref.input(d)
+    o2, od2 = ref.output(body_ops(ref, body_ops(ref, x2, y2), y2))
+    data = {k: rand_u8((32, 192), 60 + k) for k in range(2)}
+    got, _ = run(ctx, cmd, {x: d, y: d}, {x: data[0], y: data[1]}, {o: od})
+    want, _ = run(ctx, ref, {x2: d, y2: d}, {x2: data[0], y2: data[1]}, {o2: od2})
+    assert np.array_equal(got[o], want[o2])
+
+
 @pytest.mark.parametrize("out_wh", [(83, 41), (20, 11), (50, 30)])
 def test_resize_nearest(ctx, out_wh):
     """CommandBuffer::resize (command.rs:1675-1705): the coordinate ramp's

@@ -267,6 +267,34 @@ def test_extract_inject_commands():
     assert "channel" in text and "inject" in text
 
 
+def _over_p3(cmd, a, b):
+    """A small function body: b converted to Display-P3 and blended over a."""
+    p3 = cmd.color_convert(b, z.Color(primaries=z.Primaries.DisplayP3), z.Texel(10, 8))
+    back = cmd.color_convert(p3, z.Color.SRGB, z.Texel(10, 8))
+    return cmd.blend(a, z.Rectangle(0, 0, 16, 8), back)
+
+
+def test_invoke_inlines_the_function():
+    """CommandBuffer::function + invoke (command.rs:856-920) folded into the
+    caller: two calls plan exactly as the same ops written inline; argument
+    count and descriptors are checked."""
+    body = z.CommandBuffer()
+    ba, bb = body.input(rgba8(16, 8)), body.input(rgba8(16, 8))
+    body.output(_over_p3(body, ba, bb))
+    cmd = z.CommandBuffer()
+    x, y = cmd.input(rgba8(16, 8)), cmd.input(rgba8(16, 8))
+    (r1,) = cmd.invoke(body, [x, y])
+    (r2,) = cmd.invoke(body, [r1, y])
+    cmd.output(r2)
+    ref = z.CommandBuffer()
+    x2, y2 = ref.input(rgba8(16, 8)), ref.input(rgba8(16, 8))
+    ref.output(_over_p3(ref, _over_p3(ref, x2, y2), y2))
+    assert plan(cmd).describe() == plan(ref).describe()
+    expect(z.Status.TYPE, cmd.invoke, body, [x])
+    expect(z.Status.TYPE, cmd.invoke, body, [x, cmd.input(rgba8(8, 8))])
+    expect(z.Status.TYPE, cmd.invoke, body, [x, y], 2)
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:

@@ -293,6 +293,15 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
+    def invoke(self, body: "CommandBuffer", args: Sequence[int], num_results: int = 1) -> List[int]:
+        """CommandBuffer::function + invoke (command.rs:856-920), folded
+        into this buffer: ``body``'s inputs take ``args`` in order; returns
+        the registers of its outputs, in order."""
+        a = (ctypes.c_uint32 * max(1, len(args)))(*args)
+        out = (ctypes.c_uint32 * max(1, num_results))()
+        _check(self._lib.zs_command_invoke(self._h, body._h, a, len(args), out, num_results))
+        return list(out)[:num_results]
+
     def resize(self, below: int, width: int, height: int) -> int:
         """CommandBuffer::resize (command.rs:1675-1705): a bilinear coordinate
         ramp of the target size looked up in ``below`` as a palette."""

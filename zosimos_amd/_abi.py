@@ -123,6 +123,7 @@ _SIGNATURES = {
     "zs_command_palette": (i32, [vp, u32, ctypes.POINTER(PaletteC), u32, ctypes.POINTER(u32)]),
     "zs_command_extract": (i32, [vp, u32, i32, ctypes.POINTER(u32)]),
     "zs_command_resize": (i32, [vp, u32, u32, u32, ctypes.POINTER(u32)]),
+    "zs_command_invoke": (i32, [vp, vp, ctypes.POINTER(u32), u32, ctypes.POINTER(u32), u32]),
     "zs_command_inject": (i32, [vp, u32, i32, u32, ctypes.POINTER(u32)]),
     "zs_command_buffer_init": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),

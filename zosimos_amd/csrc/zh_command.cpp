@@ -787,6 +787,56 @@ zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_desc
   return ZH_OK;
 }
 
+static bool same_descriptor(const zh_descriptor& a, const zh_descriptor& b) {
+  return a.width == b.width && a.height == b.height && a.texel.bits == b.texel.bits &&
+         a.texel.parts == b.texel.parts && a.color.model == b.color.model && a.color.transfer == b.color.transfer &&
+         a.color.primaries == b.color.primaries && a.color.whitepoint == b.color.whitepoint;
+}
+
+zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint32_t* args, uint32_t num_args,
+                            uint32_t* results, uint32_t num_results) {
+  if (!cmd || !body || (num_args && !args) || (num_results && !results) || cmd == body)
+    return fail(ZH_ERR_INVALID_ARGUMENT, "null or self-referencing argument");
+  uint32_t inputs = 0, outputs = 0;
+  for (const CmdOp& op : body->ops) {
+    inputs += op.kind == K_INPUT;
+    outputs += op.kind == K_OUTPUT;
+  }
+  if (inputs != num_args) return fail(ZH_ERR_TYPE, "function takes " + std::to_string(inputs) + " arguments");
+  if (outputs != num_results) return fail(ZH_ERR_TYPE, "function returns " + std::to_string(outputs) + " results");
+  // Check every argument before touching the caller.
+  uint32_t k = 0;
+  for (const CmdOp& op : body->ops) {
+    if (op.kind != K_INPUT) continue;
+    zh_descriptor d;
+    zh_status st = reg_desc(cmd, args[k], &d);
+    if (st != ZH_OK) return st;
+    if (!same_descriptor(d, op.desc))
+      return fail(ZH_ERR_TYPE, "argument " + std::to_string(k) + " does not match the function's input");
+    ++k;
+  }
+  // Copy the body with its registers renumbered into the caller's.
+  std::vector<uint32_t> remap(body->ops.size(), 0);
+  uint32_t a = 0, r = 0;
+  for (size_t i = 0; i < body->ops.size(); ++i) {
+    const CmdOp& op = body->ops[i];
+    if (op.kind == K_INPUT) {
+      remap[i] = args[a++];
+      continue;
+    }
+    if (op.kind == K_OUTPUT) {
+      results[r++] = remap[op.src];
+      continue;
+    }
+    CmdOp c = op;
+    c.src = remap[op.src];
+    c.above = remap[op.above];
+    c.knob = -1;
+    remap[i] = push(cmd, c);
+  }
+  return ZH_OK;
+}
+
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob) {
   if (!cmd || !knob || reg >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
   CmdOp& op = cmd->ops[reg];
