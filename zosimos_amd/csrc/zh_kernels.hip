@@ -441,8 +441,15 @@ __device__ __forceinline__ void run(CDesc d, uint32_t begin, uint32_t end, Ctx& 
         break;
       }
       case OP_CBRT:
+        if (in.a) {  // input proved non-negative
 #pragma unroll
-        for (int i = 0; i < kPix; ++i) X[i] = make_float4(cbrt_s(X[i].x), cbrt_s(X[i].y), cbrt_s(X[i].z), X[i].w);
+          for (int i = 0; i < kPix; ++i)
+            X[i] = make_float4(pow_pos(X[i].x, 1.0f / 3.0f), pow_pos(X[i].y, 1.0f / 3.0f), pow_pos(X[i].z, 1.0f / 3.0f),
+                               X[i].w);
+        } else {
+#pragma unroll
+          for (int i = 0; i < kPix; ++i) X[i] = make_float4(cbrt_s(X[i].x), cbrt_s(X[i].y), cbrt_s(X[i].z), X[i].w);
+        }
         break;
       case OP_CUBE:
 #pragma unroll

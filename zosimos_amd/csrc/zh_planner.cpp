@@ -706,6 +706,19 @@ struct Emitter {
     ins(o, a, b, arg);
   }
 
+  // Whether X's colour is known non-negative: the program so far ends with
+  // a matrix of non-negative entries applied to a load of an 8-bit unorm
+  // texel (whose decoded colour lies in [0, 1]).
+  bool rgb_nonneg() const {
+    const size_t n = code.size();
+    if (n < 2 || code[n - 1].op != OP_MAT3 || code[n - 2].op != OP_LOAD) return false;
+    for (int k = 0; k < 9; ++k)
+      if (!(params[code[n - 1].arg + k] >= 0.0f)) return false;
+    const uint32_t f = sp.desc.slot[code[n - 2].a].fmt;
+    return f == FMT_U8_SRGB_RGBA || f == FMT_U8_SRGB_BGRA || f == FMT_U8_LIN_RGBA || f == FMT_U8_LIN_BGRA;
+  }
+  void cbrt() { op(OP_CBRT, rgb_nonneg() ? 1 : 0); }
+
   // X <- X * mul + add per component (M_CHANNEL).
   void channel(const Mat3& m) {
     float f[8];
@@ -796,7 +809,7 @@ struct Emitter {
           if (pm) { op(OP_UNPREMUL); pm = false; }
           switch (m.k) {
             case M_MAT3: mat(m.m); break;
-            case M_CBRT: op(OP_CBRT); break;
+            case M_CBRT: cbrt(); break;
             case M_CUBE: op(OP_CUBE); break;
             case M_CLAMP: op(OP_CLAMP01); break;
             case M_PREMUL: op(OP_PREMUL); break;
@@ -1257,7 +1270,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         for (uint32_t i = 0; i < sp.desc.n_instr; ++i) {
           const Instr& in = sp.desc.code[i];
           const bool io = in.op == OP_LOAD || in.op == OP_STORE;
-          sig[i] = sg(in.op, io ? sp.desc.slot[in.a].fmt : 0u, in.op == OP_LOAD && in.b ? 1u : 0u);
+          sig[i] = sg(in.op, io ? sp.desc.slot[in.a].fmt : in.op == OP_CBRT ? in.a : 0u,
+                      in.op == OP_LOAD && in.b ? 1u : 0u);
         }
         sp.spec = zh_spec_lookup(sig, (int)sp.desc.n_instr);
       }
@@ -1438,7 +1452,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         if (pm) { E.op(OP_UNPREMUL); pm = false; }
         switch (m.k) {
           case M_MAT3: E.mat(m.m); break;
-          case M_CBRT: E.op(OP_CBRT); break;
+          case M_CBRT: E.cbrt(); break;
           case M_CUBE: E.op(OP_CUBE); break;
           case M_CLAMP: E.op(OP_CLAMP01); break;
           case M_PREMUL: E.op(OP_PREMUL); break;

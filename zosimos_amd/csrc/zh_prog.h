@@ -230,6 +230,11 @@ __device__ __forceinline__ void exec(const Uni<P>& u, uint64_t x0, int n, const 
         const float b = m[6] * X[i].x + m[7] * X[i].y + m[8] * X[i].z;
         X[i] = make_float4(r, g, b, X[i].w);
       }
+    } else if constexpr (op == OP_CBRT && el_fmt(e) == 1) {  // input proved non-negative
+      constexpr float t = 1.0f / 3.0f;
+#pragma unroll
+      for (int i = 0; i < kPix; ++i)
+        X[i] = make_float4(pow_pos(X[i].x, t), pow_pos(X[i].y, t), pow_pos(X[i].z, t), X[i].w);
     } else if constexpr (op == OP_CBRT) {
 #pragma unroll
       for (int i = 0; i < kPix; ++i) X[i] = make_float4(cbrt_s(X[i].x), cbrt_s(X[i].y), cbrt_s(X[i].z), X[i].w);

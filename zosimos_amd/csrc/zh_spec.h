@@ -24,6 +24,7 @@ constexpr uint16_t sg(uint32_t op, uint32_t fmt = 0, uint32_t push = 0) {
 #define ZH_ST(f) sg(OP_STORE, f)
 #define ZH_M sg(OP_MAT3)
 #define ZH_CB sg(OP_CBRT)
+#define ZH_CBP sg(OP_CBRT, 1)  // cube root of a value the planner proved non-negative
 #define ZH_CU sg(OP_CUBE)
 #define ZH_CL sg(OP_CLAMP01)
 #define ZH_PM sg(OP_PREMUL)
@@ -40,12 +41,16 @@ constexpr uint16_t sg(uint32_t op, uint32_t fmt = 0, uint32_t push = 0) {
   /* Rec.709 -> P3 -> Oklab -> chroma -> back -> 'over' an f16 layer -> RGBA8 sRGB */                   \
   X(c2_over, ZH_LD(ZH_S8), ZH_M, ZH_CB, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_PM, ZH_LDP(ZH_H), ZH_PM, ZH_OV, ZH_UP, \
     ZH_ST(ZH_S8))                                                                                         \
+  X(c2_over_p, ZH_LD(ZH_S8), ZH_M, ZH_CBP, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_PM, ZH_LDP(ZH_H), ZH_PM, ZH_OV,      \
+    ZH_UP, ZH_ST(ZH_S8))                                                                                  \
   /* Oklab round trip with chroma / hue (the learned-chain workload's plain chain) */                    \
   X(oklab_rt_s8, ZH_LD(ZH_S8), ZH_M, ZH_CB, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_S8))                       \
+  X(oklab_rt_s8p, ZH_LD(ZH_S8), ZH_M, ZH_CBP, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_S8))                     \
   X(oklab_rt_h, ZH_LD(ZH_H), ZH_M, ZH_CB, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_H))                          \
   X(oklab_rt_f, ZH_LD(ZH_F), ZH_M, ZH_CB, ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_F))                          \
   /* to / from Oklab */                                                                                  \
   X(oklab_enc_s8h, ZH_LD(ZH_S8), ZH_M, ZH_CB, ZH_M, ZH_ST(ZH_H))                                          \
+  X(oklab_enc_s8hp, ZH_LD(ZH_S8), ZH_M, ZH_CBP, ZH_M, ZH_ST(ZH_H))                                        \
   X(oklab_dec_hs8, ZH_LD(ZH_H), ZH_M, ZH_CU, ZH_M, ZH_CL, ZH_ST(ZH_S8))                                   \
   /* colour matrices */                                                                                  \
   X(mat_s8s8, ZH_LD(ZH_S8), ZH_M, ZH_ST(ZH_S8))                                                           \

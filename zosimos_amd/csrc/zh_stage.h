@@ -37,7 +37,7 @@ enum Opcode : uint8_t {
   OP_LOAD,      // a = slot; b = push.  X <- decode(slot at the pixel)
   OP_SOLID,     // arg = param index of 4 floats; b = push
   OP_MAT3,      // arg = param index of 9 floats (row-major); X.rgb <- M X.rgb
-  OP_CBRT,      // X.rgb <- sign(x) |x|^(1/3)
+  OP_CBRT,      // X.rgb <- sign(x) |x|^(1/3); a = 1: the planner proved X.rgb >= 0 (no sign handling)
   OP_CUBE,      // X.rgb <- x^3
   OP_CLAMP01,   // X.rgb <- clamp(x, 0, 1)
   OP_PREMUL,    // X.rgb <- X.rgb * X.a
