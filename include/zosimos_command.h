@@ -50,6 +50,10 @@ zh_status zs_command_destroy(zs_command* cmd);
 zh_status zs_command_input(zs_command* cmd, const zh_descriptor* desc, uint32_t* reg);
 /* CommandBuffer::output, command.rs:1707-1712; `outformat` may be NULL */
 zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat);
+/* CommandBuffer::render, command.rs:1719-1736: an output meant for a render
+ * target (Environment::bind_render); here an output slot like any other,
+ * bound with zh_binding.is_output = 1. */
+zh_status zs_command_render(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat);
 /* CommandBuffer::crop, command.rs:971-978 (result keeps the source size) */
 zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg);
 /* CommandBuffer::color_convert, command.rs:986-1107 */

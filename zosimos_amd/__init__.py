@@ -258,6 +258,12 @@ class CommandBuffer:
         _check(self._lib.zs_command_output(self._h, src, ctypes.byref(r), ctypes.byref(d)))
         return r.value, Descriptor.from_c(d)
 
+    def render(self, src: int):
+        r = ctypes.c_uint32()
+        d = DescriptorC()
+        _check(self._lib.zs_command_render(self._h, src, ctypes.byref(r), ctypes.byref(d)))
+        return r.value, Descriptor.from_c(d)
+
     def crop(self, src: int, rect: Rectangle) -> int:
         return self._reg(self._lib.zs_command_crop, src, rect.c())
 

@@ -101,6 +101,7 @@ _SIGNATURES = {
     "zs_command_destroy": (i32, [vp]),
     "zs_command_input": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
     "zs_command_output": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(DescriptorC)]),
+    "zs_command_render": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(DescriptorC)]),
     "zs_command_crop": (i32, [vp, u32, RectC, ctypes.POINTER(u32)]),
     "zs_command_color_convert": (i32, [vp, u32, ctypes.POINTER(ColorC), ctypes.POINTER(Texel), ctypes.POINTER(u32)]),
     "zs_command_inscribe": (i32, [vp, u32, RectC, u32, ctypes.POINTER(u32)]),

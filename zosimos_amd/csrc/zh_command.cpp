@@ -195,6 +195,10 @@ zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_des
   return ZH_OK;
 }
 
+zh_status zs_command_render(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat) {
+  return zs_command_output(cmd, src, reg, outformat);
+}
+
 zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg) {
   if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   zh_descriptor d;
