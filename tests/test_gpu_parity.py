@@ -429,6 +429,27 @@ def test_c3_batch_small(ctx):
             assert ulp[ok].max(initial=0) <= 1
 
 
+@pytest.mark.parametrize("w,h,ow,oh", [(1, 7, 3, 21), (7, 1, 4, 1), (1, 1, 3, 3), (2, 2, 1, 1), (5, 3, 2, 1)])
+def test_resample_degenerate_sizes(ctx, w, h, ow, oh):
+    """The per-image program of the resampling batch on degenerate shapes:
+    sources one texel wide or tall (too narrow for the tap-pair kernels, so
+    the general resampler runs), single pixels, and outputs of one pixel;
+    within 1 code of the FP64 oracle where alpha is non-zero."""
+    p3 = z.Color(primaries=z.Primaries.DisplayP3, transfer=z.Transfer.Srgb)
+    din = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    r = cmd.input(din)
+    c = cmd.color_convert(r, p3, RGBA8)
+    canvas = cmd.solid_rgba(z.Descriptor(ow, oh, RGBA8, p3), (0.0, 0.0, 0.0, 0.0))
+    out_r = cmd.affine(canvas, z.Affine.new(z.AffineSample.BiLinear).scale(ow / w, oh / h), c)
+    o, od = cmd.output(out_r)
+    data = rand_u8((h, w * 4), 70 + w + 3 * h)
+    out, _ = run(ctx, cmd, {r: din}, {r: data}, {o: od})
+    lin, want = PL.c3_image(data, 0, w, h, ow, oh)
+    mr, ma, _ = compare_u8(f"degenerate_{w}x{h}_to_{ow}x{oh}", out[o], want, alpha_ok=want[..., 3] > 0)
+    assert mr <= 1 and ma <= 1
+
+
 def test_c5_layers_small(ctx):
     wl = W.c5(layers=8, width=192, height=128, seed=5)
     host = wl.make_inputs()
