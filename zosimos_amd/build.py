@@ -42,12 +42,17 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Per-source device flags.  The MLP kernels keep their MFMA accumulators in
+# ordinary VGPRs (no AGPR round trip: v_accvgpr_read per result element).
+SOURCE_FLAGS = {"zh_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), *COMMON, "-c", path, "-o", obj]
     if src.endswith(".hip"):
-        cmd += DEVICE_FLAGS
+        cmd += DEVICE_FLAGS + SOURCE_FLAGS.get(src, [])
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
