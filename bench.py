@@ -121,7 +121,7 @@ def cpu_baseline(config: str):
 def surrogate_leg(ctx, torch, steps: int):
     """The learned colour chain: 16 x 3840x2160 RGBA8 through the 7-step
     pointwise chain, run by the plain fused kernel and by the trained
-    3-64-64-3 surrogate (forced on, for timing), with the lattice judge's
+    3-64-64-3 surrogate (forced on, for timing), with the lattice check's
     verdict (max Delta-E OK < 0.005 on the 65^3 lattice)."""
     import zosimos_amd as z
     from zosimos_amd import workloads as W

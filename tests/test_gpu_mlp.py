@@ -5,7 +5,7 @@
   gradient is pinned by central differences in ``test_oracle.py``.
 * The matrix-core inference kernel against the oracle's forward pass.
 * Learning on a program (the learned-chain workload's pointwise chain): what
-  training gains over the untrained network, the lattice judge's decision,
+  training gains over the untrained network, the lattice check's decision,
   the plain kernel as the fallback, and the surrogate kernel inside the
   program against the oracle's forward pass on the trained weights.
 
@@ -65,7 +65,7 @@ def test_inference_matches_oracle_forward(ctx):
     assert np.quantile(d, 0.999) < 2e-3 and d.max() < 2e-2
 
 
-def test_learned_chain_gain_judge_and_fallback(ctx):
+def test_learned_chain_gain_check_and_fallback(ctx):
     w, h = 256, 128
     wl = W.c4(images=1, width=w, height=h, seed=9)
     (ri, din), = wl.inputs.items()
@@ -92,7 +92,7 @@ def test_learned_chain_gain_judge_and_fallback(ctx):
     assert r["final_loss"] < r["initial_loss"] / 100
     assert r["max_delta_e"] < r["max_delta_e_initial"] / 5
     assert r["mean_delta_e"] < 0.01
-    # The judge's decision follows its bound.
+    # The check's decision follows its bound.
     assert r["accepted"] == (1 if r["max_delta_e"] < 0.005 else 0)
 
     # Plain kernels: bit-identical to the program before learning.

@@ -136,7 +136,7 @@ struct zh_program {
   // learned surrogate (an image the faster kernel cannot address forces 0).
   std::vector<int> variant;
   // Learned colour surrogates per stage (zh_surrogate.cpp): packed matrix-core
-  // operands (nullptr = none), the lattice judge's verdict, master weights.
+  // operands (nullptr = none), the lattice check's verdict, master weights.
   std::vector<uint16_t*> d_surrogate;
   std::vector<uint8_t> surrogate_ok;
   std::vector<std::vector<float>> surrogate_params;

@@ -8,7 +8,7 @@
 //      continuous inputs);
 //   2. training: Adam on the mean squared error, minibatches from a seeded
 //      permutation, cosine learning-rate decay, all on the GPU;
-//   3. judge: the trained network on a lattice^3 grid of the input cube
+//   3. check: the trained network on a lattice^3 grid of the input cube
 //      (matrix-core inference kernel) against the stage's program evaluated
 //      on the host in FP64, compared in Oklab; the stage switches to the
 //      network only if the max Delta-E stays below the bound.
@@ -280,7 +280,7 @@ zh_status train_stage(zh_program* p, uint32_t stage, const zh_learn_options& o, 
   ZS_HIP(hipStreamSynchronize(s));
   const auto t1 = std::chrono::steady_clock::now();
 
-  // 3. judge on the lattice, for the trained and the initial network.
+  // 3. check on the lattice, for the trained and the initial network.
   Mat3 to_xyz = mat_identity();
   uint32_t prim = ZH_PRIMARIES_BT709, wp = ZH_WHITEPOINT_D65;
   const int64_t ref = sp.slot_ref[sp.desc.code[sp.desc.n_instr - 1].a];
@@ -306,7 +306,7 @@ zh_status train_stage(zh_program* p, uint32_t stage, const zh_learn_options& o, 
   ZS_HIP(T.alloc(&d_ly, nl * 4));
   ZS_HIP(T.alloc(&d_ops, kOps));
   ZS_HIP(hipMemcpyAsync(d_l, hl.data(), nl * 16, hipMemcpyHostToDevice, s));
-  auto judge = [&](const float* params_dev, double* mx, double* mean) -> zh_status {
+  auto lattice_check = [&](const float* params_dev, double* mx, double* mean) -> zh_status {
     ZS_HIP(zh_mlp_pack(params_dev, d_ops, s));
     std::vector<float> got;
     zh_status st = infer_host(s, cus, d_ops, d_l, d_ly, nl, got);
@@ -325,9 +325,9 @@ zh_status train_stage(zh_program* p, uint32_t stage, const zh_learn_options& o, 
   double mx0 = 0, mean0 = 0, mx = 0, mean = 0;
   float* d_init = d_q;  // reuse: the initial parameters
   ZS_HIP(hipMemcpyAsync(d_init, init.data(), kParams * 4, hipMemcpyHostToDevice, s));
-  zh_status st = judge(d_init, &mx0, &mean0);
+  zh_status st = lattice_check(d_init, &mx0, &mean0);
   if (st != ZH_OK) return st;
-  st = judge(d_p, &mx, &mean);  // leaves the trained operands in d_ops
+  st = lattice_check(d_p, &mx, &mean);  // leaves the trained operands in d_ops
   if (st != ZH_OK) return st;
 
   rep->stage = stage;
