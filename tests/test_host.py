@@ -295,6 +295,20 @@ def test_invoke_inlines_the_function():
     expect(z.Status.TYPE, cmd.invoke, body, [x, y], 2)
 
 
+def test_cube_root_sign_handling_only_where_needed():
+    """The cube root skips its sign handling only where the planner proves the
+    input non-negative: an 8-bit load through a matrix of non-negative
+    entries (sRGB to Oklab's cone space).  An RGBA16F source may hold negative
+    values, so its chain keeps the signed cube root."""
+    def kernel(desc):
+        cmd = z.CommandBuffer()
+        lab = cmd.color_convert(cmd.input(desc), z.Color.OKLAB, z.Texel(z.SampleBits.Float16x4, z.SampleParts.LabA))
+        cmd.output(lab)
+        return re.findall(r"\[kernel (\w+)\]", plan(cmd).describe())
+    assert kernel(rgba8(64, 8)) == ["oklab_enc_s8hp"]
+    assert kernel(z.Descriptor.rgba16f_linear(64, 8)) != ["oklab_enc_s8hp"]
+
+
 def test_launch_without_device_fails_loudly():
     e = plan(W.c2(8, 8))
     with pytest.raises(z.ZosimosError) as err:
