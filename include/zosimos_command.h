@@ -52,7 +52,8 @@ zh_status zs_command_input(zs_command* cmd, const zh_descriptor* desc, uint32_t*
 zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat);
 /* CommandBuffer::render, command.rs:1719-1736: an output meant for a render
  * target (Environment::bind_render); here an output slot like any other,
- * bound with zh_binding.is_output = 1. */
+ * bound with zh_binding.is_output = 1.  ZH_ERR_TYPE when the register's
+ * texel has no texture form in the reference (3- or 6-byte texels). */
 zh_status zs_command_render(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat);
 /* CommandBuffer::crop, command.rs:971-978 (result keeps the source size) */
 zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg);

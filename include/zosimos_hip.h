@@ -280,8 +280,31 @@ typedef struct zh_binding {
 
 /* Bind and launch asynchronously (Executable::launch + Execution::step
  * until done, run.rs:1016-1464).  Every input and output slot must be bound
- * to an image whose descriptor matches. */
+ * to an image whose descriptor matches.  Successive launches may bind other
+ * images without a zh_sync in between (double buffering): each launch keeps
+ * the images it was bound with.  The host waits only when three launches
+ * with changed bindings are still queued. */
 zh_status zh_program_launch(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings);
+/* hipGraph state of a program (no counterpart in the reference, which
+ * records a wgpu command buffer per launch, run.rs:1394-1464).  A program of
+ * two or more kernel launches is captured into a hipGraph at its first
+ * launch and replayed after; a rebinding that changes which kernels run
+ * captures again.  A capture that fails is reported here (and in
+ * zh_program_describe) and the program launches kernel by kernel. */
+#define ZH_GRAPH_NOT_YET 0       /* several launches, not launched yet */
+#define ZH_GRAPH_CAPTURED 1
+#define ZH_GRAPH_FAILED 2
+#define ZH_GRAPH_DISABLED 3      /* built with ZH_PLAN_NO_GRAPH */
+#define ZH_GRAPH_SINGLE_LAUNCH 4 /* one kernel launch: launched directly */
+typedef struct zh_graph_info {
+  uint32_t state;    /* ZH_GRAPH_* */
+  uint32_t nodes;    /* nodes of the captured graph */
+  uint32_t captures; /* captures so far */
+  uint32_t pad;
+  uint64_t replays;  /* launches that replayed the graph */
+} zh_graph_info;
+zh_status zh_program_graph_info(zh_program* prog, zh_graph_info* info);
+
 /* Average device time (ms) of each stage over the last `zh_program_time`
  * call: launches the program `iterations` times and records HIP events
  * around every stage on the context stream. */

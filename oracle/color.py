@@ -384,6 +384,32 @@ def affine_inverse(m):
     return np.linalg.inv(np.asarray(m, dtype=np.float64).reshape(3, 3))
 
 
+def bilinear_affine_at(texels, src_hw, matrix, xs, ys):
+    """The bilinear affine sample of ``bilinear_affine`` at the output pixels
+    (xs[k], ys[k]) only.  ``texels(yi, xi)`` returns the premultiplied
+    values of the source texels at integer arrays (yi, xi) (all in range);
+    the source is ``src_hw`` = (H, W).  Returns (N, C)."""
+    sh, sw = src_hw
+    inv = affine_inverse(matrix)
+    px = np.asarray(xs, dtype=np.float64) + 0.5
+    py = np.asarray(ys, dtype=np.float64) + 0.5
+    u = inv[0, 0] * px + inv[0, 1] * py + inv[0, 2]
+    v = inv[1, 0] * px + inv[1, 1] * py + inv[1, 2]
+    q = inv[2, 0] * px + inv[2, 1] * py + inv[2, 2]
+    u, v = u / q - 0.5, v / q - 0.5
+    x0, y0 = np.floor(u), np.floor(v)
+    fx, fy = (u - x0)[..., None], (v - y0)[..., None]
+    x0, y0 = x0.astype(np.int64), y0.astype(np.int64)
+
+    def tap(xi, yi):
+        ok = (xi >= 0) & (xi < sw) & (yi >= 0) & (yi < sh)
+        vals = texels(np.clip(yi, 0, sh - 1), np.clip(xi, 0, sw - 1))
+        return np.where(ok[..., None], vals, 0.0)
+
+    return ((1 - fx) * (1 - fy) * tap(x0, y0) + fx * (1 - fy) * tap(x0 + 1, y0)
+            + (1 - fx) * fy * tap(x0, y0 + 1) + fx * fy * tap(x0 + 1, y0 + 1))
+
+
 def bilinear_affine(src_premul, matrix, out_hw):
     """Resample ``src_premul`` (H, W, C) onto an output grid of ``out_hw``.
 
@@ -395,26 +421,10 @@ def bilinear_affine(src_premul, matrix, out_hw):
     its centre at (i + 0.5, j + 0.5); samples outside the source read zero.
     """
     h, w = out_hw
-    inv = affine_inverse(matrix)
-    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
-    px, py = xs + 0.5, ys + 0.5
-    u = inv[0, 0] * px + inv[0, 1] * py + inv[0, 2]
-    v = inv[1, 0] * px + inv[1, 1] * py + inv[1, 2]
-    q = inv[2, 0] * px + inv[2, 1] * py + inv[2, 2]
-    u, v = u / q - 0.5, v / q - 0.5
-    x0, y0 = np.floor(u), np.floor(v)
-    fx, fy = u - x0, v - y0
-    x0, y0 = x0.astype(np.int64), y0.astype(np.int64)
-    sh, sw = src_premul.shape[:2]
-
-    def tap(xi, yi):
-        ok = (xi >= 0) & (xi < sw) & (yi >= 0) & (yi < sh)
-        vals = src_premul[np.clip(yi, 0, sh - 1), np.clip(xi, 0, sw - 1)]
-        return np.where(ok[..., None], vals, 0.0)
-
-    fx, fy = fx[..., None], fy[..., None]
-    return ((1 - fx) * (1 - fy) * tap(x0, y0) + fx * (1 - fy) * tap(x0 + 1, y0)
-            + (1 - fx) * fy * tap(x0, y0 + 1) + fx * fy * tap(x0 + 1, y0 + 1))
+    ys, xs = np.mgrid[0:h, 0:w]
+    src = np.asarray(src_premul)
+    out = bilinear_affine_at(lambda yi, xi: src[yi, xi], src.shape[:2], matrix, xs.ravel(), ys.ravel())
+    return out.reshape(h, w, -1)
 
 
 def affine_matrix(shift_pre=(0.0, 0.0), scale=(1.0, 1.0), rotate_rad=0.0, shift_post=(0.0, 0.0)):

@@ -158,27 +158,48 @@ def c4(src_u8):
     return res, C.encode_u8(res, 5)
 
 
-def c3_image(data, fmt, w, h, ow, oh):
-    """decode -> Rec.709 to Display-P3 -> bilinear resample to (ow, oh) ->
-    encode in the input's format.  Returns (linear, encoded bytes)."""
+def c3_image_at(data, fmt, w, h, ow, oh, xs, ys):
+    """``c3_image`` at the output pixels (xs[k], ys[k]) only: the source
+    texels each sample needs are decoded, converted and premultiplied on
+    demand.  Returns (linear (N, 4), encoded (N, 4))."""
+    m3 = C.convert_matrix(3, 7, 5)
     if fmt == 0:
-        lin = C.decode_u8(u8_image(data, w, h), 5)
+        img = u8_image(data, w, h)
+        texels = lambda yi, xi: C.premultiply(apply3(m3, C.decode_u8(img[yi, xi], 5)))
     else:
-        lin = f16_image(data, w, h)
-    p3 = apply3(C.convert_matrix(3, 7, 5), lin)
+        img = np.asarray(data).view(np.float16).reshape(h, w, 4)
+        texels = lambda yi, xi: C.premultiply(apply3(m3, img[yi, xi].astype(np.float64)))
     m = np.diag([np.float32(ow / w), np.float32(oh / h), 1.0])
-    res = C.unpremultiply(C.bilinear_affine(C.premultiply(p3), m, (oh, ow)))
+    res = C.unpremultiply(C.bilinear_affine_at(texels, (h, w), m, xs, ys))
     if fmt == 0:
         return res, C.encode_u8(res, 5)
     return res, C.encode_f16(res)
 
 
-def c5(layers_f16, matrices, out_hw):
-    acc = np.zeros(out_hw + (4,), dtype=np.float64)
-    for lay, m in zip(layers_f16, matrices):
-        acc = C.over(C.bilinear_affine(C.premultiply(lay), mat_f32(m), out_hw), acc)
+def c3_image(data, fmt, w, h, ow, oh):
+    """decode -> Rec.709 to Display-P3 -> bilinear resample to (ow, oh) ->
+    encode in the input's format.  Returns (linear, encoded bytes)."""
+    ys, xs = np.mgrid[0:oh, 0:ow]
+    res, enc = c3_image_at(data, fmt, w, h, ow, oh, xs.ravel(), ys.ravel())
+    return res.reshape(oh, ow, 4), enc.reshape(oh, ow, 4)
+
+
+def c5_at(layers, matrices, xs, ys):
+    """``c5`` at the output pixels (xs[k], ys[k]) only.  ``layers`` are
+    (H, W, 4) arrays of straight-alpha values (any float dtype)."""
+    acc = None
+    for lay, m in zip(layers, matrices):
+        texels = lambda yi, xi, lay=lay: C.premultiply(np.asarray(lay[yi, xi], dtype=np.float64))
+        s = C.bilinear_affine_at(texels, lay.shape[:2], mat_f32(m), xs, ys)
+        acc = s if acc is None else C.over(s, acc)
     res = chain(C.unpremultiply(acc))
     return res, C.encode_u8(res, 5)
+
+
+def c5(layers_f16, matrices, out_hw):
+    ys, xs = np.mgrid[0:out_hw[0], 0:out_hw[1]]
+    res, enc = c5_at(layers_f16, matrices, xs.ravel(), ys.ravel())
+    return res.reshape(out_hw + (4,)), enc.reshape(out_hw + (4,))
 
 
 def bilinear_grid(w, h, params):

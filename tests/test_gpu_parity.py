@@ -463,28 +463,6 @@ def test_c5_layers_small(ctx):
 
 
 # ---- runtime behaviour ------------------------------------------------------------
-def test_graph_replays_match_eager(ctx):
-    """The captured hipGraph replayed several times, with eager work and
-    synchronisation between the replays, gives the eager result each time."""
-    wl = W.c1(128, 96, seed=6)
-    host = wl.make_inputs()
-    eager, _ = run(ctx, wl.commands, wl.inputs, host, wl.outputs, flags=z.PLAN_NO_GRAPH)
-    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
-    ins = {}
-    for r, d in wl.inputs.items():
-        ins[r] = ctx.image(d)
-        ins[r].upload(host[r])
-    (o, od), = wl.outputs.items()
-    out = ctx.image(od)
-    for i in range(4):
-        exe.launch(ins, {o: out})
-        ctx.sync()
-        assert np.array_equal(out.download(), eager[o]), f"replay {i}"
-        # eager work on the same stream between replays
-        scratch = ctx.image(od)
-        scratch.upload(np.zeros((96, 128 * 4), np.uint8))
-
-
 def test_knob_matches_rebuilt_program(ctx):
     """tests/knobs.rs: one executable, several parameterisations through a
     knob, each equal to a program built with that parameter."""
@@ -649,10 +627,11 @@ def test_invoke_matches_inline_program(ctx):
 @pytest.mark.parametrize("out_wh", [(83, 41), (20, 11), (50, 30)])
 def test_resize_nearest(ctx, out_wh):
     """CommandBuffer::resize (command.rs:1675-1705): the coordinate ramp's
-    pixel-centre values index the source as a palette, half a ramp texel
-    added (palette.frag's bias), so output pixel x reads source texel
-    floor((x + 1) * W_in / W_out) clamped: equal to the oracle byte for
-    byte (the ramp's FP32 values, as the fused program keeps them)."""
+    pixel-centre values, stored in the ramp register's RGBA8 Scalars texel
+    (a staged format: truncating ``uint(c * 255)``, command.rs:1677), index
+    the source as a palette with half a ramp texel added (palette.frag's
+    bias): equal to the oracle byte for byte, in the default plan (the fused
+    ramp is rounded through its format) and with strict registers."""
     w, h = 50, 30
     ow, oh = out_wh
     d = z.Descriptor.rgba8_srgb(w, h)
@@ -661,14 +640,18 @@ def test_resize_nearest(ctx, out_wh):
     a = cmd.input(d)
     o, od = cmd.output(cmd.resize(a, ow, oh))
     assert (od.width, od.height) == (ow, oh)
-    out, _ = run(ctx, cmd, {a: d}, {a: data}, {o: od})
+    out, exe = run(ctx, cmd, {a: d}, {a: data}, {o: od})
+    strict, _ = run(ctx, cmd, {a: d}, {a: data}, {o: od}, flags=z.PLAN_STRICT_REGISTERS)
     f = np.float32
     ramp = np.zeros((oh, ow, 4), f)
     ramp[..., 0] = ((np.arange(ow, dtype=f) + f(0.5)) / f(ow))[None, :]
     ramp[..., 1] = ((np.arange(oh, dtype=f) + f(0.5)) / f(oh))[:, None]
     ramp[..., 3] = 1.0
-    want = PL.palette_lookup(data.reshape(h, w, 4), ramp, [1, 0, 0, 0], [0, 1, 0, 0])
+    ramp8 = (np.clip(ramp, 0, 1) * f(255.0)).astype(np.uint32).astype(f) / f(255.0)
+    want = PL.palette_lookup(data.reshape(h, w, 4), ramp8, [1, 0, 0, 0], [0, 1, 0, 0])
+    assert "quant" in exe.describe()
     assert np.array_equal(out[o].reshape(oh, ow, 4), want)
+    assert np.array_equal(strict[o], out[o])
 
 
 @pytest.mark.parametrize("channel", [z.ColorChannel.R, z.ColorChannel.G, z.ColorChannel.B])

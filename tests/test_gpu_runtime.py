@@ -1,0 +1,297 @@
+"""GPU tests of the runtime around the kernels, through the C ABI:
+
+- the hipGraph of a multi-launch program is captured (asserted through
+  ``zh_program_graph_info``) and every replay equals the eager launches;
+- launches that rebind other images without a sync in between each keep
+  their own images (double buffering);
+- a High op list built on the caller's side (the path a Rust ``-sys`` crate
+  takes: ``zh_op[]`` mirroring program.rs:88-240, ``zh_program_build``,
+  ``zh_binding[]``, ``zh_program_launch``) gives the bytes of the front
+  end's ``lower_to``, and its error statuses are the reference's classes.
+"""
+import ctypes
+import struct
+
+import numpy as np
+import pytest
+
+import zosimos_amd as z
+from zosimos_amd import _abi
+from zosimos_amd import workloads as W
+from oracle import color as C
+from oracle import pipelines as PL
+
+from gpu_util import compare_u8, report, run
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_u8(shape, seed):
+    return np.random.default_rng(seed).integers(0, 256, size=shape, dtype=np.uint8)
+
+
+def upload_all(ctx, wl, host):
+    ins = {}
+    for r, d in wl.inputs.items():
+        ins[r] = ctx.image(d)
+        ins[r].upload(host[r])
+    return ins
+
+
+# ---- hipGraph ---------------------------------------------------------------------------
+def test_graph_captured_and_replays_match_eager(ctx):
+    """A batch of both formats plans to two launches (one tile queue per
+    form), so the program is captured as a hipGraph at its first launch.
+    Each replay, with eager work and syncs in between, equals the eager run."""
+    wl = W.c3(n=8, seed=11, max_side=260)
+    host = wl.make_inputs()
+    eager, eexe = run(ctx, wl.commands, wl.inputs, host, wl.outputs, flags=z.PLAN_NO_GRAPH)
+    assert eexe.graph_info()["state"] == "disabled"
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    launches, _ = exe.stats()
+    assert launches >= 2
+    assert exe.graph_info()["state"] == "not yet"
+    ins = upload_all(ctx, wl, host)
+    outs = {o: ctx.image(d) for o, d in wl.outputs.items()}
+    for i in range(4):
+        for o, img in outs.items():
+            img.upload(np.zeros((img.desc.height, img.desc.width * img.desc.texel.bytes), np.uint8))
+        exe.launch(ins, outs)
+        ctx.sync()
+        g = exe.graph_info()
+        assert g["state"] == "captured", exe.describe()
+        assert g["nodes"] >= launches and g["captures"] == 1 and g["replays"] == i + 1
+        for o, img in outs.items():
+            assert np.array_equal(img.download(), eager[o]), f"replay {i}, output {o}"
+    assert "graph: captured" in exe.describe()
+    report("graph_replays", **exe.graph_info())
+
+
+def test_single_launch_program_has_no_graph(ctx):
+    wl = W.c1(128, 96, seed=6)
+    host = wl.make_inputs()
+    out, exe = run(ctx, wl.commands, wl.inputs, host, wl.outputs, launches=2)
+    g = exe.graph_info()
+    assert g["state"] == "single launch" and g["captures"] == 0
+    assert "graph: none (one launch)" in exe.describe()
+
+
+# ---- rebinding without a sync -------------------------------------------------------
+def _c2_refs(wl, host, w, h):
+    src_r, layer_r = list(wl.inputs)
+    _, want = PL.c2(PL.u8_image(host[src_r], w, h), PL.f16_image(host[layer_r], w, h))
+    return want
+
+
+def test_rebind_without_sync_keeps_each_launch_images(ctx):
+    """Double buffering: one executable launched 6 times, alternating two
+    input sets and three output images, with no sync between launches.  A
+    long first launch keeps the later rebinds queued behind it.  Each output
+    holds its own launch's result."""
+    w, h = 1024, 512
+    wl = W.c2(w, h, seed=21)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    hosts = [wl.make_inputs()]
+    alt = {}
+    for k, (r, d) in enumerate(wl.inputs.items()):
+        alt[r] = (rand_u8((h, w * 4), 500 + k) if d.texel.bits == z.SampleBits.UInt8x4 else
+                  W.random_rgba16f(np.random.default_rng(600 + k), w, h))
+    hosts.append(alt)
+    ins = [upload_all(ctx, wl, hh) for hh in hosts]
+    (o, od), = wl.outputs.items()
+    outs = [ctx.image(od) for _ in range(3)]
+    big = W.c2(3840, 2160, seed=3)  # a long launch first, on the same stream
+    bexe = z.Linker.from_included().compile(big.commands).lower_to(ctx)
+    bin_ = {r: ctx.image(d) for r, d in big.inputs.items()}
+    bout = {ob: ctx.image(d) for ob, d in big.outputs.items()}
+    ctx.sync()
+    plan = [(0, 0), (1, 1), (0, 2), (1, 0), (0, 1), (1, 2)]
+    bexe.launch(bin_, bout)
+    for i_in, i_out in plan:
+        exe.launch(ins[i_in], {o: outs[i_out]})
+    ctx.sync()
+    # the last writer of each output: (1, 0), (0, 1), (1, 2)
+    final = {0: 1, 1: 0, 2: 1}
+    wants = [_c2_refs(wl, hh, w, h) for hh in hosts]
+    for k, img in enumerate(outs):
+        want = wants[final[k]]
+        mr, ma, _ = compare_u8(f"rebind_out{k}", img.download(), want, alpha_ok=want[..., 3] > 0)
+        assert mr <= 1 and ma <= 1
+
+
+def test_rebind_without_sync_multi_launch_graph(ctx):
+    """The same for a graph-captured program of two launches."""
+    wl = W.c3(n=6, seed=12, max_side=200)
+    host0 = wl.make_inputs()
+    host1 = {r: (rand_u8(v.shape, 700 + k) if wl.inputs[r].texel.bits == z.SampleBits.UInt8x4 else
+                 W.random_rgba16f(np.random.default_rng(800 + k), wl.inputs[r].width, wl.inputs[r].height))
+             for k, (r, v) in enumerate(host0.items())}
+    ref0, _ = run(ctx, wl.commands, wl.inputs, host0, wl.outputs, flags=z.PLAN_NO_GRAPH)
+    ref1, _ = run(ctx, wl.commands, wl.inputs, host1, wl.outputs, flags=z.PLAN_NO_GRAPH)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    ins = [upload_all(ctx, wl, hh) for hh in (host0, host1)]
+    outs = [{o: ctx.image(d) for o, d in wl.outputs.items()} for _ in range(2)]
+    ctx.sync()
+    for i in range(5):
+        exe.launch(ins[i % 2], outs[i % 2])
+    ctx.sync()
+    assert exe.graph_info()["state"] == "captured"
+    for k, ref in enumerate((ref0, ref1)):
+        for o, img in outs[k].items():
+            assert np.array_equal(img.download(), ref[o]), f"set {k}, output {o}"
+
+
+# ---- a caller-built High op list ----------------------------------------------------
+def _desc_c(d: z.Descriptor) -> _abi.DescriptorC:
+    return d.c()
+
+
+def _op(high, dst=0, src=0, target=0, init=0, shader=0, operands=(), params=None, sel=None, quad=None, vp=None):
+    op = _abi.OpC()
+    op.high, op.dst, op.src, op.target, op.initializer, op.shader = high, dst, src, target, init, shader
+    op.num_operands = len(operands)
+    for i, v in enumerate(operands):
+        op.operands[i] = v
+    op.knob = -1
+    if params is not None:
+        op.params = ctypes.cast(params, ctypes.c_void_p)
+        op.params_len = len(params)
+    if sel:
+        op.selection = _abi.RectC(*sel)
+    if quad:
+        for i, v in enumerate(quad):
+            op.quad[i] = v
+    if vp:
+        op.viewport = _abi.RectC(*vp)
+    return op
+
+
+def _std140_mat3(m):
+    """mat3 in std140: three columns of vec3 padded to vec4
+    (color_matrix.rs:151-161)."""
+    m = np.asarray(m, np.float32).reshape(3, 3)
+    cols = [list(m[:, j]) + [0.0] for j in range(3)]
+    return ctypes.create_string_buffer(struct.pack("12f", *sum(cols, [])), 48)
+
+
+def _caller_ops(w, h, mat_blob, ident=None):
+    """High IR of: input 0 (RGBA8 sRGB) -> Rec.709-to-P3 matrix into RGBA16F ->
+    'over' input 1 (RGBA16F P3) -> matrix (identity) into RGBA8 P3 sRGB ->
+    output 0 (program.rs:88-240: Input, PushOperand + DrawInto per op,
+    Output)."""
+    lib = _abi.lib()
+    d8 = z.Descriptor.rgba8_srgb(w, h)
+    dl = z.Descriptor(w, h, W.RGBA16F, W.P3_LINEAR)
+    d8p3 = z.Descriptor(w, h, W.RGBA8, W.P3_SRGB)
+    textures = [d8, dl, dl, dl, d8p3]
+    ident = ident if ident is not None else _std140_mat3(np.eye(3))
+    full = (0, 0, w, h)
+    quad = (0.0, 0.0, float(w), 0.0, float(w), float(h), 0.0, float(h))
+    ops = [
+        _op(0, dst=0, src=0),                                     # Input: texture 0 <- slot 0
+        _op(0, dst=1, src=1),                                     # Input: texture 1 <- slot 1
+        _op(2, dst=2, target=0, init=0, shader=1, operands=(0,), params=mat_blob),  # matrix, Discard
+        _op(2, dst=3, target=0, init=1, shader=0, operands=(2,), sel=full, quad=quad, vp=full),  # copy below
+        _op(2, dst=3, target=1, init=1, shader=17, operands=(1,), sel=full, quad=quad, vp=full),  # over, Load
+        _op(2, dst=4, target=0, init=0, shader=1, operands=(3,), params=ident),  # to RGBA8 P3 sRGB
+        _op(1, dst=0, src=4),                                     # Output: slot 0 <- texture 4
+    ]
+    keep = [mat_blob, ident]
+    tex_arr = (_abi.DescriptorC * len(textures))(*[t.c() for t in textures])
+    op_arr = (_abi.OpC * len(ops))(*ops)
+    return lib, textures, tex_arr, op_arr, keep
+
+
+def _launch_raw(lib, ctx, tex_arr, op_arr, in_imgs, out_img):
+    prog = ctypes.c_void_p()
+    st = lib.zh_program_build(ctx._h, tex_arr, len(tex_arr), op_arr, len(op_arr), 0, ctypes.byref(prog))
+    assert st == 0, lib.zh_last_error().decode()
+    binds = [_abi.BindingC(0, 0, in_imgs[0]._h), _abi.BindingC(1, 0, in_imgs[1]._h), _abi.BindingC(0, 1, out_img._h)]
+    arr = (_abi.BindingC * 3)(*binds)
+    st = lib.zh_program_launch(prog, arr, 3)
+    assert st == 0, lib.zh_last_error().decode()
+    assert lib.zh_sync(ctx._h) == 0
+    return prog
+
+
+def test_caller_built_high_ops_match_lower_to(ctx):
+    w, h = 200, 120
+    # the same program through the command front end
+    cmd = z.CommandBuffer()
+    d8 = z.Descriptor.rgba8_srgb(w, h)
+    dl = z.Descriptor(w, h, W.RGBA16F, W.P3_LINEAR)
+    a = cmd.input(d8)
+    l = cmd.input(dl)
+    p3 = cmd.color_convert(a, W.P3_LINEAR, W.RGBA16F)
+    ov = cmd.blend(p3, z.Rectangle(0, 0, w, h), l)
+    o, od = cmd.output(cmd.color_convert(ov, W.P3_SRGB, W.RGBA8))
+    host = {a: rand_u8((h, w * 4), 31), l: W.random_rgba16f(np.random.default_rng(32), w, h)}
+    ref, _ = run(ctx, cmd, {a: d8, l: dl}, host, {o: od})
+    prog = z.Linker.from_included().compile(cmd)
+    _, high = prog.high()
+    front_mat = ctypes.create_string_buffer(ctypes.string_at(high[2].params, 48), 48)
+    front_ident = ctypes.create_string_buffer(ctypes.string_at(high[5].params, 48), 48)
+    assert [op.shader for op in high[2:6]] == [1, 0, 17, 1]
+
+    img_a, img_l = ctx.image(d8), ctx.image(dl)
+    img_a.upload(host[a])
+    img_l.upload(host[l])
+    # 1: the front end's matrix bytes -> byte-identical output
+    lib, textures, tex_arr, op_arr, keep = _caller_ops(w, h, front_mat, front_ident)
+    out = ctx.image(textures[4])
+    p = _launch_raw(lib, ctx, tex_arr, op_arr, (img_a, img_l), out)
+    assert np.array_equal(out.download(), ref[o])
+    lib.zh_program_destroy(p)
+    # 2: the matrix derived by the caller in FP64 -> within 1 code of the oracle
+    m = C.convert_matrix(z.Primaries.Bt709, z.Primaries.DisplayP3, z.Whitepoint.D65)
+    lib, textures, tex_arr, op_arr, keep = _caller_ops(w, h, _std140_mat3(m))
+    out2 = ctx.image(textures[4])
+    p = _launch_raw(lib, ctx, tex_arr, op_arr, (img_a, img_l), out2)
+    lin = C.decode_u8(PL.u8_image(host[a], w, h), 5)
+    up = C.premultiply(PL.f16_image(host[l], w, h))
+    dn = C.premultiply(PL.apply3(m, lin))
+    want = C.encode_u8(C.unpremultiply(C.over(up, dn)), 5)
+    mr, ma, _ = compare_u8("caller_ops_fp64", out2.download(), want, alpha_ok=want[..., 3] > 0)
+    assert mr <= 1 and ma <= 1
+
+    # error classes through the raw ABI (run.rs:369-392, 1295-1298)
+    one = (_abi.BindingC * 1)(_abi.BindingC(0, 1, out2._h))
+    assert lib.zh_program_launch(p, one, 1) == z.Status.MISSING_KEY
+    wrong = ctx.image(z.Descriptor.rgba8_srgb(w + 1, h))
+    bad = (_abi.BindingC * 3)(_abi.BindingC(0, 0, wrong._h), _abi.BindingC(1, 0, img_l._h),
+                              _abi.BindingC(0, 1, out2._h))
+    assert lib.zh_program_launch(p, bad, 3) == z.Status.MISMATCHED_DESCRIPTOR
+    buf = ctypes.create_string_buffer(8)
+    assert lib.zh_program_set_knob(p, 0, buf, 8) == z.Status.INVALID_ARGUMENT  # no op carries knob 0
+    lib.zh_program_destroy(p)
+
+
+def test_caller_built_knob_length(ctx):
+    """A knob on the caller's matrix op: a blob of another length is
+    ZH_ERR_KNOB_LENGTH (run.rs:1295-1298); one of the right length takes
+    effect at the next launch."""
+    w, h = 64, 32
+    lib, textures, tex_arr, op_arr, keep = _caller_ops(w, h, _std140_mat3(np.eye(3)))
+    op_arr[2].knob = 7
+    img_a, img_l = ctx.image(textures[0]), ctx.image(textures[1])
+    host_a = rand_u8((h, w * 4), 41)
+    img_a.upload(host_a)
+    img_l.upload(np.zeros((h, w * 8), np.uint8))  # a transparent layer: 'over' is the identity
+    out = ctx.image(textures[4])
+    p = _launch_raw(lib, ctx, tex_arr, op_arr, (img_a, img_l), out)
+    short = ctypes.create_string_buffer(36)
+    assert lib.zh_program_set_knob(p, 7, short, 36) == z.Status.KNOB_LENGTH
+    swap = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], float)
+    assert lib.zh_program_set_knob(p, 7, _std140_mat3(swap), 48) == 0
+    binds = (_abi.BindingC * 3)(_abi.BindingC(0, 0, img_a._h), _abi.BindingC(1, 0, img_l._h),
+                                _abi.BindingC(0, 1, out._h))
+    assert lib.zh_program_launch(p, binds, 3) == 0
+    assert lib.zh_sync(ctx._h) == 0
+    got = out.download().reshape(h, w, 4)
+    src = host_a.reshape(h, w, 4)
+    # identity primaries (the output texture is declared P3 but the caller's
+    # matrices say what happens): R and G exchanged, within 1 code
+    d = np.abs(got[..., [1, 0, 2]].astype(int) - src[..., :3].astype(int))
+    ok = src[..., 3] > 0
+    assert d[ok].max() <= 1
+    lib.zh_program_destroy(p)

@@ -72,6 +72,16 @@ def test_command_type_errors():
     expect(z.Status.OTHER, cmd.crop, 99, z.Rectangle(0, 0, 1, 1))
     # inconsistent descriptor
     expect(z.Status.TYPE, cmd.input, z.Descriptor(0, 8, z.Texel(10, 8)))
+    # render needs a texel with a texture form (command.rs:1719-1736): the
+    # 3-byte UInt8x3 and 6-byte UInt16x3 texels have no StageKind
+    # (shaders/stage.rs:63-72); an output of them is fine
+    for bits in (z.SampleBits.UInt8x3, z.SampleBits.UInt16x3):
+        r = cmd.color_convert(a, z.Color.SRGB, z.Texel(bits, z.SampleParts.Rgb))
+        expect(z.Status.TYPE, cmd.render, r)
+        cmd.output(r)
+    for t in (z.Texel(z.SampleBits.UInt8x4, z.SampleParts.RgbA), z.Texel(z.SampleBits.Float16x4, z.SampleParts.RgbA),
+              z.Texel(z.SampleBits.UInt565, z.SampleParts.Rgb)):
+        cmd.render(cmd.color_convert(a, z.Color.SRGB, t))
 
 
 def test_inscribe_high_emission_keeps_normalize_quirk():

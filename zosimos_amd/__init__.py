@@ -573,6 +573,16 @@ class Executable:
         _check(self._lib.zh_program_stats(self._h, ctypes.byref(n), ctypes.byref(b)))
         return n.value, b.value
 
+    GRAPH_STATES = {0: "not yet", 1: "captured", 2: "failed", 3: "disabled", 4: "single launch"}
+
+    def graph_info(self) -> dict:
+        """hipGraph state (``zh_program_graph_info``): state name, nodes,
+        captures so far and launches that replayed the graph."""
+        g = _abi.GraphInfoC()
+        _check(self._lib.zh_program_graph_info(self._h, ctypes.byref(g)))
+        return {"state": self.GRAPH_STATES.get(g.state, str(g.state)), "nodes": g.nodes, "captures": g.captures,
+                "replays": g.replays}
+
     def knob(self, knob: int, data: bytes):
         """``Environment::knob`` (run.rs:1292-1306)."""
         buf = ctypes.create_string_buffer(bytes(data), len(data))

@@ -50,6 +50,10 @@ class BindingC(ctypes.Structure):
     _fields_ = [("slot", u32), ("is_output", u32), ("image", vp)]
 
 
+class GraphInfoC(ctypes.Structure):
+    _fields_ = [("state", u32), ("nodes", u32), ("captures", u32), ("pad", u32), ("replays", u64)]
+
+
 class LearnOptionsC(ctypes.Structure):
     _fields_ = [("uniform_points", u32), ("face_points", u32), ("steps", u32), ("batch", u32), ("lr", f32),
                 ("seed", u32), ("lattice", u32), ("max_delta_e", f32)]
@@ -83,6 +87,7 @@ _SIGNATURES = {
     "zh_program_set_knob": (i32, [vp, i32, vp, u32]),
     "zh_program_launch_bytes": (i32, [vp, ctypes.POINTER(u64), u32, ctypes.POINTER(u32)]),
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
+    "zh_program_graph_info": (i32, [vp, ctypes.POINTER(GraphInfoC)]),
     "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
                               ctypes.POINTER(u32)]),
     "zh_program_autotune": (i32, [vp, ctypes.POINTER(BindingC), u32, u32]),

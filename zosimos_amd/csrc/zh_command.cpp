@@ -196,6 +196,17 @@ zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_des
 }
 
 zh_status zs_command_render(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat) {
+  if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_descriptor d;
+  zh_status st = reg_desc(cmd, src, &d);
+  if (st != ZH_OK) return st;
+  // A render target needs a texture form (ImageDescriptor::new,
+  // program.rs:782-946): native RGBA8/BGRA8, or a staged texel of 1, 2, 4, 8
+  // or 16 bytes (StageKind, shaders/stage.rs:63-72); 3- and 6-byte texels
+  // have none (command.rs:1729-1731).
+  const uint32_t b = bits_bytes(d.texel.bits);
+  if (!is_native(d) && b != 1 && b != 2 && b != 4 && b != 8 && b != 16)
+    return fail(ZH_ERR_TYPE, "render target has no texture format");
   return zs_command_output(cmd, src, reg, outformat);
 }
 

@@ -120,7 +120,16 @@ struct zh_program {
   zh::Plan plan;
   std::vector<Unit> units;
   zh::StageDesc* d_stages = nullptr;       // device copies, one per stage
-  zh::StageDesc* h_stages = nullptr;       // pinned host staging
+  zh::StageDesc* h_stages = nullptr;       // host copy of the descriptors (pinned)
+  // Descriptor uploads of successive launches go through a ring of pinned
+  // staging copies, each reused only after the event recorded behind its
+  // last upload has passed: a launch still queued keeps reading the
+  // descriptors it was bound with, whatever later launches bind.
+  static constexpr int kRing = 3;
+  zh::StageDesc* h_ring[kRing] = {nullptr, nullptr, nullptr};
+  hipEvent_t ring_ev[kRing] = {nullptr, nullptr, nullptr};
+  bool ring_used[kRing] = {false, false, false};
+  int ring_next = 0;
   std::vector<void*> d_intermediates;
   std::vector<zh_image> buffer_images;   // the program's buffers (Plan::buffer_ops), device memory owned here
   std::vector<uint32_t*> d_tile_layers;  // per stage: the tile-layer masks (warp stages)
@@ -130,6 +139,10 @@ struct zh_program {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_failed = false;
+  std::string graph_error;     // why the capture failed (graph_failed)
+  uint32_t graph_nodes = 0;    // kernel nodes of the captured graph
+  uint32_t graph_captures = 0; // captures so far (a rebinding that changes kernels captures again)
+  uint64_t graph_replays = 0;  // launches that replayed the graph
   uint32_t max_lds = 0;
   // Kernel per stage for the bound images: 0 general interpreter, 1
   // interpreter with in-line codecs, 2 specialised pointwise, 3 warp, 5 the

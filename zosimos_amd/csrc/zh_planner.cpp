@@ -798,6 +798,15 @@ struct Emitter {
       case N_PALETTE: {
         // The indices' straight colour replaces itself with the palette's.
         if (value(n.b, root)) op(OP_UNPREMUL);
+        // The lookup is discontinuous in its index: an index computed in
+        // this stage (a fused generator) is rounded through its register's
+        // declared format, as the reference's register texture holds it
+        // (resize's RGBA8 ramp, command.rs:1677).  Strict mode rounded it
+        // already.
+        if (!strict && !leaf_physical(n.b)) {
+          const int qs = slot(-1000000 - (int64_t)n.b, B.nodes[n.b].desc);
+          op(OP_QUANT, (uint8_t)qs);
+        }
         const int s = slot(phys_ref(n.a), phys_desc(n.a));
         pending_mat_valid = false;
         ins(OP_PALETTE, (uint8_t)s, 0, param(n.grid, 10));

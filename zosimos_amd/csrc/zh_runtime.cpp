@@ -310,6 +310,14 @@ static void free_device(zh_program* p) {
   if (p->h_stages) hipHostFree(p->h_stages);
   p->d_stages = nullptr;
   p->h_stages = nullptr;
+  for (int k = 0; k < zh_program::kRing; ++k) {
+    if (p->h_ring[k]) hipHostFree(p->h_ring[k]);
+    if (p->ring_ev[k]) hipEventDestroy(p->ring_ev[k]);
+    p->h_ring[k] = nullptr;
+    p->ring_ev[k] = nullptr;
+    p->ring_used[k] = false;
+  }
+  p->ring_next = 0;
 }
 
 // Group stages into launch units.  A run of independent resample stages
@@ -439,6 +447,10 @@ static zh_status alloc_device(zh_program* p) {
   const size_t n = p->plan.stages.size();
   ZH_HIP(hipMalloc(&p->d_stages, std::max<size_t>(1, n) * sizeof(StageDesc)));
   ZH_HIP(hipHostMalloc(&p->h_stages, std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
+  for (int k = 0; k < zh_program::kRing; ++k) {
+    ZH_HIP(hipHostMalloc(&p->h_ring[k], std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
+    ZH_HIP(hipEventCreateWithFlags(&p->ring_ev[k], hipEventDisableTiming));
+  }
   for (const zh_descriptor& d : p->plan.intermediates) {
     void* b = nullptr;
     hipError_t e = hipMalloc(&b, image_bytes(d));
@@ -571,6 +583,11 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
     const zh_descriptor& d = p->plan.intermediates[i];
     o << "intermediate " << i << ": " << d.width << "x" << d.height << " bits " << d.texel.bits << "\n";
   }
+  if (p->flags & ZH_PLAN_NO_GRAPH) o << "graph: disabled\n";
+  else if (p->graph_failed) o << "graph: capture failed (" << p->graph_error << "); launching kernel by kernel\n";
+  else if (p->graph_exec) o << "graph: captured, " << p->graph_nodes << " nodes, " << p->graph_replays << " replays\n";
+  else if (p->units.size() <= 1) o << "graph: none (one launch)\n";
+  else o << "graph: captured at the first launch\n";
   const std::string s = o.str();
   if (needed) *needed = s.size() + 1;
   if (buf && len) {
@@ -762,8 +779,18 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
     p->graph = nullptr;
     p->variant = variant;
   }
-  ZH_HIP(hipMemcpyAsync(p->d_stages, p->h_stages, plan.stages.size() * sizeof(StageDesc), hipMemcpyHostToDevice,
-                        p->ctx->stream));
+  {
+    // Stage through the next ring copy; its previous upload must have left
+    // the stream before the copy is overwritten.
+    const int k = p->ring_next;
+    p->ring_next = (k + 1) % zh_program::kRing;
+    if (p->ring_used[k]) ZH_HIP(hipEventSynchronize(p->ring_ev[k]));
+    const size_t bytes = plan.stages.size() * sizeof(StageDesc);
+    std::memcpy(p->h_ring[k], p->h_stages, bytes);
+    ZH_HIP(hipMemcpyAsync(p->d_stages, p->h_ring[k], bytes, hipMemcpyHostToDevice, p->ctx->stream));
+    ZH_HIP(hipEventRecord(p->ring_ev[k], p->ctx->stream));
+    p->ring_used[k] = true;
+  }
   p->bound_inputs = ip;
   p->bound_outputs = op;
   p->uploaded = true;
@@ -850,33 +877,59 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
   if (use_graph && !p->graph_exec) {
     // Capture once; descriptor contents change through their device copies,
     // never through the graph, so the executable graph stays valid.
+    const char* step = "hipStreamBeginCapture";
     hipError_t e = hipStreamBeginCapture(p->ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       st = launch_all(p);
       hipGraph_t g = nullptr;
+      step = "hipStreamEndCapture";
       e = hipStreamEndCapture(p->ctx->stream, &g);
       if (st != ZH_OK) { if (g) hipGraphDestroy(g); return st; }
-      if (e == hipSuccess && g) {
+      if (e == hipSuccess && !g) e = hipErrorUnknown;
+      if (e == hipSuccess) {
+        size_t nodes = 0;
+        hipGraphGetNodes(g, nullptr, &nodes);
         hipGraphExec_t ge = nullptr;
+        step = "hipGraphInstantiate";
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         if (e == hipSuccess) {
           p->graph = g;
           p->graph_exec = ge;
+          p->graph_nodes = (uint32_t)nodes;
+          p->graph_captures++;
         } else {
           hipGraphDestroy(g);
         }
       }
     }
     if (!p->graph_exec) {
+      // Reported through zh_program_graph_info and zh_program_describe; the
+      // program launches its kernels one by one from now on.
       (void)hipGetLastError();
       p->graph_failed = true;
+      p->graph_error = std::string(step) + ": " + hipGetErrorString(e);
     }
   }
   if (p->graph_exec) {
     ZH_HIP(hipGraphLaunch(p->graph_exec, p->ctx->stream));
+    p->graph_replays++;
     return ZH_OK;
   }
   return launch_all(p);
+}
+
+zh_status zh_program_graph_info(zh_program* p, zh_graph_info* info) {
+  if (!p || !info) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  if (p->flags & ZH_PLAN_NO_GRAPH) info->state = ZH_GRAPH_DISABLED;
+  else if (p->graph_failed) info->state = ZH_GRAPH_FAILED;
+  else if (p->graph_exec) info->state = ZH_GRAPH_CAPTURED;
+  else if (p->units.size() <= 1) info->state = ZH_GRAPH_SINGLE_LAUNCH;
+  else info->state = ZH_GRAPH_NOT_YET;
+  info->nodes = p->graph_nodes;
+  info->captures = p->graph_captures;
+  info->replays = p->graph_replays;
+  return ZH_OK;
 }
 
 // ---- autotuner ---------------------------------------------------------------
