@@ -12,7 +12,7 @@ Bars:
   code off only where the restatement's value sits within 1e-4 of a code
   edge;
 - native BGRA8: decode equal to the FP64 sRGB EOTF rounded once to f32 (sRGB)
-  or within 1 ulp of k / 255 (linear); encode equal to the FP64 oracle's
+  or to k * fl(1/255) (linear channels and alpha); encode equal to the FP64 oracle's
   rounded code for every value; BGRA8 <-> RGBA8 a byte permutation;
 - 16x3 / 16x4 (above 32 bits, where stage.frag has no codec): IEEE
   v / 65535 and truncating ``uint(c * 65535)``, every code round-trips.
@@ -272,12 +272,16 @@ def test_native_bgra8_decode_encode(ctx, transfer):
     bgra = raw.reshape(h, w, 4)
     rgba = bgra[..., [2, 1, 0, 3]]
     lin = C.decode_u8(rgba, transfer)
+    # Linear channels and alpha decode as k * fl(1/255) in f32 (one rounded
+    # product: within 1.5 ulp of the FP64 k / 255); sRGB colour through the
+    # exact 8-bit EOTF table (the FP64 value rounded once).
+    unorm = rgba.astype(np.float32) * np.float32(1.0 / 255.0)
+    assert np.all(np.abs(unorm.astype(np.float64) - rgba / 255.0) <= 1.5 * 2.0 ** -24 * (rgba / 255.0))
     if transfer == T.Srgb:
         assert np.array_equal(got[..., :3], lin[..., :3].astype(np.float32))
     else:
-        ulp = np.spacing(lin[..., :3].astype(np.float32))
-        assert np.all(np.abs(got[..., :3] - lin[..., :3]) <= ulp)
-    assert np.all(np.abs(got[..., 3] - lin[..., 3]) <= np.spacing(lin[..., 3].astype(np.float32)))
+        assert np.array_equal(got[..., :3], unorm[..., :3])
+    assert np.array_equal(got[..., 3], unorm[..., 3])
 
     # encode: random linear values (some outside [0, 1]) to BGRA8
     v = rng.uniform(-0.1, 1.1, (h, w, 4)).astype(np.float32)
