@@ -272,11 +272,11 @@ def test_native_bgra8_decode_encode(ctx, transfer):
     bgra = raw.reshape(h, w, 4)
     rgba = bgra[..., [2, 1, 0, 3]]
     lin = C.decode_u8(rgba, transfer)
-    # Linear channels and alpha decode as k * fl(1/255) in f32 (one rounded
-    # product: within 1.5 ulp of the FP64 k / 255); sRGB colour through the
-    # exact 8-bit EOTF table (the FP64 value rounded once).
+    # Linear channels and alpha decode as k * fl(1/255) in f32 (two
+    # roundings: within 2^-23 relative of the FP64 k / 255); sRGB colour
+    # through the exact 8-bit EOTF table (the FP64 value rounded once).
     unorm = rgba.astype(np.float32) * np.float32(1.0 / 255.0)
-    assert np.all(np.abs(unorm.astype(np.float64) - rgba / 255.0) <= 1.5 * 2.0 ** -24 * (rgba / 255.0))
+    assert np.all(np.abs(unorm.astype(np.float64) - rgba / 255.0) <= 2.0 ** -23 * (rgba / 255.0))
     if transfer == T.Srgb:
         assert np.array_equal(got[..., :3], lin[..., :3].astype(np.float32))
     else:
