@@ -305,6 +305,27 @@ typedef struct zh_graph_info {
 } zh_graph_info;
 zh_status zh_program_graph_info(zh_program* prog, zh_graph_info* info);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): lower the
+ * High program (as zh_program_build) and launch it once (as
+ * zh_program_launch) with every device table of the plan -- stage
+ * descriptors, tile queues and their counters, tile-layer masks, buffer
+ * registers and intermediates -- placed in the library's static scratch
+ * array of ZH_STATIC_SCRATCH_BYTES per device instead of allocated.  The
+ * call makes no hipMalloc, hipFree or other allocating call; inputs and
+ * outputs are the caller's images (zh_image_wrap adopts existing device
+ * memory).  It waits for the work already queued on the context's stream,
+ * uploads the tables, and launches the kernels asynchronously, without a
+ * hipGraph.  A program whose tables exceed the scratch fails with
+ * ZH_ERR_OUT_OF_MEMORY and launches nothing.  Every call on a device shares
+ * its scratch, which keeps the last call's tables: calls through one context
+ * are ordered by its stream, but calls that overlap on one GPU through
+ * different contexts or streams race. */
+#define ZH_STATIC_SCRATCH_BYTES (32u << 20)
+zh_status zh_program_launch_static(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures,
+                                   const zh_op* ops, uint32_t num_ops, uint32_t flags, const zh_binding* bindings,
+                                   uint32_t num_bindings);
+
 /* Average device time (ms) of each stage over the last `zh_program_time`
  * call: launches the program `iterations` times and records HIP events
  * around every stage on the context stream. */

@@ -540,6 +540,19 @@ class Program:
         _check(self._lib.zs_linked_output_slot(self._h, reg, ctypes.byref(s)))
         return s.value
 
+    def launch_static(self, ctx: "Context", inputs: Dict[int, "Image"], outputs: Dict[int, "Image"],
+                      flags: int = PLAN_DEFAULT):
+        """Lower and launch once without allocating device memory: the plan's
+        device tables go to the library's static scratch
+        (``zh_program_launch_static``; asynchronous, no hipGraph)."""
+        tex, ops = self.high()
+        t = (DescriptorC * max(1, len(tex)))(*tex)
+        o = (_abi.OpC * max(1, len(ops)))(*ops)
+        binds = [BindingC(self.input_slot(r), 0, img._h) for r, img in inputs.items()]
+        binds += [BindingC(self.output_slot(r), 1, img._h) for r, img in outputs.items()]
+        b = (BindingC * max(1, len(binds)))(*binds)
+        _check(self._lib.zh_program_launch_static(ctx._h, t, len(tex), o, len(ops), flags, b, len(binds)))
+
     def lower_to(self, ctx: Optional["Context"] = None, flags: int = PLAN_DEFAULT) -> "Executable":
         """``Program::lower_to`` (program.rs:1304).  ``ctx=None`` plans only."""
         h = ctypes.c_void_p()

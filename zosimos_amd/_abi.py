@@ -88,6 +88,8 @@ _SIGNATURES = {
     "zh_program_launch_bytes": (i32, [vp, ctypes.POINTER(u64), u32, ctypes.POINTER(u32)]),
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
     "zh_program_graph_info": (i32, [vp, ctypes.POINTER(GraphInfoC)]),
+    "zh_program_launch_static": (i32, [vp, ctypes.POINTER(DescriptorC), u32, ctypes.POINTER(OpC), u32, u32,
+                                       ctypes.POINTER(BindingC), u32]),
     "zh_program_time": (i32, [vp, ctypes.POINTER(BindingC), u32, u32, ctypes.POINTER(f32), u32,
                               ctypes.POINTER(u32)]),
     "zh_program_autotune": (i32, [vp, ctypes.POINTER(BindingC), u32, u32]),

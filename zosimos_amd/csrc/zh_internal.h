@@ -81,6 +81,8 @@ struct zh_context {
   void* d_lut = nullptr;         // zh::Luts (8-bit sRGB tables)
   int cus = 256;                 // compute units of the device
   uint32_t* d_error = nullptr;   // kernel error flag
+  void* scratch = nullptr;       // the device's static scratch (zh_program_launch_static)
+  size_t scratch_bytes = 0;
   // Images and programs hold the context; zh_context_destroy of a context
   // with live objects defers the teardown to the release of the last one.
   int refs = 0;
@@ -154,6 +156,13 @@ struct zh_program {
   std::vector<uint8_t> surrogate_ok;
   std::vector<std::vector<float>> surrogate_params;
   int use_surrogates = 1;  // 0 plain kernels, 1 accepted surrogates, 2 every fitted surrogate
+  // A program launched from the static scratch (zh_program_launch_static):
+  // its device tables are carved from `arena` in 256-byte steps instead of
+  // allocated, its host copy of the descriptors lives in h_stage_vec, and
+  // nothing is freed.
+  uint8_t* arena = nullptr;
+  size_t arena_size = 0, arena_used = 0;
+  std::vector<zh::StageDesc> h_stage_vec;
 };
 
 

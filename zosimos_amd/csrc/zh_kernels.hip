@@ -10,7 +10,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
+#include "../../include/zosimos_hip.h"
 #include "zh_device.h"
 
 namespace zh {
@@ -949,6 +951,37 @@ hipError_t zh_launch_copy(const zh::StageDesc* d_dev, uint64_t row_bytes, uint32
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(zh::k_copy, dim3((uint32_t)blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();
+}
+
+// The static scratch of zh_program_launch_static: device tables of a
+// program launched without allocating (descriptors, tile queues and their
+// counters, tile-layer masks, buffer registers, intermediates).  Each device
+// holds its own copy from the library's load to the end of the process.
+static __device__ __align__(256) uint8_t scratch[ZH_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on `device`, which must be current.  In
+// host code the array's name is only a stand-in for the symbol, so its
+// address is resolved once per device here and kept; callers pass this
+// pointer, never the name, to kernels and copies.
+hipError_t zh_scratch_setup(int device, void** ptr, size_t* bytes) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  *bytes = sizeof(scratch);
+  return hipSuccess;
 }
 
 hipError_t zh_set_resample_lds(uint32_t bytes) {

@@ -36,6 +36,7 @@ hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32
                            hipStream_t stream);
 hipError_t zh_launch_copy(const zh::StageDesc* d_dev, uint64_t row_bytes, uint32_t rows, hipStream_t stream);
 hipError_t zh_set_resample_lds(uint32_t bytes);
+hipError_t zh_scratch_setup(int device, void** ptr, size_t* bytes);
 }
 #include "zh_mlp.h"
 
@@ -133,6 +134,9 @@ zh_status zh_context_create(int32_t device, void* stream, zh_context** out) {
   uint32_t zero = 0;
   if (e == hipSuccess) e = hipMemcpy(ctx->d_error, &zero, sizeof(zero), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = zh_set_resample_lds(kTileLdsBytes);
+  // The device address of the static scratch, resolved once per device
+  // with the device current (zh_program_launch_static).
+  if (e == hipSuccess) e = zh_scratch_setup(device, &ctx->scratch, &ctx->scratch_bytes);
   if (e != hipSuccess) {
     zh_context_destroy(ctx);
     return hip_fail(e, "context setup");
@@ -276,6 +280,28 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 }
 
 // ---- programs -----------------------------------------------------------------
+// Device memory of a program's tables: allocated, or carved in 256-byte steps
+// from the static scratch for a program of zh_program_launch_static.
+static zh_status dev_alloc(zh_program* p, size_t bytes, void** out) {
+  if (p->arena) {
+    const size_t at = (p->arena_used + 255) & ~(size_t)255;
+    if (at > p->arena_size || bytes > p->arena_size - at)
+      return fail(ZH_ERR_OUT_OF_MEMORY, "the program's device tables need more than the static scratch of " +
+                                            std::to_string(p->arena_size >> 20) + " MiB (ZH_STATIC_SCRATCH_BYTES)");
+    *out = p->arena + at;
+    p->arena_used = at + bytes;
+    return ZH_OK;
+  }
+  const hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) return fail(ZH_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return ZH_OK;
+}
+#define ZH_ALLOC(p, bytes, ptr)                                      \
+  do {                                                                \
+    zh_status _s = dev_alloc((p), (bytes), (void**)(ptr));            \
+    if (_s != ZH_OK) return _s;                                       \
+  } while (0)
+
 static void drop_surrogates(zh_program* p) {
   for (uint16_t*& b : p->d_surrogate)
     if (b) { hipFree(b); b = nullptr; }
@@ -286,6 +312,18 @@ static void drop_surrogates(zh_program* p) {
 
 static void free_device(zh_program* p) {
   drop_surrogates(p);
+  if (p->arena) {
+    // Static-scratch tables are never freed; only the host side is dropped.
+    p->units.clear();
+    p->d_intermediates.clear();
+    p->buffer_images.clear();
+    p->d_tile_layers.clear();
+    p->tile_layer_words.clear();
+    p->d_stages = nullptr;
+    p->h_stages = nullptr;
+    p->h_stage_vec.clear();
+    return;
+  }
   if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
   if (p->graph) hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;
@@ -418,10 +456,10 @@ static zh_status tile_layer_masks(zh_program* p, size_t s) {
   // Launches still in flight may read the old masks.
   ZH_HIP(hipStreamSynchronize(p->ctx->stream));
   if (p->tile_layer_words[s] < m.size()) {
-    if (p->d_tile_layers[s]) hipFree(p->d_tile_layers[s]);
+    if (p->d_tile_layers[s] && !p->arena) hipFree(p->d_tile_layers[s]);
     p->d_tile_layers[s] = nullptr;
     p->tile_layer_words[s] = 0;
-    ZH_HIP(hipMalloc(&p->d_tile_layers[s], m.size() * sizeof(uint32_t)));
+    ZH_ALLOC(p, m.size() * sizeof(uint32_t), &p->d_tile_layers[s]);
     p->tile_layer_words[s] = (uint32_t)m.size();
   }
   ZH_HIP(hipMemcpy(p->d_tile_layers[s], m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -445,16 +483,21 @@ static zh_status alloc_device(zh_program* p) {
   zh_context* ctx = p->ctx;
   ZH_HIP(hipSetDevice(ctx->device));
   const size_t n = p->plan.stages.size();
-  ZH_HIP(hipMalloc(&p->d_stages, std::max<size_t>(1, n) * sizeof(StageDesc)));
-  ZH_HIP(hipHostMalloc(&p->h_stages, std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
-  for (int k = 0; k < zh_program::kRing; ++k) {
-    ZH_HIP(hipHostMalloc(&p->h_ring[k], std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
-    ZH_HIP(hipEventCreateWithFlags(&p->ring_ev[k], hipEventDisableTiming));
+  ZH_ALLOC(p, std::max<size_t>(1, n) * sizeof(StageDesc), &p->d_stages);
+  if (p->arena) {
+    // Host memory of the caller's process heap; uploads are synchronous.
+    p->h_stage_vec.assign(std::max<size_t>(1, n), StageDesc{});
+    p->h_stages = p->h_stage_vec.data();
+  } else {
+    ZH_HIP(hipHostMalloc(&p->h_stages, std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
+    for (int k = 0; k < zh_program::kRing; ++k) {
+      ZH_HIP(hipHostMalloc(&p->h_ring[k], std::max<size_t>(1, n) * sizeof(StageDesc), hipHostMallocDefault));
+      ZH_HIP(hipEventCreateWithFlags(&p->ring_ev[k], hipEventDisableTiming));
+    }
   }
   for (const zh_descriptor& d : p->plan.intermediates) {
     void* b = nullptr;
-    hipError_t e = hipMalloc(&b, image_bytes(d));
-    if (e != hipSuccess) return fail(ZH_ERR_OUT_OF_MEMORY, "intermediate buffer allocation failed");
+    ZH_ALLOC(p, image_bytes(d), &b);
     p->d_intermediates.push_back(b);
   }
   make_units(p);
@@ -467,7 +510,7 @@ static zh_status alloc_device(zh_program* p) {
     b.ctx = ctx;
     b.desc = p->plan.input_desc[p->plan.user_inputs + k];
     b.bytes = image_bytes(b.desc);
-    ZH_HIP(hipMalloc(&b.ptr, b.bytes));
+    ZH_ALLOC(p, b.bytes, &b.ptr);
     p->buffer_images.push_back(b);
   }
   {
@@ -484,10 +527,10 @@ static zh_status alloc_device(zh_program* p) {
       prefix[k + 1] = prefix[k] + (u.spec ? (uint32_t)chunks : d.tiles_x * d.tiles_y);
       u.lds = std::max(u.lds, p->plan.stages[u.ids[k]].lds_bytes);
     }
-    ZH_HIP(hipMalloc(&u.d_batch, sizeof(BatchDesc)));
-    ZH_HIP(hipMalloc(&u.d_counter, sizeof(uint32_t)));
-    ZH_HIP(hipMalloc(&u.d_prefix, prefix.size() * sizeof(uint32_t)));
-    ZH_HIP(hipMalloc(&u.d_ids, u.ids.size() * sizeof(uint32_t)));
+    ZH_ALLOC(p, sizeof(BatchDesc), &u.d_batch);
+    ZH_ALLOC(p, sizeof(uint32_t), &u.d_counter);
+    ZH_ALLOC(p, prefix.size() * sizeof(uint32_t), &u.d_prefix);
+    ZH_ALLOC(p, u.ids.size() * sizeof(uint32_t), &u.d_ids);
     ZH_HIP(hipMemcpy(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     ZH_HIP(hipMemcpy(u.d_ids, u.ids.data(), u.ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     b.n_stages = (uint32_t)u.ids.size();
@@ -508,10 +551,9 @@ static zh_status alloc_device(zh_program* p) {
   return ZH_OK;
 }
 
-zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures, const zh_op* ops,
-                           uint32_t num_ops, uint32_t flags, zh_program** out) {
-  if (!out || (num_textures && !textures) || (num_ops && !ops)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
-  zh_program* p = new zh_program();
+// The program's own copy of the High list (parameter blobs included) and its plan.
+static zh_status init_program(zh_program* p, zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures,
+                              const zh_op* ops, uint32_t num_ops, uint32_t flags) {
   p->ctx = ctx;
   p->flags = flags;
   p->textures.assign(textures, textures + num_textures);
@@ -519,13 +561,20 @@ zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint3
   p->params.resize(num_ops);
   for (uint32_t i = 0; i < num_ops; ++i) {
     if (ops[i].params_len) {
-      if (!ops[i].params) { delete p; return fail(ZH_ERR_INVALID_ARGUMENT, "params pointer is null"); }
+      if (!ops[i].params) return fail(ZH_ERR_INVALID_ARGUMENT, "params pointer is null");
       const uint8_t* b = static_cast<const uint8_t*>(ops[i].params);
       p->params[i].assign(b, b + ops[i].params_len);
     }
     p->ops[i].params = nullptr;
   }
-  zh_status st = plan_program(p->textures, p->ops, p->params, flags, &p->plan);
+  return plan_program(p->textures, p->ops, p->params, flags, &p->plan);
+}
+
+zh_status zh_program_build(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures, const zh_op* ops,
+                           uint32_t num_ops, uint32_t flags, zh_program** out) {
+  if (!out || (num_textures && !textures) || (num_ops && !ops)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zh_program* p = new zh_program();
+  zh_status st = init_program(p, ctx, textures, num_textures, ops, num_ops, flags);
   if (st != ZH_OK) { delete p; return st; }
   if (ctx) {
     if (ctx->closing) { delete p; return fail(ZH_ERR_INVALID_ARGUMENT, "context is being destroyed"); }
@@ -779,7 +828,10 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
     p->graph = nullptr;
     p->variant = variant;
   }
-  {
+  if (p->arena) {
+    // Static-scratch programs upload synchronously (the stream is idle).
+    ZH_HIP(hipMemcpy(p->d_stages, p->h_stages, plan.stages.size() * sizeof(StageDesc), hipMemcpyHostToDevice));
+  } else {
     // Stage through the next ring copy; its previous upload must have left
     // the stream before the copy is overwritten.
     const int k = p->ring_next;
@@ -916,6 +968,33 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
     return ZH_OK;
   }
   return launch_all(p);
+}
+
+zh_status zh_program_launch_static(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures,
+                                   const zh_op* ops, uint32_t num_ops, uint32_t flags, const zh_binding* bindings,
+                                   uint32_t num_bindings) {
+  if (!ctx || (num_textures && !textures) || (num_ops && !ops) || (num_bindings && !bindings))
+    return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->closing) return fail(ZH_ERR_INVALID_ARGUMENT, "context is being destroyed");
+  if (!ctx->scratch || ctx->scratch_bytes == 0)
+    return fail(ZH_ERR_HIP, "the device's static scratch was not resolved at context creation");
+  ZH_HIP(hipSetDevice(ctx->device));
+  zh_program prog;
+  zh_program* p = &prog;
+  // No hipGraph: instantiating one allocates.
+  zh_status st = init_program(p, ctx, textures, num_textures, ops, num_ops, flags | ZH_PLAN_NO_GRAPH);
+  if (st != ZH_OK) return st;
+  // An earlier call's kernels on this stream may still read the scratch:
+  // let them finish before its tables are overwritten.
+  ZH_HIP(hipStreamSynchronize(ctx->stream));
+  p->arena = static_cast<uint8_t*>(ctx->scratch);
+  p->arena_size = ctx->scratch_bytes;
+  p->arena_used = 0;
+  st = alloc_device(p);
+  if (st == ZH_OK) st = bind(p, bindings, num_bindings);
+  if (st == ZH_OK) st = launch_all(p);
+  free_device(p);  // host side only: the tables stay in the scratch for the queued kernels
+  return st;
 }
 
 zh_status zh_program_graph_info(zh_program* p, zh_graph_info* info) {
