@@ -125,7 +125,7 @@ def test_plan_c3_batches_into_one_tile_queue_per_form():
     tile-queue launch per form, every stage in exactly one batch."""
     e = plan(W.c3(12, max_side=300))
     text = e.describe()
-    forms = set(re.findall(r"\[kernel (\w+)\]", text))
+    forms = set(re.findall(r"^stage .*\[kernel (\w+)", text, re.M))
     batches = re.findall(r"batch: stages ([\d,]+) as one tile queue", text)
     ids = sorted(int(i) for b in batches for i in b.split(","))
     assert ids == list(range(12)) and len(batches) == len(forms) <= 2
@@ -314,7 +314,7 @@ def test_cube_root_sign_handling_only_where_needed():
         cmd = z.CommandBuffer()
         lab = cmd.color_convert(cmd.input(desc), z.Color.OKLAB, z.Texel(z.SampleBits.Float16x4, z.SampleParts.LabA))
         cmd.output(lab)
-        return re.findall(r"\[kernel (\w+)\]", plan(cmd).describe())
+        return re.findall(r"^stage .*\[kernel (\w+)", plan(cmd).describe(), re.M)
     assert kernel(rgba8(64, 8)) == ["oklab_enc_s8hp"]
     assert kernel(z.Descriptor.rgba16f_linear(64, 8)) != ["oklab_enc_s8hp"]
 

@@ -263,7 +263,15 @@ zh_status zh_image_upload(zh_image* img, const void* host, uint64_t host_row_str
   if (host_row_stride == 0) host_row_stride = row;
   if (host_row_stride < row) return fail(ZH_ERR_INVALID_ARGUMENT, "host row stride shorter than a row");
   ZH_HIP(hipSetDevice(img->ctx->device));
-  ZH_HIP(hipMemcpy2DAsync(img->ptr, row_stride_of(img->desc), host, host_row_stride, row, img->desc.height,
+  const uint64_t dev_stride = row_stride_of(img->desc);
+  if (host_row_stride == dev_stride) {
+    // Same pitch on both sides: one linear copy (a pitched copy from
+    // pageable memory moves row by row).
+    const uint64_t n = dev_stride * (img->desc.height - 1) + row;
+    ZH_HIP(hipMemcpyAsync(img->ptr, host, n, hipMemcpyHostToDevice, img->ctx->stream));
+    return ZH_OK;
+  }
+  ZH_HIP(hipMemcpy2DAsync(img->ptr, dev_stride, host, host_row_stride, row, img->desc.height,
                           hipMemcpyHostToDevice, img->ctx->stream));
   return ZH_OK;
 }
@@ -274,7 +282,13 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
   if (host_row_stride == 0) host_row_stride = row;
   if (host_row_stride < row) return fail(ZH_ERR_INVALID_ARGUMENT, "host row stride shorter than a row");
   ZH_HIP(hipSetDevice(img->ctx->device));
-  ZH_HIP(hipMemcpy2DAsync(host, host_row_stride, img->ptr, row_stride_of(img->desc), row, img->desc.height,
+  const uint64_t dev_stride = row_stride_of(img->desc);
+  if (host_row_stride == dev_stride) {
+    const uint64_t n = dev_stride * (img->desc.height - 1) + row;
+    ZH_HIP(hipMemcpyAsync(host, img->ptr, n, hipMemcpyDeviceToHost, img->ctx->stream));
+    return ZH_OK;
+  }
+  ZH_HIP(hipMemcpy2DAsync(host, host_row_stride, img->ptr, dev_stride, row, img->desc.height,
                           hipMemcpyDeviceToHost, img->ctx->stream));
   return ZH_OK;
 }
@@ -618,6 +632,11 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
   std::ostringstream o;
   for (size_t i = 0; i < p->plan.stages.size(); ++i) o << "stage " << i << ": " << p->plan.stages[i].text << "\n";
   for (size_t i = 0; i < p->units.size(); ++i) {
+    const Unit& u = p->units[i];
+    const std::string& t = p->plan.stages[u.ids[0]].text;
+    const size_t k = t.rfind("[kernel ");
+    o << "launch " << i << ": " << (k == std::string::npos ? std::string("stage ") + std::to_string(u.ids[0]) : t.substr(k))
+      << " over " << u.ids.size() << (u.ids.size() == 1 ? " stage" : " stages") << "\n";
     if (p->units[i].batch) {
       o << "batch: stages ";
       for (size_t k = 0; k < p->units[i].ids.size(); ++k) o << (k ? "," : "") << p->units[i].ids[k];
