@@ -19,6 +19,7 @@ extern "C" {
 
 typedef struct zs_command zs_command; /* CommandBuffer, command.rs:60-68 */
 typedef struct zs_linked zs_linked;   /* linked Program, program.rs:24-39 */
+typedef struct zs_signature zs_signature; /* CommandSignature, command.rs:819-849 */
 
 /* AffineSample, command.rs:313-327 */
 #define ZS_AFFINE_NEAREST 0
@@ -169,13 +170,14 @@ zh_status zs_command_buffer_size(zs_command* cmd, uint32_t buffer, uint64_t* len
  * ZH_ERR_INVALID_ARGUMENT (CommandError::INVALID_CALL) when the buffer is
  * shorter than height padded rows.  The register's own descriptor is tight. */
 zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_descriptor* desc, uint32_t* reg);
-/* CommandBuffer::function + invoke, command.rs:856-920, folded into the
- * caller (the reference links the function as its own High function,
- * command.rs:2083-2150; inlining puts its stages into the caller's fused
- * plan and graph).  The inputs of `body`, in declaration order, take the
- * registers `args`, whose descriptors must equal them (ZH_ERR_TYPE
- * otherwise); its outputs, in order, come back in `results`.  Knobs inside
- * `body` are not carried over.  Generic inputs are not supported. */
+/* A call of `body` folded into the caller at once (the shortcut of
+ * zs_command_function + zs_command_invoke_function + zs_link for a body
+ * without generics or calls of its own).  The inputs of `body`, in
+ * declaration order, take the registers `args`, whose descriptors (size,
+ * texel, colour) must equal them (ZH_ERR_TYPE otherwise); its outputs, in
+ * order, come back in `results`.  Knobs inside `body` are not carried over
+ * (zs_link carries them); a generic body or one that calls functions is
+ * ZH_ERR_TYPE. */
 zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint32_t* args, uint32_t num_args,
                             uint32_t* results, uint32_t num_results);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
@@ -186,8 +188,55 @@ zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);
 /* Descriptor of a register (RegisterDescription::Texture). */
 zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc);
 
-/* Linker::compile, command.rs:2069-2071 and link_in's High emission. */
+/* ---- generics and functions (command.rs:753-923) ---------------------------
+ * A generic parameter stands for a whole descriptor (size and chroma);
+ * descriptor variables name generic or derived descriptors. */
+/* CommandBuffer::generic (command.rs:772-791): a new generic parameter;
+ * `var` is the descriptor variable standing for it. */
+zh_status zs_command_generic(zs_command* cmd, uint32_t* var);
+/* CommandBuffer::input_generic (command.rs:756-762). */
+zh_status zs_command_input_generic(zs_command* cmd, uint32_t var, uint32_t* reg);
+/* CommandBuffer::register_descriptor (command.rs:811-821): a variable
+ * describing a register (generic or not). */
+zh_status zs_command_register_descriptor(zs_command* cmd, uint32_t reg, uint32_t* var);
+/* CommandBuffer::derive_descriptor (command.rs:794-808): `base` with its
+ * size replaced by size[0] x size[1], or unchanged when `size` is NULL. */
+zh_status zs_command_derive_descriptor(zs_command* cmd, uint32_t base, const uint32_t* size, uint32_t* var);
+/* CommandBuffer::computed_signature (command.rs:824-852). */
+zh_status zs_command_signature(const zs_command* body, zs_signature** out);
+zh_status zs_signature_destroy(zs_signature* sig);
+/* Generic parameters, inputs and outputs of a signature. */
+zh_status zs_signature_counts(const zs_signature* sig, uint32_t* vars, uint32_t* inputs, uint32_t* outputs);
+/* CommandBuffer::function (command.rs:855-860): declare a function symbol
+ * of that signature, defined at link time. */
+zh_status zs_command_function(zs_command* cmd, const zs_signature* sig, uint32_t* fn);
+/* CommandBuffer::invoke (command.rs:863-923): call symbol `fn` with the
+ * descriptor variables `generics` (one per generic parameter) and the
+ * registers `args`, each equal to its parameter rewritten with the generics
+ * (ZH_ERR_OTHER, CommandError::INVALID_CALL, otherwise); `results` receive
+ * the result registers. */
+zh_status zs_command_invoke_function(zs_command* cmd, uint32_t fn, const uint32_t* generics, uint32_t num_generics,
+                                     const uint32_t* args, uint32_t num_args, uint32_t* results, uint32_t num_results);
+
+/* Linker::compile, command.rs:2069-2071 and link_in's High emission.  A
+ * buffer with unlinked function calls fails (ZH_ERR_OTHER). */
 zh_status zs_command_compile(zs_command* cmd, zs_linked** out);
+/* Linker::link (command.rs:2083-2185): `main` (buffer 0) instantiated with
+ * the descriptors `tys` (one per generic parameter of main), and the function
+ * buffers `functions` (buffers 1 ..).  For buffer b, link_counts[b] entries
+ * of link_targets (consecutive, buffer by buffer) name the buffer defining
+ * each of its function symbols in declaration order (counts of 0 for a
+ * buffer without symbols; link_counts may be NULL when no buffer calls).
+ * Every call is monomorphized at its generic arguments and folded into the
+ * caller, so the High list has no calls; the definition must have the
+ * declared signature. */
+zh_status zs_link(const zs_command* main, const zh_descriptor* tys, uint32_t num_tys, const zs_command* const* functions,
+                  uint32_t num_functions, const uint32_t* link_counts, const uint32_t* link_targets, zs_linked** out);
+/* Executable::query_knob (run.rs:1012-1014) with RegisterKnob {link_idx,
+ * register} (command.rs:2128-2140): the knob id of a knob register of buffer
+ * `link_idx` (0 = main) in the linked program (the last instantiation's,
+ * for a function called more than once). */
+zh_status zs_linked_knob(zs_linked* prog, uint32_t link_idx, uint32_t reg, int32_t* knob);
 zh_status zs_linked_destroy(zs_linked* prog);
 /* Borrow the High list (valid until zs_linked_destroy). */
 zh_status zs_linked_high(zs_linked* prog, const zh_descriptor** textures, uint32_t* num_textures, const zh_op** ops,

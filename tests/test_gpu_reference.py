@@ -441,3 +441,51 @@ def test_derivative_matches_reference_hash(ctx, method, name):
     assert d.max() <= 1
     check(f"derived_{name}", out)
 
+
+
+def test_generic_palette_reference_program(ctx):
+    """tests/generic.rs: a generic function -- its input's descriptor is a
+    generic parameter -- samples its input as a palette at a 2048x2048 ramp
+    of u v (``uv_max = (1, 1, 0, 0)`` into an RGBA8 Scalars texel, whose
+    staged store truncates); the main buffer instantiates it with the
+    background's descriptor and the linker monomorphizes the call
+    (``Linker::link(&main, &[], &[f], &[&[1], &[]])``).  The output equals
+    the oracle byte for byte (the ramp in IEEE f32, as the device computes
+    it) and reproduces a blessed ``generic`` hash exactly."""
+    bg = rgba("background")
+    h, w = bg.shape[:2]
+    f = z.CommandBuffer()
+    a = f.generic(z.GenericDeclaration())
+    img_in = f.input_generic(a)
+    ramp_d = z.Descriptor(2048, 2048, z.Texel(z.SampleBits.UInt8x4, z.SampleParts.RgbA),
+                          z.Color(model=z.ColorModel.Scalars, transfer=z.Transfer.Linear))
+    idx = f.bilinear(ramp_d, z.Bilinear(uv_max=(1.0, 1.0, 0.0, 0.0)))
+    f.output(f.palette(img_in, z.Palette(height=z.ColorChannel.R, width=z.ColorChannel.G), idx))
+    main = z.CommandBuffer()
+    conv = main.function(f.computed_signature())
+    d = z.Descriptor.rgba8_srgb(w, h)
+    i = main.input(d)
+    (res,) = main.invoke(conv, z.InvocationArguments(generics=(main.register_descriptor(i),), arguments=(i,)))
+    o, od = main.output(res)
+    assert (od.width, od.height) == (2048, 2048)
+    exe = z.Linker().link(main, [], [f], [[1], []]).lower_to(ctx)
+    img = ctx.image(d)
+    img.upload(bg.reshape(-1))
+    out_img = ctx.image(od)
+    exe.launch({i: img}, {o: out_img})
+    ctx.sync()
+    out = out_img.download().reshape(2048, 2048, 4)
+    f32 = np.float32
+    u = (np.arange(2048, dtype=f32) + f32(0.5)) / f32(2048)
+    uv = (u[:, None] * u[None, :]).astype(f32)
+    code = (uv * f32(255.0)).astype(np.uint32).astype(f32) / f32(255.0)
+    ramp = np.zeros((2048, 2048, 4), f32)
+    ramp[..., 0] = code
+    ramp[..., 1] = code
+    want = PL.palette_lookup(bg, ramp, [0, 1, 0, 0], [1, 0, 0, 0])
+    diff = int(np.any(out != want, axis=-1).sum())
+    got = B.blockhash256(out)
+    dist = min(bin(int(got, 16) ^ int(b, 16)).count("1") for b in HASHES["generic"])
+    report("generic_program", differing=diff, hamming=dist)
+    assert diff == 0
+    assert got in HASHES["generic"]

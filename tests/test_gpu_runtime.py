@@ -295,3 +295,36 @@ def test_caller_built_knob_length(ctx):
     ok = src[..., 3] > 0
     assert d[ok].max() <= 1
     lib.zh_program_destroy(p)
+
+
+def test_knob_inside_linked_function(ctx):
+    """A knob on a conversion inside a function body survives the link
+    (RegisterKnob {link_idx: 1, register}, command.rs:2128-2140): setting it
+    to the identity matrix turns the conversion into a byte copy."""
+    w, h = 64, 32
+    d = z.Descriptor.rgba8_srgb(w, h)
+    body = z.CommandBuffer()
+    x = body.input(d)
+    c = body.color_convert(x, z.Color(primaries=z.Primaries.DisplayP3), d.texel)
+    body.knob(c)
+    body.output(c)
+    main = z.CommandBuffer()
+    fn = main.function(body.computed_signature())
+    i = main.input(d)
+    (r,) = main.invoke(fn, z.InvocationArguments(arguments=(i,)))
+    o, od = main.output(r)
+    prog = z.Linker().link(main, [], [body], [[1], []])
+    k = prog.query_knob(1, c)
+    exe = prog.lower_to(ctx)
+    data = rand_u8((h, w * 4), 61)
+    img = ctx.image(d)
+    img.upload(data)
+    out = ctx.image(od)
+    exe.launch({i: img}, {o: out})
+    ctx.sync()
+    converted = out.download()
+    assert not np.array_equal(converted, data)
+    exe.knob(k, _std140_mat3(np.eye(3)).raw)
+    exe.launch({i: img}, {o: out})
+    ctx.sync()
+    assert np.array_equal(out.download(), data)

@@ -593,3 +593,29 @@ def test_channel_swap_reproduces_blessed_hash():
     hashes = json.load(open(os.path.join(here, "reference_hashes.json")))
     bg = np.asarray(Image.open(os.path.join(here, "background.png")).convert("RGBA"))
     assert B.blockhash256(bg[..., [1, 0, 2, 3]]) in hashes["swapped"]
+
+
+def test_generic_palette_reproduces_blessed_hash():
+    """tests/generic.rs: the background sampled as a palette at the 2048x2048
+    ramp u v stored in an RGBA8 Scalars texel (truncating staged store; the
+    ramp in IEEE f32 as bilinear.frag computes it), height from R, width
+    from G.  The oracle's image hashes to a blessed line exactly; rounding the
+    store instead of truncating it does not."""
+    import json
+    import os
+    from PIL import Image
+    from oracle import blockhash as B
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    blessed = json.load(open(os.path.join(golden, "reference_hashes.json")))["generic"]
+    bg = np.asarray(Image.open(os.path.join(golden, "background.png")).convert("RGBA"))
+    f32 = np.float32
+    u = (np.arange(2048, dtype=f32) + f32(0.5)) / f32(2048)
+    uv = (u[:, None] * u[None, :]).astype(f32)
+
+    def image(codes):
+        ramp = np.zeros((2048, 2048, 4), f32)
+        ramp[..., 0] = ramp[..., 1] = codes.astype(f32) / f32(255.0)
+        return PL.palette_lookup(bg, ramp, [0, 1, 0, 0], [1, 0, 0, 0])
+
+    assert B.blockhash256(image((uv * f32(255.0)).astype(np.uint32))) in blessed
+    assert B.blockhash256(image(np.rint(uv * f32(255.0)).astype(np.uint32))) not in blessed

@@ -299,14 +299,63 @@ class CommandBuffer:
         p = (ctypes.c_float * 8)(*distribution.params())
         return self._reg(self._lib.zs_command_distribution_normal2d, ctypes.byref(desc.c()), p)
 
-    def invoke(self, body: "CommandBuffer", args: Sequence[int], num_results: int = 1) -> List[int]:
-        """CommandBuffer::function + invoke (command.rs:856-920), folded
-        into this buffer: ``body``'s inputs take ``args`` in order; returns
-        the registers of its outputs, in order."""
+    def invoke(self, function, args=None, num_results: int = 1) -> List[int]:
+        """Call a function.
+
+        ``invoke(fn, InvocationArguments(generics, arguments))``: the
+        reference's ``CommandBuffer::invoke`` (command.rs:863-923) of a
+        declared ``FunctionVar``, defined at ``Linker.link``; returns the
+        result registers.  ``invoke(body, args)``: ``body`` (a command
+        buffer without generics or calls) folded into this one at once."""
+        if isinstance(function, FunctionVar):
+            inv = args if isinstance(args, InvocationArguments) else InvocationArguments(arguments=args or ())
+            g = (ctypes.c_uint32 * max(1, len(inv.generics)))(*inv.generics)
+            a = (ctypes.c_uint32 * max(1, len(inv.arguments)))(*inv.arguments)
+            out = (ctypes.c_uint32 * max(1, function.outputs))()
+            _check(self._lib.zs_command_invoke_function(self._h, function.index, g, len(inv.generics), a,
+                                                        len(inv.arguments), out, function.outputs))
+            return list(out)[:function.outputs]
+        body = function
         a = (ctypes.c_uint32 * max(1, len(args)))(*args)
         out = (ctypes.c_uint32 * max(1, num_results))()
         _check(self._lib.zs_command_invoke(self._h, body._h, a, len(args), out, num_results))
         return list(out)[:num_results]
+
+    def generic(self, declaration: Optional["GenericDeclaration"] = None) -> int:
+        """CommandBuffer::generic (command.rs:772-791): a generic descriptor
+        parameter; returns its descriptor variable."""
+        v = ctypes.c_uint32()
+        _check(self._lib.zs_command_generic(self._h, ctypes.byref(v)))
+        return v.value
+
+    def input_generic(self, var: int) -> int:
+        """CommandBuffer::input_generic (command.rs:756-762)."""
+        return self._reg(self._lib.zs_command_input_generic, var)
+
+    def register_descriptor(self, reg: int) -> int:
+        """CommandBuffer::register_descriptor (command.rs:811-821)."""
+        v = ctypes.c_uint32()
+        _check(self._lib.zs_command_register_descriptor(self._h, reg, ctypes.byref(v)))
+        return v.value
+
+    def derive_descriptor(self, base: int, size: Optional[Tuple[int, int]] = None) -> int:
+        """CommandBuffer::derive_descriptor (command.rs:794-808)."""
+        v = ctypes.c_uint32()
+        sz = (ctypes.c_uint32 * 2)(*size) if size is not None else None
+        _check(self._lib.zs_command_derive_descriptor(self._h, base, sz, ctypes.byref(v)))
+        return v.value
+
+    def computed_signature(self) -> "CommandSignature":
+        """CommandBuffer::computed_signature (command.rs:824-852)."""
+        h = ctypes.c_void_p()
+        _check(self._lib.zs_command_signature(self._h, ctypes.byref(h)))
+        return CommandSignature(h)
+
+    def function(self, signature: "CommandSignature") -> "FunctionVar":
+        """CommandBuffer::function (command.rs:855-860)."""
+        v = ctypes.c_uint32()
+        _check(self._lib.zs_command_function(self._h, signature._h, ctypes.byref(v)))
+        return FunctionVar(v.value, signature.vars, signature.inputs, signature.outputs)
 
     def resize(self, below: int, width: int, height: int) -> int:
         """CommandBuffer::resize (command.rs:1675-1705): a bilinear coordinate
@@ -378,6 +427,44 @@ class CommandBuffer:
         d = DescriptorC()
         _check(self._lib.zs_command_describe(self._h, reg, ctypes.byref(d)))
         return Descriptor.from_c(d)
+
+
+@dataclass(frozen=True)
+class GenericDeclaration:
+    """command.rs:762-770: bounds of a generic parameter (none exist yet)."""
+    bounds: Tuple = ()
+
+
+@dataclass(frozen=True)
+class InvocationArguments:
+    """command.rs:703-710: generic arguments (descriptor variables) and
+    argument registers of a call."""
+    generics: Tuple[int, ...] = ()
+    arguments: Tuple[int, ...] = ()
+
+
+@dataclass(frozen=True)
+class FunctionVar:
+    index: int
+    vars: int
+    inputs: int
+    outputs: int
+
+
+class CommandSignature:
+    """CommandSignature (command.rs:680-700) of a command buffer."""
+
+    def __init__(self, handle):
+        self._lib = _abi.lib()
+        self._h = handle
+        v, i, o = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.zs_signature_counts(handle, ctypes.byref(v), ctypes.byref(i), ctypes.byref(o)))
+        self.vars, self.inputs, self.outputs = v.value, i.value, o.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zs_signature_destroy(self._h)
+            self._h = None
 
 
 @dataclass
@@ -503,6 +590,24 @@ class Linker:
         _check(_abi.lib().zs_command_compile(commands._h, ctypes.byref(h)))
         return Program(h)
 
+    def link(self, main: CommandBuffer, tys: Sequence[Descriptor], functions: Sequence[CommandBuffer],
+             links: Sequence[Sequence[int]]) -> "Program":
+        """Linker::link (command.rs:2083-2185): ``main`` instantiated with the
+        descriptors ``tys``; ``links[b][k]`` names the buffer (0 = main, 1 +
+        i = functions[i]) defining function symbol k of buffer b."""
+        n = 1 + len(functions)
+        t = (DescriptorC * max(1, len(tys)))(*[d.c() for d in tys])
+        f = (ctypes.c_void_p * max(1, len(functions)))(*[fb._h.value for fb in functions])
+        counts = [len(links[b]) if b < len(links) else 0 for b in range(n)]
+        targets = [int(x) for b in range(n) for x in (links[b] if b < len(links) else [])]
+        c = (ctypes.c_uint32 * n)(*counts)
+        g = (ctypes.c_uint32 * max(1, len(targets)))(*targets)
+        h = ctypes.c_void_p()
+        _check(_abi.lib().zs_link(main._h, t, len(tys), f, len(functions), c, g, ctypes.byref(h)))
+        prog = Program(h)
+        prog._keep = (main, list(functions))
+        return prog
+
 
 # Plan flags (zosimos_hip.h)
 PLAN_DEFAULT = 0
@@ -539,6 +644,14 @@ class Program:
         s = ctypes.c_uint32()
         _check(self._lib.zs_linked_output_slot(self._h, reg, ctypes.byref(s)))
         return s.value
+
+    def query_knob(self, link_idx: int, register: int) -> int:
+        """Executable::query_knob(RegisterKnob {link_idx, register})
+        (run.rs:1012-1014): the knob id of a knob register of buffer
+        ``link_idx`` (0 = the main buffer) in this linked program."""
+        k = ctypes.c_int32()
+        _check(self._lib.zs_linked_knob(self._h, link_idx, register, ctypes.byref(k)))
+        return k.value
 
     def launch_static(self, ctx: "Context", inputs: Dict[int, "Image"], outputs: Dict[int, "Image"],
                       flags: int = PLAN_DEFAULT):

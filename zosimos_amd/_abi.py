@@ -139,7 +139,20 @@ _SIGNATURES = {
     "zs_command_from_buffer": (i32, [vp, u32, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
     "zs_command_knob": (i32, [vp, u32, ctypes.POINTER(i32)]),
     "zs_command_describe": (i32, [vp, u32, ctypes.POINTER(DescriptorC)]),
+    "zs_command_generic": (i32, [vp, ctypes.POINTER(u32)]),
+    "zs_command_input_generic": (i32, [vp, u32, ctypes.POINTER(u32)]),
+    "zs_command_register_descriptor": (i32, [vp, u32, ctypes.POINTER(u32)]),
+    "zs_command_derive_descriptor": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "zs_command_signature": (i32, [vp, ctypes.POINTER(vp)]),
+    "zs_signature_destroy": (i32, [vp]),
+    "zs_signature_counts": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "zs_command_function": (i32, [vp, vp, ctypes.POINTER(u32)]),
+    "zs_command_invoke_function": (i32, [vp, u32, ctypes.POINTER(u32), u32, ctypes.POINTER(u32), u32,
+                                         ctypes.POINTER(u32), u32]),
     "zs_command_compile": (i32, [vp, ctypes.POINTER(vp)]),
+    "zs_link": (i32, [vp, ctypes.POINTER(DescriptorC), u32, ctypes.POINTER(vp), u32, ctypes.POINTER(u32),
+                      ctypes.POINTER(u32), ctypes.POINTER(vp)]),
+    "zs_linked_knob": (i32, [vp, u32, u32, ctypes.POINTER(i32)]),
     "zs_linked_destroy": (i32, [vp]),
     "zs_linked_high": (i32, [vp, ctypes.POINTER(ctypes.POINTER(DescriptorC)), ctypes.POINTER(u32),
                              ctypes.POINTER(ctypes.POINTER(OpC)), ctypes.POINTER(u32)]),

@@ -18,7 +18,23 @@ namespace {
 
 enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
-  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE, K_EXTRACT, K_INJECT
+  K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE, K_EXTRACT, K_INJECT,
+  K_RESULT,  // Op::InvokedResult (command.rs:911-916): a result register of the Invoke that follows
+  K_INVOKE   // Op::Invoke (command.rs:918-923)
+};
+
+// GenericDescriptor (command.rs:89-92): size and chroma (texel + colour),
+// each concrete or a generic parameter of the command buffer.
+struct GenDesc {
+  int32_t size_var = -1;    // -1: desc.width / desc.height
+  int32_t chroma_var = -1;  // -1: desc.texel / desc.color
+  zh_descriptor desc{};     // the concrete parts (placeholders where generic)
+};
+
+// CommandSignature (command.rs:819-849): generic parameters, inputs, outputs.
+struct Signature {
+  uint32_t vars = 0;
+  std::vector<GenDesc> inputs, outputs;
 };
 
 struct CmdOp {
@@ -31,6 +47,12 @@ struct CmdOp {
   float matrix[9] = {0};
   uint32_t sampling = 0;
   int32_t knob = -1;
+  int32_t gsize = -1, gchroma = -1;  // generic size / chroma of the register (GenDesc)
+  // K_INVOKE: the function symbol, its arguments, result registers and the
+  // generic arguments.
+  uint32_t fn = 0;
+  std::vector<uint32_t> args, results;
+  std::vector<GenDesc> generics;
 };
 
 // CIE 1931 2-degree chromaticities of the white points A .. F11 (the order of
@@ -130,21 +152,81 @@ bool zh::primaries_to_xyz(uint32_t primaries, uint32_t whitepoint, Mat3* out) {
 struct zs_command {
   std::vector<CmdOp> ops;
   int32_t next_knob = 0;
+  uint32_t vars = 0;                // generic parameters declared (command.rs:772-791)
+  std::vector<GenDesc> tys;         // DescriptorVar -> descriptor
+  std::vector<Signature> symbols;   // FunctionVar -> declared signature
+};
+
+struct zs_signature {
+  Signature sig;
 };
 
 struct zs_linked {
   std::vector<zh_descriptor> textures;
   std::vector<zh_op> ops;
   std::vector<std::vector<uint8_t>> blobs;
-  std::vector<int64_t> input_slot, output_slot;  // per register
+  std::vector<int64_t> input_slot, output_slot;  // per register of the entry buffer
+  // RegisterKnob {link_idx, register} -> knob (command.rs:2128-2140).
+  std::vector<std::array<int64_t, 3>> knobs;
 };
 
-static zh_status reg_desc(zs_command* c, uint32_t r, zh_descriptor* d) {
+// The register's descriptor, generic or not (describe_reg(..).as_texture()).
+static zh_status reg_gen(const zs_command* c, uint32_t r, GenDesc* g) {
   if (!c || r >= c->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
-  if (c->ops[r].kind == K_OUTPUT) return fail(ZH_ERR_TYPE, "register is an output, not a texture");
-  if (c->ops[r].kind == K_BUFFER) return fail(ZH_ERR_TYPE, "register is a buffer, not a texture");
-  *d = c->ops[r].desc;
+  const CmdOp& op = c->ops[r];
+  if (op.kind == K_OUTPUT) return fail(ZH_ERR_TYPE, "register is an output, not a texture");
+  if (op.kind == K_BUFFER) return fail(ZH_ERR_TYPE, "register is a buffer, not a texture");
+  if (op.kind == K_INVOKE) return fail(ZH_ERR_TYPE, "register is an invocation, not a texture");
+  g->desc = op.desc;
+  g->size_var = op.gsize;
+  g->chroma_var = op.gchroma;
   return ZH_OK;
+}
+
+// The register's concrete descriptor; a generic one is a type error
+// (as_concrete(), CommandErrorKind::ConcreteDescriptorRequired).
+static zh_status reg_desc(const zs_command* c, uint32_t r, zh_descriptor* d) {
+  GenDesc g;
+  zh_status st = reg_gen(c, r, &g);
+  if (st != ZH_OK) return st;
+  if (g.size_var >= 0 || g.chroma_var >= 0)
+    return fail(ZH_ERR_TYPE, "operation requires a concrete descriptor; register " + std::to_string(r) + " is generic");
+  *d = g.desc;
+  return ZH_OK;
+}
+
+static bool gen_equal(const GenDesc& a, const GenDesc& b) {
+  if (a.size_var != b.size_var || a.chroma_var != b.chroma_var) return false;
+  if (a.size_var < 0 && (a.desc.width != b.desc.width || a.desc.height != b.desc.height)) return false;
+  if (a.chroma_var < 0) {
+    const zh_texel &ta = a.desc.texel, &tb = b.desc.texel;
+    const zh_color &ca = a.desc.color, &cb = b.desc.color;
+    if (ta.bits != tb.bits || ta.parts != tb.parts || ca.model != cb.model || ca.primaries != cb.primaries ||
+        ca.transfer != cb.transfer || ca.whitepoint != cb.whitepoint || ca.luminance != cb.luminance)
+      return false;
+  }
+  return true;
+}
+
+// GenericDescriptor::rewrite (command.rs:3143-3154): generic parts replaced
+// by the corresponding parts of `decl`.
+static GenDesc rewrite(const GenDesc& g, const std::vector<GenDesc>& decl) {
+  GenDesc r = g;
+  if (g.size_var >= 0) {
+    const GenDesc& s = decl[(size_t)g.size_var];
+    r.size_var = s.size_var;
+    r.desc.width = s.desc.width;
+    r.desc.height = s.desc.height;
+  }
+  if (g.chroma_var >= 0) {
+    const GenDesc& s = decl[(size_t)g.chroma_var];
+    r.chroma_var = s.chroma_var;
+    r.desc.texel = s.desc.texel;
+    r.desc.color = s.desc.color;
+    r.desc.texel_stride = s.desc.texel_stride;
+  }
+  if (r.size_var < 0 && r.chroma_var < 0) r.desc.row_stride = 0;
+  return r;
 }
 
 static uint32_t push(zs_command* c, CmdOp op) {
@@ -183,15 +265,17 @@ zh_status zs_command_input(zs_command* cmd, const zh_descriptor* desc, uint32_t*
 
 zh_status zs_command_output(zs_command* cmd, uint32_t src, uint32_t* reg, zh_descriptor* outformat) {
   if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
-  zh_descriptor d;
-  zh_status st = reg_desc(cmd, src, &d);
+  GenDesc g;  // a function's output may be generic (command.rs:1707-1712)
+  zh_status st = reg_gen(cmd, src, &g);
   if (st != ZH_OK) return st;
   CmdOp op;
   op.kind = K_OUTPUT;
   op.src = src;
-  op.desc = d;
+  op.desc = g.desc;
+  op.gsize = g.size_var;
+  op.gchroma = g.chroma_var;
   *reg = push(cmd, op);
-  if (outformat) *outformat = d;
+  if (outformat) *outformat = g.desc;
   return ZH_OK;
 }
 
@@ -212,14 +296,16 @@ zh_status zs_command_render(zs_command* cmd, uint32_t src, uint32_t* reg, zh_des
 
 zh_status zs_command_crop(zs_command* cmd, uint32_t src, zh_rect rect, uint32_t* reg) {
   if (!reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
-  zh_descriptor d;
-  zh_status st = reg_desc(cmd, src, &d);
+  GenDesc g;
+  zh_status st = reg_gen(cmd, src, &g);
   if (st != ZH_OK) return st;
   CmdOp op;
   op.kind = K_CROP;
   op.src = src;
   op.rect = rect;
-  op.desc = d;  // command.rs:972-977 keeps the source descriptor
+  op.desc = g.desc;  // command.rs:972-977 keeps the source descriptor, generic or not
+  op.gsize = g.size_var;
+  op.gchroma = g.chroma_var;
   *reg = push(cmd, op);
   return ZH_OK;
 }
@@ -571,10 +657,12 @@ zh_status zs_command_distribution_normal2d(zs_command* cmd, const zh_descriptor*
 zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette* config, uint32_t indices,
                              uint32_t* reg) {
   if (!config || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
-  zh_descriptor pd, id;
-  zh_status st = reg_desc(cmd, palette, &pd);
-  if (st == ZH_OK) st = reg_desc(cmd, indices, &id);
+  GenDesc pg, ig;  // either may be generic (command.rs:1448-1470)
+  zh_status st = reg_gen(cmd, palette, &pg);
+  if (st == ZH_OK) st = reg_gen(cmd, indices, &ig);
   if (st != ZH_OK) return st;
+  const zh_descriptor& pd = pg.desc;
+  const zh_descriptor& id = ig.desc;
   // ChannelPosition::into_vec4: a one-hot row per axis, or zeros.
   float x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
   for (int axis = 0; axis < 2; ++axis) {
@@ -592,6 +680,8 @@ zh_status zs_command_palette(zs_command* cmd, uint32_t palette, const zs_palette
   op.desc.width = id.width;
   op.desc.height = id.height;
   op.desc.row_stride = 0;
+  op.gchroma = pg.chroma_var;
+  op.gsize = ig.size_var;
   // binary_data: the mat4x2's columns (x_k, y_k) (shaders/palette.rs:57-66).
   const float m[8] = {x[0], y[0], x[1], y[1], x[2], y[2], x[3], y[3]};
   op.blob.resize(32);
@@ -802,12 +892,6 @@ zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_desc
   return ZH_OK;
 }
 
-static bool same_descriptor(const zh_descriptor& a, const zh_descriptor& b) {
-  return a.width == b.width && a.height == b.height && a.texel.bits == b.texel.bits &&
-         a.texel.parts == b.texel.parts && a.color.model == b.color.model && a.color.transfer == b.color.transfer &&
-         a.color.primaries == b.color.primaries && a.color.whitepoint == b.color.whitepoint;
-}
-
 zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint32_t* args, uint32_t num_args,
                             uint32_t* results, uint32_t num_results) {
   if (!cmd || !body || (num_args && !args) || (num_results && !results) || cmd == body)
@@ -819,14 +903,19 @@ zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint3
   }
   if (inputs != num_args) return fail(ZH_ERR_TYPE, "function takes " + std::to_string(inputs) + " arguments");
   if (outputs != num_results) return fail(ZH_ERR_TYPE, "function returns " + std::to_string(outputs) + " results");
+  if (body->vars || !body->symbols.empty())
+    return fail(ZH_ERR_TYPE, "a generic function or one that calls others is linked with zs_link");
   // Check every argument before touching the caller.
   uint32_t k = 0;
   for (const CmdOp& op : body->ops) {
     if (op.kind != K_INPUT) continue;
-    zh_descriptor d;
-    zh_status st = reg_desc(cmd, args[k], &d);
+    // The argument's full descriptor (size, texel, colour) must be the
+    // parameter's (command.rs:895-902).
+    GenDesc have, want;
+    zh_status st = reg_gen(cmd, args[k], &have);
     if (st != ZH_OK) return st;
-    if (!same_descriptor(d, op.desc))
+    want.desc = op.desc;
+    if (!gen_equal(want, have))
       return fail(ZH_ERR_TYPE, "argument " + std::to_string(k) + " does not match the function's input");
     ++k;
   }
@@ -856,7 +945,8 @@ zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob) {
   if (!cmd || !knob || reg >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
   CmdOp& op = cmd->ops[reg];
   if (op.kind == K_INPUT || op.kind == K_OUTPUT || op.kind == K_TRANSMUTE || op.kind == K_CROP ||
-      op.kind == K_INSCRIBE || op.kind == K_BLEND || op.kind == K_FROM_BUFFER)
+      op.kind == K_INSCRIBE || op.kind == K_BLEND || op.kind == K_FROM_BUFFER || op.kind == K_RESULT ||
+      op.kind == K_INVOKE)
     return fail(ZH_ERR_TYPE, "operation has no parameter data to override");
   if (op.knob < 0) op.knob = cmd->next_knob++;
   *knob = op.knob;
@@ -868,8 +958,11 @@ zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc
   return reg_desc(cmd, reg, desc);
 }
 
-zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
-  if (!cmd || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+}  // extern "C"
+
+// The High list of a flat, concrete command buffer (Linker::link_in,
+// command.rs:2345-2880): one texture per register.
+static zh_status compile_flat(const zs_command* cmd, zs_linked** out) {
   zs_linked* L = new zs_linked();
   const size_t n = cmd->ops.size();
   L->textures.resize(n);
@@ -1043,6 +1136,10 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
         op.viewport = with_size(b.width, b.height);
         break;
       }
+      case K_RESULT:
+      case K_INVOKE:
+        delete L;
+        return fail(ZH_ERR_OTHER, "internal: call left in a linked program");
     }
   }
   for (size_t i = 0; i < L->ops.size(); ++i) {
@@ -1050,6 +1147,304 @@ zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
     L->ops[i].params_len = (uint32_t)L->blobs[i].size();
   }
   *out = L;
+  return ZH_OK;
+}
+
+namespace {
+
+// Linker::link's monomorphization (command.rs:2083-2185), folded: every
+// invocation is expanded in place with its function's body instantiated at
+// the call's generic arguments, so the program handed to the backend has no
+// calls and no generic descriptors left, and a call's stages fuse with the
+// caller's.  Knobs of a function body get fresh knob ids per instantiation;
+// the last one of each (buffer, register) is what zs_linked_knob reports, as
+// the reference's map keeps the last insertion.
+struct Expander {
+  std::vector<const zs_command*> bufs;          // 0: the entry buffer, then the functions
+  std::vector<std::vector<uint32_t>> links;     // per buffer: function symbol -> buffer index
+  zs_command out;                               // the flat program
+  std::vector<std::array<int64_t, 3>> knobs;    // (buffer, register, knob)
+  std::vector<int64_t> entry_remap;             // entry register -> register of `out`
+  int depth = 0;
+
+  static Signature signature_of(const zs_command* c) {
+    Signature s;
+    s.vars = c->vars;
+    for (const CmdOp& op : c->ops) {
+      GenDesc g;
+      g.desc = op.desc;
+      g.size_var = op.gsize;
+      g.chroma_var = op.gchroma;
+      if (op.kind == K_INPUT) s.inputs.push_back(g);
+      if (op.kind == K_OUTPUT) s.outputs.push_back(g);
+    }
+    return s;
+  }
+
+  zh_status expand(uint32_t b, const std::vector<GenDesc>& tys, const std::vector<uint32_t>* args,
+                   std::vector<uint32_t>* results) {
+    if (++depth > 64) return fail(ZH_ERR_OTHER, "function calls nest deeper than 64 (recursive links?)");
+    const zs_command* c = bufs[b];
+    if (tys.size() != c->vars)
+      return fail(ZH_ERR_OTHER, "buffer " + std::to_string(b) + " takes " + std::to_string(c->vars) +
+                                    " generic arguments, " + std::to_string(tys.size()) + " given");
+    std::vector<uint32_t> remap(c->ops.size(), 0);
+    uint32_t a = 0;
+    for (uint32_t i = 0; i < c->ops.size(); ++i) {
+      const CmdOp& op = c->ops[i];
+      GenDesc g;
+      g.desc = op.desc;
+      g.size_var = op.gsize;
+      g.chroma_var = op.gchroma;
+      const GenDesc cg = rewrite(g, tys);
+      const bool concrete = cg.size_var < 0 && cg.chroma_var < 0;
+      if (op.kind == K_RESULT) continue;  // set by the Invoke that follows
+      if (op.kind == K_INVOKE) {
+        if (b >= links.size() || op.fn >= links[b].size())
+          return fail(ZH_ERR_OTHER, "function symbol " + std::to_string(op.fn) + " of buffer " + std::to_string(b) +
+                                        " is not linked");
+        const uint32_t callee = links[b][op.fn];
+        if (callee == 0 || callee >= bufs.size()) return fail(ZH_ERR_OTHER, "link target out of range");
+        // The definition must have the declared signature (command.rs:2076-2081).
+        const Signature want = c->symbols[op.fn], have = signature_of(bufs[callee]);
+        bool same = want.vars == have.vars && want.inputs.size() == have.inputs.size() &&
+                    want.outputs.size() == have.outputs.size();
+        for (size_t k = 0; same && k < want.inputs.size(); ++k) same = gen_equal(want.inputs[k], have.inputs[k]);
+        for (size_t k = 0; same && k < want.outputs.size(); ++k) same = gen_equal(want.outputs[k], have.outputs[k]);
+        if (!same) return fail(ZH_ERR_OTHER, "linked function does not have the declared signature");
+        std::vector<GenDesc> gens;
+        for (const GenDesc& gg : op.generics) {
+          GenDesc r = rewrite(gg, tys);
+          if (r.size_var >= 0 || r.chroma_var >= 0) return fail(ZH_ERR_OTHER, "generic argument left unbound");
+          gens.push_back(r);
+        }
+        std::vector<uint32_t> in, res;
+        for (uint32_t r : op.args) in.push_back(remap[r]);
+        zh_status st = expand(callee, gens, &in, &res);
+        if (st != ZH_OK) return st;
+        for (size_t k = 0; k < op.results.size() && k < res.size(); ++k) remap[op.results[k]] = res[k];
+        continue;
+      }
+      if (!concrete) return fail(ZH_ERR_OTHER, "generic descriptor left unbound in buffer " + std::to_string(b));
+      if (op.kind == K_INPUT && args) {
+        remap[i] = (*args)[a++];
+        continue;
+      }
+      if (op.kind == K_OUTPUT && args) {
+        results->push_back(remap[op.src]);
+        continue;
+      }
+      CmdOp n = op;
+      n.src = op.src < remap.size() ? remap[op.src] : 0;
+      n.above = op.above < remap.size() ? remap[op.above] : 0;
+      n.desc = cg.desc;
+      n.desc.texel_stride = bits_bytes(n.desc.texel.bits);
+      n.gsize = n.gchroma = -1;
+      n.args.clear();
+      n.results.clear();
+      n.generics.clear();
+      if (op.knob >= 0) {
+        // The entry buffer keeps its knob ids; function bodies get fresh ones.
+        n.knob = b == 0 ? op.knob : out.next_knob++;
+        knobs.push_back({(int64_t)b, (int64_t)i, (int64_t)n.knob});
+      }
+      remap[i] = push(&out, n);
+    }
+    if (b == 0) entry_remap.assign(remap.begin(), remap.end());
+    --depth;
+    return ZH_OK;
+  }
+};
+
+}  // namespace
+
+static zh_status link_impl(const zs_command* main, const zh_descriptor* tys, uint32_t num_tys,
+                           const zs_command* const* functions, uint32_t num_functions, const uint32_t* link_counts,
+                           const uint32_t* link_targets, zs_linked** out) {
+  if (!main || !out || (num_tys && !tys) || (num_functions && !functions)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  Expander E;
+  E.bufs.push_back(main);
+  for (uint32_t k = 0; k < num_functions; ++k) {
+    if (!functions[k]) return fail(ZH_ERR_INVALID_ARGUMENT, "null function buffer");
+    E.bufs.push_back(functions[k]);
+  }
+  E.links.resize(E.bufs.size());
+  if (link_counts) {
+    size_t at = 0;
+    for (size_t bi = 0; bi < E.bufs.size(); ++bi) {
+      for (uint32_t k = 0; k < link_counts[bi]; ++k) E.links[bi].push_back(link_targets[at++]);
+      if (!E.links[bi].empty() && E.links[bi].size() != E.bufs[bi]->symbols.size())
+        return fail(ZH_ERR_OTHER, "buffer " + std::to_string(bi) + " declares " +
+                                      std::to_string(E.bufs[bi]->symbols.size()) + " functions, links name " +
+                                      std::to_string(E.links[bi].size()));
+    }
+  }
+  // Knob ids of function bodies start above the entry buffer's.
+  E.out.next_knob = main->next_knob;
+  std::vector<GenDesc> entry_tys;
+  for (uint32_t k = 0; k < num_tys; ++k) {
+    GenDesc g;
+    g.desc = normalized(tys[k]);
+    entry_tys.push_back(g);
+  }
+  zh_status st = E.expand(0, entry_tys, nullptr, nullptr);
+  if (st != ZH_OK) return st;
+  zs_linked* flat = nullptr;
+  st = compile_flat(&E.out, &flat);
+  if (st != ZH_OK) return st;
+  // Slots per register of the entry buffer.
+  std::vector<int64_t> in(main->ops.size(), -1), outs(main->ops.size(), -1);
+  for (size_t r = 0; r < main->ops.size(); ++r) {
+    const int64_t t = E.entry_remap[r];
+    if (main->ops[r].kind == K_INPUT) in[r] = flat->input_slot[(size_t)t];
+    if (main->ops[r].kind == K_OUTPUT) outs[r] = flat->output_slot[(size_t)t];
+  }
+  flat->input_slot = in;
+  flat->output_slot = outs;
+  flat->knobs = E.knobs;
+  *out = flat;
+  return ZH_OK;
+}
+
+extern "C" {
+
+zh_status zs_command_compile(zs_command* cmd, zs_linked** out) {
+  if (!cmd || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  return link_impl(cmd, nullptr, 0, nullptr, 0, nullptr, nullptr, out);
+}
+
+zh_status zs_link(const zs_command* main, const zh_descriptor* tys, uint32_t num_tys, const zs_command* const* functions,
+                  uint32_t num_functions, const uint32_t* link_counts, const uint32_t* link_targets, zs_linked** out) {
+  return link_impl(main, tys, num_tys, functions, num_functions, link_counts, link_targets, out);
+}
+
+zh_status zs_linked_knob(zs_linked* L, uint32_t link_idx, uint32_t reg, int32_t* knob) {
+  if (!L || !knob) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  for (size_t k = L->knobs.size(); k-- > 0;)
+    if (L->knobs[k][0] == (int64_t)link_idx && L->knobs[k][1] == (int64_t)reg) {
+      *knob = (int32_t)L->knobs[k][2];
+      return ZH_OK;
+    }
+  return fail(ZH_ERR_INVALID_ARGUMENT, "no knob on that register");
+}
+
+// ---- generics (command.rs:753-923) --------------------------------------------
+zh_status zs_command_generic(zs_command* cmd, uint32_t* var) {
+  if (!cmd || !var) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  GenDesc g;
+  g.size_var = g.chroma_var = (int32_t)cmd->vars++;
+  cmd->tys.push_back(g);
+  *var = (uint32_t)cmd->tys.size() - 1;
+  return ZH_OK;
+}
+
+zh_status zs_command_input_generic(zs_command* cmd, uint32_t var, uint32_t* reg) {
+  if (!cmd || !reg) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (var >= cmd->tys.size()) return fail(ZH_ERR_OTHER, "bad descriptor variable");
+  const GenDesc& g = cmd->tys[var];
+  CmdOp op;
+  op.kind = K_INPUT;
+  op.desc = g.desc;
+  op.gsize = g.size_var;
+  op.gchroma = g.chroma_var;
+  *reg = push(cmd, op);
+  return ZH_OK;
+}
+
+zh_status zs_command_register_descriptor(zs_command* cmd, uint32_t reg, uint32_t* var) {
+  if (!var) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  GenDesc g;
+  zh_status st = reg_gen(cmd, reg, &g);
+  if (st != ZH_OK) return st;
+  g.desc.row_stride = 0;
+  cmd->tys.push_back(g);
+  *var = (uint32_t)cmd->tys.size() - 1;
+  return ZH_OK;
+}
+
+zh_status zs_command_derive_descriptor(zs_command* cmd, uint32_t base, const uint32_t* size, uint32_t* var) {
+  if (!cmd || !var) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (base >= cmd->tys.size()) return fail(ZH_ERR_OTHER, "bad descriptor variable");
+  GenDesc g = cmd->tys[base];
+  if (size) {
+    if (size[0] == 0 || size[1] == 0) return fail(ZH_ERR_TYPE, "empty size");
+    g.size_var = -1;
+    g.desc.width = size[0];
+    g.desc.height = size[1];
+  }
+  cmd->tys.push_back(g);
+  *var = (uint32_t)cmd->tys.size() - 1;
+  return ZH_OK;
+}
+
+zh_status zs_command_signature(const zs_command* body, zs_signature** out) {
+  if (!body || !out) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  zs_signature* s = new zs_signature();
+  s->sig = Expander::signature_of(body);
+  *out = s;
+  return ZH_OK;
+}
+
+zh_status zs_signature_counts(const zs_signature* sig, uint32_t* vars, uint32_t* inputs, uint32_t* outputs) {
+  if (!sig) return fail(ZH_ERR_INVALID_ARGUMENT, "null signature");
+  if (vars) *vars = sig->sig.vars;
+  if (inputs) *inputs = (uint32_t)sig->sig.inputs.size();
+  if (outputs) *outputs = (uint32_t)sig->sig.outputs.size();
+  return ZH_OK;
+}
+
+zh_status zs_signature_destroy(zs_signature* sig) {
+  delete sig;
+  return ZH_OK;
+}
+
+zh_status zs_command_function(zs_command* cmd, const zs_signature* sig, uint32_t* fn) {
+  if (!cmd || !sig || !fn) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  cmd->symbols.push_back(sig->sig);
+  *fn = (uint32_t)cmd->symbols.size() - 1;
+  return ZH_OK;
+}
+
+zh_status zs_command_invoke_function(zs_command* cmd, uint32_t fn, const uint32_t* generics, uint32_t num_generics,
+                                     const uint32_t* args, uint32_t num_args, uint32_t* results,
+                                     uint32_t num_results) {
+  if (!cmd || (num_generics && !generics) || (num_args && !args) || (num_results && !results))
+    return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (fn >= cmd->symbols.size()) return fail(ZH_ERR_OTHER, "bad function variable");
+  const Signature sig = cmd->symbols[fn];
+  // CommandError::INVALID_CALL (command.rs:866-900) for arity and types.
+  if (sig.inputs.size() != num_args) return fail(ZH_ERR_OTHER, "invalid call: wrong number of arguments");
+  if (sig.vars != num_generics) return fail(ZH_ERR_OTHER, "invalid call: wrong number of generic arguments");
+  if (sig.outputs.size() != num_results) return fail(ZH_ERR_OTHER, "invalid call: wrong number of results");
+  std::vector<GenDesc> gens;
+  for (uint32_t k = 0; k < num_generics; ++k) {
+    if (generics[k] >= cmd->tys.size()) return fail(ZH_ERR_OTHER, "bad descriptor variable");
+    gens.push_back(cmd->tys[generics[k]]);
+  }
+  for (uint32_t k = 0; k < num_args; ++k) {
+    GenDesc have;
+    zh_status st = reg_gen(cmd, args[k], &have);
+    if (st != ZH_OK) return st;
+    if (!gen_equal(rewrite(sig.inputs[k], gens), have))
+      return fail(ZH_ERR_OTHER, "invalid call: argument " + std::to_string(k) + " does not match the parameter");
+  }
+  // Result registers first, then the invocation (command.rs:907-923).
+  CmdOp inv;
+  inv.kind = K_INVOKE;
+  inv.fn = fn;
+  inv.args.assign(args, args + num_args);
+  inv.generics = gens;
+  for (uint32_t k = 0; k < num_results; ++k) {
+    const GenDesc r = rewrite(sig.outputs[k], gens);
+    CmdOp res;
+    res.kind = K_RESULT;
+    res.desc = r.desc;
+    res.gsize = r.size_var;
+    res.gchroma = r.chroma_var;
+    results[k] = push(cmd, res);
+    inv.results.push_back(results[k]);
+  }
+  push(cmd, inv);
   return ZH_OK;
 }
 
