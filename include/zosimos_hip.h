@@ -400,6 +400,73 @@ zh_status zh_mlp_evaluate(zh_context* ctx, const float* params, uint32_t n, cons
  * No counterpart in the reference, whose sRGB codec is the texture unit's. */
 zh_status zh_srgb_tables(void* out, size_t len);
 
+/* ---- image pool (replaces zosimos::pool::Pool, pool.rs:19-156) ----------
+ * A pool holds images under keys: host bytes (ImageData::Host), a HIP device
+ * image (ImageData::GpuBuffer) or only a descriptor (ImageData::LateBound).
+ * Keys are never 0; 0 binds an output to no pool image. */
+typedef struct zh_pool zh_pool;
+typedef uint64_t zh_pool_key;
+#define ZH_IMAGE_HOST 0
+#define ZH_IMAGE_HIP_BUFFER 1
+#define ZH_IMAGE_LATE_BOUND 2
+zh_status zh_pool_create(zh_pool** out);                        /* Pool::new, pool.rs:182 */
+zh_status zh_pool_destroy(zh_pool* pool);
+/* Pool::insert / insert_srgb (pool.rs:244-280): copies `bytes` (rows
+ * `row_stride` apart, 0 = tight) into the pool as host bytes. */
+zh_status zh_pool_insert_host(zh_pool* pool, const zh_descriptor* desc, const void* bytes, uint64_t row_stride,
+                              zh_pool_key* key);
+/* Pool::declare (pool.rs:281-291): a LateBound image, a destination only. */
+zh_status zh_pool_declare(zh_pool* pool, const zh_descriptor* desc, zh_pool_key* key);
+/* ImageData::GpuBuffer (pool.rs:133-137): the pool takes ownership of `img`. */
+zh_status zh_pool_insert_device(zh_pool* pool, zh_image* img, zh_pool_key* key);
+/* PoolImage::descriptor (pool.rs:661-668). */
+zh_status zh_pool_image_info(zh_pool* pool, zh_pool_key key, zh_descriptor* desc, int32_t* kind);
+/* PoolImage::as_bytes (pool.rs:669-680): tight host rows, valid until the
+ * image changes. */
+zh_status zh_pool_host_bytes(zh_pool* pool, zh_pool_key key, const void** data, uint64_t* len);
+/* The device image of a ZH_IMAGE_HIP_BUFFER entry; the pool keeps ownership. */
+zh_status zh_pool_device_image(zh_pool* pool, zh_pool_key key, zh_image** img);
+/* ImageMeta::no_read (pool.rs:104-109, 912-916): an input launched from this image
+ * moves out of the pool until the execution retires it or finishes. */
+zh_status zh_pool_set_no_read(zh_pool* pool, zh_pool_key key, int32_t no_read);
+/* Pool::upload (pool.rs:292-370) and its reverse: move the image to the device of `ctx`, or
+ * back to host bytes (synchronous). */
+zh_status zh_pool_upload(zh_pool* pool, zh_pool_key key, zh_context* ctx);
+zh_status zh_pool_download(zh_pool* pool, zh_pool_key key);
+zh_status zh_pool_remove(zh_pool* pool, zh_pool_key key);      /* Pool::remove */
+/* Pool::clear_cache (pool.rs:450): free the device buffers kept from
+ * finished executions for reuse. */
+zh_status zh_pool_clear_cache(zh_pool* pool);
+zh_status zh_pool_cached_buffers(zh_pool* pool, uint32_t* count);
+
+/* Launch from the pool (Executable::launch with a pool, run.rs:1016-1167):
+ * binding {slot, 0, key} reads input `slot` from pool image `key`; {slot, 1,
+ * key} writes output `slot` into `key` (0 = a new image at retirement).  All
+ * checks run before anything moves: ZH_ERR_MISSING_KEY for an unbound used
+ * slot or an input whose image holds no data, ZH_ERR_MISMATCHED_DESCRIPTOR
+ * when the image's descriptor differs from the slot's, ZH_ERR_INVALID_ARGUMENT
+ * for an unknown key or an image bound twice.  Device images are used in
+ * place; host bytes are uploaded into device buffers the pool caches for the
+ * next launch.  The launch is asynchronous; the pool must outlive the
+ * execution. */
+typedef struct zh_pool_binding {
+  uint32_t slot;
+  uint32_t is_output;
+  zh_pool_key key;
+} zh_pool_binding;
+typedef struct zh_execution zh_execution;
+zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
+                                 uint32_t num_bindings, zh_execution** out);
+/* Retire::output (run.rs:2809-2875): move output `slot` into the pool, as
+ * host bytes (`to_host`, waits for the launch) or as the device image itself
+ * (no copy, no wait).  `key` receives the pool key. */
+zh_status zh_execution_retire_output(zh_execution* exec, uint32_t slot, int32_t to_host, zh_pool_key* key);
+/* Retire::input (run.rs:2791-2808): give a moved (no_read) input back. */
+zh_status zh_execution_retire_input(zh_execution* exec, uint32_t slot, zh_pool_key* key);
+/* Retire::finish (run.rs:2962-2998): wait, return what is not retired to the
+ * pool and its cache, free the execution. */
+zh_status zh_execution_finish(zh_execution* exec);
+
 /* Message of the last failed call on this thread (never NULL). */
 const char* zh_last_error(void);
 /* Library version string. */

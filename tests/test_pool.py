@@ -1,0 +1,103 @@
+"""The image pool on the host (no GPU): keys and their data
+(pool.rs:19-156, 244-291), and the checks a launch from the pool makes
+before anything moves (Executable::check_satisfiable, run.rs:1085-1167)."""
+import numpy as np
+import pytest
+
+import zosimos_amd as z
+
+
+def expect(code, fn, *args):
+    with pytest.raises(z.ZosimosError) as e:
+        fn(*args)
+    assert e.value.code == code, e.value
+    return e.value
+
+
+def test_keys_and_host_bytes():
+    pool = z.Pool()
+    d = z.Descriptor.rgba8_srgb(5, 3)
+    data = np.arange(5 * 3 * 4, dtype=np.uint8)
+    k = pool.insert(d, data)
+    assert k != 0
+    got, kind = pool.info(k)
+    assert kind == "host" and (got.width, got.height) == (5, 3)
+    assert np.array_equal(pool.as_bytes(k), data)
+    late = pool.declare(z.Descriptor.rgba16f_linear(7, 2))
+    assert late not in (0, k) and pool.kind(late) == "late bound"
+    expect(z.Status.INVALID_ARGUMENT, pool.as_bytes, late)
+    expect(z.Status.INVALID_ARGUMENT, pool.device_ptr, k)
+    # download of host bytes is a no-op; a late-bound image has nothing to move
+    pool.download(k)
+    expect(z.Status.INVALID_ARGUMENT, pool.download, late)
+    pool.set_no_read(k)
+    pool.remove(k)
+    expect(z.Status.INVALID_ARGUMENT, pool.info, k)
+    expect(z.Status.INVALID_ARGUMENT, pool.remove, k)
+    with pytest.raises(ValueError):
+        pool.insert(d, data[:-1])
+    assert pool.cached_buffers() == 0
+
+
+def test_insert_with_row_stride_keeps_tight_rows():
+    """Rows given with a pitch are stored tight (ImageBuffer rows)."""
+    import ctypes
+    pool = z.Pool()
+    d = z.Descriptor.rgba8_srgb(3, 2)
+    pitched = np.zeros((2, 20), dtype=np.uint8)
+    pitched[:, :12] = np.arange(24, dtype=np.uint8).reshape(2, 12)
+    k = ctypes.c_uint64()
+    z._check(z._abi.lib().zh_pool_insert_host(pool._h, ctypes.byref(d.c()), pitched.ctypes.data, 20,
+                                              ctypes.byref(k)))
+    assert np.array_equal(pool.as_bytes(k.value), np.arange(24, dtype=np.uint8))
+    # a pitch shorter than a row is refused
+    code = z._abi.lib().zh_pool_insert_host(pool._h, ctypes.byref(d.c()), pitched.ctypes.data, 8, ctypes.byref(k))
+    assert code == z.Status.INVALID_ARGUMENT
+
+
+def inscribe_program():
+    bg, fg = z.Descriptor.rgba8_srgb(16, 16), z.Descriptor.rgba8_srgb(8, 8)
+    cmd = z.CommandBuffer()
+    b, f = cmd.input(bg), cmd.input(fg)
+    o, od = cmd.output(cmd.inscribe(b, z.Rectangle(0, 0, 8, 8), f))
+    return z.Linker.from_included().compile(cmd).lower_to(None), (b, f, o), (bg, fg, od)
+
+
+def test_launch_checks_before_anything_moves():
+    exe, (b, f, o), (bg, fg, od) = inscribe_program()
+    pool = z.Pool()
+    kb = pool.insert(bg, np.zeros(bg.nbytes, np.uint8))
+    kf = pool.insert(fg, np.zeros(fg.nbytes, np.uint8))
+    wrong = pool.insert(z.Descriptor.rgba8_srgb(8, 9), np.zeros(8 * 9 * 4, np.uint8))
+    late = pool.declare(fg)
+    # an unbound input (StartError::MissingKey)
+    expect(z.Status.MISSING_KEY, exe.launch_pool, pool, {b: kb}, {o: 0})
+    # an input whose image holds only a descriptor
+    expect(z.Status.MISSING_KEY, exe.launch_pool, pool, {b: kb, f: late}, {o: 0})
+    # a descriptor other than the register's (StartError::MismatchedDescriptor)
+    expect(z.Status.MISMATCHED_DESCRIPTOR, exe.launch_pool, pool, {b: kb, f: wrong}, {o: 0})
+    expect(z.Status.MISMATCHED_DESCRIPTOR, exe.launch_pool, pool, {b: kb, f: kf}, {o: kf})
+    # the whole descriptor is compared, colour included (run.rs:1099-1106)
+    d50 = pool.insert(fg.with_(color=z.Color(whitepoint=z.Whitepoint.D50)), np.zeros(fg.nbytes, np.uint8))
+    expect(z.Status.MISMATCHED_DESCRIPTOR, exe.launch_pool, pool, {b: kb, f: d50}, {o: 0})
+    # an input also written, an unknown key
+    same = pool.insert(bg, np.zeros(bg.nbytes, np.uint8))
+    expect(z.Status.INVALID_ARGUMENT, exe.launch_pool, pool, {b: kb, f: kf}, {o: kb})
+    expect(z.Status.INVALID_ARGUMENT, exe.launch_pool, pool, {b: 12345, f: kf}, {o: 0})
+    # everything checks out: the program has no device to run on
+    expect(z.Status.NO_DEVICE, exe.launch_pool, pool, {b: kb, f: kf}, {o: same})
+    expect(z.Status.NO_DEVICE, exe.launch_pool, pool, {b: kb, f: kf}, {o: pool.declare(od)})
+    # nothing moved
+    assert [pool.kind(k) for k in (kb, kf, late)] == ["host", "host", "late bound"]
+
+
+def test_one_image_is_not_bound_twice():
+    d = z.Descriptor.rgba8_srgb(8, 8)
+    cmd = z.CommandBuffer()
+    b, f = cmd.input(d), cmd.input(d)
+    o, _ = cmd.output(cmd.inscribe(b, z.Rectangle(0, 0, 8, 8), f))
+    exe = z.Linker.from_included().compile(cmd).lower_to(None)
+    pool = z.Pool()
+    k, k2 = pool.insert(d, np.zeros(d.nbytes, np.uint8)), pool.insert(d, np.zeros(d.nbytes, np.uint8))
+    expect(z.Status.INVALID_ARGUMENT, exe.launch_pool, pool, {b: k, f: k}, {o: 0})
+    expect(z.Status.NO_DEVICE, exe.launch_pool, pool, {b: k, f: k2}, {o: 0})

@@ -16,7 +16,7 @@ from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _abi
-from ._abi import BindingC, ColorC, DescriptorC, RectC, Texel as TexelC
+from ._abi import BindingC, ColorC, DescriptorC, PoolBindingC, RectC, Texel as TexelC
 
 # ---- identifiers (lib/std/src/stage.frag:107-170) ---------------------------
 class Transfer:
@@ -729,6 +729,17 @@ class Executable:
         arr, n = self._bindings(inputs, outputs)
         _check(self._lib.zh_program_launch(self._h, arr, n))
 
+    def launch_pool(self, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
+        """Launch with inputs read from pool keys and outputs written to pool
+        keys (0: a new key at retirement); ``Executable::launch`` with a pool
+        (run.rs:1016-1167).  Returns the running execution."""
+        binds = [PoolBindingC(self.program.input_slot(r), 0, k) for r, k in inputs.items()]
+        binds += [PoolBindingC(self.program.output_slot(r), 1, k) for r, k in outputs.items()]
+        arr = (PoolBindingC * max(len(binds), 1))(*binds)
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_program_launch_pool(self._h, pool._h, arr, len(binds), ctypes.byref(h)))
+        return Execution(h, self, pool)
+
     def time_stages(self, inputs, outputs, iterations: int = 10):
         arr, n = self._bindings(inputs, outputs)
         out = (ctypes.c_float * 64)()
@@ -851,6 +862,124 @@ class Image:
         _check(_abi.lib().zh_image_download(self._h, out.ctypes.data, row))
         self.ctx.sync()
         return out
+
+
+class Pool:
+    """``zosimos::pool::Pool`` (pool.rs:19-156): images under keys, as host
+    bytes, a device image, or only a descriptor (late bound)."""
+
+    KINDS = {0: "host", 1: "device", 2: "late bound"}
+
+    def __init__(self):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_pool_create(ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _abi is not None:
+            self._lib.zh_pool_destroy(self._h)
+            self._h = None
+
+    def insert(self, desc: Descriptor, data) -> int:
+        """``Pool::insert`` (pool.rs:244-252): tightly packed host bytes."""
+        arr = np.ascontiguousarray(data)
+        if arr.nbytes != desc.width * desc.texel.bytes * desc.height:
+            raise ValueError(f"expected {desc.width * desc.texel.bytes * desc.height} bytes, got {arr.nbytes}")
+        k = ctypes.c_uint64()
+        _check(self._lib.zh_pool_insert_host(self._h, ctypes.byref(desc.c()), arr.ctypes.data, 0, ctypes.byref(k)))
+        return k.value
+
+    def declare(self, desc: Descriptor) -> int:
+        """``Pool::declare`` (pool.rs:281-291)."""
+        k = ctypes.c_uint64()
+        _check(self._lib.zh_pool_declare(self._h, ctypes.byref(desc.c()), ctypes.byref(k)))
+        return k.value
+
+    def insert_device(self, image: "Image") -> int:
+        """Hand a device image to the pool (``ImageData::GpuBuffer``); the
+        Image object gives up its handle."""
+        k = ctypes.c_uint64()
+        _check(self._lib.zh_pool_insert_device(self._h, image._h, ctypes.byref(k)))
+        image._h = None
+        return k.value
+
+    def info(self, key: int) -> Tuple[Descriptor, str]:
+        d, kind = DescriptorC(), ctypes.c_int32()
+        _check(self._lib.zh_pool_image_info(self._h, key, ctypes.byref(d), ctypes.byref(kind)))
+        return Descriptor.from_c(d), self.KINDS.get(kind.value, str(kind.value))
+
+    def descriptor(self, key: int) -> Descriptor:
+        return self.info(key)[0]
+
+    def kind(self, key: int) -> str:
+        return self.info(key)[1]
+
+    def as_bytes(self, key: int) -> np.ndarray:
+        """A copy of the host bytes (``PoolImage::as_bytes``, pool.rs:669)."""
+        p, n = ctypes.c_void_p(), ctypes.c_uint64()
+        _check(self._lib.zh_pool_host_bytes(self._h, key, ctypes.byref(p), ctypes.byref(n)))
+        return np.frombuffer(ctypes.string_at(p.value, n.value), dtype=np.uint8).copy()
+
+    def device_ptr(self, key: int) -> int:
+        img = ctypes.c_void_p()
+        _check(self._lib.zh_pool_device_image(self._h, key, ctypes.byref(img)))
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(self._lib.zh_image_device_ptr(img, ctypes.byref(p), ctypes.byref(n)))
+        return p.value
+
+    def set_no_read(self, key: int, no_read: bool = True):
+        _check(self._lib.zh_pool_set_no_read(self._h, key, int(no_read)))
+
+    def upload(self, key: int, ctx: "Context"):
+        _check(self._lib.zh_pool_upload(self._h, key, ctx._h))
+
+    def download(self, key: int):
+        _check(self._lib.zh_pool_download(self._h, key))
+
+    def remove(self, key: int):
+        _check(self._lib.zh_pool_remove(self._h, key))
+
+    def clear_cache(self):
+        _check(self._lib.zh_pool_clear_cache(self._h))
+
+    def cached_buffers(self) -> int:
+        n = ctypes.c_uint32()
+        _check(self._lib.zh_pool_cached_buffers(self._h, ctypes.byref(n)))
+        return n.value
+
+
+class Execution:
+    """A launch from a pool and its ``Retire`` (run.rs:2786-2998)."""
+
+    def __init__(self, handle, exe: Executable, pool: Pool):
+        self._lib = _abi.lib()
+        self._h = handle
+        self.exe = exe
+        self.pool = pool  # the pool outlives the execution
+
+    def retire_output(self, reg: int, to_host: bool = True) -> int:
+        """``Retire::output`` (run.rs:2809-2875): the output's pool key."""
+        k = ctypes.c_uint64()
+        _check(self._lib.zh_execution_retire_output(self._h, self.exe.program.output_slot(reg), int(to_host),
+                                                    ctypes.byref(k)))
+        return k.value
+
+    def retire_input(self, reg: int) -> int:
+        k = ctypes.c_uint64()
+        _check(self._lib.zh_execution_retire_input(self._h, self.exe.program.input_slot(reg), ctypes.byref(k)))
+        return k.value
+
+    def finish(self):
+        """``Retire::finish`` (run.rs:2962-2998)."""
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            _check(self._lib.zh_execution_finish(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _abi is not None:
+            self._lib.zh_execution_finish(self._h)
+            self._h = None
 
 
 MLP_PARAMS = 4611

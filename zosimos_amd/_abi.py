@@ -54,6 +54,10 @@ class GraphInfoC(ctypes.Structure):
     _fields_ = [("state", u32), ("nodes", u32), ("captures", u32), ("pad", u32), ("replays", u64)]
 
 
+class PoolBindingC(ctypes.Structure):
+    _fields_ = [("slot", u32), ("is_output", u32), ("key", u64)]
+
+
 class LearnOptionsC(ctypes.Structure):
     _fields_ = [("uniform_points", u32), ("face_points", u32), ("steps", u32), ("batch", u32), ("lr", f32),
                 ("seed", u32), ("lattice", u32), ("max_delta_e", f32)]
@@ -101,6 +105,24 @@ _SIGNATURES = {
                               ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "zh_mlp_evaluate": (i32, [vp, ctypes.POINTER(f32), u32, ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "zh_srgb_tables": (i32, [vp, sz]),
+    "zh_pool_create": (i32, [ctypes.POINTER(vp)]),
+    "zh_pool_destroy": (i32, [vp]),
+    "zh_pool_insert_host": (i32, [vp, ctypes.POINTER(DescriptorC), vp, u64, ctypes.POINTER(u64)]),
+    "zh_pool_declare": (i32, [vp, ctypes.POINTER(DescriptorC), ctypes.POINTER(u64)]),
+    "zh_pool_insert_device": (i32, [vp, vp, ctypes.POINTER(u64)]),
+    "zh_pool_image_info": (i32, [vp, u64, ctypes.POINTER(DescriptorC), ctypes.POINTER(i32)]),
+    "zh_pool_host_bytes": (i32, [vp, u64, ctypes.POINTER(vp), ctypes.POINTER(u64)]),
+    "zh_pool_device_image": (i32, [vp, u64, ctypes.POINTER(vp)]),
+    "zh_pool_set_no_read": (i32, [vp, u64, i32]),
+    "zh_pool_upload": (i32, [vp, u64, vp]),
+    "zh_pool_download": (i32, [vp, u64]),
+    "zh_pool_remove": (i32, [vp, u64]),
+    "zh_pool_clear_cache": (i32, [vp]),
+    "zh_pool_cached_buffers": (i32, [vp, ctypes.POINTER(u32)]),
+    "zh_program_launch_pool": (i32, [vp, vp, ctypes.POINTER(PoolBindingC), u32, ctypes.POINTER(vp)]),
+    "zh_execution_retire_output": (i32, [vp, u32, i32, ctypes.POINTER(u64)]),
+    "zh_execution_retire_input": (i32, [vp, u32, ctypes.POINTER(u64)]),
+    "zh_execution_finish": (i32, [vp]),
     "zh_last_error": (ctypes.c_char_p, []),
     "zh_version": (ctypes.c_char_p, []),
     # zosimos_command.h

@@ -26,6 +26,7 @@ SOURCES = [
     "zh_planner.cpp",
     "zh_command.cpp",
     "zh_surrogate.cpp",
+    "zh_pool.cpp",
 ]
 HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_internal.h", "zh_mlp.h"]
 ARCH = "gfx950"
