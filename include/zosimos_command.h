@@ -180,10 +180,28 @@ zh_status zs_command_from_buffer(zs_command* cmd, uint32_t buffer, const zh_desc
  * ZH_ERR_TYPE. */
 zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint32_t* args, uint32_t num_args,
                             uint32_t* results, uint32_t num_results);
+/* User stage kernels (CommandBuffer::{construct,unary,binary}_dynamic,
+ * command.rs:2933-3048): what ShaderCommand::source and ShaderCommand::data
+ * (command/dynamic.rs:7-28) return -- the shader, its data and the result's
+ * descriptor.  The operands must be concrete texture registers (buffers and
+ * generic registers are ZH_ERR_TYPE); the result may have any size.  The
+ * command buffer holds a reference to the shader.  zh_dynamic_args in
+ * zosimos_hip.h is what the kernel receives. */
+typedef struct zs_shader_command {
+  zh_shader* shader;  /* ShaderSource: a HIP code object and kernel */
+  const void* data;   /* ShaderData::set_data bytes (NULL when data_len is 0) */
+  uint32_t data_len;
+  zh_descriptor desc; /* the descriptor data() returns */
+} zs_shader_command;
+zh_status zs_command_construct_dynamic(zs_command* cmd, const zs_shader_command* dynamic, uint32_t* reg);
+zh_status zs_command_unary_dynamic(zs_command* cmd, uint32_t src, const zs_shader_command* dynamic, uint32_t* reg);
+zh_status zs_command_binary_dynamic(zs_command* cmd, uint32_t lhs, uint32_t rhs, const zs_shader_command* dynamic,
+                                    uint32_t* reg);
 /* Mark the operation that defined `reg` as a knob (CommandBuffer::with_knob,
  * command.rs:1865-1905); `knob` receives its id for zh_program_set_knob.  A
  * buffer register's knob takes the buffer's bytes, of the same length
- * (WithKnob::buffer_init, command.rs:1937-1940). */
+ * (WithKnob::buffer_init, command.rs:1937-1940).  Dynamic shaders take no
+ * knob (WithKnob has no dynamic ops, command.rs:1865-2060): ZH_ERR_TYPE. */
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob);
 /* Descriptor of a register (RegisterDescription::Texture). */
 zh_status zs_command_describe(zs_command* cmd, uint32_t reg, zh_descriptor* desc);

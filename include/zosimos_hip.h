@@ -201,6 +201,10 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
  *  AFFINE_BILINEAR       mat3 std140 of the row-major Affine matrix, 48 B;
  *                        operand = layer, composited over the target (Load)
  *  PREMULTIPLY, UNPREMULTIPLY  none
+ *  DYNAMIC               a zh_dynamic_call followed by its data_len bytes of
+ *                        shader data: a user stage kernel (High::DrawInto of
+ *                        FragmentShaderInvocation::Runtime, command.rs:
+ *                        2767-2830); operands = its num_args sampled images
  */
 #define ZH_SHADER_PAINT_COPY 0
 #define ZH_SHADER_LINEAR_COLOR_MATRIX 1
@@ -221,6 +225,49 @@ zh_status zh_image_download(zh_image* img, void* host, uint64_t host_row_stride)
 #define ZH_SHADER_AFFINE_BILINEAR 18
 #define ZH_SHADER_PREMULTIPLY 19
 #define ZH_SHADER_UNPREMULTIPLY 20
+#define ZH_SHADER_DYNAMIC 21
+
+/* ---- user stage kernels (ShaderCommand, command/dynamic.rs:7-51) -------
+ * The HIP form of ShaderSource (dynamic.rs:31-34, a #[non_exhaustive] enum
+ * whose one variant is SPIR-V): a code object for gfx950 (`hipcc --genco
+ * --offload-arch=gfx950`) and the name of a kernel in it declared
+ *     extern "C" __global__ void name(zh_dynamic_args args)
+ * It runs where the reference runs the fragment shader: over every texel of
+ * the result, with 16 x 16 workgroups and a grid of ceil(width / 16) x
+ * ceil(height / 16) (the kernel tests x < width, y < height itself).  Its
+ * operands and its result are RGBA f32 in tight rows, straight alpha, in the
+ * linear values of each register's colour space -- the values a fragment
+ * shader reads through its sampler and writes to its target, before the
+ * register's texel encoding.  The texture coordinate of texel (x, y) is
+ * ((x + 0.5) / width, (y + 0.5) / height); the reference samples operands
+ * with a nearest, clamp-to-edge sampler (encoder.rs:1533-1537). */
+typedef struct zh_dynamic_args {
+  uint32_t width, height;       /* the result's size in texels */
+  uint32_t num_inputs;          /* operands: 0, 1 or 2 */
+  uint32_t data_len;            /* bytes of shader data at `data` */
+  float* out;                   /* width * height RGBA f32 texels */
+  const float* in[2];           /* the operands, in_width[k] * in_height[k] texels each */
+  uint32_t in_width[2], in_height[2];
+  const void* data;             /* device copy of the shader data (ShaderData::set_data) */
+} zh_dynamic_args;
+
+typedef struct zh_shader zh_shader;
+/* Copy a code object and name its kernel.  Nothing is loaded until a program
+ * that runs the shader is built on a context (hipModuleLoadData on the
+ * context's device, once per device).  The shader is reference counted:
+ * command buffers, linked programs and programs that use it hold it, and
+ * zh_shader_release drops the creator's reference. */
+zh_status zh_shader_create(const void* code_object, size_t len, const char* kernel, zh_shader** out);
+zh_status zh_shader_retain(zh_shader* shader);
+zh_status zh_shader_release(zh_shader* shader);
+
+/* The leading bytes of a ZH_SHADER_DYNAMIC parameter blob (the
+ * ShaderInvocation of command.rs:2943-2958). */
+typedef struct zh_dynamic_call {
+  zh_shader* shader;
+  uint32_t num_args;  /* operands of the draw: 0, 1 or 2 */
+  uint32_t data_len;  /* shader data bytes following this header */
+} zh_dynamic_call;
 
 typedef struct zh_op {
   uint32_t high;        /* ZH_HIGH_* */

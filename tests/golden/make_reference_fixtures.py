@@ -16,7 +16,8 @@ NAMES = ["composed", "affine", "convert_bt709", "swapped", "transmute", "oklab",
          "bilinear-knob-0", "bilinear-knob-1", "bilinear-knob-2", "bilinear-knob-3", "bilinear-knob-4",
          "bilinear_from_buffer", "from_buffer", "from_buffer-with-knob", "distribution_normal2d",
          "distribution_normal1d", "distribution_u8", "distribution_fractal2d", "srlab2", "derived_Scharr3", "derived_Scharr3To4Bit",
-         "derived_Scharr3To8Bit", "derived_Prewitt", "derived_Sobel", "palette", "generic"]
+         "derived_Scharr3To8Bit", "derived_Prewitt", "derived_Sobel", "palette", "generic", "mandelbrot", "crt",
+         "flat_field"]
 
 
 def main():

@@ -619,3 +619,94 @@ def test_generic_palette_reproduces_blessed_hash():
 
     assert B.blockhash256(image((uv * f32(255.0)).astype(np.uint32))) in blessed
     assert B.blockhash256(image(np.rint(uv * f32(255.0)).astype(np.uint32))) not in blessed
+
+
+# ---- the reference's extension shaders (oracle/shaders.py) --------------------
+def _blessed(name):
+    import json
+    import os
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return json.load(open(os.path.join(golden, "reference_hashes.json")))[name]
+
+
+def _mandelbrot_srgb(lab):
+    """The shader's Lab stored in the test's Oklab LchA 8-bit register, then
+    converted to sRGB (custom.rs:76-90)."""
+    u8 = S.stage_encode(lab.reshape(-1, 4), 10, 20, 0x100).view(np.uint8).reshape(lab.shape[0], lab.shape[1], 4)
+    return PL.oklab_lch8_to_srgb(u8)[1]
+
+
+def test_mandelbrot_reproduces_blessed_hash():
+    """custom.rs:14-96: a 256x256 construct_dynamic at scale (3, 3), position
+    (0.6, 0.5).  Swapping a and b or flipping rows leaves the blessed set."""
+    from oracle import blockhash as B
+    from oracle import shaders as SH
+    lab = SH.mandelbrot(256, 256)
+    assert B.blockhash256(_mandelbrot_srgb(lab)) in _blessed("mandelbrot")
+    assert B.blockhash256(_mandelbrot_srgb(lab[..., [0, 2, 1, 3]].copy())) not in _blessed("mandelbrot")
+    assert B.blockhash256(_mandelbrot_srgb(lab[::-1].copy())) not in _blessed("mandelbrot")
+
+
+def test_mandelbrot_known_orbits():
+    """Orbits with closed forms, at pixel centres chosen so that c is exact
+    (u = 0.125 + 0.25 k, c = (u - position) * 8): c = -1 cycles 0, -1, so
+    |z_2048| = 0, light 0.7, mean -1/2 and a = (-1/2 + 1) / 2 = 1/4;
+    c = -2 lands on the fixed point 2 after two steps, light 0 and
+    a = ((-2 + 2 * 2046) / 2048 + 2) / 2 exactly; c = 1 escapes (light 0)."""
+    from oracle import shaders as SH
+    lab = SH.mandelbrot(4, 4, scale=(8.0, 8.0), position=(0.25, 0.375))
+    # row 1: v = 0.375, c.y = 0; columns: c.x = -1, 1, 3, 5
+    assert tuple(lab[1, 0]) == (np.float32(0.7), 0.25, 0.0, 1.0)
+    assert lab[1, 1, 0] == 0.0
+    lab2 = SH.mandelbrot(4, 4, scale=(8.0, 8.0), position=(0.375, 0.375))
+    # column 0: c.x = (0.125 - 0.375) * 8 = -2
+    assert lab2[1, 0, 0] == 0.0 and lab2[1, 0, 1] == np.float32((4090.0 / 2048.0 + 2.0) * 0.5) and lab2[1, 0, 2] == 0
+    # c = 0 (pixel 1 of position 0.375): z stays 0, colour (0.7, 0, 0, 1)
+    assert tuple(lab2[1, 1]) == (np.float32(0.7), 0.0, 0.0, 1.0)
+
+
+def test_crt_reproduces_blessed_hash_and_mask_texels():
+    """custom.rs:98-185: the background magnified 3x through the shadow mask.
+    On a constant grey the mask's texels follow the 3 x 6 staggered grid:
+    (0, 0) cell 0 -> 0; (0, 2) cell 2 -> red 0.96; (4, 0) cell 3 -> green
+    0.96; (4, 2) cell 5 -> 0; (2, 1) cell 1 -> blue 0.64; alpha kept."""
+    from oracle import blockhash as B
+    from oracle import shaders as SH
+    bg = _golden_rgba("background")
+    lin = C.decode_u8(bg, 5).astype(np.float32)
+    out = SH.crt(lin, 3 * bg.shape[1], 3 * bg.shape[0])
+    assert B.blockhash256(C.encode_u8(out, 5)) in _blessed("crt")
+    grey = np.full((4, 4, 4), 0.5, np.float32)
+    m = SH.crt(grey, 12, 12)
+    f = np.float32
+    assert tuple(m[0, 0]) == (0.0, 0.0, 0.0, 0.5)
+    assert tuple(m[2, 0]) == (f(0.5) * (f(0.16) * f(6.0)), 0.0, 0.0, 0.5)
+    assert tuple(m[0, 4]) == (0.0, f(0.5) * (f(0.16) * f(6.0)), 0.0, 0.5)
+    assert tuple(m[2, 4]) == (0.0, 0.0, 0.0, 0.5)
+    assert tuple(m[1, 2]) == (0.0, 0.0, f(0.5) * (f(0.16) * f(4.0)), 0.5)
+    # the operand is sampled nearest: result texel (x, y) reads (x / 3, y / 3)
+    ramp = np.zeros((4, 4, 4), np.float32)
+    ramp[..., 3] = np.arange(16, dtype=np.float32).reshape(4, 4)
+    assert np.array_equal(SH.crt(ramp, 12, 12)[..., 3], np.repeat(np.repeat(ramp[..., 3], 3, 0), 3, 1))
+
+
+def test_flat_field_reproduces_blessed_hash():
+    """custom.rs:187-285: the background corrected by 8-octave fractal noise
+    (amplitude 0.1, damping 0.2, grid 10) stored as Luma8 sRGB, mean 0.062.
+    Multiplying by the field instead of dividing, or transposing it, leaves
+    the blessed set."""
+    from oracle import blockhash as B
+    from oracle import shaders as SH
+    bg = _golden_rgba("background")
+    lin = C.decode_u8(bg, 5).astype(np.float32)
+    noise = PL.fractal_noise(512, 512, 8, 0.1, 0.2, scale=(10.0, 10.0))
+    field = S.stage_decode(S.stage_encode(noise.reshape(-1, 4), 0, 4, 5), 0, 4, 5).reshape(512, 512, 4)
+    assert B.blockhash256(C.encode_u8(SH.flat_field(lin, field, 0.124 / 2.0), 5)) in _blessed("flat_field")
+    wrong = lin.copy()
+    wrong[..., :3] = lin[..., :3] * field[..., :1] / np.float32(0.062)
+    assert B.blockhash256(C.encode_u8(wrong, 5)) not in _blessed("flat_field")
+    assert B.blockhash256(C.encode_u8(SH.flat_field(lin, field.transpose(1, 0, 2).copy(), 0.062), 5)) \
+        not in _blessed("flat_field")
+    # a field of exactly the mean leaves the image unchanged
+    flat = np.full_like(field, np.float32(0.062))
+    assert np.array_equal(SH.flat_field(lin, flat, 0.062), lin)

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 from dataclasses import dataclass, field
 from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
@@ -427,6 +428,81 @@ class CommandBuffer:
         d = DescriptorC()
         _check(self._lib.zs_command_describe(self._h, reg, ctypes.byref(d)))
         return Descriptor.from_c(d)
+
+    def _dynamic(self, fn, regs, dynamic: "ShaderCommand") -> int:
+        src = dynamic.source()
+        sd = ShaderData()
+        desc = dynamic.data(sd)
+        buf = ctypes.create_string_buffer(sd.bytes, max(len(sd.bytes), 1))
+        c = _abi.ShaderCommandC(src._h, ctypes.cast(buf, ctypes.c_void_p) if sd.bytes else None, len(sd.bytes),
+                                desc.c())
+        return self._reg(fn, *regs, ctypes.byref(c))
+
+    def construct_dynamic(self, dynamic: "ShaderCommand") -> int:
+        """``CommandBuffer::construct_dynamic`` (command.rs:2933-2961)."""
+        return self._dynamic(self._lib.zs_command_construct_dynamic, (), dynamic)
+
+    def unary_dynamic(self, op: int, dynamic: "ShaderCommand") -> int:
+        """``CommandBuffer::unary_dynamic`` (command.rs:2963-3004)."""
+        return self._dynamic(self._lib.zs_command_unary_dynamic, (op,), dynamic)
+
+    def binary_dynamic(self, lhs: int, rhs: int, dynamic: "ShaderCommand") -> int:
+        """``CommandBuffer::binary_dynamic`` (command.rs:3006-3048)."""
+        return self._dynamic(self._lib.zs_command_binary_dynamic, (lhs, rhs), dynamic)
+
+
+SHADER_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shaders")
+
+
+class ShaderSource:
+    """``ShaderSource`` (command/dynamic.rs:31-34) in its HIP form: a gfx950
+    code object and the name of its stage kernel (``zh_shader_create``)."""
+
+    def __init__(self, code: bytes, kernel: str):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.zh_shader_create(code, len(code), kernel.encode(), ctypes.byref(h)))
+        self._h = h
+        self.kernel = kernel
+
+    @staticmethod
+    def hip_code_object(path: str, kernel: str) -> "ShaderSource":
+        with open(path, "rb") as f:
+            return ShaderSource(f.read(), kernel)
+
+    @staticmethod
+    def bundled(name: str) -> "ShaderSource":
+        """One of the package's own stage kernels (``shaders/<name>.co``,
+        kernel ``name``)."""
+        return ShaderSource.hip_code_object(os.path.join(SHADER_DIR, name + ".co"), name)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _abi is not None:
+            self._lib.zh_shader_release(self._h)
+            self._h = None
+
+
+class ShaderData:
+    """``ShaderData`` (command/dynamic.rs:40-51): the shader's argument bytes."""
+
+    def __init__(self):
+        self.bytes = b""
+
+    def set_data(self, values, dtype=np.float32):
+        """``ShaderData::set_data``: the values as plain bytes of ``dtype``."""
+        self.bytes = np.ascontiguousarray(values, dtype=dtype).tobytes()
+
+
+class ShaderCommand:
+    """The ``ShaderCommand`` trait (command/dynamic.rs:7-28): ``source`` gives
+    the kernel, ``data`` sets its argument bytes and returns the result's
+    descriptor."""
+
+    def source(self) -> ShaderSource:
+        raise NotImplementedError
+
+    def data(self, data: ShaderData) -> Descriptor:
+        raise NotImplementedError
 
 
 @dataclass(frozen=True)

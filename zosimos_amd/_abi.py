@@ -58,6 +58,10 @@ class PoolBindingC(ctypes.Structure):
     _fields_ = [("slot", u32), ("is_output", u32), ("key", u64)]
 
 
+class ShaderCommandC(ctypes.Structure):
+    _fields_ = [("shader", vp), ("data", vp), ("data_len", u32), ("desc", DescriptorC)]
+
+
 class LearnOptionsC(ctypes.Structure):
     _fields_ = [("uniform_points", u32), ("face_points", u32), ("steps", u32), ("batch", u32), ("lr", f32),
                 ("seed", u32), ("lattice", u32), ("max_delta_e", f32)]
@@ -123,6 +127,9 @@ _SIGNATURES = {
     "zh_execution_retire_output": (i32, [vp, u32, i32, ctypes.POINTER(u64)]),
     "zh_execution_retire_input": (i32, [vp, u32, ctypes.POINTER(u64)]),
     "zh_execution_finish": (i32, [vp]),
+    "zh_shader_create": (i32, [vp, sz, ctypes.c_char_p, ctypes.POINTER(vp)]),
+    "zh_shader_retain": (i32, [vp]),
+    "zh_shader_release": (i32, [vp]),
     "zh_last_error": (ctypes.c_char_p, []),
     "zh_version": (ctypes.c_char_p, []),
     # zosimos_command.h
@@ -159,6 +166,9 @@ _SIGNATURES = {
     "zs_command_buffer_zero": (i32, [vp, ctypes.c_uint64, ctypes.POINTER(u32)]),
     "zs_command_buffer_size": (i32, [vp, u32, ctypes.POINTER(ctypes.c_uint64)]),
     "zs_command_from_buffer": (i32, [vp, u32, ctypes.POINTER(DescriptorC), ctypes.POINTER(u32)]),
+    "zs_command_construct_dynamic": (i32, [vp, ctypes.POINTER(ShaderCommandC), ctypes.POINTER(u32)]),
+    "zs_command_unary_dynamic": (i32, [vp, u32, ctypes.POINTER(ShaderCommandC), ctypes.POINTER(u32)]),
+    "zs_command_binary_dynamic": (i32, [vp, u32, u32, ctypes.POINTER(ShaderCommandC), ctypes.POINTER(u32)]),
     "zs_command_knob": (i32, [vp, u32, ctypes.POINTER(i32)]),
     "zs_command_describe": (i32, [vp, u32, ctypes.POINTER(DescriptorC)]),
     "zs_command_generic": (i32, [vp, ctypes.POINTER(u32)]),

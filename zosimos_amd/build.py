@@ -27,6 +27,7 @@ SOURCES = [
     "zh_command.cpp",
     "zh_surrogate.cpp",
     "zh_pool.cpp",
+    "zh_dynamic.cpp",
 ]
 HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_internal.h", "zh_mlp.h"]
 ARCH = "gfx950"
@@ -60,8 +61,34 @@ def _compile(src: str) -> str:
     return obj
 
 
+SHADER_DIR = os.path.join(HERE, "shaders")
+
+
+def build_shaders() -> list:
+    """Compile each user stage kernel under ``shaders/`` into a gfx950 code
+    object next to its source (``hipcc --genco``), the form
+    ``zh_shader_create`` takes."""
+    out = []
+    inc_dir = os.path.join(os.path.dirname(HERE), "include")
+    for f in sorted(os.listdir(SHADER_DIR)):
+        if not f.endswith(".hip"):
+            continue
+        co = os.path.join(SHADER_DIR, os.path.splitext(f)[0] + ".co")
+        # Contraction as fast as usual, but a shader's own
+        # `#pragma clang fp contract(off)` is honoured.
+        cmd = [hipcc(), "--genco", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=fast-honor-pragmas",
+               *DEVICE_FLAGS, "-I", inc_dir,
+               os.path.join(SHADER_DIR, f), "-o", co]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc --genco failed for {f}:\n{res.stderr}")
+        out.append(co)
+    return out
+
+
 def build(force: bool = True, jobs: int = 6) -> str:
     """Compile every source (in parallel) and link the shared library."""
+    build_shaders()
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     hdr_mtime = max(os.path.getmtime(os.path.join(CSRC, h)) for h in HEADERS if os.path.exists(os.path.join(CSRC, h)))

@@ -20,7 +20,8 @@ enum OpKind {
   K_INPUT, K_OUTPUT, K_CROP, K_CONVERT, K_INSCRIBE, K_TRANSMUTE, K_AFFINE, K_BLEND, K_SOLID, K_ADJUST, K_BILINEAR,
   K_DERIVATIVE, K_BUFFER, K_FROM_BUFFER, K_NORMAL2D, K_NOISE, K_PALETTE, K_EXTRACT, K_INJECT,
   K_RESULT,  // Op::InvokedResult (command.rs:911-916): a result register of the Invoke that follows
-  K_INVOKE   // Op::Invoke (command.rs:918-923)
+  K_INVOKE,  // Op::Invoke (command.rs:918-923)
+  K_DYNAMIC  // Op::DynamicImage (command.rs:180-185): a user stage kernel
 };
 
 // GenericDescriptor (command.rs:89-92): size and chroma (texel + colour),
@@ -53,6 +54,8 @@ struct CmdOp {
   uint32_t fn = 0;
   std::vector<uint32_t> args, results;
   std::vector<GenDesc> generics;
+  // K_DYNAMIC: the shader (blob = zh_dynamic_call + the shader data).
+  zh::ShaderRef dyn_shader;
 };
 
 // CIE 1931 2-degree chromaticities of the white points A .. F11 (the order of
@@ -168,6 +171,7 @@ struct zs_linked {
   std::vector<int64_t> input_slot, output_slot;  // per register of the entry buffer
   // RegisterKnob {link_idx, register} -> knob (command.rs:2128-2140).
   std::vector<std::array<int64_t, 3>> knobs;
+  std::vector<zh::ShaderRef> shaders;  // held for the zh_dynamic_call blobs
 };
 
 // The register's descriptor, generic or not (describe_reg(..).as_texture()).
@@ -941,12 +945,59 @@ zh_status zs_command_invoke(zs_command* cmd, const zs_command* body, const uint3
   return ZH_OK;
 }
 
+// construct/unary/binary_dynamic (command.rs:2933-3048): the operands must
+// describe as textures; the shader's data() gives the result's descriptor.
+static zh_status dynamic_op(zs_command* cmd, uint32_t nargs, uint32_t lhs, uint32_t rhs,
+                            const zs_shader_command* dyn, uint32_t* reg) {
+  if (!cmd || !dyn || !reg || !dyn->shader || (dyn->data_len && !dyn->data))
+    return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  const uint32_t regs[2] = {lhs, rhs};
+  for (uint32_t k = 0; k < nargs; ++k) {
+    GenDesc g;
+    zh_status st = reg_gen(cmd, regs[k], &g);
+    if (st != ZH_OK) return st;
+    if (g.size_var >= 0 || g.chroma_var >= 0)
+      return fail(ZH_ERR_TYPE, "a dynamic shader's operand needs a concrete descriptor");
+  }
+  std::string why;
+  if (!descriptor_valid(dyn->desc, &why)) return fail(ZH_ERR_TYPE, "dynamic shader result: " + why);
+  CmdOp op;
+  op.kind = K_DYNAMIC;
+  op.src = lhs;
+  op.above = rhs;
+  op.desc = normalized(dyn->desc);
+  op.dyn_shader = zh::ShaderRef(dyn->shader);
+  zh_dynamic_call call;
+  std::memset(&call, 0, sizeof(call));
+  call.shader = dyn->shader;
+  call.num_args = nargs;
+  call.data_len = dyn->data_len;
+  op.blob.resize(sizeof(call) + dyn->data_len);
+  std::memcpy(op.blob.data(), &call, sizeof(call));
+  if (dyn->data_len) std::memcpy(op.blob.data() + sizeof(call), dyn->data, dyn->data_len);
+  *reg = push(cmd, std::move(op));
+  return ZH_OK;
+}
+
+zh_status zs_command_construct_dynamic(zs_command* cmd, const zs_shader_command* dynamic, uint32_t* reg) {
+  return dynamic_op(cmd, 0, 0, 0, dynamic, reg);
+}
+
+zh_status zs_command_unary_dynamic(zs_command* cmd, uint32_t src, const zs_shader_command* dynamic, uint32_t* reg) {
+  return dynamic_op(cmd, 1, src, 0, dynamic, reg);
+}
+
+zh_status zs_command_binary_dynamic(zs_command* cmd, uint32_t lhs, uint32_t rhs, const zs_shader_command* dynamic,
+                                    uint32_t* reg) {
+  return dynamic_op(cmd, 2, lhs, rhs, dynamic, reg);
+}
+
 zh_status zs_command_knob(zs_command* cmd, uint32_t reg, int32_t* knob) {
   if (!cmd || !knob || reg >= cmd->ops.size()) return fail(ZH_ERR_OTHER, "bad register");
   CmdOp& op = cmd->ops[reg];
   if (op.kind == K_INPUT || op.kind == K_OUTPUT || op.kind == K_TRANSMUTE || op.kind == K_CROP ||
       op.kind == K_INSCRIBE || op.kind == K_BLEND || op.kind == K_FROM_BUFFER || op.kind == K_RESULT ||
-      op.kind == K_INVOKE)
+      op.kind == K_INVOKE || op.kind == K_DYNAMIC)
     return fail(ZH_ERR_TYPE, "operation has no parameter data to override");
   if (op.knob < 0) op.knob = cmd->next_knob++;
   *knob = op.knob;
@@ -1062,6 +1113,17 @@ static zh_status compile_flat(const zs_command* cmd, zs_linked** out) {
       case K_NOISE:
         draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_FRACTAL_NOISE, {}, c.blob, c.knob);
         break;
+      case K_DYNAMIC: {
+        // command.rs:2767-2830: the operands pushed, one full-screen draw.
+        zh_dynamic_call call;
+        std::memcpy(&call, c.blob.data(), sizeof(call));
+        std::vector<uint32_t> operands;
+        if (call.num_args > 0) operands.push_back(c.src);
+        if (call.num_args > 1) operands.push_back(c.above);
+        draw(r, ZH_TARGET_DISCARD, ZH_INIT_PAINT_FULLSCREEN, ZH_SHADER_DYNAMIC, operands, c.blob, -1);
+        L->shaders.push_back(c.dyn_shader);
+        break;
+      }
       case K_BUFFER:
         break;  // read through the images constructed from it
       case K_FROM_BUFFER: {

@@ -5,6 +5,7 @@
 
 #include <array>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,34 @@ Mat3 mat_identity();
 bool primaries_to_xyz(uint32_t primaries, uint32_t whitepoint, Mat3* out);
 Mat3 mat_from_std140(const void* blob);  // 12 floats, column-major with padding
 
+// ---- user stage kernels (zh_dynamic.cpp) ------------------------------------
+// A counted reference to a zh_shader (retained on copy, released on
+// destruction).
+class ShaderRef {
+ public:
+  ShaderRef() = default;
+  explicit ShaderRef(zh_shader* s);  // takes a new reference
+  ShaderRef(const ShaderRef& o);
+  ShaderRef& operator=(const ShaderRef& o);
+  ~ShaderRef();
+  zh_shader* get() const { return s_; }
+ private:
+  zh_shader* s_ = nullptr;
+};
+// The kernel of `shader` on `device` (the current device), its code object
+// loaded on first use.
+zh_status shader_function(zh_shader* shader, int device, hipFunction_t* fn);
+// Is the code object already loaded on `device`?
+bool shader_loaded(zh_shader* shader, int device);
+const char* shader_kernel_name(zh_shader* shader);
+
+// A user stage of a plan: the shader and its data.
+struct DynamicCall {
+  ShaderRef shader;
+  std::vector<uint8_t> data;
+  uint32_t num_args = 0;
+};
+
 // ---- planned program -------------------------------------------------------
 struct StagePlan {
   StageDesc desc;                 // host copy; slot ptrs patched per launch
@@ -49,6 +78,7 @@ struct StagePlan {
   // slot -> what it refers to: >=0 input slot, or ~output slot (<0), or
   // INTERMEDIATE_BASE + buffer id.
   std::vector<int64_t> slot_ref;
+  int dynamic = -1;               // STAGE_DYNAMIC: index into Plan::dynamics
   std::string text;               // description
 };
 
@@ -65,6 +95,7 @@ struct Plan {
   uint32_t user_inputs = 0;
   std::vector<uint32_t> buffer_ops;
   bool batched = false;                      // resample stages run as one tile queue
+  std::vector<DynamicCall> dynamics;         // user stages (STAGE_DYNAMIC)
   std::string text;
 };
 
@@ -163,6 +194,10 @@ struct zh_program {
   uint8_t* arena = nullptr;
   size_t arena_size = 0, arena_used = 0;
   std::vector<zh::StageDesc> h_stage_vec;
+  // User stages: the loaded kernel and the device copy of the shader data,
+  // per entry of plan.dynamics.
+  std::vector<hipFunction_t> dyn_fn;
+  std::vector<void*> dyn_data;
 };
 
 

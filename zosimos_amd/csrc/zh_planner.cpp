@@ -122,7 +122,11 @@ static const Mat3 kM2 = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951
 // texture (its operand is materialised).  Both are always the root of a stage
 // of their own, which runs on the 2-D interpreter: they need pixel
 // coordinates.
-enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL, N_PALETTE, N_INJECT };
+// N_DYNAMIC: a user stage kernel over RGBA f32 copies of its operands (a, b:
+// identity nodes stored as RGBA f32), itself stored as RGBA f32; the
+// register's value is an identity node over it that rounds through the
+// register's declared texel, as the reference's target texture holds it.
+enum NodeKind { N_INPUT, N_SOLID, N_UNARY, N_BLEND, N_PAINT, N_BYTES, N_GRID, N_STENCIL, N_PALETTE, N_INJECT, N_DYNAMIC };
 enum MicroKind { M_MAT3, M_CBRT, M_CUBE, M_CLAMP, M_PREMUL, M_UNPREMUL, M_SRLAB, M_SRLAB_INV, M_CHANNEL };
 
 struct Micro {
@@ -153,6 +157,9 @@ struct Node {
   bool mat = false;     // materialised by its own stage
   bool sampled = false; // read as a texture by a stencil: stored even when constant
   int buffer = -1;      // intermediate buffer id
+  int dyn = -1;         // N_DYNAMIC: index into Plan::dynamics
+  bool f32_buffer = false;  // stored as RGBA f32 linear (a user kernel's operand or result)
+  bool declared = false;    // rounded through `desc` wherever it is read, in every mode
 };
 
 struct Builder {
@@ -314,6 +321,66 @@ static zh_status parse(const std::vector<zh_descriptor>& tex, const std::vector<
         n.h = td.height;
         if (op.initializer == ZH_INIT_PAINT_FULLSCREEN) {
           const int src = op.num_operands > 0 ? B.cur[op.operands[0]] : -1;
+          if (op.shader == ZH_SHADER_DYNAMIC) {
+            // FragmentShaderInvocation::Runtime (command.rs:2767-2830).
+            zh_dynamic_call call;
+            if (!need_blob(sizeof(call))) return fail(ZH_ERR_INVALID_ARGUMENT, "dynamic blob shorter than its header" + at);
+            std::memcpy(&call, blob.data(), sizeof(call));
+            if (!call.shader) return fail(ZH_ERR_INVALID_ARGUMENT, "dynamic call without a shader" + at);
+            if (op.knob >= 0) return fail(ZH_ERR_UNSUPPORTED, "a dynamic shader's data cannot be a knob" + at);
+            if (call.num_args > 2 || call.num_args != op.num_operands)
+              return fail(ZH_ERR_INVALID_ARGUMENT, "dynamic shader takes " + std::to_string(call.num_args) +
+                                                       " operands, the draw pushes " + std::to_string(op.num_operands) + at);
+            if (blob.size() < sizeof(call) + (size_t)call.data_len)
+              return fail(ZH_ERR_INVALID_ARGUMENT, "dynamic blob shorter than its shader data" + at);
+            DynamicCall dc;
+            dc.shader = ShaderRef(call.shader);
+            dc.data.assign(blob.begin() + sizeof(call), blob.begin() + sizeof(call) + call.data_len);
+            dc.num_args = call.num_args;
+            plan.dynamics.push_back(std::move(dc));
+            auto f32 = [](uint32_t w, uint32_t h) {
+              zh_descriptor d;
+              std::memset(&d, 0, sizeof(d));
+              d.width = w;
+              d.height = h;
+              d.texel = zh_texel{ZH_BITS_FLOAT32X4, ZH_PARTS_RGBA};
+              d.texel_stride = 16;
+              d.color.model = ZH_COLOR_SCALARS;
+              d.color.transfer = ZH_TRANSFER_LINEAR;
+              return d;
+            };
+            Node r;
+            r.kind = N_DYNAMIC;
+            r.dyn = (int)plan.dynamics.size() - 1;
+            r.desc = f32(td.width, td.height);
+            r.w = td.width;
+            r.h = td.height;
+            r.f32_buffer = true;
+            r.label = std::string("dynamic:") + shader_kernel_name(call.shader);
+            for (uint32_t k = 0; k < call.num_args; ++k) {
+              const int x = B.cur[op.operands[k]];
+              Node c;
+              c.kind = N_UNARY;
+              c.a = x;
+              c.desc = f32(B.nodes[x].w, B.nodes[x].h);
+              c.w = B.nodes[x].w;
+              c.h = B.nodes[x].h;
+              c.f32_buffer = true;
+              c.label = "dynamic-operand";
+              (k == 0 ? r.a : r.b) = B.add(c);
+            }
+            const int rid = B.add(r);
+            Node u;
+            u.kind = N_UNARY;
+            u.a = rid;
+            u.desc = td;
+            u.w = td.width;
+            u.h = td.height;
+            u.declared = true;
+            u.label = "dynamic-result";
+            B.cur[op.dst] = B.add(u);
+            continue;
+          }
           if (op.shader != ZH_SHADER_SOLID_RGB && op.shader != ZH_SHADER_BILINEAR &&
               op.shader != ZH_SHADER_DISTRIBUTION_NORMAL2D && op.shader != ZH_SHADER_FRACTAL_NOISE) {
             if (src < 0) return fail(ZH_ERR_INVALID_ARGUMENT, "full-screen shader without operand" + at);
@@ -850,6 +917,9 @@ struct Emitter {
       case N_PAINT:
         err = "internal: paint node inside a per-pixel expression";
         return false;
+      case N_DYNAMIC:
+        err = "internal: user kernel inside a per-pixel expression";
+        return false;
     }
     return false;
   }
@@ -859,7 +929,7 @@ struct Emitter {
   // formats themselves.  Returns whether X is still premultiplied.
   bool quantise(const Node& n, bool pm, int root) {
     const int64_t id = (int64_t)(&n - &B.nodes[0]);
-    if (!strict || id == root) return pm;
+    if ((!strict && !n.declared) || id == root) return pm;
     if (pm) op(OP_UNPREMUL);
     const int s = slot(-1000000 - id, n.desc);
     op(OP_QUANT, (uint8_t)s);
@@ -1057,6 +1127,11 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       mark(n.a);
     }
     if (n.kind == N_PALETTE) mark((int)id);
+    if (n.kind == N_DYNAMIC) {
+      mark((int)id);
+      if (n.a >= 0) mark(n.a);
+      if (n.b >= 0) mark(n.b);
+    }
     if (n.kind == N_BYTES && B.nodes[n.a].kind != N_INPUT) mark(n.a);
     if (n.uses > 1 && n.kind != N_BYTES) mark((int)id);
   }
@@ -1132,7 +1207,9 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       if (live[j] && B.nodes[j].kind == N_BYTES && B.nodes[j].a == (int)id) bytes_child = true;
     // Strict registers keep their declared texels, so a later stage reads
     // exactly what the reference's register would hold.
-    if (!bytes_child && !strict) {
+    if (n.f32_buffer) {
+      d = n.desc;  // RGBA f32 linear, the user kernel's layout
+    } else if (!bytes_child && !strict && !n.declared) {
       // FP16 where the declared register allows it, FP32 otherwise.
       const bool wide = d.texel.bits == ZH_BITS_FLOAT32X4 || d.texel.bits == ZH_BITS_INT16X4 ||
                         d.texel.bits == ZH_BITS_INT16 || d.texel.bits == 11 || d.texel.bits == 12;
@@ -1208,6 +1285,26 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       continue;
     }
 
+    if (n.kind == N_DYNAMIC) {
+      // The user kernel: operands and result in RGBA f32 intermediates.
+      sp.desc.kind = STAGE_DYNAMIC;
+      sp.dynamic = n.dyn;
+      std::ostringstream t;
+      t << "dynamic " << n.w << "x" << n.h << ": ";
+      for (int operand : {n.a, n.b}) {
+        if (operand < 0) continue;
+        const Node& c = B.nodes[operand];
+        sp.slot_ref.push_back(kIntermediateBase + c.buffer);
+        sp.alg_bytes += (uint64_t)c.w * c.h * 16;
+      }
+      if (n.buffer < 0) return fail(ZH_ERR_UNSUPPORTED, "internal: a user kernel's result has no buffer");
+      sp.slot_ref.push_back(kIntermediateBase + n.buffer);
+      sp.alg_bytes += (uint64_t)n.w * n.h * 16;
+      t << plan.dynamics[n.dyn].num_args << " operand(s) [kernel " << n.label << "]";
+      sp.text = t.str();
+      plan.stages.push_back(sp);
+      continue;
+    }
     if (n.kind == N_INPUT) {
       // Passthrough of an input (tests/color_support.rs): row copies when the
       // texels agree (the row pitches may differ), a decode/encode stage

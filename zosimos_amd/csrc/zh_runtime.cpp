@@ -330,6 +330,8 @@ static void free_device(zh_program* p) {
     // Static-scratch tables are never freed; only the host side is dropped.
     p->units.clear();
     p->d_intermediates.clear();
+    p->dyn_data.clear();
+    p->dyn_fn.clear();
     p->buffer_images.clear();
     p->d_tile_layers.clear();
     p->tile_layer_words.clear();
@@ -351,6 +353,9 @@ static void free_device(zh_program* p) {
   p->units.clear();
   for (void* b : p->d_intermediates) hipFree(b);
   p->d_intermediates.clear();
+  for (void* b : p->dyn_data) hipFree(b);
+  p->dyn_data.clear();
+  p->dyn_fn.clear();
   for (zh_image& b : p->buffer_images)
     if (b.ptr) hipFree(b.ptr);
   p->buffer_images.clear();
@@ -513,6 +518,17 @@ static zh_status alloc_device(zh_program* p) {
     void* b = nullptr;
     ZH_ALLOC(p, image_bytes(d), &b);
     p->d_intermediates.push_back(b);
+  }
+  // User stages: the kernel on this device and a device copy of the data.
+  for (const DynamicCall& dc : p->plan.dynamics) {
+    hipFunction_t fn = nullptr;
+    zh_status st = shader_function(dc.shader.get(), ctx->device, &fn);
+    if (st != ZH_OK) return st;
+    void* b = nullptr;
+    ZH_ALLOC(p, std::max<size_t>(dc.data.size(), 16), &b);
+    if (!dc.data.empty()) ZH_HIP(hipMemcpy(b, dc.data.data(), dc.data.size(), hipMemcpyHostToDevice));
+    p->dyn_fn.push_back(fn);
+    p->dyn_data.push_back(b);
   }
   make_units(p);
   for (size_t s = 0; s < n; ++s) {
@@ -790,7 +806,9 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
       *stride = 0;
       return 0;  // format-only slot (strict quantisation)
     };
-    if (d.kind == STAGE_COPY) {
+    if (d.kind == STAGE_DYNAMIC) {
+      // A user kernel reads program-owned intermediates, not a StageDesc.
+    } else if (d.kind == STAGE_COPY) {
       uint64_t s0, s1;
       d.copy_src = resolve(refs[0], &s0);
       d.copy_dst = resolve(refs[1], &s1);
@@ -925,6 +943,28 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     case STAGE_COPY:
       ZH_HIP(zh_launch_copy(d, sp.desc.copy_row_bytes, sp.desc.copy_rows, s));
       break;
+    case STAGE_DYNAMIC: {
+      // The user kernel over the result, 16 x 16 texels per workgroup.
+      const DynamicCall& dc = p->plan.dynamics[sp.dynamic];
+      zh_dynamic_args a;
+      std::memset(&a, 0, sizeof(a));
+      a.width = sp.desc.width;
+      a.height = sp.desc.height;
+      a.num_inputs = dc.num_args;
+      a.data_len = (uint32_t)dc.data.size();
+      for (uint32_t k = 0; k < dc.num_args; ++k) {
+        const size_t b = (size_t)(sp.slot_ref[k] - kIntermediateBase);
+        a.in[k] = static_cast<const float*>(p->d_intermediates[b]);
+        a.in_width[k] = p->plan.intermediates[b].width;
+        a.in_height[k] = p->plan.intermediates[b].height;
+      }
+      a.out = static_cast<float*>(p->d_intermediates[(size_t)(sp.slot_ref[dc.num_args] - kIntermediateBase)]);
+      a.data = p->dyn_data[sp.dynamic];
+      void* args[] = {&a};
+      ZH_HIP(hipModuleLaunchKernel(p->dyn_fn[sp.dynamic], (a.width + 15) / 16, (a.height + 15) / 16, 1, 16, 16, 1, 0,
+                                   s, args, nullptr));
+      break;
+    }
   }
   return ZH_OK;
 }
@@ -1003,6 +1043,9 @@ zh_status zh_program_launch_static(zh_context* ctx, const zh_descriptor* texture
   // No hipGraph: instantiating one allocates.
   zh_status st = init_program(p, ctx, textures, num_textures, ops, num_ops, flags | ZH_PLAN_NO_GRAPH);
   if (st != ZH_OK) return st;
+  // Loading a user kernel's code object allocates.
+  if (!p->plan.dynamics.empty())
+    return fail(ZH_ERR_UNSUPPORTED, "user stage kernels cannot run from the static scratch (loading them allocates)");
   // An earlier call's kernels on this stream may still read the scratch:
   // let them finish before its tables are overwritten.
   ZH_HIP(hipStreamSynchronize(ctx->stream));

@@ -14,6 +14,8 @@
 //                    and sampled through an affine map (bilinear, staged in
 //                    LDS per output tile, or nearest with quad coverage).
 //   STAGE_COPY       byte copy of rows (transmute / same-format inscribe).
+//   STAGE_DYNAMIC    a user kernel loaded from a code object; it reads and
+//                    writes RGBA f32 intermediates, not a StageDesc.
 #pragma once
 #include <stdint.h>
 
@@ -128,7 +130,7 @@ struct Layer {
   double cover[4];
 };
 
-enum StageKind : uint32_t { STAGE_POINTWISE = 0, STAGE_RESAMPLE = 1, STAGE_COPY = 2 };
+enum StageKind : uint32_t { STAGE_POINTWISE = 0, STAGE_RESAMPLE = 1, STAGE_COPY = 2, STAGE_DYNAMIC = 3 };
 
 struct StageDesc {
   uint32_t kind;
