@@ -332,6 +332,16 @@ typedef struct zh_binding {
  * the images it was bound with.  The host waits only when three launches
  * with changed bindings are still queued. */
 zh_status zh_program_launch(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings);
+/* Executable::launch_once (run.rs:1050-1079): one launch that consumes the
+ * program -- kernel by kernel (no graph is captured for a single run), then
+ * the program is destroyed once its kernels are done; `prog` is invalid
+ * after the call, whatever it returns. */
+zh_status zh_program_launch_once(zh_program* prog, const zh_binding* bindings, uint32_t num_bindings);
+/* Executable::dot (run.rs:490-1010): a Graphviz digraph of the plan -- input,
+ * output and intermediate images as boxes, stages (with their kernels) as
+ * ellipses, an edge per read and per write.  Same buffer protocol as
+ * zh_program_describe. */
+zh_status zh_program_dot(zh_program* prog, char* buf, size_t len, size_t* needed);
 /* hipGraph state of a program (no counterpart in the reference, which
  * records a wgpu command buffer per launch, run.rs:1394-1464).  A program of
  * two or more kernel launches is captured into a hipGraph at its first
@@ -504,15 +514,43 @@ typedef struct zh_pool_binding {
 typedef struct zh_execution zh_execution;
 zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
                                  uint32_t num_bindings, zh_execution** out);
+/* Executable::launch without stepping (run.rs:1016-1050): everything
+ * above is checked, traded and bound, but no kernel is queued until
+ * zh_execution_step.  Until its last launch is issued the program runs no
+ * other execution (ZH_ERR_INVALID_ARGUMENT), and retiring is refused. */
+zh_status zh_program_start_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
+                                uint32_t num_bindings, zh_execution** out);
+/* Execution::step_to(StepLimits { instructions }) (run.rs:1394-1464):
+ * queue up to `max_launches` more kernel launches (kernel by kernel, no
+ * hipGraph); `launched` receives how many (0 once all are queued). */
+zh_status zh_execution_step(zh_execution* exec, uint32_t max_launches, uint32_t* launched);
+/* Execution::is_running (run.rs:1381-1384): 1 while launches are left to
+ * step or the context's stream still has work. */
+zh_status zh_execution_is_running(zh_execution* exec, int32_t* running);
+/* Execution::resources_used (ResourcesUsed, run.rs:212-221, 1495-1498). */
+typedef struct zh_resources_used {
+  uint64_t buffer_mem;     /* device bytes newly allocated for the launch's images */
+  uint64_t buffer_reused;  /* device bytes reused: the pool's cache, images used in place */
+  uint64_t program_mem;    /* the program's own device memory: intermediates, buffers, tables */
+  uint32_t buffers_allocated;
+  uint32_t buffers_reused;
+} zh_resources_used;
+zh_status zh_execution_resources(zh_execution* exec, zh_resources_used* used);
 /* Retire::output (run.rs:2809-2875): move output `slot` into the pool, as
  * host bytes (`to_host`, waits for the launch) or as the device image itself
  * (no copy, no wait).  `key` receives the pool key. */
 zh_status zh_execution_retire_output(zh_execution* exec, uint32_t slot, int32_t to_host, zh_pool_key* key);
 /* Retire::input (run.rs:2791-2808): give a moved (no_read) input back. */
 zh_status zh_execution_retire_input(zh_execution* exec, uint32_t slot, zh_pool_key* key);
+/* Retire::prune (run.rs:2944-2952; a no-op there): the execution's own device buffers are
+ * freed at finish instead of kept in the pool's cache. */
+zh_status zh_execution_prune(zh_execution* exec);
 /* Retire::finish (run.rs:2962-2998): wait, return what is not retired to the
  * pool and its cache, free the execution. */
 zh_status zh_execution_finish(zh_execution* exec);
+/* Retire::finish_by_discarding (run.rs:2989-2997): wait and drop everything
+ * not retired -- moved inputs stay out of the pool, buffers are freed. */
+zh_status zh_execution_discard(zh_execution* exec);
 
 /* Message of the last failed call on this thread (never NULL). */
 const char* zh_last_error(void);

@@ -154,3 +154,101 @@ def test_no_read_input_moves_out_for_the_launch(ctx):
     with pytest.raises(z.ZosimosError):
         exe.launch_pool(pool, {b: kb}, {o: 0})
     assert pool.kind(kf) == "host"
+
+
+def crt_chain(ctx, w=96, h=64):
+    """Three launches: the operand copy, a user kernel, the encode."""
+    class Crt(z.ShaderCommand):
+        def source(self):
+            return z.ShaderSource.bundled("crt")
+
+        def data(self, d):
+            d.set_data([3 * w, 3 * h], np.uint32)
+            return z.Descriptor.rgba8_srgb(3 * w, 3 * h)
+
+    d = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    i = cmd.input(d)
+    o, od = cmd.output(cmd.unary_dynamic(i, Crt()))
+    prog = z.Linker.from_included().compile(cmd)
+    return prog, prog.lower_to(ctx), i, o, d, od
+
+
+def test_stepped_execution(ctx):
+    """Execution::step_to: nothing runs before the first step, each step
+    queues one launch, retiring waits for the last; the result equals the
+    launch stepped at once."""
+    prog, exe, i, o, d, od = crt_chain(ctx)
+    rng = np.random.default_rng(2)
+    pool = z.Pool()
+    k = pool.insert(d, rng.integers(0, 256, d.nbytes, dtype=np.uint8))
+    run = exe.launch_pool(pool, {i: k}, {o: 0})
+    want = pool.as_bytes(run.retire_output(o))
+    run.finish()
+
+    run = exe.start_pool(pool, {i: k}, {o: 0})
+    assert run.is_running()
+    with pytest.raises(z.ZosimosError):
+        run.retire_output(o)  # launches left to step
+    with pytest.raises(z.ZosimosError):
+        exe.launch_pool(pool, {i: k}, {o: 0})  # the program is busy stepping
+    steps = []
+    while True:
+        n = run.step(1)
+        if n == 0:
+            break
+        steps.append(n)
+    assert steps == [1, 1, 1]
+    got = pool.as_bytes(run.retire_output(o))
+    run.finish()
+    assert np.array_equal(got, want)
+    # a larger limit queues the rest at once
+    run = exe.start_pool(pool, {i: k}, {o: 0})
+    assert run.step(100) == 3 and run.step(5) == 0
+    assert np.array_equal(pool.as_bytes(run.retire_output(o)), want)
+    run.finish()
+    report("pool_stepped", steps=len(steps))
+
+
+def test_resources_used_and_reuse(ctx):
+    """Execution::resources_used: the first launch allocates its upload and
+    output buffers, the next one reuses them from the pool's cache; a pruned
+    execution gives nothing back to the cache."""
+    prog, exe, i, o, d, od = crt_chain(ctx, 64, 32)
+    pool = z.Pool()
+    k = pool.insert(d, np.full(d.nbytes, 7, np.uint8))
+    run = exe.launch_pool(pool, {i: k}, {o: 0})
+    u1 = run.resources_used()
+    run.finish()
+    assert u1["buffers_allocated"] == 2 and u1["buffers_reused"] == 0
+    assert u1["buffer_mem"] == d.nbytes + od.nbytes and u1["program_mem"] > 0
+    run = exe.launch_pool(pool, {i: k}, {o: 0})
+    u2 = run.resources_used()
+    run.prune()
+    run.finish()
+    assert u2["buffers_allocated"] == 0 and u2["buffers_reused"] == 2 and u2["buffer_reused"] == u1["buffer_mem"]
+    assert pool.cached_buffers() == 0
+    report("pool_resources", first=u1, second=u2)
+
+
+def test_finish_by_discarding_and_launch_once(ctx):
+    prog, exe, i, o, d, od = crt_chain(ctx, 32, 32)
+    pool = z.Pool()
+    data = np.arange(d.nbytes, dtype=np.uint32).astype(np.uint8)
+    k = pool.insert(d, data)
+    run = exe.launch_pool(pool, {i: k}, {o: 0})
+    want = pool.as_bytes(run.retire_output(o))
+    run.finish()
+    pool.set_no_read(k)
+    run = exe.launch_pool(pool, {i: k}, {o: 0})
+    cached = pool.cached_buffers()
+    run.finish_by_discarding()
+    # the moved input is dropped with the execution; no buffer went back
+    assert pool.kind(k) == "late bound" and pool.cached_buffers() == cached
+    # launch_once consumes the executable
+    once = prog.lower_to(ctx)
+    src, out = ctx.image(d), ctx.image(od)
+    src.upload(data)
+    once.launch_once({i: src}, {o: out})
+    assert once._h is None
+    assert np.array_equal(out.download().reshape(-1), want)

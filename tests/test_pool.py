@@ -101,3 +101,32 @@ def test_one_image_is_not_bound_twice():
     k, k2 = pool.insert(d, np.zeros(d.nbytes, np.uint8)), pool.insert(d, np.zeros(d.nbytes, np.uint8))
     expect(z.Status.INVALID_ARGUMENT, exe.launch_pool, pool, {b: k, f: k}, {o: 0})
     expect(z.Status.NO_DEVICE, exe.launch_pool, pool, {b: k, f: k2}, {o: 0})
+
+
+def test_dot_names_every_image_and_stage():
+    """Executable::dot (run.rs:490-1010): images as boxes, stages as
+    ellipses, an edge per read and write."""
+    exe, (b, f, o), _ = inscribe_program()
+    dot = exe.dot()
+    assert dot.startswith("digraph program {") and dot.rstrip().endswith("}")
+    # the inscribe of two RGBA8 inputs: one resample stage reading both
+    assert "in0 -> s0;" in dot and "in1 -> s0;" in dot and "s0 -> out0;" in dot
+    # a program with an intermediate: a shared value read by two outputs' stages
+    d = z.Descriptor.rgba8_srgb(32, 32)
+    c = z.CommandBuffer()
+    i = c.input(d)
+    lin = c.color_convert(i, z.Color(transfer=z.Transfer.Linear), z.Texel(z.SampleBits.Float16x4, z.SampleParts.RgbA))
+    c.output(c.crop(lin, z.Rectangle(0, 0, 16, 16)))
+    c.output(c.crop(lin, z.Rectangle(16, 16, 32, 32)))
+    dot2 = z.Linker.from_included().compile(c).lower_to(None).dot()
+    assert "s0 -> t0;" in dot2 and "t0 -> s1;" in dot2 and "t0 -> s2;" in dot2
+    assert dot2.count("shape=ellipse") == 3
+
+
+def test_stepped_start_needs_a_device():
+    exe, (b, f, o), (bg, fg, od) = inscribe_program()
+    pool = z.Pool()
+    kb = pool.insert(bg, np.zeros(bg.nbytes, np.uint8))
+    kf = pool.insert(fg, np.zeros(fg.nbytes, np.uint8))
+    expect(z.Status.NO_DEVICE, exe.start_pool, pool, {b: kb, f: kf}, {o: 0})
+    expect(z.Status.MISSING_KEY, exe.start_pool, pool, {b: kb}, {o: 0})

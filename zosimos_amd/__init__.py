@@ -805,16 +805,39 @@ class Executable:
         arr, n = self._bindings(inputs, outputs)
         _check(self._lib.zh_program_launch(self._h, arr, n))
 
-    def launch_pool(self, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
-        """Launch with inputs read from pool keys and outputs written to pool
-        keys (0: a new key at retirement); ``Executable::launch`` with a pool
-        (run.rs:1016-1167).  Returns the running execution."""
+    def _pool_launch(self, fn, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
         binds = [PoolBindingC(self.program.input_slot(r), 0, k) for r, k in inputs.items()]
         binds += [PoolBindingC(self.program.output_slot(r), 1, k) for r, k in outputs.items()]
         arr = (PoolBindingC * max(len(binds), 1))(*binds)
         h = ctypes.c_void_p()
-        _check(self._lib.zh_program_launch_pool(self._h, pool._h, arr, len(binds), ctypes.byref(h)))
+        _check(fn(self._h, pool._h, arr, len(binds), ctypes.byref(h)))
         return Execution(h, self, pool)
+
+    def launch_pool(self, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
+        """Launch with inputs read from pool keys and outputs written to pool
+        keys (0: a new key at retirement); ``Executable::launch`` with a pool
+        (run.rs:1016-1167), stepped to the end.  Returns the execution."""
+        return self._pool_launch(self._lib.zh_program_launch_pool, pool, inputs, outputs)
+
+    def start_pool(self, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
+        """``Executable::launch`` before any step: bound and traded, nothing
+        queued until ``Execution.step``."""
+        return self._pool_launch(self._lib.zh_program_start_pool, pool, inputs, outputs)
+
+    def launch_once(self, inputs: Dict[int, "Image"], outputs: Dict[int, "Image"]):
+        """``Executable::launch_once`` (run.rs:1050-1079): one launch that
+        consumes this executable."""
+        arr, n = self._bindings(inputs, outputs)
+        h, self._h = self._h, None
+        _check(self._lib.zh_program_launch_once(h, arr, n))
+
+    def dot(self) -> str:
+        """``Executable::dot`` (run.rs:490-1010): a Graphviz digraph of the plan."""
+        need = ctypes.c_size_t()
+        _check(self._lib.zh_program_dot(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self._lib.zh_program_dot(self._h, buf, need.value, None))
+        return buf.value.decode()
 
     def time_stages(self, inputs, outputs, iterations: int = 10):
         arr, n = self._bindings(inputs, outputs)
@@ -1046,11 +1069,39 @@ class Execution:
         _check(self._lib.zh_execution_retire_input(self._h, self.exe.program.input_slot(reg), ctypes.byref(k)))
         return k.value
 
+    def step(self, limit: int = 1) -> int:
+        """``Execution::step_to(StepLimits { instructions: limit })``: queue up
+        to ``limit`` more kernel launches; returns how many were queued."""
+        n = ctypes.c_uint32()
+        _check(self._lib.zh_execution_step(self._h, limit, ctypes.byref(n)))
+        return n.value
+
+    def is_running(self) -> bool:
+        r = ctypes.c_int32()
+        _check(self._lib.zh_execution_is_running(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def resources_used(self) -> dict:
+        """``Execution::resources_used`` (run.rs:212-221)."""
+        u = _abi.ResourcesUsedC()
+        _check(self._lib.zh_execution_resources(self._h, ctypes.byref(u)))
+        return {f: getattr(u, f) for f, _ in _abi.ResourcesUsedC._fields_}
+
+    def prune(self):
+        """``Retire::prune``: free this execution's buffers at finish."""
+        _check(self._lib.zh_execution_prune(self._h))
+
     def finish(self):
         """``Retire::finish`` (run.rs:2962-2998)."""
         if getattr(self, "_h", None):
             h, self._h = self._h, None
             _check(self._lib.zh_execution_finish(h))
+
+    def finish_by_discarding(self):
+        """``Retire::finish_by_discarding``: drop everything not retired."""
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            _check(self._lib.zh_execution_discard(h))
 
     def __del__(self):
         if getattr(self, "_h", None) and _abi is not None:

@@ -62,6 +62,11 @@ class ShaderCommandC(ctypes.Structure):
     _fields_ = [("shader", vp), ("data", vp), ("data_len", u32), ("desc", DescriptorC)]
 
 
+class ResourcesUsedC(ctypes.Structure):
+    _fields_ = [("buffer_mem", u64), ("buffer_reused", u64), ("program_mem", u64), ("buffers_allocated", u32),
+                ("buffers_reused", u32)]
+
+
 class LearnOptionsC(ctypes.Structure):
     _fields_ = [("uniform_points", u32), ("face_points", u32), ("steps", u32), ("batch", u32), ("lr", f32),
                 ("seed", u32), ("lattice", u32), ("max_delta_e", f32)]
@@ -127,6 +132,14 @@ _SIGNATURES = {
     "zh_execution_retire_output": (i32, [vp, u32, i32, ctypes.POINTER(u64)]),
     "zh_execution_retire_input": (i32, [vp, u32, ctypes.POINTER(u64)]),
     "zh_execution_finish": (i32, [vp]),
+    "zh_program_start_pool": (i32, [vp, vp, ctypes.POINTER(PoolBindingC), u32, ctypes.POINTER(vp)]),
+    "zh_execution_step": (i32, [vp, u32, ctypes.POINTER(u32)]),
+    "zh_execution_is_running": (i32, [vp, ctypes.POINTER(i32)]),
+    "zh_execution_resources": (i32, [vp, ctypes.POINTER(ResourcesUsedC)]),
+    "zh_execution_prune": (i32, [vp]),
+    "zh_execution_discard": (i32, [vp]),
+    "zh_program_launch_once": (i32, [vp, ctypes.POINTER(BindingC), u32]),
+    "zh_program_dot": (i32, [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]),
     "zh_shader_create": (i32, [vp, sz, ctypes.c_char_p, ctypes.POINTER(vp)]),
     "zh_shader_retain": (i32, [vp]),
     "zh_shader_release": (i32, [vp]),

@@ -198,6 +198,17 @@ struct zh_program {
   // per entry of plan.dynamics.
   std::vector<hipFunction_t> dyn_fn;
   std::vector<void*> dyn_data;
+  // The stepped execution (zh_program_start_pool) still launching this
+  // program's kernels; other launches wait for it to finish.
+  zh_execution* stepping = nullptr;
 };
+
+namespace zh {
+// Launch pieces for stepped executions (zh_runtime.cpp).
+zh_status program_bind(zh_program* p, const zh_binding* bindings, uint32_t num_bindings);
+size_t program_units(const zh_program* p);
+zh_status program_launch_units(zh_program* p, size_t first, size_t count);
+uint64_t program_device_bytes(const zh_program* p);
+}  // namespace zh
 
 

@@ -18,6 +18,7 @@
 // are reused by the next launch with the same layout.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -89,21 +90,36 @@ struct zh_execution {
     PoolImage moved_image;
   };
   std::vector<Slot> in, out;
+  // Stepped executions (Execution::step_to): launch units [next, units) are
+  // still to be issued.
+  bool stepped = false;
+  size_t next = 0, units = 0;
+  bool prune = false;  // Retire::prune: free the execution's buffers at finish
+  zh_resources_used usage{};
 };
 
-static DevPtr take_buffer(zh_pool* pool, zh_context* ctx, const zh_descriptor& d, zh_status* st) {
+// A device buffer of layout d: from the pool's cache, else newly allocated
+// (ResourcesUsed::buffer_mem / buffer_reused, run.rs:212-221).
+static DevPtr take_buffer(zh_execution* e, const zh_descriptor& d, zh_status* st) {
+  zh_pool* pool = e->pool;
+  zh_context* ctx = e->prog->ctx;
   for (size_t k = 0; k < pool->cache.size(); ++k) {
     const DevPtr& b = pool->cache[k];
     if (b->img->ctx == ctx && same_layout(b->img->desc, d)) {
       DevPtr r = b;
       pool->cache.erase(pool->cache.begin() + (long)k);
+      e->usage.buffer_reused += r->img->bytes;
+      e->usage.buffers_reused++;
       *st = ZH_OK;
       return r;
     }
   }
   zh_image* img = nullptr;
   *st = zh_image_alloc(ctx, &d, &img);
-  return *st == ZH_OK ? std::make_shared<DeviceBuf>(img) : nullptr;
+  if (*st != ZH_OK) return nullptr;
+  e->usage.buffer_mem += img->bytes;
+  e->usage.buffers_allocated++;
+  return std::make_shared<DeviceBuf>(img);
 }
 
 static zh_status bad_key(zh_pool_key key) {
@@ -258,22 +274,28 @@ zh_status zh_pool_cached_buffers(zh_pool* pool, uint32_t* count) {
 
 // Return what the execution still holds to the pool: moved inputs go back,
 // device buffers it allocated go to the cache.
-static void release(zh_execution* e) {
+// Return what the execution still holds: moved inputs go back to the pool
+// (unless discarded), device buffers it allocated go to the cache (unless
+// pruned or discarded).
+static void release(zh_execution* e, bool discard) {
   for (auto* slots : {&e->in, &e->out})
     for (zh_execution::Slot& s : *slots) {
-      if (s.moved) {
+      if (s.moved && !discard) {
         auto it = e->pool->items.find(s.key);
         if (it != e->pool->items.end()) it->second = std::move(s.moved_image);
-        s.moved = false;
       }
-      if (s.dev && !s.in_place) e->pool->cache.push_back(s.dev);
+      s.moved = false;
+      if (s.dev && !s.in_place && !discard && !e->prune) e->pool->cache.push_back(s.dev);
       s.dev.reset();
       s.host_keep.reset();
     }
+  if (e->prog && e->prog->stepping == e) e->prog->stepping = nullptr;
 }
 
-zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
-                                 uint32_t num_bindings, zh_execution** out) {
+// Executable::launch from a pool (run.rs:1016-1167): check, trade, bind, and
+// either launch everything or leave the launches to zh_execution_step.
+static zh_status start(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings, uint32_t num_bindings,
+                       bool launch, zh_execution** out) {
   if (!prog || !pool || !out || (num_bindings && !bindings)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   const Plan& plan = prog->plan;
   std::unique_ptr<zh_execution> e(new zh_execution());
@@ -319,6 +341,7 @@ zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_
       return fail(ZH_ERR_MISSING_KEY, "output slot " + std::to_string(i) + " unbound");
 
   if (!prog->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  if (prog->stepping) return fail(ZH_ERR_INVALID_ARGUMENT, "a stepped execution of this program is unfinished");
 
   // Trade (pool.rs:912-941): device images are used in place, host bytes are
   // uploaded into a (cached) device buffer; a no_read image moves out of the
@@ -333,8 +356,10 @@ zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_
         im.dev->img->ctx == prog->ctx) {
       sl.dev = im.dev;
       sl.in_place = true;
+      e->usage.buffer_reused += im.dev->img->bytes;
+      e->usage.buffers_reused++;
     } else {
-      sl.dev = take_buffer(pool, prog->ctx, tight(plan.input_desc[s]), &st);
+      sl.dev = take_buffer(e.get(), tight(plan.input_desc[s]), &st);
       if (st != ZH_OK) break;
       if (im.kind == ZH_IMAGE_HOST) {
         sl.host_keep = im.host;
@@ -363,21 +388,87 @@ zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_
         same_layout(it->second.dev->img->desc, plan.output_desc[s]) && it->second.dev->img->ctx == prog->ctx) {
       sl.dev = it->second.dev;
       sl.in_place = true;
+      e->usage.buffer_reused += sl.dev->img->bytes;
+      e->usage.buffers_reused++;
     } else {
-      sl.dev = take_buffer(pool, prog->ctx, tight(plan.output_desc[s]), &st);
+      sl.dev = take_buffer(e.get(), tight(plan.output_desc[s]), &st);
       if (st != ZH_OK) break;
     }
     binds.push_back(zh_binding{s, 1, sl.dev->img});
   }
-  if (st == ZH_OK) st = zh_program_launch(prog, binds.data(), (uint32_t)binds.size());
+  e->usage.program_mem = program_device_bytes(prog);
+  if (st == ZH_OK) {
+    if (launch) {
+      st = zh_program_launch(prog, binds.data(), (uint32_t)binds.size());
+    } else {
+      st = program_bind(prog, binds.data(), (uint32_t)binds.size());
+      e->stepped = true;
+      e->units = program_units(prog);
+    }
+  }
   if (st != ZH_OK) {
     const std::string msg = zh_last_error();
     hipStreamSynchronize(prog->ctx->stream);
-    release(e.get());
+    release(e.get(), false);
     return fail(st, msg);
   }
+  if (e->stepped) prog->stepping = e.get();
   *out = e.release();
   return ZH_OK;
+}
+
+zh_status zh_program_launch_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
+                                 uint32_t num_bindings, zh_execution** out) {
+  return start(prog, pool, bindings, num_bindings, true, out);
+}
+
+zh_status zh_program_start_pool(zh_program* prog, zh_pool* pool, const zh_pool_binding* bindings,
+                                uint32_t num_bindings, zh_execution** out) {
+  return start(prog, pool, bindings, num_bindings, false, out);
+}
+
+zh_status zh_execution_step(zh_execution* e, uint32_t max_launches, uint32_t* launched) {
+  if (!e) return fail(ZH_ERR_INVALID_ARGUMENT, "null execution");
+  if (max_launches == 0) return fail(ZH_ERR_INVALID_ARGUMENT, "a step launches at least one kernel");
+  uint32_t n = 0;
+  if (e->stepped && e->next < e->units) {
+    n = (uint32_t)std::min<size_t>(max_launches, e->units - e->next);
+    zh_status st = program_launch_units(e->prog, e->next, n);
+    if (st != ZH_OK) return st;
+    e->next += n;
+    if (e->next == e->units && e->prog->stepping == e) e->prog->stepping = nullptr;
+  }
+  if (launched) *launched = n;
+  return ZH_OK;
+}
+
+zh_status zh_execution_is_running(zh_execution* e, int32_t* running) {
+  if (!e || !running) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  if (e->stepped && e->next < e->units) {
+    *running = 1;
+    return ZH_OK;
+  }
+  return zh_query(e->prog->ctx, running);
+}
+
+zh_status zh_execution_resources(zh_execution* e, zh_resources_used* used) {
+  if (!e || !used) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  *used = e->usage;
+  return ZH_OK;
+}
+
+zh_status zh_execution_prune(zh_execution* e) {
+  if (!e) return fail(ZH_ERR_INVALID_ARGUMENT, "null execution");
+  e->prune = true;
+  return ZH_OK;
+}
+
+zh_status zh_execution_discard(zh_execution* e) {
+  if (!e) return ZH_OK;
+  zh_status st = zh_sync(e->prog->ctx);
+  release(e, true);
+  delete e;
+  return st;
 }
 
 zh_status zh_execution_retire_output(zh_execution* e, uint32_t slot, int32_t to_host, zh_pool_key* key) {
@@ -385,6 +476,7 @@ zh_status zh_execution_retire_output(zh_execution* e, uint32_t slot, int32_t to_
   if (slot >= e->out.size() || !e->out[slot].bound) return fail(ZH_ERR_INVALID_ARGUMENT, "no such output");
   zh_execution::Slot& sl = e->out[slot];
   if (sl.retired) return fail(ZH_ERR_INVALID_ARGUMENT, "output already retired");
+  if (e->stepped && e->next < e->units) return fail(ZH_ERR_INVALID_ARGUMENT, "the execution has launches left to step");
   zh_pool* pool = e->pool;
   zh_context* ctx = e->prog->ctx;
   // The image the output goes to: its key, or a new one (run.rs:2809-2875).
@@ -413,7 +505,7 @@ zh_status zh_execution_retire_output(zh_execution* e, uint32_t slot, int32_t to_
     im.host = host;
     im.dev.reset();
     im.desc = tight(im.desc);
-    if (!sl.in_place) pool->cache.push_back(sl.dev);
+    if (!sl.in_place && !e->prune) pool->cache.push_back(sl.dev);
     sl.dev.reset();
   }
   sl.retired = true;
@@ -425,6 +517,7 @@ zh_status zh_execution_retire_input(zh_execution* e, uint32_t slot, zh_pool_key*
   if (!e || !key) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   if (slot >= e->in.size() || !e->in[slot].bound) return fail(ZH_ERR_INVALID_ARGUMENT, "no such input");
   zh_execution::Slot& sl = e->in[slot];
+  if (e->stepped && e->next < e->units) return fail(ZH_ERR_INVALID_ARGUMENT, "the execution has launches left to step");
   if (sl.moved) {
     zh_status st = zh_sync(e->prog->ctx);
     if (st != ZH_OK) return st;
@@ -440,7 +533,7 @@ zh_status zh_execution_retire_input(zh_execution* e, uint32_t slot, zh_pool_key*
 zh_status zh_execution_finish(zh_execution* e) {
   if (!e) return ZH_OK;
   zh_status st = zh_sync(e->prog->ctx);
-  release(e);
+  release(e, false);
   delete e;
   return st;
 }

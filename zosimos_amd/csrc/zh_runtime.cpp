@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <sstream>
 
 #include "zh_internal.h"
@@ -977,9 +978,45 @@ static zh_status launch_all(zh_program* p) {
   return ZH_OK;
 }
 
+}  // extern "C"
+
+// ---- stepped executions (Execution::step_to, run.rs:1394-1464) -------------
+namespace zh {
+
+zh_status program_bind(zh_program* p, const zh_binding* bindings, uint32_t num_bindings) {
+  if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  return bind(p, bindings, num_bindings);
+}
+
+size_t program_units(const zh_program* p) { return p->units.size(); }
+
+zh_status program_launch_units(zh_program* p, size_t first, size_t count) {
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  for (size_t i = first; i < first + count && i < p->units.size(); ++i) {
+    zh_status st = launch_unit(p, p->units[i]);
+    if (st != ZH_OK) return st;
+  }
+  return ZH_OK;
+}
+
+uint64_t program_device_bytes(const zh_program* p) {
+  uint64_t b = (uint64_t)p->plan.stages.size() * sizeof(StageDesc);
+  for (const zh_descriptor& d : p->plan.intermediates) b += image_bytes(d);
+  for (const zh_image& img : p->buffer_images) b += img.bytes;
+  for (uint32_t w : p->tile_layer_words) b += (uint64_t)w * 4;
+  for (const DynamicCall& dc : p->plan.dynamics) b += std::max<size_t>(dc.data.size(), 16);
+  return b;
+}
+
+}  // namespace zh
+
+extern "C" {
+
 zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t num_bindings) {
   if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
   if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  if (p->stepping) return fail(ZH_ERR_INVALID_ARGUMENT, "a stepped execution of this program is unfinished");
   ZH_HIP(hipSetDevice(p->ctx->device));
   zh_status st = bind(p, bindings, num_bindings);
   if (st != ZH_OK) return st;
@@ -1027,6 +1064,85 @@ zh_status zh_program_launch(zh_program* p, const zh_binding* bindings, uint32_t 
     return ZH_OK;
   }
   return launch_all(p);
+}
+
+zh_status zh_program_launch_once(zh_program* p, const zh_binding* bindings, uint32_t num_bindings) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (!p->ctx) return fail(ZH_ERR_NO_DEVICE, "program was built without a context");
+  if (p->stepping) return fail(ZH_ERR_INVALID_ARGUMENT, "a stepped execution of this program is unfinished");
+  ZH_HIP(hipSetDevice(p->ctx->device));
+  zh_status st = bind(p, bindings, num_bindings);
+  // One run: kernel by kernel (a graph pays off only when replayed).
+  if (st == ZH_OK) st = launch_all(p);
+  const std::string msg = st == ZH_OK ? std::string() : std::string(zh_last_error());
+  zh_program_destroy(p);  // waits for the queued kernels first
+  return st == ZH_OK ? ZH_OK : fail(st, msg);
+}
+
+// Graphviz text of the plan: images and stages, an edge per read and write.
+zh_status zh_program_dot(zh_program* p, char* buf, size_t len, size_t* needed) {
+  if (!p) return fail(ZH_ERR_INVALID_ARGUMENT, "null program");
+  if (p->units.empty() && !p->plan.stages.empty()) make_units(p);
+  const Plan& plan = p->plan;
+  std::ostringstream o;
+  o << "digraph program {\n  rankdir=LR;\n";
+  auto image = [&](int64_t ref) -> std::string {
+    if (ref >= kIntermediateBase) return "t" + std::to_string(ref - kIntermediateBase);
+    if (ref >= 0) return "in" + std::to_string(ref);
+    return "out" + std::to_string(~ref);
+  };
+  auto label = [&](int64_t ref) -> std::string {
+    const zh_descriptor& d = ref >= kIntermediateBase ? plan.intermediates[(size_t)(ref - kIntermediateBase)]
+                             : ref >= 0               ? plan.input_desc[(size_t)ref]
+                                                      : plan.output_desc[(size_t)~ref];
+    const char* kind = ref >= kIntermediateBase ? "intermediate" : ref >= 0 ? "input" : "output";
+    return std::string(kind) + " " + std::to_string(ref >= kIntermediateBase ? ref - kIntermediateBase
+                                                    : ref >= 0               ? ref
+                                                                             : ~ref) +
+           "\\n" + std::to_string(d.width) + "x" + std::to_string(d.height) + " bits " + std::to_string(d.texel.bits);
+  };
+  std::set<int64_t> seen;
+  auto node = [&](int64_t ref) {
+    if (seen.insert(ref).second)
+      o << "  " << image(ref) << " [shape=box, label=\"" << label(ref) << "\"];\n";
+  };
+  for (size_t s = 0; s < plan.stages.size(); ++s) {
+    const StagePlan& sp = plan.stages[s];
+    std::string kernel = "stage " + std::to_string(s);
+    const size_t k = sp.text.rfind("[kernel ");
+    if (k != std::string::npos) kernel += "\\n" + sp.text.substr(k + 8, sp.text.size() - k - 9);
+    o << "  s" << s << " [shape=ellipse, label=\"" << kernel << "\"];\n";
+    std::vector<bool> read(sp.slot_ref.size(), false), written(sp.slot_ref.size(), false);
+    if (sp.desc.kind == STAGE_COPY) {
+      read[0] = written[1] = true;
+    } else if (sp.desc.kind == STAGE_DYNAMIC) {
+      for (size_t i = 0; i + 1 < sp.slot_ref.size(); ++i) read[i] = true;
+      written.back() = true;
+    } else {
+      for (uint32_t i = 0; i < sp.desc.n_instr; ++i) {
+        const Instr& in = sp.desc.code[i];
+        if (in.op == OP_STORE) written[in.a] = true;
+        if (in.op == OP_LOAD || in.op == OP_BOX3 || in.op == OP_PALETTE) read[in.a] = true;
+      }
+    }
+    for (size_t i = 0; i < sp.slot_ref.size(); ++i) {
+      const int64_t ref = sp.slot_ref[i];
+      if (ref < 0 && ref <= -1000000) continue;  // quantisation-only slot
+      if (!read[i] && !written[i]) continue;
+      node(ref);
+      if (read[i]) o << "  " << image(ref) << " -> s" << s << ";\n";
+      if (written[i]) o << "  s" << s << " -> " << image(ref) << ";\n";
+    }
+  }
+  o << "}\n";
+  const std::string t = o.str();
+  if (needed) *needed = t.size() + 1;
+  if (buf && len) {
+    const size_t n = std::min(len - 1, t.size());
+    std::memcpy(buf, t.data(), n);
+    buf[n] = 0;
+  }
+  return ZH_OK;
 }
 
 zh_status zh_program_launch_static(zh_context* ctx, const zh_descriptor* textures, uint32_t num_textures,
