@@ -260,3 +260,11 @@ def test_missing_kernel_name_is_an_error(ctx):
     with pytest.raises(z.ZosimosError) as e:
         prog.lower_to(ctx)
     assert e.value.code == z.Status.INVALID_ARGUMENT and "no_such_kernel" in e.value.message
+    # the failed lookup leaves no error behind for the next launch to report
+    d = z.Descriptor.rgba8_srgb(64, 16)
+    c2 = z.CommandBuffer()
+    i2 = c2.input(d)
+    o2, od2 = c2.output(c2.color_convert(i2, z.Color(transfer=z.Transfer.Linear),
+                                         z.Texel(SB.UInt16x4, SP.RgbA)))
+    raw, _ = run(ctx, c2, {i2: (d, np.full(d.nbytes, 255, np.uint8))}, o2, od2)
+    assert np.all(raw.view(np.uint16) == 65535)

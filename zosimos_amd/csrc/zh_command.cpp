@@ -229,7 +229,9 @@ static GenDesc rewrite(const GenDesc& g, const std::vector<GenDesc>& decl) {
     r.desc.color = s.desc.color;
     r.desc.texel_stride = s.desc.texel_stride;
   }
-  if (r.size_var < 0 && r.chroma_var < 0) r.desc.row_stride = 0;
+  // A substituted part makes the placeholder's row layout meaningless: the
+  // instance has tight rows.  A concrete descriptor keeps its own.
+  if ((g.size_var >= 0 || g.chroma_var >= 0) && r.size_var < 0 && r.chroma_var < 0) r.desc.row_stride = 0;
   return r;
 }
 

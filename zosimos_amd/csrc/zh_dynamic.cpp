@@ -65,11 +65,14 @@ zh_status shader_function(zh_shader* s, int device, hipFunction_t* fn) {
     }
   hipModule_t m = nullptr;
   hipError_t e = hipModuleLoadData(&m, s->code.data());
-  if (e != hipSuccess)
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // the runtime keeps the error for the next launch's check otherwise
     return fail(ZH_ERR_HIP, std::string("hipModuleLoadData of the shader's code object failed: ") + hipGetErrorString(e));
+  }
   hipFunction_t f = nullptr;
   e = hipModuleGetFunction(&f, m, s->kernel.c_str());
   if (e != hipSuccess) {
+    (void)hipGetLastError();
     hipModuleUnload(m);
     return fail(ZH_ERR_INVALID_ARGUMENT, "the shader's code object has no kernel named '" + s->kernel + "'");
   }

@@ -167,6 +167,7 @@ struct zh_program {
   std::vector<zh_image> buffer_images;   // the program's buffers (Plan::buffer_ops), device memory owned here
   std::vector<uint32_t*> d_tile_layers;  // per stage: the tile-layer masks (warp stages)
   std::vector<uint32_t> tile_layer_words;  // their capacities in words
+  std::vector<std::vector<uint32_t>> h_tile_layers;  // host copies of the masks uploaded
   std::vector<void*> bound_inputs, bound_outputs;
   bool uploaded = false;
   hipGraph_t graph = nullptr;

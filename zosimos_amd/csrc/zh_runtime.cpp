@@ -336,6 +336,7 @@ static void free_device(zh_program* p) {
     p->buffer_images.clear();
     p->d_tile_layers.clear();
     p->tile_layer_words.clear();
+    p->h_tile_layers.clear();
     p->d_stages = nullptr;
     p->h_stages = nullptr;
     p->h_stage_vec.clear();
@@ -364,6 +365,7 @@ static void free_device(zh_program* p) {
     if (m) hipFree(m);
   p->d_tile_layers.clear();
   p->tile_layer_words.clear();
+  p->h_tile_layers.clear();
   if (p->d_stages) hipFree(p->d_stages);
   if (p->h_stages) hipHostFree(p->h_stages);
   p->d_stages = nullptr;
@@ -473,6 +475,13 @@ static zh_status tile_layer_masks(zh_program* p, size_t s) {
       const uint32_t t = ty * d.tiles_x + tx;
       m[t >> 2] |= bits << ((t & 3u) * 8u);
     }
+  if (p->h_tile_layers.size() < p->plan.stages.size()) p->h_tile_layers.resize(p->plan.stages.size());
+  if (p->d_tile_layers[s] && p->h_tile_layers[s] == m) {
+    // Unchanged (a knob that moved no layer): nothing to upload.
+    d.tile_layers = p->d_tile_layers[s];
+    if (p->h_stages) p->h_stages[s].tile_layers = d.tile_layers;
+    return ZH_OK;
+  }
   // Launches still in flight may read the old masks.
   ZH_HIP(hipStreamSynchronize(p->ctx->stream));
   if (p->tile_layer_words[s] < m.size()) {
@@ -483,6 +492,7 @@ static zh_status tile_layer_masks(zh_program* p, size_t s) {
     p->tile_layer_words[s] = (uint32_t)m.size();
   }
   ZH_HIP(hipMemcpy(p->d_tile_layers[s], m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  p->h_tile_layers[s] = std::move(m);
   d.tile_layers = p->d_tile_layers[s];
   if (p->h_stages) p->h_stages[s].tile_layers = d.tile_layers;
   return ZH_OK;
@@ -714,6 +724,25 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
   Plan np;
   zh_status st = plan_program(p->textures, p->ops, params, p->flags, &np);
   if (st != ZH_OK) return st;
+  // Warp stages keep the tiles they run on (the autotuner's choice): a warp
+  // kernel's tiles do not depend on the layers' maps.
+  if (np.stages.size() == p->plan.stages.size())
+    for (size_t i = 0; i < np.stages.size(); ++i) {
+      StagePlan& a = np.stages[i];
+      const StagePlan& b = p->plan.stages[i];
+      if (a.desc.kind != STAGE_RESAMPLE || b.desc.kind != STAGE_RESAMPLE || a.desc.variant < 0 ||
+          a.desc.variant != b.desc.variant || a.desc.width != b.desc.width || a.desc.height != b.desc.height)
+        continue;
+      a.desc.tile_w = b.desc.tile_w;
+      a.desc.tile_h = b.desc.tile_h;
+      a.desc.band_h = b.desc.band_h;
+      a.desc.tiles_x = b.desc.tiles_x;
+      a.desc.tiles_y = b.desc.tiles_y;
+      a.desc.lds_texels = b.desc.lds_texels;
+      a.tile_mode = b.tile_mode;
+      a.lds_bytes = b.lds_bytes;
+      a.text = b.text;
+    }
   // Same structure (stage kinds, tiles, slots): keep buffers and the graph and
   // refresh the descriptors at the next launch.
   bool same = np.stages.size() == p->plan.stages.size() && np.intermediates.size() == p->plan.intermediates.size();
@@ -725,8 +754,13 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
            a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat && a.desc.variant == b.desc.variant;
   }
   p->params = params;
-  // A surrogate was fitted to the old parameters.
-  if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+  // A surrogate was fitted to the old parameters; a buffer knob rewrites the
+  // buffer in place: both wait for the launches in flight.  Parameters the
+  // stage descriptors carry go through the descriptor ring instead.
+  bool buffer_knob = false;
+  for (size_t i = 0; i < p->ops.size(); ++i)
+    buffer_knob = buffer_knob || (p->ops[i].knob == knob && p->ops[i].high == ZH_HIGH_BUFFER);
+  if (p->ctx && (buffer_knob || !p->d_surrogate.empty() || !same)) hipStreamSynchronize(p->ctx->stream);
   drop_surrogates(p);
   if (same) {
     for (size_t i = 0; i < np.stages.size(); ++i) np.stages[i].lds_bytes = p->plan.stages[i].lds_bytes;
@@ -738,8 +772,10 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
         zh_status ms = tile_layer_masks(p, s);
         if (ms != ZH_OK) return ms;
       }
-      zh_status bs = upload_buffers(p);
-      if (bs != ZH_OK) return bs;
+      if (buffer_knob) {
+        zh_status bs = upload_buffers(p);
+        if (bs != ZH_OK) return bs;
+      }
     }
     return ZH_OK;
   }
@@ -1204,6 +1240,9 @@ static bool warp_unit(const zh_program* p, const Unit& u) {
 // copies and, for a batch, the tile prefix).  The interpreter fallback of a
 // re-tiled stage gathers its taps directly (TILE_DIRECT accepts any tile).
 static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th) {
+  // The warp kernels split a tile's lanes by shifts: tw / 4 lanes per row,
+  // a power of two, and 256 lanes per tile.
+  if (tw < 4 || (tw & (tw - 1)) || tw * th != 1024) return fail(ZH_ERR_INVALID_ARGUMENT, "unsupported warp tile shape");
   std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
   for (size_t k = 0; k < u.ids.size(); ++k) {
     const size_t s = u.ids[k];

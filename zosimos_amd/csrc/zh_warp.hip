@@ -8,13 +8,13 @@
 //     of 256 lanes, kPix = 4 horizontally adjacent pixels per lane); a
 //     persistent grid walks the tiles with a stride;
 //   * per tile and layer, the tile's footprint is mapped in FP64 (four
-//     corners) and the layer is skipped when it misses the source; for 8-bit
-//     outputs the map of the tile's first pixel centre is split into an
-//     integer texel and an FP32 remainder and every pixel adds its FP32
-//     offset (|offset| < 100 px: ~4e-6 px error instead of FP32's 5e-4 px at
-//     8K); float outputs map every pixel in FP64, since a bilinear weight
-//     error of ~4e-6 reaches a few f16 ulps after un-premultiplying a small
-//     alpha;
+//     corners) and the layer is skipped when it misses the source; each
+//     lane's first pixel centre is mapped in FP64 and held, like the step
+//     between the lane's pixels, in 32.32 fixed point, so every pixel's map
+//     is one 64-bit add: the integer texel is the high word and the bilinear
+//     fraction the low word (2^-32 px resolution at any image size; FP32
+//     steps would err by ~1e-5 px at 8K, which un-premultiplying a small
+//     alpha turns into several f16 ulps);
 //   * the four taps are read straight from memory (the L1/L2 caches serve
 //     the overlap between neighbouring pixels) through buffer loads whose
 //     range is the source image, decoded in-line, premultiplied and
@@ -143,18 +143,30 @@ struct Warp {
   typedef Post_ Post;
 };
 
-// Map of a lane's pixels into one layer's source.  FP32 form: (lu, lv) is the
-// lane's first pixel centre mapped, minus the half texel, relative to the
-// integer texel (iu0, iv0), and (su, sv) the step between the lane's pixels
-// (~4e-6 px error: enough for 8-bit outputs).  FP64 form (float outputs,
-// where a weight error of that size shows after un-premultiplying a small
-// alpha): every pixel is mapped from (du, dv) = the lane's first pixel centre
-// mapped minus the half texel, steps (eu, ev).
+// Map of a lane's pixels into one layer's source, in 32.32 fixed point:
+// (u0, v0) = the lane's first pixel centre mapped, minus the half texel;
+// (su, sv) = the step between the lane's pixels (lanes_x output pixels).
 struct LaneMap {
-  int iu0, iv0;
-  float lu, lv, su, sv;
-  double du, dv, eu, ev;
+  int64_t u0, v0, su, sv;
 };
+
+// x in 32.32 fixed point (|x| < 2^31; the fraction truncated to 2^-32).
+__device__ __forceinline__ int64_t fixed32(double x) {
+  const double f = floor(x);
+  const uint32_t lo = (uint32_t)((x - f) * 4294967296.0);
+  return (int64_t)(((uint64_t)(uint32_t)(int)f << 32) | lo);
+}
+// Integer part and fraction of a 32.32 value.  The fraction rounds to the
+// nearest float and can reach 1.0 from just below it: the weights are
+// continuous there, so the taps' blend is the same.
+__device__ __forceinline__ int fx_int(int64_t v) { return (int)(v >> 32); }
+// A value equal in every lane, moved to scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float fx_frac(int64_t v) { return (float)(uint32_t)v * 0x1p-32f; }
 
 // Composite one layer's bilinear samples over the lane's kPix pixels.
 // Taps of all pixels are addressed first (weight 0 for taps outside the
@@ -177,7 +189,7 @@ __device__ __forceinline__ typename RawT<F>::type pair_half(const typename PairT
   }
 }
 
-template <class V, bool SEP, bool P64>
+template <class V, bool SEP>
 __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp, int n, Px& X, const Luts* L) {
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
@@ -196,17 +208,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
   uint32_t r0s = 0, r1s = 0;
   bool yas = false, ybs = false;
   if constexpr (SEP) {
-    int j;
-    float ay;
-    if constexpr (P64) {
-      const double fv = floor(mp.dv);
-      j = (int)fv;
-      ay = (float)(mp.dv - fv);
-    } else {
-      const float fv = floorf(mp.lv);
-      j = mp.iv0 + (int)fv;
-      ay = mp.lv - fv;
-    }
+    const int j = fx_int(mp.v0);
+    const float ay = fx_frac(mp.v0);
     yas = (unsigned)j < (unsigned)a.sh;
     ybs = (unsigned)(j + 1) < (unsigned)a.sh;
     wy0s = yas ? 1.0f - ay : 0.0f;
@@ -216,19 +219,9 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
   }
 #pragma unroll
   for (int k = 0; k < kPix; ++k) {
-    int i;
-    float ax;
-    if constexpr (P64) {
-      const double u = fma(mp.eu, (double)k, mp.du);
-      const double fu = floor(u);
-      i = (int)fu;
-      ax = (float)(u - fu);
-    } else {
-      const float u = mp.lu + mp.su * (float)k;
-      const float fu = floorf(u);
-      i = mp.iu0 + (int)fu;
-      ax = u - fu;
-    }
+    const int64_t uk = mp.u0 + mp.su * k;
+    const int i = fx_int(uk);
+    const float ax = fx_frac(uk);
     const bool in = k < n;
     const bool xa = in && (unsigned)i < (unsigned)a.sw, xb = in && (unsigned)(i + 1) < (unsigned)a.sw;
     bool ya, yb;
@@ -242,19 +235,9 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
       r0 = r0s;
       r1 = r1s;
     } else {
-      int j;
-      float ay;
-      if constexpr (P64) {
-        const double v = fma(mp.ev, (double)k, mp.dv);
-        const double fv = floor(v);
-        j = (int)fv;
-        ay = (float)(v - fv);
-      } else {
-        const float v = mp.lv + mp.sv * (float)k;
-        const float fv = floorf(v);
-        j = mp.iv0 + (int)fv;
-        ay = v - fv;
-      }
+      const int64_t vk = mp.v0 + mp.sv * k;
+      const int j = fx_int(vk);
+      const float ay = fx_frac(vk);
       ya = (unsigned)j < (unsigned)a.sh;
       yb = (unsigned)(j + 1) < (unsigned)a.sh;
       wy0 = ya ? 1.0f - ay : 0.0f;
@@ -350,11 +333,14 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
   typedef typename RawT<V::F>::type R;
   constexpr uint32_t B = fmt_bpp(V::F);
   const int TW = (int)d->tile_w, TH = (int)d->tile_h;
+  // Tile widths are powers of two of at least kPix pixels (the planner's
+  // and the autotuner's shapes): the lane split is a shift.
   const int lanes_x = TW / kPix;
-  const int lx = (int)threadIdx.x % lanes_x, ly = (int)threadIdx.x / lanes_x;
+  const int lsh = __builtin_ctz((unsigned)lanes_x);
+  const int lx = (int)threadIdx.x & (lanes_x - 1), ly = (int)threadIdx.x >> lsh;
   const int W = (int)d->width, H = (int)d->height;
   const int x0 = (int)tx * TW + lx, y = (int)ty * TH + ly;
-  const int n = (y < H && x0 < W) ? min(kPix, (W - x0 + lanes_x - 1) / lanes_x) : 0;
+  const int n = (y < H && x0 < W) ? min(kPix, (W - x0 + lanes_x - 1) >> lsh) : 0;
 
   // Base: a premultiplied solid colour (transparent when the program is empty).
   float4 base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -379,31 +365,19 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
     const Layer Ly = ldc(&d->layer[l]);
     if (!tl && !Ly.covers_all && !layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;
     const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
-    // Map of the tile's first pixel centre: integer texel + FP32 remainder
-    // (8-bit outputs), or the lane's first pixel in FP64 (float outputs).
-    constexpr bool P64 = el_fmt(Post::el[Post::n - 1]) == FMT_F16 || el_fmt(Post::el[Post::n - 1]) == FMT_F32;
+    // The tile's first pixel centre in FP64, then each lane's first pixel
+    // and the step between a lane's pixels in fixed point.
     const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
     const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
-    LaneMap mp{};
-    if constexpr (P64) {
-      mp.du = fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5;
-      mp.dv = fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5;
-      mp.eu = Ly.inv[0] * (double)lanes_x;
-      mp.ev = Ly.inv[3] * (double)lanes_x;
-    } else {
-      const double fuo = floor(uo), fvo = floor(vo);
-      mp.iu0 = (int)fuo;
-      mp.iv0 = (int)fvo;
-      const float m0 = (float)Ly.inv[0], m1 = (float)Ly.inv[1], m3 = (float)Ly.inv[3], m4 = (float)Ly.inv[4];
-      mp.lu = (float)(uo - fuo) + m0 * (float)lx + m1 * (float)ly - 0.5f;
-      mp.lv = (float)(vo - fvo) + m3 * (float)lx + m4 * (float)ly - 0.5f;
-      mp.su = m0 * (float)lanes_x;
-      mp.sv = m3 * (float)lanes_x;
-    }
+    LaneMap mp;
+    mp.u0 = fixed32(fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5);
+    mp.v0 = fixed32(fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5);
+    mp.su = uniform64(fixed32(Ly.inv[0] * (double)lanes_x));
+    mp.sv = uniform64(fixed32(Ly.inv[3] * (double)lanes_x));
     // Axis-aligned maps (pure scale and shift; the planner picks the SEP
     // variant only when every layer has one) are separable: the lane's
     // pixels share one source row pair, computed once.
-    layer_taps<V, V::SEP, P64>(a, mp, n, X, L);
+    layer_taps<V, V::SEP>(a, mp, n, X, L);
   }
 
   if (n == 0) return;
