@@ -36,8 +36,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c5", "copy"],
+    ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c5", "c2r", "copy"],
                     help="all: the batched program as the headline, the other workloads as sub-objects; "
+                         "c2r: c2's operations reordered (general pointwise interpreter); "
                          "copy: the byte-copy calibration workload of the HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -295,7 +296,7 @@ def main():
     ctx = z.Context(0, stream=stream.cuda_stream)
     traffic = load_traffic()
     head_cfg = HEADLINE if args.config == "all" else args.config
-    wl = W.copy_frame() if head_cfg == "copy" else W.CONFIGS[head_cfg]()
+    wl = W.copy_frame() if head_cfg == "copy" else {**W.CONFIGS, **W.EXTRA}[head_cfg]()
     head = measure(wl, ctx, stream, torch, args.steps, args.warmup, traffic)
     line = {
         "metric": METRIC,
@@ -327,11 +328,14 @@ def main():
             if not args.no_cpu_baseline:
                 subs[c]["cpu_baseline"] = cpu_baseline(c, 3.0)
         line["configs"] = subs
+        # The same operations as c2 in an order no specialised kernel covers:
+        # the general pointwise interpreter's rate beside c2's.
+        line["general"] = measure(W.c2_reordered(), ctx, stream, torch, args.steps, args.warmup, traffic)
     if not args.no_surrogate and args.config in ("all", "c4"):
         line["surrogate"] = surrogate_leg(ctx, stream, torch, args.steps)
     if not args.no_relaunch and args.config == "all":
         line["relaunch"] = relaunch_leg(ctx)
-    if not args.no_cpu_baseline and head_cfg != "copy":
+    if not args.no_cpu_baseline and head_cfg in W.CONFIGS:
         line["cpu_baseline"] = cpu_baseline(head_cfg, 10.0)
     print(json.dumps(line), flush=True)
     ctx.close()

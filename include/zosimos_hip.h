@@ -297,6 +297,9 @@ typedef struct zh_op {
 #define ZH_PLAN_STRICT_REGISTERS 1u
 /* Do not capture the program's launches into a hipGraph. */
 #define ZH_PLAN_NO_GRAPH 2u
+/* Run pointwise chains that have no registered specialised kernel on the
+ * general interpreter instead of compiling one for them at run time. */
+#define ZH_PLAN_NO_JIT 4u
 
 /* Lower a High program (Program::lower_to, program.rs:1304).  `textures`
  * holds the descriptor of every texture id.  `ctx` may be NULL: the plan is

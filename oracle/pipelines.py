@@ -142,6 +142,19 @@ def c2(src_u8, layer_f16):
     return res, C.encode_u8(res, 5)
 
 
+def c2_reordered(src_u8, layer_f16):
+    """C2's operations with the 'over' first: Rec.709 -> Display-P3, the
+    RGBA16F layer 'over' it, then Oklab -> chroma x1.2 -> back (with the
+    decode clamp) -> RGBA8 (P3 primaries, sRGB transfer)."""
+    lin = C.decode_u8(src_u8, 5)
+    p3 = apply3(C.convert_matrix(3, 7, 5), lin)
+    ov = C.unpremultiply(C.over(C.premultiply(layer_f16), C.premultiply(p3)))
+    lab = C.lch_adjust(C.rgb_to_oklab(ov[..., :3], 7), 1.2, 0.0)
+    res = ov.copy()
+    res[..., :3] = C.oklab_to_rgb(lab, 7, clamp=True)
+    return res, C.encode_u8(res, 5)
+
+
 def chain(lin_rgba, hue_deg=30.0, chroma=1.2):
     """sRGB-primaries linear RGB -> XYZ -> Oklab -> hue -> chroma -> back
     (with the Oklab decode clamp), alpha untouched."""

@@ -689,6 +689,7 @@ class Linker:
 PLAN_DEFAULT = 0
 PLAN_STRICT_REGISTERS = 1
 PLAN_NO_GRAPH = 2
+PLAN_NO_JIT = 4
 
 
 class Program:

@@ -28,8 +28,10 @@ SOURCES = [
     "zh_surrogate.cpp",
     "zh_pool.cpp",
     "zh_dynamic.cpp",
+    "zh_jit.cpp",
 ]
-HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_internal.h", "zh_mlp.h"]
+HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_spec_kernels.h", "zh_internal.h",
+           "zh_mlp.h"]
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) is disabled for every device compile.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -106,7 +108,8 @@ def build(force: bool = True, jobs: int = 6) -> str:
         list(ex.map(_compile, todo))
     objs = [os.path.join(OUT_DIR, os.path.splitext(s)[0] + ".o") for s in srcs]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # hiprtc compiles the run-time kernels of unregistered signatures (zh_jit.cpp).
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

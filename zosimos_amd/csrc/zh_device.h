@@ -130,6 +130,7 @@ __device__ __forceinline__ float srlab_nl_inv(float v) {
   return p * p * p;
 }
 
+#ifndef __HIPCC_RTC__  // host code: kernels compiled at run time (zh_jit.cpp) see device code only
 // Workgroups of `kernel` (kBlock lanes) resident on a device of `cus` compute
 // units: the grid of a persistent kernel, which then has no second partial
 // round of workgroups.
@@ -138,6 +139,7 @@ inline uint32_t resident_blocks(const void* kernel, uint32_t cus, size_t dyn_lds
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kBlock, dyn_lds) != hipSuccess || n <= 0) n = 1;
   return (uint32_t)n * cus;
 }
+#endif
 
 // Copy the context's sRGB tables (zh_codec.h Luts) into LDS.
 __device__ __forceinline__ void load_lut(CDesc d, Luts* lut) {

@@ -60,6 +60,16 @@ zh_status shader_function(zh_shader* shader, int device, hipFunction_t* fn);
 bool shader_loaded(zh_shader* shader, int device);
 const char* shader_kernel_name(zh_shader* shader);
 
+// ---- kernels compiled at run time (zh_jit.cpp) -------------------------------
+// Can the specialised bodies run this signature (zh_spec.h elements)?
+bool jit_supported(const uint16_t* sig, int n);
+// The process-wide id of a signature (nothing is compiled yet).
+int jit_signature(const uint16_t* sig, int n);
+// The stage and batch kernels of signature `id` on `device` (the current
+// device), compiled on first use; `why` receives the reason of a failure.
+zh_status jit_kernels(int id, int device, hipFunction_t* stage, hipFunction_t* batch, std::string* why);
+double jit_compile_seconds(int id);
+
 // A user stage of a plan: the shader and its data.
 struct DynamicCall {
   ShaderRef shader;
@@ -73,6 +83,7 @@ struct StagePlan {
   int tile_mode = 0;              // resample: TILE_* of zh_kernels.hip
   bool fast = false;              // pointwise: every slot has an in-line codec (k_pointwise<true>)
   int spec = -1;                  // pointwise: index of the specialised kernel (zh_spec.h), -1 = none
+  int jit = -1;                   // pointwise: run-time kernel of the signature (zh_jit.cpp), -1 = none
   uint32_t lds_bytes = 0;
   uint64_t alg_bytes = 0;         // algorithmic HBM bytes (reads once + writes once)
   // slot -> what it refers to: >=0 input slot, or ~output slot (<0), or
@@ -199,6 +210,10 @@ struct zh_program {
   // per entry of plan.dynamics.
   std::vector<hipFunction_t> dyn_fn;
   std::vector<void*> dyn_data;
+  // Run-time kernels per stage (StagePlan::jit): stage and batch forms and
+  // their resident workgroups per CU.
+  std::vector<hipFunction_t> jit_stage_fn, jit_batch_fn;
+  std::vector<uint32_t> jit_resident, jit_batch_resident;
   // The stepped execution (zh_program_start_pool) still launching this
   // program's kernels; other launches wait for it to finish.
   zh_execution* stepping = nullptr;

@@ -1380,6 +1380,10 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
                       in.op == OP_LOAD && in.b ? 1u : 0u);
         }
         sp.spec = zh_spec_lookup(sig, (int)sp.desc.n_instr);
+        // Any other chain of the operations the specialised bodies run gets
+        // its kernel compiled at run time (zh_jit.cpp).
+        if (sp.spec < 0 && !(flags & ZH_PLAN_NO_JIT) && jit_supported(sig, (int)sp.desc.n_instr))
+          sp.jit = jit_signature(sig, (int)sp.desc.n_instr);
       }
       if (flat) {
         // The first two distinct 4- or 8-byte load slots are prefetched one
@@ -1413,6 +1417,8 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
         << describe_code(sp.desc, 0, sp.desc.n_instr);
       if (sp.spec >= 0)
         t << " [kernel " << zh_spec_name(sp.spec) << "]";
+      else if (sp.jit >= 0)
+        t << " [kernel jit#" << sp.jit << "]";
       else if (sp.fast)
         t << " [kernel interp-inline]";
       sp.text = t.str();

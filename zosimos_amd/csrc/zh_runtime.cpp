@@ -390,7 +390,7 @@ static void make_units(zh_program* p) {
   // 1: warp stages (partitioned by form), 2: LDS-staged resample stages,
   // 3: flat pointwise stages with a specialised kernel (by signature).
   auto family = [&](const StagePlan& sp) {
-    if (sp.desc.kind == STAGE_POINTWISE) return sp.spec >= 0 ? 3 : 0;
+    if (sp.desc.kind == STAGE_POINTWISE) return sp.spec >= 0 || sp.jit >= 0 ? 3 : 0;
     if (sp.desc.kind != STAGE_RESAMPLE) return 0;
     if (sp.desc.variant >= 0) return 1;
     return sp.tile_mode == 0 ? 2 : 0;
@@ -417,7 +417,7 @@ static void make_units(zh_program* p) {
     // Partition [i, j): by warp form, or as one group.
     std::vector<std::pair<int, std::vector<uint32_t>>> groups;
     for (size_t k = i; k < j; ++k) {
-      const int key = fam == 1 ? st[k].desc.variant : fam == 3 ? st[k].spec : -1;
+      const int key = fam == 1 ? st[k].desc.variant : fam == 3 ? (st[k].spec >= 0 ? st[k].spec : 100000 + st[k].jit) : -1;
       auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
       if (it == groups.end()) groups.push_back({key, {(uint32_t)k}});
       else it->second.push_back((uint32_t)k);
@@ -529,6 +529,35 @@ static zh_status alloc_device(zh_program* p) {
     void* b = nullptr;
     ZH_ALLOC(p, image_bytes(d), &b);
     p->d_intermediates.push_back(b);
+  }
+  // Run-time kernels of unregistered pointwise signatures (none from the
+  // static scratch: loading a code object allocates).  A kernel that cannot
+  // be had leaves its stage on the interpreter, and the description says why.
+  p->jit_stage_fn.assign(p->plan.stages.size(), nullptr);
+  p->jit_batch_fn.assign(p->plan.stages.size(), nullptr);
+  p->jit_resident.assign(p->plan.stages.size(), 1);
+  p->jit_batch_resident.assign(p->plan.stages.size(), 1);
+  for (size_t s = 0; s < p->plan.stages.size(); ++s) {
+    StagePlan& sp = p->plan.stages[s];
+    if (sp.jit < 0) continue;
+    std::string why;
+    hipFunction_t f = nullptr, fb = nullptr;
+    const size_t k = sp.text.rfind(" [kernel jit#");
+    if (p->arena || jit_kernels(sp.jit, ctx->device, &f, &fb, &why) != ZH_OK) {
+      sp.jit = -1;
+      if (k != std::string::npos)
+        sp.text = sp.text.substr(0, k) + " [kernel interp-inline]" +
+                  (p->arena ? std::string() : " (run-time kernel unavailable: " + why.substr(0, 120) + ")");
+      continue;
+    }
+    int n = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kBlock, 0) == hipSuccess && n > 0)
+      p->jit_resident[s] = (uint32_t)n;
+    n = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb, kBlock, 0) == hipSuccess && n > 0)
+      p->jit_batch_resident[s] = (uint32_t)n;
+    p->jit_stage_fn[s] = f;
+    p->jit_batch_fn[s] = fb;
   }
   // User stages: the kernel on this device and a device copy of the data.
   for (const DynamicCall& dc : p->plan.dynamics) {
@@ -673,6 +702,14 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
     if (p->units[i].strip_w)
       o << "launch " << i << ": tiles in strips " << p->units[i].strip_w << " tiles wide\n";
+  }
+  {
+    std::set<int> shown;
+    for (size_t i = 0; i < p->plan.stages.size(); ++i) {
+      const int j = p->plan.stages[i].jit;
+      if (j >= 0 && i < p->jit_stage_fn.size() && p->jit_stage_fn[i] && shown.insert(j).second)
+        o << "jit#" << j << ": compiled at run time in " << jit_compile_seconds(j) << " s\n";
+    }
   }
   for (size_t i = 0; i < p->plan.intermediates.size(); ++i) {
     const zh_descriptor& d = p->plan.intermediates[i];
@@ -889,7 +926,8 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
         bool aligned = true;
         for (size_t k = 0; k < refs.size(); ++k) aligned = aligned && d.slot[k].vec_ok;
         const StagePlan& sp = plan.stages[s];
-        variant[s] = !aligned ? 0 : sp.spec >= 0 ? 2 : sp.fast ? 1 : 0;
+        const bool jit = sp.jit >= 0 && s < p->jit_stage_fn.size() && p->jit_stage_fn[s];
+        variant[s] = !aligned ? 0 : sp.spec >= 0 ? 2 : jit ? 6 : sp.fast ? 1 : 0;
       }
     }
     p->h_stages[s] = d;
@@ -926,10 +964,22 @@ static zh_status bind(zh_program* p, const zh_binding* b, uint32_t n) {
 static zh_status launch_unit(zh_program* p, const Unit& u) {
   hipStream_t s = p->ctx->stream;
   if (u.batch) {
-    bool all_spec = u.spec;
-    for (uint32_t id : u.ids) all_spec = all_spec && p->variant[id] == 2;
+    bool all_spec = u.spec, all_jit = u.spec;
+    for (uint32_t id : u.ids) {
+      all_spec = all_spec && p->variant[id] == 2;
+      all_jit = all_jit && p->variant[id] == 6;
+    }
     if (all_spec) {
       ZH_HIP(zh_launch_spec_batch(p->plan.stages[u.ids[0]].spec, u.d_batch, u.total_tiles, (uint32_t)p->ctx->cus, s));
+      return ZH_OK;
+    }
+    if (all_jit) {
+      const uint32_t first = u.ids[0];
+      const uint32_t blocks = std::min<uint32_t>(u.total_tiles, p->jit_batch_resident[first] * (uint32_t)p->ctx->cus);
+      if (blocks == 0) return ZH_OK;
+      const BatchDesc* bd = u.d_batch;
+      void* args[] = {&bd};
+      ZH_HIP(hipModuleLaunchKernel(p->jit_batch_fn[first], blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
       return ZH_OK;
     }
     bool all_warp = u.warp;
@@ -965,6 +1015,17 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
                                   (uint32_t)p->ctx->cus, s));
       else if (v == 2)
         ZH_HIP(zh_launch_spec(sp.spec, d, sp.desc.width, sp.desc.height, (uint32_t)p->ctx->cus, s));
+      else if (v == 6) {
+        // A run-time kernel: the persistent grid of zh_launch_spec.
+        const uint64_t groups = ((uint64_t)sp.desc.width * sp.desc.height + kPix - 1) / kPix;
+        const uint64_t blocks =
+            std::min<uint64_t>((groups + kBlock - 1) / kBlock, (uint64_t)p->jit_resident[first] * p->ctx->cus);
+        if (blocks == 0) break;
+        const StageDesc* dp = d;
+        void* args[] = {&dp};
+        ZH_HIP(hipModuleLaunchKernel(p->jit_stage_fn[first], (uint32_t)blocks, 1, 1, kBlock, 1, 1, 0, s, args,
+                                     nullptr));
+      }
       else
         ZH_HIP(zh_launch_pointwise(d, sp.desc.width, sp.desc.height, (int)sp.desc.flat, v, (uint32_t)p->ctx->cus,
                                    s));

@@ -124,6 +124,30 @@ def c2(width: int = 7680, height: int = 4320, seed: int = 0) -> Workload:
     return Workload("c2_color_over_7680x4320", cmd, {src: d8, layer: dl}, {o: od}, make, width * height)
 
 
+def c2_reordered(width: int = 7680, height: int = 4320, seed: int = 0) -> Workload:
+    """C2 with the over-blend moved before the Oklab adjustment: the same
+    operations in an order no specialised kernel is registered for, so the
+    stage runs on the general pointwise interpreter."""
+    cmd = CommandBuffer()
+    d8 = Descriptor.rgba8_srgb(width, height)
+    dl = Descriptor(width, height, RGBA16F, P3_LINEAR)
+    src = cmd.input(d8)
+    layer = cmd.input(dl)
+    p3 = cmd.color_convert(src, P3_LINEAR, RGBA16F)
+    over = cmd.blend(p3, Rectangle(0, 0, width, height), layer)
+    lab = cmd.color_convert(over, Color.OKLAB, LABA16F)
+    lab = cmd.oklab_adjust(lab, chroma=1.2, hue_rad=0.0)
+    back = cmd.color_convert(lab, P3_LINEAR, RGBA16F)
+    enc = cmd.color_convert(back, P3_SRGB, RGBA8)
+    o, od = cmd.output(enc)
+
+    def make():
+        r0, r1 = (np.random.default_rng([seed, k]) for k in range(2))
+        return {src: random_rgba8(r0, width, height), layer: random_rgba16f(r1, width, height)}
+
+    return Workload("c2_reordered_7680x4320", cmd, {src: d8, layer: dl}, {o: od}, make, width * height)
+
+
 def c3_shapes(n: int = 256, seed: int = 0, lo: int = 64, hi: int = 4096):
     rng = np.random.default_rng([seed, 3])
     w = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(int)
@@ -240,3 +264,5 @@ def copy_frame(width: int = 7680, height: int = 4320, seed: int = 0) -> Workload
 
 
 CONFIGS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
+# Workloads measured beside the five configurations (not headline lines).
+EXTRA = {"c2r": c2_reordered}
