@@ -94,3 +94,41 @@ def test_compiled_kernel_inside_a_graph(ctx):
     g = exe.graph_info()
     report("jit_graph", **g)
     assert g["state"] == "captured" and g["replays"] >= 2
+
+
+def test_unregistered_warp_form_runs_a_compiled_kernel(ctx):
+    """A rotated RGBA16F layer composited onto a transparent canvas and
+    converted from Display-P3 to sRGB: a post program (un-premultiply, matrix,
+    store) no registered warp kernel has.  The compiled warp kernel against
+    the FP64 oracle and the resample interpreter, alone and as a batch of two
+    images (one launch)."""
+    from oracle import color as C
+    rng = np.random.default_rng(12)
+    cmd = z.CommandBuffer()
+    regs, host = [], {}
+    for w, h, ow, oh in [(200, 120, 260, 200), (97, 64, 150, 101)]:
+        d = z.Descriptor(w, h, W.RGBA16F, W.P3_LINEAR)
+        a = cmd.input(d)
+        canvas = cmd.solid_rgba(z.Descriptor(ow, oh, W.RGBA16F, W.P3_LINEAR), (0, 0, 0, 0))
+        aff = z.Affine.new(z.AffineSample.BiLinear).rotate(0.3).shift(40.0, 10.0)
+        r = cmd.affine(canvas, aff, a)
+        o, od = cmd.output(cmd.color_convert(r, z.Color.SRGB, W.RGBA8))
+        host[a] = W.random_rgba16f(rng, w, h)
+        regs.append((a, d, o, od, aff, w, h, ow, oh))
+    inputs = {a: d for (a, d, *_rest) in regs}
+    outputs = {o: od for (_a, _d, o, od, *_rest) in regs}
+    out, exe = run(ctx, cmd, inputs, host, outputs)
+    text = exe.describe()
+    assert "[kernel jitw#" in text and "over 2 stages" in text and "unavailable" not in text, text
+    interp, exe2 = run(ctx, cmd, inputs, host, outputs, flags=z.PLAN_NO_JIT)
+    assert "jitw#" not in exe2.describe()
+    m709 = C.convert_matrix(7, 3, 5)
+    for (a, d, o, od, aff, w, h, ow, oh) in regs:
+        src = PL.f16_image(host[a], w, h)
+        res = C.unpremultiply(C.bilinear_affine(C.premultiply(src), PL.mat_f32(aff.transformation), (oh, ow)))
+        want = C.encode_u8(PL.apply3(m709, res), 5)
+        mr, ma, exact = compare_u8(f"jitw_{w}x{h}", out[o], want, alpha_ok=res[..., 3] > 0.02)
+        assert mr <= 1 and ma <= 1 and exact > 0.99
+        diff = np.abs(out[o].astype(int) - interp[o].astype(int))
+        report(f"jitw_vs_interpreter_{w}x{h}", max_code_diff=int(diff.max()), equal=float((diff == 0).mean()))
+        assert diff.max() <= 1

@@ -32,3 +32,21 @@ def test_registered_and_unsupported_chains():
     cmd.output(s)
     text = describe(cmd)
     assert "jit#" not in text
+
+
+def test_unregistered_warp_form_is_compiled_at_run_time():
+    d = z.Descriptor(200, 120, W.RGBA16F, W.P3_LINEAR)
+    cmd = z.CommandBuffer()
+    a = cmd.input(d)
+    canvas = cmd.solid_rgba(z.Descriptor(300, 180, W.RGBA16F, W.P3_LINEAR), (0, 0, 0, 0))
+    r = cmd.affine(canvas, z.Affine.new(z.AffineSample.BiLinear).rotate(0.3), a)
+    cmd.output(cmd.color_convert(r, z.Color.SRGB, W.RGBA8))
+    assert "post[unpremul mat3 store#1] [kernel jitw#" in describe(cmd)
+    assert "jitw#" not in describe(cmd, z.PLAN_NO_JIT)
+    # axis-aligned maps pick the separable form
+    cmd2 = z.CommandBuffer()
+    a2 = cmd2.input(d)
+    c2 = cmd2.solid_rgba(z.Descriptor(300, 180, W.RGBA16F, W.P3_LINEAR), (0, 0, 0, 0))
+    r2 = cmd2.affine(c2, z.Affine.new(z.AffineSample.BiLinear).scale(1.5, 1.5), a2)
+    cmd2.output(cmd2.color_convert(r2, z.Color.SRGB, W.RGBA8))
+    assert "_sep]" in describe(cmd2).split("jitw#")[1]

@@ -30,7 +30,8 @@ SOURCES = [
     "zh_dynamic.cpp",
     "zh_jit.cpp",
 ]
-HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_spec_kernels.h", "zh_internal.h",
+HEADERS = ["zh_stage.h", "zh_codec.h", "zh_device.h", "zh_spec.h", "zh_prog.h", "zh_spec_kernels.h",
+           "zh_warp_kernels.h", "zh_internal.h",
            "zh_mlp.h"]
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) is disabled for every device compile.

@@ -69,6 +69,16 @@ int jit_signature(const uint16_t* sig, int n);
 // device), compiled on first use; `why` receives the reason of a failure.
 zh_status jit_kernels(int id, int device, hipFunction_t* stage, hipFunction_t* batch, std::string* why);
 double jit_compile_seconds(int id);
+// Affine layer-stack forms without a registered warp kernel: a post program
+// of operations the warp bodies run, and the process-wide id of a form.
+bool jit_warp_supported(const uint16_t* post, int n);
+int jit_warp_form(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n);
+// The general or separable stage kernel and batch kernel of form `id`.
+zh_status jit_warp_kernels(int id, bool sep, int device, hipFunction_t* stage, hipFunction_t* batch,
+                           std::string* why);
+// StageDesc::variant of a run-time warp form: kJitWarpBase + 2 id (+ 1 for
+// separable maps); registered forms are below it.
+constexpr int kJitWarpBase = 1 << 16;
 
 // A user stage of a plan: the shader and its data.
 struct DynamicCall {

@@ -6,7 +6,10 @@
 // group.  Instead, the first program built with such a signature compiles the
 // same kernel bodies (zh_spec_kernels.h) for it with hiprtc -- the kernel is
 // `spec_stage<Prog<signature>>`, straight-line code with no dispatch -- and
-// every later program with the signature reuses the code object.  This is
+// every later program with the signature reuses the code object.  Resample
+// stages whose affine layer-stack form (source form, format, matrix, post
+// program) has no registered warp kernel get theirs the same way from the
+// warp bodies (zh_warp_kernels.h).  This is
 // the HIP form of what the reference does when wgpu builds a pipeline for a
 // shader at run time (program/encoder.rs:1375-1389): the backend's kernels
 // cover any chain of the operations the specialised bodies implement, not a
@@ -24,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -35,12 +39,17 @@ using namespace zh;
 
 namespace {
 
+// A pointwise signature (warp = false) or an affine layer-stack form: source
+// form, source format, mat3 flag and the post program's signature.
 struct JitEntry {
+  bool warp = false;
+  uint32_t src = 0, fmt = 0, mat = 0;
   std::vector<uint16_t> sig;
   bool compiled = false;
-  std::string error;             // why the compile failed (compiled and no shaders)
-  zh_shader* stage = nullptr;    // kernel zh_jit_stage (held for the process)
-  zh_shader* batch = nullptr;    // kernel zh_jit_batch
+  std::string error;  // why the compile failed (compiled and no kernels)
+  // Kernels held for the process: pointwise {stage, batch}; warp {stage,
+  // stage of separable maps, batch, batch of separable maps}.
+  std::vector<zh_shader*> kernels;
   double compile_s = 0.0;
 };
 
@@ -59,21 +68,33 @@ std::string package_dir() {
   return k == std::string::npos ? std::string(".") : p.substr(0, k);
 }
 
-std::string source_for(const std::vector<uint16_t>& sig) {
+const char* const kSpecKernels[] = {"zh_jit_stage", "zh_jit_batch"};
+const char* const kWarpKernels[] = {"zh_jit_warp", "zh_jit_warp_sep", "zh_jit_warp_batch", "zh_jit_warp_batch_sep"};
+
+std::string source_for(const JitEntry& e) {
   std::string list;
-  for (size_t i = 0; i < sig.size(); ++i) list += (i ? ", " : "") + std::to_string(sig[i]);
-  return "#include \"zh_spec_kernels.h\"\n"
-         "typedef zh::spec::Prog<" + list + "> P;\n"
-         "extern \"C\" __global__ __launch_bounds__(256) void zh_jit_stage(const zh::StageDesc* d) {\n"
-         "  zh::spec::spec_stage<P>(d);\n}\n"
-         "extern \"C\" __global__ __launch_bounds__(256) void zh_jit_batch(const zh::BatchDesc* b) {\n"
-         "  zh::spec::spec_batch<P>(b);\n}\n";
+  for (size_t i = 0; i < e.sig.size(); ++i) list += (i ? ", " : "") + std::to_string(e.sig[i]);
+  const std::string k = "extern \"C\" __global__ __launch_bounds__(256) void ";
+  if (!e.warp)
+    return "#include \"zh_spec_kernels.h\"\n"
+           "typedef zh::spec::Prog<" + list + "> P;\n" +
+           k + "zh_jit_stage(const zh::StageDesc* d) { zh::spec::spec_stage<P>(d); }\n" +
+           k + "zh_jit_batch(const zh::BatchDesc* b) { zh::spec::spec_batch<P>(b); }\n";
+  const std::string form = std::to_string(e.src) + "u, " + std::to_string(e.fmt) + "u, " + (e.mat ? "true" : "false") +
+                           ", zh::spec::Prog<" + list + ">, ";
+  return "#include \"zh_warp_kernels.h\"\n"
+         "typedef zh::spec::Warp<" + form + "false> V;\n"
+         "typedef zh::spec::Warp<" + form + "true> VS;\n" +
+         k + "zh_jit_warp(const zh::StageDesc* d) { zh::spec::warp_stage<V>(d); }\n" +
+         k + "zh_jit_warp_sep(const zh::StageDesc* d) { zh::spec::warp_stage<VS>(d); }\n" +
+         k + "zh_jit_warp_batch(const zh::BatchDesc* b) { zh::spec::warp_batch<V>(b); }\n" +
+         k + "zh_jit_warp_batch_sep(const zh::BatchDesc* b) { zh::spec::warp_batch<VS>(b); }\n";
 }
 
 // Compile entry e (called with g_mu held, once).
 void compile(JitEntry& e) {
   e.compiled = true;
-  const std::string src = source_for(e.sig);
+  const std::string src = source_for(e);
   hiprtcProgram prog = nullptr;
   if (hiprtcCreateProgram(&prog, src.c_str(), "zh_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
     e.error = "hiprtcCreateProgram failed";
@@ -102,9 +123,57 @@ void compile(JitEntry& e) {
   std::vector<char> code(n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
-  if (zh_shader_create(code.data(), code.size(), "zh_jit_stage", &e.stage) != ZH_OK ||
-      zh_shader_create(code.data(), code.size(), "zh_jit_batch", &e.batch) != ZH_OK)
-    e.error = "code object rejected";
+  const char* const* names = e.warp ? kWarpKernels : kSpecKernels;
+  const int count = e.warp ? 4 : 2;
+  for (int i = 0; i < count; ++i) {
+    zh_shader* k = nullptr;
+    if (zh_shader_create(code.data(), code.size(), names[i], &k) != ZH_OK) {
+      e.error = "code object rejected";
+      for (zh_shader* x : e.kernels) zh_shader_release(x);
+      e.kernels.clear();
+      return;
+    }
+    e.kernels.push_back(k);
+  }
+}
+
+int find_or_add(const JitEntry& want) {
+  std::vector<JitEntry>& es = entries();
+  for (size_t i = 0; i < es.size(); ++i) {
+    const JitEntry& e = es[i];
+    if (e.warp == want.warp && e.src == want.src && e.fmt == want.fmt && e.mat == want.mat && e.sig == want.sig)
+      return (int)i;
+  }
+  es.push_back(want);
+  return (int)es.size() - 1;
+}
+
+// The kernels `which` of entry `id` on `device`, compiling the entry first.
+zh_status kernels_of(int id, bool warp, std::initializer_list<int> which, int device, hipFunction_t* out,
+                     std::string* why) {
+  std::vector<zh_shader*> ks;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    std::vector<JitEntry>& es = entries();
+    if (id < 0 || id >= (int)es.size() || es[(size_t)id].warp != warp)
+      return fail(ZH_ERR_INVALID_ARGUMENT, "unknown run-time kernel");
+    JitEntry& e = es[(size_t)id];
+    if (!e.compiled) compile(e);
+    if (e.kernels.empty()) {
+      if (why) *why = e.error;
+      return fail(ZH_ERR_UNSUPPORTED, "run-time kernel unavailable: " + e.error);
+    }
+    ks = e.kernels;
+  }
+  int k = 0;
+  for (int w : which) {
+    zh_status st = shader_function(ks[(size_t)w], device, &out[k++]);
+    if (st != ZH_OK) {
+      if (why) *why = zh_last_error();
+      return st;
+    }
+  }
+  return ZH_OK;
 }
 
 }  // namespace
@@ -124,34 +193,47 @@ bool jit_supported(const uint16_t* sig, int n) {
 
 int jit_signature(const uint16_t* sig, int n) {
   std::lock_guard<std::mutex> lock(g_mu);
-  std::vector<JitEntry>& es = entries();
-  for (size_t i = 0; i < es.size(); ++i)
-    if ((int)es[i].sig.size() == n && std::memcmp(es[i].sig.data(), sig, sizeof(uint16_t) * (size_t)n) == 0)
-      return (int)i;
   JitEntry e;
   e.sig.assign(sig, sig + n);
-  es.push_back(std::move(e));
-  return (int)es.size() - 1;
+  return find_or_add(e);
 }
 
 zh_status jit_kernels(int id, int device, hipFunction_t* stage, hipFunction_t* batch, std::string* why) {
-  zh_shader *s = nullptr, *b = nullptr;
-  {
-    std::lock_guard<std::mutex> lock(g_mu);
-    std::vector<JitEntry>& es = entries();
-    if (id < 0 || id >= (int)es.size()) return fail(ZH_ERR_INVALID_ARGUMENT, "unknown run-time kernel");
-    JitEntry& e = es[(size_t)id];
-    if (!e.compiled) compile(e);
-    if (!e.stage) {
-      if (why) *why = e.error;
-      return fail(ZH_ERR_UNSUPPORTED, "run-time kernel unavailable: " + e.error);
-    }
-    s = e.stage;
-    b = e.batch;
+  hipFunction_t f[2] = {nullptr, nullptr};
+  zh_status st = kernels_of(id, false, {0, 1}, device, f, why);
+  *stage = f[0];
+  *batch = f[1];
+  return st;
+}
+
+bool jit_warp_supported(const uint16_t* post, int n) {
+  if (n <= 0 || n > kMaxInstr) return false;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t op = post[i] & 0xffu;
+    if (op != OP_STORE && op != OP_MAT3 && op != OP_CBRT && op != OP_CUBE && op != OP_CLAMP01 && op != OP_PREMUL &&
+        op != OP_UNPREMUL)
+      return false;
   }
-  zh_status st = shader_function(s, device, stage);
-  if (st == ZH_OK) st = shader_function(b, device, batch);
-  if (st != ZH_OK && why) *why = zh_last_error();
+  return (post[n - 1] & 0xffu) == OP_STORE;
+}
+
+int jit_warp_form(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  JitEntry e;
+  e.warp = true;
+  e.src = src;
+  e.fmt = fmt;
+  e.mat = mat;
+  e.sig.assign(post, post + n);
+  return find_or_add(e);
+}
+
+zh_status jit_warp_kernels(int id, bool sep, int device, hipFunction_t* stage, hipFunction_t* batch,
+                           std::string* why) {
+  hipFunction_t f[2] = {nullptr, nullptr};
+  zh_status st = sep ? kernels_of(id, true, {1, 3}, device, f, why) : kernels_of(id, true, {0, 2}, device, f, why);
+  *stage = f[0];
+  *batch = f[1];
   return st;
 }
 

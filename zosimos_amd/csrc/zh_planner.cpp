@@ -985,7 +985,7 @@ static uint64_t tile_box_texels(const Mat3& inv, int TW, int TH) {
 // Registered warp form of a bilinear resample stage (zh_spec.h), or -1:
 // affine layers, a solid (or empty) base, one source form shared by every
 // layer and a registered post program.
-static int warp_variant(const StageDesc& d) {
+static int warp_variant(const StageDesc& d, uint32_t flags) {
   const uint32_t nb = d.base_end - d.base_begin;
   if (!(nb == 0 || (nb == 2 && d.code[d.base_begin].op == OP_SOLID && d.code[d.base_begin + 1].op == OP_PREMUL)))
     return -1;
@@ -1037,7 +1037,10 @@ static int warp_variant(const StageDesc& d) {
     if (in.op == OP_STORE && (d.slot[in.a].width != d.width || d.slot[in.a].height != d.height)) return -1;
     sig[i] = sg(in.op, in.op == OP_STORE ? d.slot[in.a].fmt : 0u);
   }
-  const int v = zh_warp_lookup(src, fmt, mat, sig, (int)np);
+  int v = zh_warp_lookup(src, fmt, mat, sig, (int)np);
+  // A form without a registered kernel gets one compiled at run time.
+  if (v < 0 && !(flags & ZH_PLAN_NO_JIT) && jit_warp_supported(sig, (int)np))
+    v = kJitWarpBase + 2 * jit_warp_form(src, fmt, mat, sig, (int)np);
   if (v < 0) return -1;
   bool sep = true;
   for (uint32_t l = 0; l < d.n_layers; ++l) sep = sep && d.layer[l].inv[1] == 0.0 && d.layer[l].inv[3] == 0.0;
@@ -1585,7 +1588,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
     // A registered affine layer form runs as a warp kernel (zh_warp.hip) on
     // 64 x 16 tiles; the interpreter kernels stay valid on the same tiles as
     // the fallback for images the warp kernel cannot address.
-    sp.desc.variant = bil ? warp_variant(sp.desc) : -1;
+    sp.desc.variant = bil ? warp_variant(sp.desc, flags) : -1;
     if (sp.desc.variant >= 0) {
       max_box = 0;
       choose_tiles({{64, 16}});
@@ -1608,7 +1611,10 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
            << " " << ly.src_w << "x" << ly.src_h << ")[" << describe_code(sp.desc, ly.prog_begin, ly.prog_end) << "]";
       }
       t2 << " post[" << describe_code(sp.desc, sp.desc.post_begin, sp.desc.post_end) << "]";
-      if (sp.desc.variant >= 0) t2 << " [kernel " << zh_warp_name(sp.desc.variant) << "]";
+      if (sp.desc.variant >= kJitWarpBase)
+        t2 << " [kernel jitw#" << (sp.desc.variant - kJitWarpBase) / 2 << ((sp.desc.variant & 1) ? "_sep" : "") << "]";
+      else if (sp.desc.variant >= 0)
+        t2 << " [kernel " << zh_warp_name(sp.desc.variant) << "]";
       sp.text = t2.str();
     }
     plan.stages.push_back(sp);

@@ -539,6 +539,31 @@ static zh_status alloc_device(zh_program* p) {
   p->jit_batch_resident.assign(p->plan.stages.size(), 1);
   for (size_t s = 0; s < p->plan.stages.size(); ++s) {
     StagePlan& sp = p->plan.stages[s];
+    if (sp.desc.kind == STAGE_RESAMPLE && sp.desc.variant >= kJitWarpBase) {
+      // A run-time warp form; without it the stage runs on the resample
+      // interpreter, whose kernels accept the same tiles.
+      std::string why;
+      hipFunction_t f = nullptr, fb = nullptr;
+      const int id = (sp.desc.variant - kJitWarpBase) / 2;
+      const bool sep = (sp.desc.variant & 1) != 0;
+      if (p->arena || jit_warp_kernels(id, sep, ctx->device, &f, &fb, &why) != ZH_OK) {
+        sp.desc.variant = -1;
+        const size_t k = sp.text.rfind(" [kernel jitw#");
+        if (k != std::string::npos)
+          sp.text = sp.text.substr(0, k) +
+                    (p->arena ? std::string() : " (run-time kernel unavailable: " + why.substr(0, 120) + ")");
+        continue;
+      }
+      int n = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kBlock, 0) == hipSuccess && n > 0)
+        p->jit_resident[s] = (uint32_t)n;
+      n = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb, kBlock, 0) == hipSuccess && n > 0)
+        p->jit_batch_resident[s] = (uint32_t)n;
+      p->jit_stage_fn[s] = f;
+      p->jit_batch_fn[s] = fb;
+      continue;
+    }
     if (sp.jit < 0) continue;
     std::string why;
     hipFunction_t f = nullptr, fb = nullptr;
@@ -985,6 +1010,16 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     bool all_warp = u.warp;
     for (uint32_t id : u.ids) all_warp = all_warp && p->variant[id] == 3;
     if (all_warp) {
+      if (p->plan.stages[u.ids[0]].desc.variant >= kJitWarpBase) {
+        const uint32_t first = u.ids[0], res = p->jit_batch_resident[first];
+        const uint32_t bpc = u.blocks_per_cu ? std::min(u.blocks_per_cu, res) : res;
+        const uint32_t blocks = std::min<uint32_t>(u.total_tiles, bpc * (uint32_t)p->ctx->cus);
+        if (blocks == 0) return ZH_OK;
+        const BatchDesc* bd = u.d_batch;
+        void* args[] = {&bd};
+        ZH_HIP(hipModuleLaunchKernel(p->jit_batch_fn[first], blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
+        return ZH_OK;
+      }
       ZH_HIP(zh_launch_warp_batch(p->plan.stages[u.ids[0]].desc.variant, u.d_batch, u.total_tiles,
                                   (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
       return ZH_OK;
@@ -1032,11 +1067,21 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
       break;
     }
     case STAGE_RESAMPLE:
-      if (first < p->variant.size() && p->variant[first] == 3)
+      if (first < p->variant.size() && p->variant[first] == 3 && sp.desc.variant >= kJitWarpBase) {
+        // A run-time warp form: the persistent grid of zh_launch_warp.
+        const uint32_t res = p->jit_resident[first];
+        const uint32_t bpc = u.blocks_per_cu ? std::min(u.blocks_per_cu, res) : res;
+        const uint32_t blocks = std::min<uint32_t>(sp.desc.tiles_x * sp.desc.tiles_y, bpc * (uint32_t)p->ctx->cus);
+        if (blocks == 0) break;
+        const StageDesc* dp = d;
+        void* args[] = {&dp};
+        ZH_HIP(hipModuleLaunchKernel(p->jit_stage_fn[first], blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
+      } else if (first < p->variant.size() && p->variant[first] == 3) {
         ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus,
                               u.blocks_per_cu, s));
-      else
+      } else {
         ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
+      }
       break;
     case STAGE_COPY:
       ZH_HIP(zh_launch_copy(d, sp.desc.copy_row_bytes, sp.desc.copy_rows, s));
