@@ -175,15 +175,32 @@ __global__ __launch_bounds__(kBlock) void k_grad_part(zh_mlp_batch b, int layer,
   }
 }
 
-// Sum the chunk partials in chunk order into the gradient vector.
-__global__ __launch_bounds__(kBlock) void k_grad_reduce(const float* part, uint32_t chunks, int layer, float* grad) {
+// Sum the chunk partials into the gradient vector in a fixed two-level order
+// (deterministic, and kReduceGroups times the parallelism of one serial sum
+// per entry): group g sums chunks [g per, (g + 1) per) of every entry, then
+// each entry sums its kReduceGroups group sums in order.
+constexpr uint32_t kReduceGroups = 16;
+
+__global__ __launch_bounds__(kBlock) void k_grad_reduce_groups(const float* part, uint32_t chunks, int entries,
+                                                               float* sums) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= entries) return;
+  const uint32_t g = blockIdx.y;
+  const uint32_t per = (chunks + kReduceGroups - 1) / kReduceGroups;
+  const uint32_t c0 = g * per, c1 = min(chunks, c0 + per);
+  float s = 0.0f;
+  for (uint32_t c = c0; c < c1; ++c) s += part[(uint64_t)c * entries + e];
+  sums[(uint64_t)g * entries + e] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_reduce(const float* sums, int layer, float* grad) {
   const int n_in = layer == 1 ? kIn : kHid;
   const int n_out = layer == 3 ? kOut : kHid;
   const int entries = (n_in + 1) * n_out;
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= entries) return;
   float s = 0.0f;
-  for (uint32_t c = 0; c < chunks; ++c) s += part[(uint64_t)c * entries + e];
+  for (uint32_t g = 0; g < kReduceGroups; ++g) s += sums[(uint64_t)g * entries + e];
   const int i = e / n_out, o = e % n_out;
   const int wo = layer == 1 ? kOffW1 : (layer == 2 ? kOffW2 : kOffW3);
   const int bo = layer == 1 ? kOffB1 : (layer == 2 ? kOffB2 : kOffB3);
@@ -361,7 +378,9 @@ extern "C" {
 
 using namespace zh::mlp;
 
-size_t zh_mlp_part_floats(uint32_t n) { return (size_t)((n + kChunk - 1) / kChunk) * (kHid + 1) * kHid; }
+size_t zh_mlp_part_floats(uint32_t n) {
+  return ((size_t)((n + kChunk - 1) / kChunk) + kReduceGroups) * (kHid + 1) * kHid;
+}
 
 hipError_t zh_mlp_quantise(const float* p, float* q, hipStream_t s) {
   hipLaunchKernelGGL(k_quant, dim3((kParams + 255) / 256), dim3(256), 0, s, p, q);
@@ -382,12 +401,15 @@ hipError_t zh_mlp_loss_grad(const float* q, const zh_mlp_batch* b, float* grad, 
   hipLaunchKernelGGL(k_bwd_hidden, gh, dim3(kBlock), 0, s, *b, 2, q);
   hipLaunchKernelGGL(k_bwd_hidden, gh, dim3(kBlock), 0, s, *b, 1, q);
   const uint32_t chunks = (n + kChunk - 1) / kChunk;
-  float* part = grad + kParams;  // chunks * (kHid + 1) * kHid floats
+  float* part = grad + kParams;                        // chunks * (kHid + 1) * kHid floats
+  float* sums = part + (size_t)chunks * (kHid + 1) * kHid;  // kReduceGroups * (kHid + 1) * kHid
   for (int layer = 3; layer >= 1; --layer) {
     const int entries = ((layer == 1 ? kIn : kHid) + 1) * (layer == 3 ? kOut : kHid);
+    const uint32_t eb = (entries + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_grad_part, dim3(chunks), dim3(kBlock), 0, s, *b, layer, part);
-    hipLaunchKernelGGL(k_grad_reduce, dim3((entries + kBlock - 1) / kBlock), dim3(kBlock), 0, s, part, chunks,
-                       layer, grad);
+    hipLaunchKernelGGL(k_grad_reduce_groups, dim3(eb, kReduceGroups), dim3(kBlock), 0, s, part, chunks, entries,
+                       sums);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(eb), dim3(kBlock), 0, s, sums, layer, grad);
   }
   return hipGetLastError();
 }

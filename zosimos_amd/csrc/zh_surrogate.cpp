@@ -213,6 +213,12 @@ zh_status train_stage(zh_program* p, uint32_t stage, const zh_learn_options& o, 
     sl.vec_ok = 1;
     sl.ptr = 0;
   }
+  // The targets are the chain before its final clamps (oklab.frag:63):
+  // smooth where the clamped chain has kinks at the gamut boundary.  The
+  // inference kernel clamps the network's output when it stores 8-bit
+  // texels, which applies the clamps exactly.
+  for (uint32_t i = 0; i < td.n_instr; ++i)
+    if (td.code[i].op == OP_CLAMP01) td.code[i].op = OP_NOP;
   td.slot[td.code[0].a].ptr = (uint64_t)(uintptr_t)d_x;
   td.slot[td.code[td.n_instr - 1].a].ptr = (uint64_t)(uintptr_t)d_t;
   StageDesc* d_td = nullptr;
