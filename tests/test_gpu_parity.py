@@ -206,8 +206,7 @@ def test_nearest_affine_rotate45(ctx):
     got = out[o].reshape(512, 512, 4)
     mismatch = np.any(got != want.astype(np.uint8), axis=-1)
     report("nearest_affine_rotate45", mismatched_pixels=int(mismatch.sum()))
-    # pixel centres exactly on a quad edge may round either way
-    assert mismatch.sum() <= 8
+    assert mismatch.sum() == 0
 
 
 def test_blend_at_offset(ctx):

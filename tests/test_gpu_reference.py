@@ -307,13 +307,12 @@ def test_fractal_noise_matches_reference_hash(ctx):
 def test_palette_reference_program(ctx):
     """blend.rs:377-421 (run_palette) with strict registers: the background
     sampled at the coordinates of an 8-bit sRGB red-green ramp.  The ramp is
-    computed in FP32 on the device and in FP64 by the oracle, so a ramp value
-    within rounding of a code edge may take the neighbouring code: at most
-    0.1% of the pixels may differ from the oracle, each by the texel one
-    column or row over.  The blessed ``palette`` hash is not required: its
-    two blessed lines differ from each other in 6 of 256 bits (it depends on
-    the machine that rendered it), and this output lies within 8 bits of
-    one of them."""
+    computed in FP32 on the device and in FP64 by the oracle, then stored in
+    its 8-bit register; every output texel equals the oracle's lookup.  The
+    blessed ``palette`` hash is not reproduced exactly: its two blessed lines
+    differ from each other in 6 of 256 bits (it depends on the machine that
+    rendered it), and this output lies no farther than that from one of
+    them."""
     bg = rgba("background")
     h, w = bg.shape[:2]
     ramp_p = z.Bilinear(u_min=(0.0, 0.0, 0.0, 1.0), v_min=(0.0, 0.0, 0.0, 1.0), uv_min=(0.0, 0.0, 0.0, 1.0),
@@ -331,18 +330,15 @@ def test_palette_reference_program(ctx):
     ctx.sync()
     out = out_img.download().reshape(400, 400, 4)
     ramp_lin = C.decode_u8(C.encode_u8(PL.bilinear_grid(400, 400, ramp_p.params()), 5), 5)
-    want, (ty, tx) = PL.palette_lookup(bg, ramp_lin, [1, 0, 0, 0], [0, 1, 0, 0], return_index=True)
+    want = PL.palette_lookup(bg, ramp_lin, [1, 0, 0, 0], [0, 1, 0, 0])
     bad = np.argwhere(np.any(out != want, axis=-1))
     report("palette_program", differing=len(bad))
-    assert len(bad) <= 160
-    for y, x in bad:
-        near = [bg[min(max(ty[y, x] + dy, 0), h - 1), min(max(tx[y, x] + dx, 0), w - 1)]
-                for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
-        assert any(np.array_equal(out[y, x], n) for n in near)
+    assert len(bad) == 0
     got = B.blockhash256(out)
+    spread = bin(int(HASHES["palette"][0], 16) ^ int(HASHES["palette"][1], 16)).count("1")
     dist = min(bin(int(got, 16) ^ int(b, 16)).count("1") for b in HASHES["palette"])
-    report("palette_hash", hamming=dist)
-    assert dist <= 8
+    report("palette_hash", hamming=dist, blessed_spread=spread)
+    assert dist <= spread
 
 
 @pytest.mark.parametrize("flags", [0, 1])
