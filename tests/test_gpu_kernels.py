@@ -252,9 +252,10 @@ def test_c2_kernel_multi_step_grid(ctx):
 @pytest.mark.parametrize("scale,mode", [(0.75, "lds"), (0.02, "direct")])
 def test_resample_interpreter_fallback(ctx, scale, mode):
     """A bilinear layer stack with no registered warp form (linear RGBA8
-    canvas and output) runs on the interpreter resample kernels: LDS-staged
-    tiles, or four gathered taps per pixel when a tile's source box exceeds
-    the LDS budget."""
+    canvas and output) gets a warp kernel compiled at run time; with run-time
+    kernels off it runs on the interpreter resample kernels: LDS-staged tiles,
+    or four gathered taps per pixel when a tile's source box exceeds the LDS
+    budget.  Both match the oracle."""
     w, h = 192, 160
     d = desc_of("l8", w, h)
     cmd = z.CommandBuffer()
@@ -263,19 +264,23 @@ def test_resample_interpreter_fallback(ctx, scale, mode):
     canvas = cmd.solid_rgba(d, (0.0, 0.0, 0.0, 0.0))
     aff = W.about_centre(w, h, scale, math.radians(15.0))
     o, od = cmd.output(cmd.affine(canvas, aff, ab))
-    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    prog = z.Linker.from_included().compile(cmd)
+    exe = prog.lower_to(ctx, z.PLAN_NO_JIT)
     text = exe.describe()
     assert "[kernel" not in text and f" {mode} " in text, text
+    jit = prog.lower_to(ctx)
+    assert "[kernel jitw#" in jit.describe(), jit.describe()
     rng = np.random.default_rng(11)
     ha, hb = (rng.integers(0, 256, size=w * h * 4, dtype=np.uint8) for _ in range(2))
     ia, ib = ctx.image(d), ctx.image(d)
     ia.upload(ha)
     ib.upload(hb)
-    got = launch(ctx, exe, {a: ia, b: ib}, o, od)
     A = C.premultiply(C.decode_u8(PL.u8_image(ha, w, h), 4))
     Bv = C.premultiply(C.decode_u8(PL.u8_image(hb, w, h), 4))
     lin = C.unpremultiply(C.bilinear_affine(C.over(Bv, A), PL.mat_f32(aff.transformation), (h, w)))
-    check(f"fallback_{mode}", got, lin, "l8", w, h, lin[..., 3] > 0.5 / 255.0)
+    for tag, e in ((f"fallback_{mode}", exe), (f"fallback_{mode}_jit", jit)):
+        got = launch(ctx, e, {a: ia, b: ib}, o, od)
+        check(tag, got, lin, "l8", w, h, lin[..., 3] > 0.5 / 255.0)
 
 
 def _bind_and_run(ctx, wl, host, exe):

@@ -813,7 +813,8 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     const StagePlan& b = p->plan.stages[i];
     same = a.desc.kind == b.desc.kind && a.tile_mode == b.tile_mode && a.desc.tiles_x == b.desc.tiles_x &&
            a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes &&
-           a.spec == b.spec && a.fast == b.fast && a.desc.flat == b.desc.flat && a.desc.variant == b.desc.variant;
+           a.spec == b.spec && a.jit == b.jit && a.fast == b.fast && a.desc.flat == b.desc.flat &&
+           a.desc.variant == b.desc.variant;
   }
   p->params = params;
   // A surrogate was fitted to the old parameters; a buffer knob rewrites the
