@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-chain leg (surrogate vs plain colour chain)")
     ap.add_argument("--no-relaunch", action="store_true", help="skip the 512x512 inscribe relaunch leg")
+    ap.add_argument("--staged", action="store_true",
+                    help="plan with ZH_PLAN_STAGED: separable one-layer resample stages read their taps from "
+                         "LDS-staged source boxes instead of direct gathers (the A/B of the staged tiles)")
+    ap.add_argument("--no-jit", action="store_true",
+                    help="plan with ZH_PLAN_NO_JIT: chains without a registered kernel run on the interpreter")
     return ap.parse_args()
 
 
@@ -90,12 +95,12 @@ def timed_launches(exe, ctx, stream, torch, imgs, outs, steps):
     return start.elapsed_time(end) / steps
 
 
-def measure(wl, ctx, stream, torch, steps, warmup, traffic):
+def measure(wl, ctx, stream, torch, steps, warmup, traffic, flags=0):
     """One workload: compile, autotune on the bound images, warm up, time
     `steps` launches; the dominant launch's roofline from its share of the
     step (per-launch HIP events over as many launches)."""
     import zosimos_amd as z
-    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx, flags)
     launches, alg_bytes = exe.stats()
     imgs, outs, keep = device_inputs(wl, ctx, torch)
     exe.autotune(imgs, outs, iterations=3)
@@ -295,9 +300,10 @@ def main():
     stream = torch.cuda.Stream()  # default priority
     ctx = z.Context(0, stream=stream.cuda_stream)
     traffic = load_traffic()
+    flags = (z.PLAN_STAGED if args.staged else 0) | (z.PLAN_NO_JIT if args.no_jit else 0)
     head_cfg = HEADLINE if args.config == "all" else args.config
     wl = W.copy_frame() if head_cfg == "copy" else {**W.CONFIGS, **W.EXTRA}[head_cfg]()
-    head = measure(wl, ctx, stream, torch, args.steps, args.warmup, traffic)
+    head = measure(wl, ctx, stream, torch, args.steps, args.warmup, traffic, flags)
     line = {
         "metric": METRIC,
         "value": head["value"],
@@ -314,7 +320,8 @@ def main():
         "config": {"workload": head["workload"], "pixels_per_step": head["pixels_per_step"],
                    "launches_per_step": head["launches_per_step"],
                    "algorithmic_bytes_per_step": head["algorithmic_bytes_per_step"],
-                   "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_launch"]},
+                   "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_launch"],
+                   "plan_flags": flags},
         "hbm_gbps": head["hbm_gbps"],
         "hbm_frac": head["hbm_frac"],
         "roofline": head["roofline"],
@@ -324,13 +331,14 @@ def main():
     if args.config == "all":
         subs = {}
         for c in ("c1", "c2", "c4", "c5"):
-            subs[c] = measure(W.CONFIGS[c](), ctx, stream, torch, args.steps, args.warmup, traffic)
+            subs[c] = measure(W.CONFIGS[c](), ctx, stream, torch, args.steps, args.warmup, traffic, flags)
             if not args.no_cpu_baseline:
                 subs[c]["cpu_baseline"] = cpu_baseline(c, 3.0)
         line["configs"] = subs
-        # The same operations as c2 in an order no specialised kernel covers:
-        # the general pointwise interpreter's rate beside c2's.
-        line["general"] = measure(W.c2_reordered(), ctx, stream, torch, args.steps, args.warmup, traffic)
+        # The same operations as c2 in an order no registered kernel covers:
+        # the kernel compiled for it at run time (the interpreter with
+        # --no-jit), beside c2's.
+        line["general"] = measure(W.c2_reordered(), ctx, stream, torch, args.steps, args.warmup, traffic, flags)
     if not args.no_surrogate and args.config in ("all", "c4"):
         line["surrogate"] = surrogate_leg(ctx, stream, torch, args.steps)
     if not args.no_relaunch and args.config == "all":

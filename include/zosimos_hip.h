@@ -300,6 +300,11 @@ typedef struct zh_op {
 /* Run pointwise chains that have no registered specialised kernel on the
  * general interpreter instead of compiling one for them at run time. */
 #define ZH_PLAN_NO_JIT 4u
+/* Resample separable one-layer stages from each tile's source box staged in
+ * LDS (decoded and premultiplied once per texel) wherever the box fits,
+ * instead of gathering every tap directly.  Measured slower than the direct
+ * gathers on MI355X at every scale from 1x to 4x, so not the default. */
+#define ZH_PLAN_STAGED 8u
 
 /* Lower a High program (Program::lower_to, program.rs:1304).  `textures`
  * holds the descriptor of every texture id.  `ctx` may be NULL: the plan is

@@ -50,3 +50,4 @@ def test_unregistered_warp_form_is_compiled_at_run_time():
     r2 = cmd2.affine(c2, z.Affine.new(z.AffineSample.BiLinear).scale(1.5, 1.5), a2)
     cmd2.output(cmd2.color_convert(r2, z.Color.SRGB, W.RGBA8))
     assert "_sep]" in describe(cmd2).split("jitw#")[1]
+    assert "_sep_lds]" in describe(cmd2, z.PLAN_STAGED).split("jitw#")[1]

@@ -690,6 +690,7 @@ PLAN_DEFAULT = 0
 PLAN_STRICT_REGISTERS = 1
 PLAN_NO_GRAPH = 2
 PLAN_NO_JIT = 4
+PLAN_STAGED = 8
 
 
 class Program:

@@ -73,9 +73,12 @@ double jit_compile_seconds(int id);
 // of operations the warp bodies run, and the process-wide id of a form.
 bool jit_warp_supported(const uint16_t* post, int n);
 int jit_warp_form(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n);
-// The general or separable stage kernel and batch kernel of form `id`.
-zh_status jit_warp_kernels(int id, bool sep, int device, hipFunction_t* stage, hipFunction_t* batch,
+// The general, separable or LDS-staged (separable, one source) stage kernel
+// and batch kernel of form `id`.
+zh_status jit_warp_kernels(int id, bool sep, bool staged, int device, hipFunction_t* stage, hipFunction_t* batch,
                            std::string* why);
+// Does form `id` have LDS-staged kernels (one-source forms)?
+bool jit_warp_has_staged(int id);
 // StageDesc::variant of a run-time warp form: kJitWarpBase + 2 id (+ 1 for
 // separable maps); registered forms are below it.
 constexpr int kJitWarpBase = 1 << 16;
@@ -94,6 +97,7 @@ struct StagePlan {
   bool fast = false;              // pointwise: every slot has an in-line codec (k_pointwise<true>)
   int spec = -1;                  // pointwise: index of the specialised kernel (zh_spec.h), -1 = none
   int jit = -1;                   // pointwise: run-time kernel of the signature (zh_jit.cpp), -1 = none
+  bool staged = false;            // warp: the LDS-staged kernel (staged_tile), the box fitting its tile
   uint32_t lds_bytes = 0;
   uint64_t alg_bytes = 0;         // algorithmic HBM bytes (reads once + writes once)
   // slot -> what it refers to: >=0 input slot, or ~output slot (<0), or
@@ -119,6 +123,11 @@ struct Plan {
   std::vector<DynamicCall> dynamics;         // user stages (STAGE_DYNAMIC)
   std::string text;
 };
+
+// Does warp stage d get the LDS-staged kernel on tw x th tiles: a separable
+// one-layer one-source form that has one, every tile's source box within
+// kBoxTexels?
+bool warp_staged_fits(const StageDesc& d, uint32_t tw, uint32_t th);
 
 // Build a fused plan from a High op list.
 zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::vector<zh_op>& ops,
@@ -154,6 +163,7 @@ struct Unit {
   bool batch = false;
   bool warp = false;             // batch of one warp form (k_warp_batch<form>)
   bool spec = false;             // batch of one pointwise signature (k_spec_batch<signature>)
+  bool staged = false;           // warp stages on the LDS-staged kernels
   std::vector<uint32_t> ids;     // stages of the unit (one unless batched)
   uint32_t* d_ids = nullptr;     // device copy of ids (batches)
   zh::BatchDesc* d_batch = nullptr;  // device

@@ -48,7 +48,8 @@ struct JitEntry {
   bool compiled = false;
   std::string error;  // why the compile failed (compiled and no kernels)
   // Kernels held for the process: pointwise {stage, batch}; warp {stage,
-  // stage of separable maps, batch, batch of separable maps}.
+  // stage of separable maps, batch, batch of separable maps} and, for
+  // one-source forms, the LDS-staged {stage, batch} of separable maps.
   std::vector<zh_shader*> kernels;
   double compile_s = 0.0;
 };
@@ -69,7 +70,10 @@ std::string package_dir() {
 }
 
 const char* const kSpecKernels[] = {"zh_jit_stage", "zh_jit_batch"};
-const char* const kWarpKernels[] = {"zh_jit_warp", "zh_jit_warp_sep", "zh_jit_warp_batch", "zh_jit_warp_batch_sep"};
+const char* const kWarpKernels[] = {"zh_jit_warp",       "zh_jit_warp_sep",    "zh_jit_warp_batch",
+                                    "zh_jit_warp_batch_sep", "zh_jit_warp_staged", "zh_jit_warp_batch_staged"};
+
+int kernel_count(const JitEntry& e) { return !e.warp ? 2 : e.src == WSRC_ONE ? 6 : 4; }
 
 std::string source_for(const JitEntry& e) {
   std::string list;
@@ -82,13 +86,17 @@ std::string source_for(const JitEntry& e) {
            k + "zh_jit_batch(const zh::BatchDesc* b) { zh::spec::spec_batch<P>(b); }\n";
   const std::string form = std::to_string(e.src) + "u, " + std::to_string(e.fmt) + "u, " + (e.mat ? "true" : "false") +
                            ", zh::spec::Prog<" + list + ">, ";
-  return "#include \"zh_warp_kernels.h\"\n"
-         "typedef zh::spec::Warp<" + form + "false> V;\n"
-         "typedef zh::spec::Warp<" + form + "true> VS;\n" +
-         k + "zh_jit_warp(const zh::StageDesc* d) { zh::spec::warp_stage<V>(d); }\n" +
-         k + "zh_jit_warp_sep(const zh::StageDesc* d) { zh::spec::warp_stage<VS>(d); }\n" +
-         k + "zh_jit_warp_batch(const zh::BatchDesc* b) { zh::spec::warp_batch<V>(b); }\n" +
-         k + "zh_jit_warp_batch_sep(const zh::BatchDesc* b) { zh::spec::warp_batch<VS>(b); }\n";
+  std::string src = "#include \"zh_warp_kernels.h\"\n"
+                    "typedef zh::spec::Warp<" + form + "false> V;\n"
+                    "typedef zh::spec::Warp<" + form + "true> VS;\n" +
+                    k + "zh_jit_warp(const zh::StageDesc* d) { zh::spec::warp_stage<V>(d); }\n" +
+                    k + "zh_jit_warp_sep(const zh::StageDesc* d) { zh::spec::warp_stage<VS>(d); }\n" +
+                    k + "zh_jit_warp_batch(const zh::BatchDesc* b) { zh::spec::warp_batch<V>(b); }\n" +
+                    k + "zh_jit_warp_batch_sep(const zh::BatchDesc* b) { zh::spec::warp_batch<VS>(b); }\n";
+  if (e.src == WSRC_ONE)
+    src += k + "zh_jit_warp_staged(const zh::StageDesc* d) { zh::spec::warp_staged_stage<VS>(d); }\n" +
+           k + "zh_jit_warp_batch_staged(const zh::BatchDesc* b) { zh::spec::warp_staged_batch<VS>(b); }\n";
+  return src;
 }
 
 // Compile entry e (called with g_mu held, once).
@@ -124,7 +132,7 @@ void compile(JitEntry& e) {
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
   const char* const* names = e.warp ? kWarpKernels : kSpecKernels;
-  const int count = e.warp ? 4 : 2;
+  const int count = kernel_count(e);
   for (int i = 0; i < count; ++i) {
     zh_shader* k = nullptr;
     if (zh_shader_create(code.data(), code.size(), names[i], &k) != ZH_OK) {
@@ -228,13 +236,22 @@ int jit_warp_form(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post
   return find_or_add(e);
 }
 
-zh_status jit_warp_kernels(int id, bool sep, int device, hipFunction_t* stage, hipFunction_t* batch,
+zh_status jit_warp_kernels(int id, bool sep, bool staged, int device, hipFunction_t* stage, hipFunction_t* batch,
                            std::string* why) {
   hipFunction_t f[2] = {nullptr, nullptr};
-  zh_status st = sep ? kernels_of(id, true, {1, 3}, device, f, why) : kernels_of(id, true, {0, 2}, device, f, why);
+  if (staged && !jit_warp_has_staged(id)) return fail(ZH_ERR_INVALID_ARGUMENT, "run-time warp form without staged tiles");
+  zh_status st = staged ? kernels_of(id, true, {4, 5}, device, f, why)
+                 : sep  ? kernels_of(id, true, {1, 3}, device, f, why)
+                        : kernels_of(id, true, {0, 2}, device, f, why);
   *stage = f[0];
   *batch = f[1];
   return st;
+}
+
+bool jit_warp_has_staged(int id) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  const std::vector<JitEntry>& es = entries();
+  return id >= 0 && id < (int)es.size() && es[(size_t)id].warp && es[(size_t)id].src == WSRC_ONE;
 }
 
 double jit_compile_seconds(int id) {

@@ -1047,6 +1047,21 @@ static int warp_variant(const StageDesc& d, uint32_t flags) {
   return sep ? v + 1 : v;
 }
 
+bool warp_staged_fits(const StageDesc& d, uint32_t tw, uint32_t th) {
+  if (d.kind != STAGE_RESAMPLE || d.variant < 0 || !(d.variant & 1) || d.n_layers != 1) return false;
+  const bool has = d.variant >= kJitWarpBase ? jit_warp_has_staged((d.variant - kJitWarpBase) / 2)
+                                             : zh_warp_has_staged(d.variant) != 0;
+  if (!has || tw == 0 || th == 0) return false;
+  // The kernel's box spans floor(u_max) - floor(u_min) + 4 columns with
+  // u_max - u_min = |m0| (tw - 1) up to rounding (rows likewise): at most
+  // floor(|m0| (tw - 1)) + 6.
+  const Layer& L = d.layer[0];
+  const double bw = std::floor(std::fabs(L.inv[0]) * (tw - 1)) + 6.0;
+  const double bh = std::floor(std::fabs(L.inv[4]) * (th - 1)) + 6.0;
+  // Rows are laid out an odd number of texels apart: one more at most.
+  return bw <= kBoxTexels / 4 && bh <= kBoxTexels / 4 && (bw + 1) * bh <= kBoxTexels;
+}
+
 static double visible_fraction(const Mat3& inv, uint32_t sw, uint32_t sh, uint32_t W, uint32_t H) {
   Mat3 fwd;
   if (!mat_inv(inv, &fwd)) return 1.0;
@@ -1599,6 +1614,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       sp.desc.tiles_y = (n.h + TH - 1) / TH;
       sp.desc.lds_texels = (uint32_t)std::min<uint64_t>(max_box, kTileLdsBytes / 16);
       sp.lds_bytes = sp.tile_mode == 0 ? (uint32_t)(sp.desc.lds_texels * 16) : 0;
+      sp.staged = (flags & ZH_PLAN_STAGED) && warp_staged_fits(sp.desc, TW, TH);
     }
     {
       std::ostringstream t2;
@@ -1611,10 +1627,12 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
            << " " << ly.src_w << "x" << ly.src_h << ")[" << describe_code(sp.desc, ly.prog_begin, ly.prog_end) << "]";
       }
       t2 << " post[" << describe_code(sp.desc, sp.desc.post_begin, sp.desc.post_end) << "]";
+      const char* lds = sp.staged ? "_lds" : "";
       if (sp.desc.variant >= kJitWarpBase)
-        t2 << " [kernel jitw#" << (sp.desc.variant - kJitWarpBase) / 2 << ((sp.desc.variant & 1) ? "_sep" : "") << "]";
+        t2 << " [kernel jitw#" << (sp.desc.variant - kJitWarpBase) / 2 << ((sp.desc.variant & 1) ? "_sep" : "") << lds
+           << "]";
       else if (sp.desc.variant >= 0)
-        t2 << " [kernel " << zh_warp_name(sp.desc.variant) << "]";
+        t2 << " [kernel " << zh_warp_name(sp.desc.variant) << lds << "]";
       sp.text = t2.str();
     }
     plan.stages.push_back(sp);

@@ -27,10 +27,10 @@ hipError_t zh_launch_spec(int index, const zh::StageDesc* d_dev, uint32_t width,
                           uint32_t cus, hipStream_t stream);
 hipError_t zh_launch_spec_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_chunks, uint32_t cus,
                                 hipStream_t stream);
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
-                          hipStream_t stream);
-hipError_t zh_launch_warp_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus,
-                                uint32_t per_cu, hipStream_t stream);
+hipError_t zh_launch_warp(int index, int staged, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus,
+                          uint32_t per_cu, hipStream_t stream);
+hipError_t zh_launch_warp_batch(int index, int staged, const zh::BatchDesc* b_dev, uint32_t total_tiles,
+                                uint32_t cus, uint32_t per_cu, hipStream_t stream);
 hipError_t zh_launch_resample(const zh::StageDesc* d_dev, int mode, uint32_t tiles_x, uint32_t tiles_y,
                               uint32_t lds_bytes, hipStream_t stream);
 hipError_t zh_launch_batch(const zh::BatchDesc* b_dev, uint32_t* counter, uint32_t grid, uint32_t lds_bytes,
@@ -417,7 +417,9 @@ static void make_units(zh_program* p) {
     // Partition [i, j): by warp form, or as one group.
     std::vector<std::pair<int, std::vector<uint32_t>>> groups;
     for (size_t k = i; k < j; ++k) {
-      const int key = fam == 1 ? st[k].desc.variant : fam == 3 ? (st[k].spec >= 0 ? st[k].spec : 100000 + st[k].jit) : -1;
+      const int key = fam == 1   ? 2 * st[k].desc.variant + (st[k].staged ? 1 : 0)
+                      : fam == 3 ? (st[k].spec >= 0 ? st[k].spec : 100000 + st[k].jit)
+                                 : -1;
       auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
       if (it == groups.end()) groups.push_back({key, {(uint32_t)k}});
       else it->second.push_back((uint32_t)k);
@@ -428,6 +430,7 @@ static void make_units(zh_program* p) {
       u.batch = fam && u.ids.size() >= 2;
       u.warp = u.batch && fam == 1;
       u.spec = u.batch && fam == 3;
+      u.staged = fam == 1 && st[u.ids[0]].staged;
       p->units.push_back(u);
     }
     i = j;
@@ -546,8 +549,9 @@ static zh_status alloc_device(zh_program* p) {
       hipFunction_t f = nullptr, fb = nullptr;
       const int id = (sp.desc.variant - kJitWarpBase) / 2;
       const bool sep = (sp.desc.variant & 1) != 0;
-      if (p->arena || jit_warp_kernels(id, sep, ctx->device, &f, &fb, &why) != ZH_OK) {
+      if (p->arena || jit_warp_kernels(id, sep, sp.staged, ctx->device, &f, &fb, &why) != ZH_OK) {
         sp.desc.variant = -1;
+        sp.staged = false;
         const size_t k = sp.text.rfind(" [kernel jitw#");
         if (k != std::string::npos)
           sp.text = sp.text.substr(0, k) +
@@ -766,6 +770,18 @@ zh_status zh_program_stats(zh_program* p, uint32_t* launches, uint64_t* algorith
   return ZH_OK;
 }
 
+// A warp stage's kernel tag with or without the "_lds" suffix of the staged
+// kernels.
+static void set_staged_tag(std::string& text, bool staged) {
+  const size_t k = text.rfind(" [kernel ");
+  if (k == std::string::npos) return;
+  const size_t e = text.find(']', k);
+  if (e == std::string::npos) return;
+  const bool has = e >= k + 4 && text.compare(e - 4, 4, "_lds") == 0;
+  if (staged && !has) text.insert(e, "_lds");
+  else if (!staged && has) text.erase(e - 4, 4);
+}
+
 zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uint32_t len) {
   if (!p || (!data && len)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
   bool found = false;
@@ -803,7 +819,10 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
       a.desc.lds_texels = b.desc.lds_texels;
       a.tile_mode = b.tile_mode;
       a.lds_bytes = b.lds_bytes;
+      // The new maps decide whether the box still fits the kept tiles.
+      a.staged = (p->flags & ZH_PLAN_STAGED) && warp_staged_fits(a.desc, a.desc.tile_w, a.desc.tile_h);
       a.text = b.text;
+      set_staged_tag(a.text, a.staged);
     }
   // Same structure (stage kinds, tiles, slots): keep buffers and the graph and
   // refresh the descriptors at the next launch.
@@ -814,7 +833,7 @@ zh_status zh_program_set_knob(zh_program* p, int32_t knob, const void* data, uin
     same = a.desc.kind == b.desc.kind && a.tile_mode == b.tile_mode && a.desc.tiles_x == b.desc.tiles_x &&
            a.desc.tiles_y == b.desc.tiles_y && a.slot_ref == b.slot_ref && a.lds_bytes <= b.lds_bytes &&
            a.spec == b.spec && a.jit == b.jit && a.fast == b.fast && a.desc.flat == b.desc.flat &&
-           a.desc.variant == b.desc.variant;
+           a.desc.variant == b.desc.variant && a.staged == b.staged;
   }
   p->params = params;
   // A surrogate was fitted to the old parameters; a buffer knob rewrites the
@@ -1021,7 +1040,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
         ZH_HIP(hipModuleLaunchKernel(p->jit_batch_fn[first], blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
         return ZH_OK;
       }
-      ZH_HIP(zh_launch_warp_batch(p->plan.stages[u.ids[0]].desc.variant, u.d_batch, u.total_tiles,
+      ZH_HIP(zh_launch_warp_batch(p->plan.stages[u.ids[0]].desc.variant, u.staged, u.d_batch, u.total_tiles,
                                   (uint32_t)p->ctx->cus, u.blocks_per_cu, s));
       return ZH_OK;
     }
@@ -1078,7 +1097,7 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
         void* args[] = {&dp};
         ZH_HIP(hipModuleLaunchKernel(p->jit_stage_fn[first], blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
       } else if (first < p->variant.size() && p->variant[first] == 3) {
-        ZH_HIP(zh_launch_warp(sp.desc.variant, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus,
+        ZH_HIP(zh_launch_warp(sp.desc.variant, sp.staged, d, sp.desc.tiles_x * sp.desc.tiles_y, (uint32_t)p->ctx->cus,
                               u.blocks_per_cu, s));
       } else {
         ZH_HIP(zh_launch_resample(d, sp.tile_mode, sp.desc.tiles_x, sp.desc.tiles_y, sp.lds_bytes, s));
@@ -1423,6 +1442,10 @@ zh_status zh_program_autotune(zh_program* p, const zh_binding* bindings, uint32_
     uint32_t bw = tw0, bh = th0, bb = u.blocks_per_cu;
     static const uint32_t kPerCu[] = {0, 4, 3, 2, 1};
     for (const auto& c : kWarpTiles) {
+      // Staged units keep to the tile shapes whose boxes fit every stage.
+      bool fits = true;
+      for (uint32_t s : u.ids) fits = fits && (!u.staged || warp_staged_fits(p->plan.stages[s].desc, c[0], c[1]));
+      if (!fits) continue;
       st = set_warp_tile(p, u, c[0], c[1]);
       for (uint32_t bpc : kPerCu) {
         if (st != ZH_OK) break;

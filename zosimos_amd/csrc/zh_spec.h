@@ -104,4 +104,6 @@ const char* zh_spec_name(int index);
 // for axis-aligned maps.
 int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* post, int n);
 const char* zh_warp_name(int index);
+// Does warp kernel `index` have an LDS-staged form (separable, one source)?
+int zh_warp_has_staged(int index);
 }

@@ -33,6 +33,9 @@ constexpr uint32_t kSpecChunkGroups = 16;
 constexpr uint64_t kSpecChunkPixels = (uint64_t)kBlock * kPix * kSpecChunkGroups;
 // LDS bytes available to one resample tile (float4 texels).
 constexpr int kTileLdsBytes = 64 * 1024;
+// Staged warp tiles: float4 texels of a tile's source box in LDS (25.6 KB;
+// a 64 x 16 tile under a magnifying map needs at most 69 x 21).
+constexpr int kBoxTexels = 1600;
 
 enum Opcode : uint8_t {
   OP_NOP = 0,

@@ -23,6 +23,28 @@ __global__ __launch_bounds__(kBlock) void k_warp_batch(const BatchDesc* bb) {
   warp_batch<V>(bb);
 }
 
+// LDS-staged tiles of one-source separable forms (staged_tile).
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_warp_staged(const StageDesc* dd) {
+  warp_staged_stage<V>(dd);
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_warp_staged_batch(const BatchDesc* bb) {
+  warp_staged_batch<V>(bb);
+}
+
+template <class V>
+constexpr void (*staged_fn())(const StageDesc*) {
+  if constexpr (V::SEP && V::SRC == WSRC_ONE) return k_warp_staged<V>;
+  else return nullptr;
+}
+template <class V>
+constexpr void (*staged_batch_fn())(const BatchDesc*) {
+  if constexpr (V::SEP && V::SRC == WSRC_ONE) return k_warp_staged_batch<V>;
+  else return nullptr;
+}
+
 // ---- registry ------------------------------------------------------------------
 #define ZH_WARP_TYPE(name, src, f, mat, ...)                         \
   typedef Warp<src, f, mat != 0, Prog<__VA_ARGS__>, false> V_##name;    \
@@ -46,13 +68,15 @@ struct WarpEntry {
   int n;
   void (*fn)(const StageDesc*);
   void (*batch)(const BatchDesc*);
+  void (*staged)(const StageDesc*);  // separable one-source forms: LDS-staged tiles
+  void (*staged_batch)(const BatchDesc*);
 };
 
 #define ZH_WARP_ENTRY(name, src, f, mat, ...)                                                          \
   {#name, src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<V_##name>,         \
-   k_warp_batch<V_##name>},                                                                              \
+   k_warp_batch<V_##name>, nullptr, nullptr},                                                            \
   {#name "_sep", src, f, mat, WP_##name, (int)(sizeof(WP_##name) / sizeof(uint16_t)), k_warp<VS_##name>, \
-   k_warp_batch<VS_##name>},
+   k_warp_batch<VS_##name>, staged_fn<VS_##name>(), staged_batch_fn<VS_##name>()},
 static const WarpEntry kWarps[] = {ZH_WARP_LIST(ZH_WARP_ENTRY)};
 #undef ZH_WARP_ENTRY
 
@@ -75,29 +99,42 @@ const char* zh_warp_name(int index) {
   return index >= 0 && index < zh::spec::kNumWarps ? zh::spec::kWarps[index].name : "";
 }
 
+int zh_warp_has_staged(int index) {
+  return index >= 0 && index < zh::spec::kNumWarps && zh::spec::kWarps[index].staged != nullptr;
+}
+
 // per_cu: workgroups per compute unit of the persistent grid (0: as many as
 // are resident; fewer keeps more of each workgroup's taps in the L1).
-hipError_t zh_launch_warp(int index, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus, uint32_t per_cu,
-                          hipStream_t stream) {
+// staged: the LDS-staged kernel of a form that has one.
+hipError_t zh_launch_warp(int index, int staged, const zh::StageDesc* d_dev, uint32_t tiles, uint32_t cus,
+                          uint32_t per_cu, hipStream_t stream) {
   if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
-  static uint32_t resident[zh::spec::kNumWarps] = {};
-  if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kWarps[index].fn, 1);
-  const uint32_t bpc = per_cu ? std::min(per_cu, resident[index]) : resident[index];
+  const auto& e = zh::spec::kWarps[index];
+  if (staged && !e.staged) return hipErrorInvalidValue;
+  void (*fn)(const zh::StageDesc*) = staged ? e.staged : e.fn;
+  static uint32_t resident[2][zh::spec::kNumWarps] = {};
+  uint32_t& res = resident[staged ? 1 : 0][index];
+  if (!res) res = zh::resident_blocks((const void*)fn, 1);
+  const uint32_t bpc = per_cu ? std::min(per_cu, res) : res;
   const uint32_t blocks = std::min(tiles, bpc * cus);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(zh::spec::kWarps[index].fn, dim3(blocks), dim3(zh::kBlock), 0, stream, d_dev);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(zh::kBlock), 0, stream, d_dev);
   return hipGetLastError();
 }
 
-hipError_t zh_launch_warp_batch(int index, const zh::BatchDesc* b_dev, uint32_t total_tiles, uint32_t cus,
-                                uint32_t per_cu, hipStream_t stream) {
+hipError_t zh_launch_warp_batch(int index, int staged, const zh::BatchDesc* b_dev, uint32_t total_tiles,
+                                uint32_t cus, uint32_t per_cu, hipStream_t stream) {
   if (index < 0 || index >= zh::spec::kNumWarps) return hipErrorInvalidValue;
-  static uint32_t resident[zh::spec::kNumWarps] = {};
-  if (!resident[index]) resident[index] = zh::resident_blocks((const void*)zh::spec::kWarps[index].batch, 1);
-  const uint32_t bpc = per_cu ? std::min(per_cu, resident[index]) : resident[index];
+  const auto& e = zh::spec::kWarps[index];
+  if (staged && !e.staged_batch) return hipErrorInvalidValue;
+  void (*fn)(const zh::BatchDesc*) = staged ? e.staged_batch : e.batch;
+  static uint32_t resident[2][zh::spec::kNumWarps] = {};
+  uint32_t& res = resident[staged ? 1 : 0][index];
+  if (!res) res = zh::resident_blocks((const void*)fn, 1);
+  const uint32_t bpc = per_cu ? std::min(per_cu, res) : res;
   const uint32_t blocks = std::min(total_tiles, bpc * cus);
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(zh::spec::kWarps[index].batch, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(zh::kBlock), 0, stream, b_dev);
   return hipGetLastError();
 }
 

@@ -433,6 +433,322 @@ constexpr bool needs_lut() {
   return false;
 }
 
+// ---- staged tiles ------------------------------------------------------------
+// A one-layer stage with a separable map (WSRC_ONE) whose per-tile source box
+// fits kBoxTexels -- magnifying maps and mild minification, the planner's
+// test -- reads each source texel for up to 4 / scale^2 taps.  Staged, a
+// tile's box (the taps of its first and last pixel centres along each axis,
+// the map being monotonic, widened by one texel per side) is decoded and
+// premultiplied once into LDS, texels outside the source stored as 0 (the
+// transparent border), and every pixel interpolates its four taps from
+// there: the decode, the tap addressing and the border tests leave the
+// per-pixel path.  The lanes' maps are the direct kernel's (32.32 fixed
+// point), so both read the same taps.
+//
+// Each workgroup pipelines its tiles: the raw texels of its next tile are
+// fetched by buffer loads that write LDS directly (no registers held), one
+// dword per texel word, while the current tile interpolates; then, behind a
+// wait and a barrier, they are decoded into the box.
+template <uint32_t F> constexpr int raw_words() { return (int)(fmt_bpp(F) / 4); }
+constexpr int kBoxLoads = (kBoxTexels + kBlock - 1) / kBlock;
+static_assert(kBoxTexels % 64 == 0, "box loads write whole 64-lane chunks");
+
+// Uniform state of one staged tile (kept in scalar registers while the next
+// tile's texels are in flight).
+struct BoxTile {
+  CDesc d;
+  uint32_t tx, ty;
+  int hit;  // the layer meets the tile (else the tile is the base colour)
+  int i0, j0, bw, bh, nb;
+  int pitch;  // texels per box row in LDS: bw rounded up to odd, so rows start on different banks
+};
+// Words of the raw texels are planes kRawPlane dwords apart (one dword past
+// the box size, so a texel's words sit in different banks).
+constexpr int kRawPlane = kBoxTexels + 1;
+
+// Every field of a tile's state is equal across the workgroup; moved to
+// scalar registers explicitly, it stays there through the loop that carries
+// it from one tile to the next (left to the compiler, the descriptor pointer
+// lands in vector registers and every descriptor read becomes a vector load).
+__device__ __forceinline__ BoxTile uniform_tile(BoxTile s) {
+  const uint64_t p = (uint64_t)(uintptr_t)s.d;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)p), hi = __builtin_amdgcn_readfirstlane((uint32_t)(p >> 32));
+  s.d = (CDesc)(uintptr_t)(((uint64_t)hi << 32) | lo);
+  s.tx = __builtin_amdgcn_readfirstlane(s.tx);
+  s.ty = __builtin_amdgcn_readfirstlane(s.ty);
+  s.hit = __builtin_amdgcn_readfirstlane(s.hit);
+  s.i0 = __builtin_amdgcn_readfirstlane(s.i0);
+  s.j0 = __builtin_amdgcn_readfirstlane(s.j0);
+  s.bw = __builtin_amdgcn_readfirstlane(s.bw);
+  s.bh = __builtin_amdgcn_readfirstlane(s.bh);
+  s.nb = __builtin_amdgcn_readfirstlane(s.nb);
+  s.pitch = __builtin_amdgcn_readfirstlane(s.pitch);
+  return s;
+}
+
+// Box texel e = r bw + c: r = floor((e + 0.5) / bw) through a 1-ulp
+// reciprocal.  For e < 2^11 and 4 <= bw <= 400 the quotient lies at least
+// 0.5 / bw = 1.25e-3 from an integer and errs by under 2^-21 e / bw.
+__device__ __forceinline__ int box_row(int e, float rbw) { return (int)(((float)e + 0.5f) * rbw); }
+
+__device__ __forceinline__ BoxTile box_tile(CDesc d, uint32_t tx, uint32_t ty) {
+  BoxTile s;
+  s.d = d;
+  s.tx = tx;
+  s.ty = ty;
+  s.i0 = s.j0 = 0;
+  s.bw = s.bh = 4;
+  s.nb = 0;
+  s.pitch = 5;
+  const auto& Ly = d->layer[0];  // constant address space: fields through ldc
+  const int TW = (int)d->tile_w, TH = (int)d->tile_h;
+  const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
+  const uint32_t* tl = d->tile_layers;
+  const uint32_t t = ty * d->tiles_x + tx;
+  if (tl) {
+    s.hit = (cword(tl, t >> 2) >> ((t & 3u) * 8u)) & 1u;
+  } else {
+    const int W = (int)d->width, H = (int)d->height;
+    const double ex1 = min((int)tx * TW + TW, W) - 0.5, ey1 = min((int)ty * TH + TH, H) - 0.5;
+    s.hit = ldc(&Ly.covers_all) || layer_hits(ldc(&Ly), ex0, ey0, ex1, ey1);
+  }
+  if (!s.hit) return s;
+  const double m0 = ldc(&Ly.inv[0]), m1 = ldc(&Ly.inv[1]), m2 = ldc(&Ly.inv[2]);
+  const double m3 = ldc(&Ly.inv[3]), m4 = ldc(&Ly.inv[4]), m5 = ldc(&Ly.inv[5]);
+  const double uo = m0 * ex0 + m1 * ey0 + m2, vo = m3 * ex0 + m4 * ey0 + m5;
+  // The box over the whole tile (pixels past the image edge included, so
+  // every lane's taps lie inside it).
+  const double ua = uo - 0.5, ub = fma(m0, (double)(TW - 1), uo) - 0.5;
+  const double va = vo - 0.5, vb = fma(m4, (double)(TH - 1), vo) - 0.5;
+  s.i0 = (int)floor(fmin(ua, ub)) - 1;
+  s.j0 = (int)floor(fmin(va, vb)) - 1;
+  s.bw = (int)floor(fmax(ua, ub)) + 3 - s.i0;
+  s.bh = (int)floor(fmax(va, vb)) + 3 - s.j0;
+  s.pitch = s.bw | 1;
+  if (s.bw > kBoxTexels / 4 || s.bh > kBoxTexels / 4 || s.pitch * s.bh > kBoxTexels) {
+    // The planner's bound excludes it; the tile is left as the base colour.
+    if (threadIdx.x == 0) *d->error_flag = 1u;
+    s.hit = 0;
+    s.bw = s.bh = 4;
+    s.pitch = 5;
+    return s;
+  }
+  s.nb = s.bw * s.bh;
+  return s;
+}
+
+// Issue the loads of tile s's box texels into `raw` (word w of texel e at
+// raw[w * kRawPlane + e]).  Lane l of a wave writes the dword at the wave's
+// LDS base + 4 l, so every 64-lane chunk of texels is one load per word.
+// Texels outside the source load a clamped, valid texel (decoded as 0).
+template <class V>
+__device__ __forceinline__ void box_issue(const BoxTile& s, uint32_t* raw) {
+  constexpr uint32_t B = fmt_bpp(V::F);
+  constexpr int NW = raw_words<V::F>();
+  if (!s.hit) return;
+  const LayerArgs a = layer_args<V::SRC, V::F, false>(s.d, ldc(&s.d->layer[0]));
+  const int tid = (int)threadIdx.x, wbase = tid & ~63;
+  const float rbw = __builtin_amdgcn_rcpf((float)s.bw);
+#pragma unroll
+  for (int q = 0; q < kBoxLoads; ++q) {
+    const int cb = q * kBlock + wbase;
+    if (cb < s.nb) {  // uniform in the wave
+      const int e = q * kBlock + tid;
+      const int r = box_row(e, rbw), c = e - r * s.bw;
+      const int si = min(max(s.i0 + c, 0), a.sw - 1), sj = min(max(s.j0 + r, 0), a.sh - 1);
+      const uint32_t off = __umul24((uint32_t)sj, a.stride) + (uint32_t)si * B;
+#pragma unroll
+      for (int w = 0; w < NW; ++w)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            a.src0, (__attribute__((address_space(3))) void*)(raw + w * kRawPlane + cb), 4, (int)(off + 4 * w), 0, 0,
+            0);
+    }
+  }
+}
+
+// Decode tile s's raw texels into the box: premultiplied float4, 0 outside
+// the source.
+template <class V>
+__device__ __forceinline__ void box_decode(const BoxTile& s, const uint32_t* raw, float4* box, const Luts* L) {
+  typedef typename RawT<V::F>::type R;
+  constexpr int NW = raw_words<V::F>();
+  if (!s.hit) return;
+  const int sw = (int)ldc(&s.d->layer[0].src_w), sh = (int)ldc(&s.d->layer[0].src_h);
+  const int tid = (int)threadIdx.x;
+  const float rbw = __builtin_amdgcn_rcpf((float)s.bw);
+#pragma unroll
+  for (int q = 0; q < kBoxLoads; ++q) {
+    const int e = q * kBlock + tid;
+    if (q * kBlock < s.nb && e < s.nb) {
+      const int r = box_row(e, rbw), c = e - r * s.bw;
+      const bool in = (unsigned)(s.i0 + c) < (unsigned)sw && (unsigned)(s.j0 + r) < (unsigned)sh;
+      uint32_t wv[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) wv[w] = raw[w * kRawPlane + e];
+      R rv;
+      if constexpr (NW == 1) rv = wv[0];
+      else if constexpr (NW == 2) { rv.x = wv[0]; rv.y = wv[1]; }
+      else { rv.x = wv[0]; rv.y = wv[1]; rv.z = wv[2]; rv.w = wv[3]; }
+      const float4 t = raw_dec<V::F>(rv, L);
+      const float al = in ? t.w : 0.0f;
+      box[r * s.pitch + c] = make_float4(t.x * al, t.y * al, t.z * al, al);
+    }
+  }
+}
+
+// The pixels of tile s: four taps from the box, the source matrix, 'over'
+// the base, then the post program (which stores them).
+template <class V>
+__device__ __forceinline__ void box_pixels(const BoxTile& s, const float4* box, const Luts* L) {
+  typedef typename V::Post Post;
+  const CDesc d = s.d;
+  const int TW = (int)d->tile_w, TH = (int)d->tile_h;
+  const int lanes_x = TW / kPix;
+  const int lsh = __builtin_ctz((unsigned)lanes_x);
+  const int tid = (int)threadIdx.x;
+  const int lx = tid & (lanes_x - 1), ly = tid >> lsh;
+  const int W = (int)d->width, H = (int)d->height;
+  const int x0 = (int)s.tx * TW + lx, y = (int)s.ty * TH + ly;
+  const int n = (y < H && x0 < W) ? min(kPix, (W - x0 + lanes_x - 1) >> lsh) : 0;
+
+  float4 base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (d->base_end > d->base_begin) {
+    const uint32_t q = d->code[d->base_begin].arg;
+    base = premul4(make_float4(d->params[q], d->params[q + 1], d->params[q + 2], d->params[q + 3]));
+  }
+  Px X;
+#pragma unroll
+  for (int k = 0; k < kPix; ++k) X[k] = base;
+  if (s.hit) {
+    const Layer Ly = ldc(&d->layer[0]);
+    float m[9];
+    if constexpr (V::MAT) {
+      const uint32_t q = d->code[Ly.prog_begin + 1].arg;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) m[k] = d->params[q + k];
+    }
+    const double ex0 = (double)s.tx * TW + 0.5, ey0 = (double)s.ty * TH + 0.5;
+    const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
+    const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
+    const int64_t u0 = fixed32(fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5);
+    const int64_t v0 = fixed32(fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5);
+    const int64_t su = uniform64(fixed32(Ly.inv[0] * (double)lanes_x));
+    const float fy = fx_frac(v0), gy = 1.0f - fy;
+    const int bw = s.bw, pitch = s.pitch;
+    const float4* row = box + min(max(fx_int(v0) - s.j0, 0), s.bh - 2) * pitch;
+#pragma unroll
+    for (int k = 0; k < kPix; ++k) {
+      const int64_t uk = u0 + su * k;
+      const float fx = fx_frac(uk), gx = 1.0f - fx;
+      const float4* p = row + min(max(fx_int(uk) - s.i0, 0), bw - 2);
+      const float4 t00 = p[0], t01 = p[1], t10 = p[pitch], t11 = p[pitch + 1];
+      const float w00 = gx * gy, w01 = fx * gy, w10 = gx * fy, w11 = fx * fy;
+      float4 c;
+      c.x = fmaf(t11.x, w11, fmaf(t10.x, w10, fmaf(t01.x, w01, t00.x * w00)));
+      c.y = fmaf(t11.y, w11, fmaf(t10.y, w10, fmaf(t01.y, w01, t00.y * w00)));
+      c.z = fmaf(t11.z, w11, fmaf(t10.z, w10, fmaf(t01.z, w01, t00.z * w00)));
+      c.w = fmaf(t11.w, w11, fmaf(t10.w, w10, fmaf(t01.w, w01, t00.w * w00)));
+      if constexpr (V::MAT) {
+        const float r = m[0] * c.x + m[1] * c.y + m[2] * c.z;
+        const float g = m[3] * c.x + m[4] * c.y + m[5] * c.z;
+        const float b = m[6] * c.x + m[7] * c.y + m[8] * c.z;
+        c = make_float4(r, g, b, c.w);
+      }
+      const float kk = 1.0f - c.w;
+      X[k] = make_float4(fmaf(X[k].x, kk, c.x), fmaf(X[k].y, kk, c.y), fmaf(X[k].z, kk, c.z), fmaf(X[k].w, kk, c.w));
+    }
+  }
+  if (n == 0) return;
+  Uni<Post> u;
+  resolve<Post>(d, u, d->post_begin);
+  const uint32_t so = instr_slot(d, d->post_end - 1);
+  const uint64_t stride_tx = d->slot[so].row_stride / fmt_bpp(el_fmt(Post::el[Post::n - 1]));
+  const uint64_t xi = (uint64_t)y * stride_tx + (uint64_t)x0;
+  Px Y, Z;
+  exec<Post, false>(u, xi, n, nullptr, X, Y, Z, L, lanes_x);
+}
+
+// The pipelined loop over a workgroup's tiles; next(j, &s) gives the tile of
+// queue position j.  Per tile: wait for its texels and for every lane to
+// leave the previous box, decode, barrier, issue the next tile's loads,
+// interpolate.
+template <class V, class Next>
+__device__ __forceinline__ void box_loop(uint32_t j, uint32_t total, Next next, uint32_t* raw, float4* box,
+                                         const Luts* L) {
+  BoxTile cur = next(j);
+  box_issue<V>(cur, raw);
+  while (true) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    box_decode<V>(cur, raw, box, L);
+    __syncthreads();
+    const uint32_t jn = j + gridDim.x;
+    const bool more = jn < total;
+    BoxTile nxt = cur;
+    if (more) {
+      nxt = next(jn);
+      box_issue<V>(nxt, raw);
+    }
+    box_pixels<V>(cur, box, L);
+    if (!more) break;
+    cur = nxt;
+    j = jn;
+  }
+}
+
+template <class V>
+__device__ __forceinline__ void warp_staged_stage(const StageDesc* dd) {
+  static_assert(V::SEP && V::SRC == WSRC_ONE, "staged tiles serve one-source separable forms");
+  const CDesc d = as_const(dd);
+  __shared__ Luts luts;
+  __shared__ float4 box[kBoxTexels];
+  __shared__ uint32_t raw[raw_words<V::F>() * kRawPlane];
+  if constexpr (needs_lut<V>()) {
+    load_lut(d, &luts);
+    __syncthreads();
+  }
+  const uint32_t tiles = d->tiles_x * d->tiles_y;
+  const uint32_t j0 = virtual_block();
+  if (j0 >= tiles) return;
+  box_loop<V>(j0, tiles,
+              [&](uint32_t t) {
+                uint32_t tx, ty;
+                tile_xy(d, t, tx, ty);
+                return uniform_tile(box_tile(d, tx, ty));
+              },
+              raw, box, &luts);
+}
+
+template <class V>
+__device__ __forceinline__ void warp_staged_batch(const BatchDesc* bb) {
+  static_assert(V::SEP && V::SRC == WSRC_ONE, "staged tiles serve one-source separable forms");
+  const CBatch b = (CBatch)(uintptr_t)bb;
+  __shared__ Luts luts;
+  __shared__ float4 box[kBoxTexels];
+  __shared__ uint32_t raw[raw_words<V::F>() * kRawPlane];
+  if constexpr (needs_lut<V>()) {
+    load_lut(as_const(b->stages), &luts);
+    __syncthreads();
+  }
+  const uint32_t total = b->total_tiles;
+  const uint32_t j0 = virtual_block();
+  if (j0 >= total) return;
+  uint32_t lo = 0, hi = b->n_stages;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (cword(b->tile_prefix, mid) <= j0) lo = mid; else hi = mid;
+  }
+  box_loop<V>(j0, total,
+              [&](uint32_t j) {
+                while (cword(b->tile_prefix, lo + 1) <= j) ++lo;  // uniform; tiles only move forward
+                const CDesc d = as_const(b->stages + cword(b->stage_ids, lo));
+                uint32_t tx, ty;
+                tile_xy(d, j - cword(b->tile_prefix, lo), tx, ty);
+                return uniform_tile(box_tile(d, tx, ty));
+              },
+              raw, box, &luts);
+}
+
 template <class V>
 __device__ __forceinline__ void warp_stage(const StageDesc* dd) {
   const CDesc d = as_const(dd);
