@@ -994,6 +994,11 @@ static int warp_variant(const StageDesc& d, uint32_t flags) {
     const Layer& L = d.layer[l];
     if (L.mode != LAYER_BILINEAR_OVER) return -1;
     if (L.inv[6] != 0.0 || L.inv[7] != 0.0 || std::fabs(L.inv[8] - 1.0) > 1e-12) return -1;
+    // The lane maps hold the steps in 8.56 fixed point and coordinates in
+    // 32.32 (Layer::fx_step, fx_org).
+    if (std::fabs(L.inv[0]) >= 127.0 || std::fabs(L.inv[1]) >= 127.0 || std::fabs(L.inv[3]) >= 127.0 ||
+        std::fabs(L.inv[4]) >= 127.0 || std::fabs(L.inv[2]) >= 1e9 || std::fabs(L.inv[5]) >= 1e9)
+      return -1;
     const Instr* c = d.code + L.prog_begin;
     const uint32_t n = L.prog_end - L.prog_begin;
     uint32_t s_, f_, m_ = 0;
@@ -1060,6 +1065,28 @@ bool warp_staged_fits(const StageDesc& d, uint32_t tw, uint32_t th) {
   const double bh = std::floor(std::fabs(L.inv[4]) * (th - 1)) + 6.0;
   // Rows are laid out an odd number of texels apart: one more at most.
   return bw <= kBoxTexels / 4 && bh <= kBoxTexels / 4 && (bw + 1) * bh <= kBoxTexels;
+}
+
+// x in 32.32 fixed point (|x| < 2^31; the fraction truncated to 2^-32), as
+// the warp kernels hold mapped coordinates.
+static int64_t fixed32(double x) {
+  const double f = std::floor(x);
+  const uint32_t lo = (uint32_t)((x - f) * 4294967296.0);
+  return (int64_t)(((uint64_t)(uint32_t)(int64_t)f << 32) | lo);
+}
+
+// The fixed-point form of a layer's affine map (Layer::fx_step, fx_org);
+// steps of 127 or more in magnitude are left 0 (warp_variant admits none).
+static void fixed_map(Layer& ly) {
+  const int idx[4] = {0, 1, 3, 4};
+  for (int k = 0; k < 4; ++k) {
+    const double m = ly.inv[idx[k]];
+    ly.fx_step[k] = std::fabs(m) < 127.0 ? (int64_t)std::llround(std::ldexp(m, 56)) : 0;
+  }
+  const double u = ly.inv[0] * 0.5 + ly.inv[1] * 0.5 + ly.inv[2] - 0.5;
+  const double v = ly.inv[3] * 0.5 + ly.inv[4] * 0.5 + ly.inv[5] - 0.5;
+  ly.fx_org[0] = std::fabs(u) < 2147483648.0 ? fixed32(u) : 0;
+  ly.fx_org[1] = std::fabs(v) < 2147483648.0 ? fixed32(v) : 0;
 }
 
 static double visible_fraction(const Mat3& inv, uint32_t sw, uint32_t sh, uint32_t W, uint32_t H) {
@@ -1541,6 +1568,7 @@ zh_status plan_program(const std::vector<zh_descriptor>& textures, const std::ve
       ly.src_h = L.src_h;
       for (int k = 0; k < 9; ++k) ly.inv[k] = L.inv[k];
       for (int k = 0; k < 4; ++k) ly.cover[k] = L.cover[k];
+      fixed_map(ly);
       // An affine map takes the output's pixel-centre hull to a
       // parallelogram; when its corners (minus the half texel) lie inside
       // (-1, w) x (-1, h), so does every tile's footprint, and the kernels'

@@ -131,6 +131,11 @@ struct Layer {
   double inv[9];
   // Nearest modes: coverage rectangle in source texel space [x0, x1) x [y0, y1).
   double cover[4];
+  // Affine maps of the warp kernels, for their integer lane maps: the steps
+  // m0, m1, m3, m4 in 8.56 fixed point (|m| < 127), and (u, v) of the pixel
+  // centre (0.5, 0.5) minus the half texel in 32.32 fixed point.
+  int64_t fx_step[4];
+  int64_t fx_org[2];
 };
 
 enum StageKind : uint32_t { STAGE_POINTWISE = 0, STAGE_RESAMPLE = 1, STAGE_COPY = 2, STAGE_DYNAMIC = 3 };

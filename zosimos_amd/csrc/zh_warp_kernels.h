@@ -9,12 +9,13 @@
 //   * the output is cut into tiles of tile_w x tile_h pixels (one workgroup
 //     of 256 lanes, kPix = 4 horizontally adjacent pixels per lane); a
 //     persistent grid walks the tiles with a stride;
-//   * per tile and layer, the tile's footprint is mapped in FP64 (four
-//     corners) and the layer is skipped when it misses the source; each
-//     lane's first pixel centre is mapped in FP64 and held, like the step
-//     between the lane's pixels, in 32.32 fixed point, so every pixel's map
-//     is one 64-bit add: the integer texel is the high word and the bilinear
-//     fraction the low word (2^-32 px resolution at any image size; FP32
+//   * per tile and layer, the layer is skipped when it misses the tile (the
+//     host's per-tile mask, or the tile's footprint mapped in FP64); each
+//     lane's first pixel centre and the step between the lane's pixels are
+//     held in 32.32 fixed point, computed with integers from the layer's map
+//     in fixed point (the tile's part uniform), so every pixel's map is one
+//     64-bit add: the integer texel is the high word and the bilinear
+//     fraction the low word (about 1e-9 px error at any image size; FP32
 //     steps would err by ~1e-5 px at 8K, which un-premultiplying a small
 //     alpha turns into several f16 ulps);
 //   * the four taps are read straight from memory (the L1/L2 caches serve
@@ -134,6 +135,16 @@ __device__ __forceinline__ bool layer_hits(const Layer& L, double ex0, double ey
   return umax > -1.0 && vmax > -1.0 && umin < (double)L.src_w && vmin < (double)L.src_h;
 }
 
+// layer_hits over the pixel-centre extent of tile (tx, ty) (clipped to the
+// image).  The fallback for stages without the host's per-tile masks; kept
+// out of line so that its FP64 set-up is not hoisted onto every tile's path.
+__device__ __attribute__((noinline)) bool tile_hits(CDesc d, uint32_t l, uint32_t tx, uint32_t ty) {
+  const int TW = (int)d->tile_w, TH = (int)d->tile_h, W = (int)d->width, H = (int)d->height;
+  const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
+  const double ex1 = min((int)tx * TW + TW, W) - 0.5, ey1 = min((int)ty * TH + TH, H) - 0.5;
+  return layer_hits(ldc(&d->layer[l]), ex0, ey0, ex1, ey1);
+}
+
 // Warp variant: source form, source format, mat3 flag, post program, and
 // whether every layer's map is axis-aligned (separable taps).
 template <uint32_t SRC_, uint32_t F_, bool MAT_, class Post_, bool SEP_>
@@ -150,12 +161,6 @@ struct LaneMap {
   int64_t u0, v0, su, sv;
 };
 
-// x in 32.32 fixed point (|x| < 2^31; the fraction truncated to 2^-32).
-__device__ __forceinline__ int64_t fixed32(double x) {
-  const double f = floor(x);
-  const uint32_t lo = (uint32_t)((x - f) * 4294967296.0);
-  return (int64_t)(((uint64_t)(uint32_t)(int)f << 32) | lo);
-}
 // Integer part and fraction of a 32.32 value.  The fraction rounds to the
 // nearest float and can reach 1.0 from just below it: the weights are
 // continuous there, so the taps' blend is the same.
@@ -167,6 +172,34 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ float fx_frac(int64_t v) { return (float)(uint32_t)v * 0x1p-32f; }
+
+// x a for a step a in 8.56 fixed point (|a| < 2^7) and 0 <= x < 2^24, in
+// 32.32 fixed point (the 24 lowest fraction bits dropped, rounding toward
+// -inf): the 88-bit product assembled from two 32 x 32 products.
+__device__ __forceinline__ int64_t mul56(uint32_t x, int64_t a) {
+  const uint64_t lo = (uint64_t)x * (uint32_t)a;         // < 2^56
+  const int64_t hi = (int64_t)x * (int32_t)(a >> 32);    // |.| < 2^55
+  return hi * 256 + (int64_t)(lo >> 24);
+}
+
+// The lane map of one layer for the tile whose first pixel is (X0, Y0) and
+// the lane's first pixel (X0 + lx, Y0 + ly): integer arithmetic on the
+// layer's fixed-point map (Layer::fx_step, fx_org), the tile's part equal in
+// every lane.  Each term errs by under 2^-32 texel, the map's steps by
+// under 2^-57 per pixel: about 1e-9 texel at any image size.
+template <bool SEP>
+__device__ __forceinline__ LaneMap lane_map(const Layer& Ly, uint32_t X0, uint32_t Y0, int lx, int ly,
+                                            int lanes_x) {
+  const int64_t ut = Ly.fx_org[0] + mul56(X0, Ly.fx_step[0]) + mul56(Y0, Ly.fx_step[1]);
+  const int64_t vt = Ly.fx_org[1] + mul56(X0, Ly.fx_step[2]) + mul56(Y0, Ly.fx_step[3]);
+  LaneMap mp;
+  // Separable maps have no cross terms (m1 = m3 = 0).
+  mp.u0 = ut + mul56((uint32_t)lx, Ly.fx_step[0]) + (SEP ? 0 : mul56((uint32_t)ly, Ly.fx_step[1]));
+  mp.v0 = vt + (SEP ? 0 : mul56((uint32_t)lx, Ly.fx_step[2])) + mul56((uint32_t)ly, Ly.fx_step[3]);
+  mp.su = uniform64(mul56((uint32_t)lanes_x, Ly.fx_step[0]));
+  mp.sv = uniform64(mul56((uint32_t)lanes_x, Ly.fx_step[2]));
+  return mp;
+}
 
 // Composite one layer's bilinear samples over the lane's kPix pixels.
 // Taps of all pixels are addressed first (weight 0 for taps outside the
@@ -200,6 +233,10 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
   // the sample's own taps, 0 for a texel the clamp substituted at a border.
   constexpr bool PAIR = B <= 8;
   typedef typename PairT<B <= 8 ? B : 8>::type PR;
+  // One-source 8-bit formats weigh each tap by its alpha byte: the 1/255 of
+  // the byte's value goes into the row weights instead of every tap.
+  constexpr bool A8 = B == 4 && V::SRC == WSRC_ONE;
+  constexpr float kA = A8 ? 1.0f / 255.0f : 1.0f;
   uint32_t off[kPix][4];
   float w[kPix][4];
   bool any = false;
@@ -212,8 +249,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
     const float ay = fx_frac(mp.v0);
     yas = (unsigned)j < (unsigned)a.sh;
     ybs = (unsigned)(j + 1) < (unsigned)a.sh;
-    wy0s = yas ? 1.0f - ay : 0.0f;
-    wy1s = ybs ? ay : 0.0f;
+    wy0s = yas ? (1.0f - ay) * kA : 0.0f;
+    wy1s = ybs ? ay * kA : 0.0f;
     r0s = (uint32_t)__mul24(j, (int)a.stride);
     r1s = r0s + a.stride;
   }
@@ -240,8 +277,8 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
       const float ay = fx_frac(vk);
       ya = (unsigned)j < (unsigned)a.sh;
       yb = (unsigned)(j + 1) < (unsigned)a.sh;
-      wy0 = ya ? 1.0f - ay : 0.0f;
-      wy1 = yb ? ay : 0.0f;
+      wy0 = ya ? (1.0f - ay) * kA : 0.0f;
+      wy1 = yb ? ay * kA : 0.0f;
       r0 = (uint32_t)__mul24(j, (int)a.stride);
       r1 = r0 + a.stride;
     }
@@ -308,7 +345,9 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
         const float4 m = make_float4(fmaf(lo.x, q, h.x), fmaf(lo.y, q, h.y), fmaf(lo.z, q, h.z), fmaf(lo.w, q, h.w));
         s = make_float4(fmaf(m.x, wt, s.x), fmaf(m.y, wt, s.y), fmaf(m.z, wt, s.z), fmaf(m.w, wt, s.w));
       } else {
-        const float wa = w[k][t] * c.w;
+        float al = c.w;
+        if constexpr (A8) al = (float)((PAIR ? pair_half<V::F>(p0[k][t >> 1], t & 1) : t0[k][t]) >> 24);
+        const float wa = w[k][t] * al;
         s = make_float4(fmaf(c.x, wa, s.x), fmaf(c.y, wa, s.y), fmaf(c.z, wa, s.z), s.w + wa);
       }
     }
@@ -352,9 +391,6 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
 #pragma unroll
   for (int k = 0; k < kPix; ++k) X[k] = base;
 
-  // Pixel-centre extent of the tile (clipped to the image).
-  const double ex0 = (double)tx * TW + 0.5, ey0 = (double)ty * TH + 0.5;
-  const double ex1 = min((int)tx * TW + TW, W) - 0.5, ey1 = min((int)ty * TH + TH, H) - 0.5;
   const uint32_t nl = d->n_layers;
   // Layers meeting this tile: the host's per-tile mask when there is one.
   const uint32_t* tl = d->tile_layers;
@@ -363,20 +399,13 @@ __device__ __forceinline__ void warp_tile(CDesc d, uint32_t tx, uint32_t ty, con
   for (uint32_t l = 0; l < nl; ++l) {
     if (!((lmask >> l) & 1u)) continue;  // uniform
     const Layer Ly = ldc(&d->layer[l]);
-    if (!tl && !Ly.covers_all && !layer_hits(Ly, ex0, ey0, ex1, ey1)) continue;
+    if (!tl && !Ly.covers_all && !tile_hits(d, l, tx, ty)) continue;
     const LayerArgs a = layer_args<V::SRC, V::F, V::MAT>(d, Ly);
-    // The tile's first pixel centre in FP64, then each lane's first pixel
-    // and the step between a lane's pixels in fixed point.
-    const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
-    const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
-    LaneMap mp;
-    mp.u0 = fixed32(fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5);
-    mp.v0 = fixed32(fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5);
-    mp.su = uniform64(fixed32(Ly.inv[0] * (double)lanes_x));
-    mp.sv = uniform64(fixed32(Ly.inv[3] * (double)lanes_x));
-    // Axis-aligned maps (pure scale and shift; the planner picks the SEP
-    // variant only when every layer has one) are separable: the lane's
+    // Each lane's first pixel and the step between a lane's pixels in fixed
+    // point.  Axis-aligned maps (pure scale and shift; the planner picks the
+    // SEP variant only when every layer has one) are separable: the lane's
     // pixels share one source row pair, computed once.
+    const LaneMap mp = lane_map<V::SEP>(Ly, tx * (uint32_t)TW, ty * (uint32_t)TH, lx, ly, lanes_x);
     layer_taps<V, V::SEP>(a, mp, n, X, L);
   }
 
@@ -627,12 +656,8 @@ __device__ __forceinline__ void box_pixels(const BoxTile& s, const float4* box, 
 #pragma unroll
       for (int k = 0; k < 9; ++k) m[k] = d->params[q + k];
     }
-    const double ex0 = (double)s.tx * TW + 0.5, ey0 = (double)s.ty * TH + 0.5;
-    const double uo = Ly.inv[0] * ex0 + Ly.inv[1] * ey0 + Ly.inv[2];
-    const double vo = Ly.inv[3] * ex0 + Ly.inv[4] * ey0 + Ly.inv[5];
-    const int64_t u0 = fixed32(fma(Ly.inv[0], (double)lx, fma(Ly.inv[1], (double)ly, uo)) - 0.5);
-    const int64_t v0 = fixed32(fma(Ly.inv[3], (double)lx, fma(Ly.inv[4], (double)ly, vo)) - 0.5);
-    const int64_t su = uniform64(fixed32(Ly.inv[0] * (double)lanes_x));
+    const LaneMap mp = lane_map<true>(Ly, s.tx * (uint32_t)TW, s.ty * (uint32_t)TH, lx, ly, lanes_x);
+    const int64_t u0 = mp.u0, v0 = mp.v0, su = mp.su;
     const float fy = fx_frac(v0), gy = 1.0f - fy;
     const int bw = s.bw, pitch = s.pitch;
     const float4* row = box + min(max(fx_int(v0) - s.j0, 0), s.bh - 2) * pitch;
