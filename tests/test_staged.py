@@ -60,3 +60,11 @@ def test_c3_batch_splits_staged_and_direct_units():
         assert (int(sw / ow * 63) + 7) * (int(sh / oh * 15) + 6) <= 1600, line
     launches = [l for l in text.splitlines() if l.startswith("launch ")]
     assert any("_lds]" in l for l in launches) and any("_sep]" in l for l in launches), launches
+
+
+def test_warp_forms_need_fixed_point_steps():
+    """The warp kernels map pixels with the layer's steps in 8.56 fixed point
+    (|m| < 127): a 1/200 reduction stays on the interpreter's resampler, a
+    1/100 one gets the warp kernel."""
+    assert "[kernel w_s8m_s8" not in plan(one(1 / 200, 1 / 200, w=1000, h=800), 0)
+    assert "[kernel w_s8m_s8_sep]" in plan(one(1 / 100, 1 / 100, w=1000, h=800), 0)
