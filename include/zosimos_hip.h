@@ -326,6 +326,11 @@ zh_status zh_program_launch_bytes(zh_program* prog, uint64_t* bytes, uint32_t ma
 /* Knobs (Environment::knob, run.rs:1292-1306): override the parameter blob of
  * every op built with this knob id; the length must equal the original. */
 zh_status zh_program_set_knob(zh_program* prog, int32_t knob, const void* data, uint32_t len);
+/* The knob's current parameter bytes: *len receives their length and the
+ * first min(cap, *len) bytes are copied to `out` (which may be NULL when cap
+ * is 0).  A run's knob data (Environment::knob, run.rs:1292-1306) holds for
+ * that run only; a caller restores the bytes read here once it finishes. */
+zh_status zh_program_knob_data(zh_program* prog, int32_t knob, void* out, uint32_t cap, uint32_t* len);
 
 typedef struct zh_binding {
   uint32_t slot;   /* input slot (ZH_HIGH_INPUT.src) or output slot (ZH_HIGH_OUTPUT.dst) */

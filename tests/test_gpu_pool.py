@@ -194,7 +194,7 @@ def test_stepped_execution(ctx):
         exe.launch_pool(pool, {i: k}, {o: 0})  # the program is busy stepping
     steps = []
     while True:
-        n = run.step(1)
+        n = run.step_to(1).launches
         if n == 0:
             break
         steps.append(n)
@@ -204,7 +204,7 @@ def test_stepped_execution(ctx):
     assert np.array_equal(got, want)
     # a larger limit queues the rest at once
     run = exe.start_pool(pool, {i: k}, {o: 0})
-    assert run.step(100) == 3 and run.step(5) == 0
+    assert run.step_to(100).launches == 3 and run.step_to(5).launches == 0
     assert np.array_equal(pool.as_bytes(run.retire_output(o)), want)
     run.finish()
     report("pool_stepped", steps=len(steps))
@@ -252,3 +252,103 @@ def test_finish_by_discarding_and_launch_once(ctx):
     once.launch_once({i: src}, {o: out})
     assert once._h is None
     assert np.array_equal(out.download().reshape(-1), want)
+
+
+def run_once_with_output(ctx, commands, pool, binds, output, knobs=()):
+    """``tests/util.rs:61-125`` (``run_executable_with_output``) through the
+    same names: from_pool, knob, bind, recover_buffers, launch, clear_cache,
+    step until done, retire_gracefully, the output callback,
+    retire_buffers, finish."""
+    exe = z.Linker.from_included().compile(commands).lower_to(ctx)
+    env = exe.from_pool(pool)
+    for knob, data in knobs:
+        env.knob(knob, data)
+    for reg, key in binds:
+        env.bind(reg, key)
+    env.recover_buffers()
+    execution = exe.launch(env)
+    pool.clear_cache()
+    marks = []
+    while execution.is_running():
+        point = execution.step()
+        point.block_on()
+        marks.append(point.debug_mark())
+        assert point.time_spent() > 0
+    retire = execution.retire_gracefully(pool)
+    result = output(retire)
+    retire.retire_buffers()
+    retire.finish()
+    return result, marks, exe
+
+
+def test_util_run_sequence_matches_reference_hash(ctx):
+    """The reference's test driver sequence on the inscribe program of
+    tests/blend.rs: the retired output's block hash is the blessed
+    ``composed`` hash, and each step's debug mark names a launch."""
+    bg, fg = rgba("background"), rgba("foreground")
+    dbg = z.Descriptor.rgba8_srgb(bg.shape[1], bg.shape[0])
+    dfg = z.Descriptor.rgba8_srgb(fg.shape[1], fg.shape[0])
+    cmd = z.CommandBuffer()
+    b, f = cmd.input(dbg), cmd.input(dfg)
+    o, od = cmd.output(cmd.inscribe(b, z.Rectangle(0, 0, fg.shape[1], fg.shape[0]), f))
+    pool = z.Pool()
+    kb, kf = pool.insert(dbg, bg), pool.insert(dfg, fg)
+    key, marks, exe = run_once_with_output(ctx, cmd, pool, [(b, kb), (f, kf)], lambda r: r.output(o))
+    assert marks and all(m and m.startswith("launch ") for m in marks), marks
+    got = pool.as_bytes(key)
+    h = B.blockhash256(got.reshape(od.height, od.width, 4))
+    report("util_run_sequence", hash=h, steps=len(marks))
+    assert h in HASHES["composed"]
+
+
+def test_environment_checks_and_run_scoped_knobs(ctx):
+    """Environment::bind rejects an image of another layout, knob data of the
+    wrong length is refused, and a run's knob data holds for that run only
+    (the next run without it uses the program's own data again);
+    Retire::output_key is the bound key."""
+    w, h = 48, 32
+    d = z.Descriptor.rgba8_srgb(w, h)
+    cmd = z.CommandBuffer()
+    i = cmd.input(d)
+    c = cmd.color_convert(i, z.Color(primaries=z.Primaries.DisplayP3), d.texel)
+    knob = cmd.knob(c)
+    o, od = cmd.output(c)
+    exe = z.Linker.from_included().compile(cmd).lower_to(ctx)
+    pool = z.Pool()
+    rng = np.random.default_rng(31)
+    data = rng.integers(0, 256, d.nbytes, dtype=np.uint8)
+    k = pool.insert(d, data)
+    wrong = pool.insert(z.Descriptor.rgba8_srgb(w + 1, h), np.zeros((w + 1) * h * 4, np.uint8))
+
+    env = exe.from_pool(pool)
+    with pytest.raises(z.ZosimosError):
+        env.bind(i, wrong)
+    with pytest.raises(z.ZosimosError) as e:
+        env.knob(knob, b"\0" * 7)
+    assert e.value.code == z.Status.KNOB_LENGTH
+
+    def run(with_identity):
+        out = pool.declare(od)
+        env = exe.from_pool(pool)
+        env.bind(i, k)
+        env.bind_output(o, out)
+        if with_identity:
+            ident = np.zeros(12, np.float32)
+            ident[[0, 5, 10]] = 1.0  # std140 mat3: three vec4 columns
+            env.knob(knob, ident.tobytes())
+        run = exe.launch(env)
+        while run.is_running():
+            run.step().block_on()
+        retire = run.retire_gracefully(pool)
+        assert retire.output_key(o) == out
+        key = retire.output(o)
+        retire.finish()
+        assert key == out
+        return pool.as_bytes(key)
+
+    converted = run(False)
+    copied = run(True)
+    again = run(False)
+    # the identity matrix turns the conversion into a byte copy
+    assert np.array_equal(copied, data)
+    assert np.array_equal(again, converted) and not np.array_equal(converted, data)

@@ -685,6 +685,10 @@ class Linker:
         return prog
 
 
+# High op kinds of the program's inputs and outputs (zosimos_hip.h)
+HIGH_INPUT = 0
+HIGH_OUTPUT = 1
+
 # Plan flags (zosimos_hip.h)
 PLAN_DEFAULT = 0
 PLAN_STRICT_REGISTERS = 1
@@ -712,6 +716,18 @@ class Program:
         _check(self._lib.zs_linked_high(self._h, ctypes.byref(tex), ctypes.byref(nt), ctypes.byref(ops),
                                         ctypes.byref(no)))
         return [tex[i] for i in range(nt.value)], [ops[i] for i in range(no.value)]
+
+    def register_descriptor(self, reg: int, output: bool) -> Descriptor:
+        """The descriptor of an input (or output) register of this program,
+        from its High::Input (High::Output) entry."""
+        slot = self.output_slot(reg) if output else self.input_slot(reg)
+        tex, ops = self.high()
+        for op in ops:
+            if not output and op.high == HIGH_INPUT and op.src == slot:
+                return Descriptor.from_c(tex[op.dst])
+            if output and op.high == HIGH_OUTPUT and op.dst == slot:
+                return Descriptor.from_c(tex[op.src])
+        raise ZosimosError(Status.INVALID_ARGUMENT, f"register {reg} is not an {'output' if output else 'input'}")
 
     def input_slot(self, reg: int) -> int:
         s = ctypes.c_uint32()
@@ -801,10 +817,28 @@ class Executable:
         arr = (BindingC * len(binds))(*binds)
         return arr, len(binds)
 
-    def launch(self, inputs: Dict[int, "Image"], outputs: Dict[int, "Image"]):
-        """Bind registers to images and launch asynchronously
-        (``Environment::bind`` + ``Executable::launch`` + stepping)."""
-        arr, n = self._bindings(inputs, outputs)
+    def knob_data(self, knob: int) -> bytes:
+        """The knob's current parameter bytes."""
+        n = ctypes.c_uint32()
+        _check(self._lib.zh_program_knob_data(self._h, knob, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(n.value, 1))
+        _check(self._lib.zh_program_knob_data(self._h, knob, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    def from_pool(self, pool: "Pool") -> "Environment":
+        """``Executable::from_pool`` (run.rs:471-488): the bindings and knob
+        data of one launch with images from ``pool``."""
+        return Environment(self, pool)
+
+    def launch(self, inputs, outputs: Optional[Dict[int, "Image"]] = None):
+        """``Executable::launch`` (run.rs:1016-1167).  With an
+        ``Environment``: bind and trade its pool images and return the
+        ``Execution``, which queues nothing until it is stepped.  With
+        dictionaries of registers to images: bind them and launch
+        asynchronously, stepped to the end."""
+        if isinstance(inputs, Environment):
+            return inputs._launch()
+        arr, n = self._bindings(inputs, outputs or {})
         _check(self._lib.zh_program_launch(self._h, arr, n))
 
     def _pool_launch(self, fn, pool: "Pool", inputs: Dict[int, int], outputs: Dict[int, int]) -> "Execution":
@@ -1053,11 +1087,15 @@ class Pool:
 class Execution:
     """A launch from a pool and its ``Retire`` (run.rs:2786-2998)."""
 
-    def __init__(self, handle, exe: Executable, pool: Pool):
+    def __init__(self, handle, exe: Executable, pool: Pool, outputs: Optional[Dict[int, int]] = None,
+                 restore: Optional[Dict[int, bytes]] = None):
         self._lib = _abi.lib()
         self._h = handle
         self.exe = exe
         self.pool = pool  # the pool outlives the execution
+        self._queued = 0
+        self._outputs = dict(outputs or {})  # register -> the pool key it was bound to (0: none)
+        self._restore = dict(restore or {})  # knob -> the bytes it held before this run
 
     def retire_output(self, reg: int, to_host: bool = True) -> int:
         """``Retire::output`` (run.rs:2809-2875): the output's pool key."""
@@ -1071,12 +1109,28 @@ class Execution:
         _check(self._lib.zh_execution_retire_input(self._h, self.exe.program.input_slot(reg), ctypes.byref(k)))
         return k.value
 
-    def step(self, limit: int = 1) -> int:
-        """``Execution::step_to(StepLimits { instructions: limit })``: queue up
-        to ``limit`` more kernel launches; returns how many were queued."""
+    def step_to(self, instructions: int = 1) -> "SyncPoint":
+        """``Execution::step_to(StepLimits { instructions })`` (run.rs:1394-
+        1464): queue up to ``instructions`` more kernel launches; the
+        ``SyncPoint`` waits for them and says how many were queued."""
+        import time
+        t0 = time.perf_counter()
+        first = self._queued
         n = ctypes.c_uint32()
-        _check(self._lib.zh_execution_step(self._h, limit, ctypes.byref(n)))
-        return n.value
+        _check(self._lib.zh_execution_step(self._h, instructions, ctypes.byref(n)))
+        self._queued += n.value
+        return SyncPoint(self, n.value, t0, first)
+
+    def step(self) -> "SyncPoint":
+        """``Execution::step`` (run.rs:1389-1391): one launch."""
+        return self.step_to(1)
+
+    def retire_gracefully(self, pool: "Pool") -> "Retire":
+        """``Execution::retire_gracefully`` (run.rs:1479-1492): stop, with the
+        images and buffers going back into ``pool``."""
+        if pool is not self.pool:
+            raise ZosimosError(Status.INVALID_ARGUMENT, "an execution retires into the pool it was launched from")
+        return Retire(self)
 
     def is_running(self) -> bool:
         r = ctypes.c_int32()
@@ -1093,22 +1147,199 @@ class Execution:
         """``Retire::prune``: free this execution's buffers at finish."""
         _check(self._lib.zh_execution_prune(self._h))
 
+    def _restore_knobs(self):
+        # A run's knob data (Environment::knob) held for that run only.
+        restore, self._restore = self._restore, {}
+        for knob, data in restore.items():
+            self.exe.knob(knob, data)
+
     def finish(self):
         """``Retire::finish`` (run.rs:2962-2998)."""
         if getattr(self, "_h", None):
             h, self._h = self._h, None
             _check(self._lib.zh_execution_finish(h))
+            self._restore_knobs()
 
     def finish_by_discarding(self):
         """``Retire::finish_by_discarding``: drop everything not retired."""
         if getattr(self, "_h", None):
             h, self._h = self._h, None
             _check(self._lib.zh_execution_discard(h))
+            self._restore_knobs()
 
     def __del__(self):
         if getattr(self, "_h", None) and _abi is not None:
             self._lib.zh_execution_finish(self._h)
             self._h = None
+
+
+class Environment:
+    """``Environment`` (run.rs:1170-1348): the pool images and knob data of
+    one launch, from ``Executable.from_pool``; ``Executable.launch(env)``
+    starts it."""
+
+    def __init__(self, exe: Executable, pool: Pool):
+        self.exe = exe
+        self.pool = pool
+        self._inputs: Dict[int, int] = {}
+        self._outputs: Dict[int, int] = {}
+        self._knobs: Dict[int, bytes] = {}
+
+    def _check_image(self, reg: int, key: int, output: bool):
+        want = self.exe.program.register_descriptor(reg, output)
+        have = self.pool.descriptor(key)
+        # As the reference: layout and texel must match, colour is not compared.
+        if (have.width, have.height, have.texel) != (want.width, want.height, want.texel) or (
+                have.row_stride and want.row_stride and have.row_stride != want.row_stride):
+            raise ZosimosError(Status.MISMATCHED_DESCRIPTOR,
+                               f"pool image {key} does not match register {reg}'s descriptor")
+
+    def bind(self, reg: int, key: int):
+        """``Environment::bind`` (run.rs:1171-1205): the input ``reg`` reads
+        pool image ``key``."""
+        self._check_image(reg, key, False)
+        self._inputs[reg] = key
+
+    def bind_output(self, reg: int, key: int):
+        """``Environment::bind_output`` (run.rs:1207-1242): the output ``reg``
+        is written into pool image ``key``."""
+        self._check_image(reg, key, True)
+        self._outputs[reg] = key
+
+    def bind_render(self, reg: int, key: int):
+        """``Environment::bind_render`` (run.rs:1244-1279): as
+        ``bind_output``, into a pool image that lives on the device (the
+        reference's render targets are GPU textures)."""
+        if self.pool.kind(key) != "device":
+            raise ZosimosError(Status.INVALID_ARGUMENT, f"render target {key} is not a device image")
+        self.bind_output(reg, key)
+
+    def knob(self, knob: int, data: bytes):
+        """``Environment::knob`` (run.rs:1292-1306): the knob's data for this
+        run only; the length must equal the knob's."""
+        have = len(self.exe.knob_data(knob))
+        if len(data) != have:
+            raise ZosimosError(Status.KNOB_LENGTH, f"knob data length {len(data)} differs from {have}")
+        self._knobs[knob] = bytes(data)
+
+    def knob_by_register(self, knob: Tuple[int, int], data: bytes):
+        """``Environment::knob_by_register`` (run.rs:1283-1290) with a
+        ``RegisterKnob`` (link index, register)."""
+        self.knob(self.exe.program.query_knob(knob[0], knob[1]), data)
+
+    def recover_buffers(self) -> dict:
+        """``Environment::recover_buffers`` (run.rs:1312-1347): the launch
+        takes device buffers of matching size from the pool's cache; this
+        reports how many the cache holds for it to take."""
+        return {"cached_buffers": self.pool.cached_buffers()}
+
+    def _launch(self) -> "Execution":
+        restore = {}
+        try:
+            for knob, data in self._knobs.items():
+                restore[knob] = self.exe.knob_data(knob)
+                self.exe.knob(knob, data)
+            # Outputs left unbound go to new pool images when retired.
+            prog = self.exe.program
+            bound = {prog.output_slot(reg): key for reg, key in self._outputs.items()}
+            _, ops = prog.high()
+            slots = sorted({op.dst for op in ops if op.high == HIGH_OUTPUT})
+            binds = [PoolBindingC(prog.input_slot(r), 0, k) for r, k in self._inputs.items()]
+            binds += [PoolBindingC(s, 1, bound.get(s, 0)) for s in slots]
+            arr = (PoolBindingC * max(len(binds), 1))(*binds)
+            h = ctypes.c_void_p()
+            _check(self.exe._lib.zh_program_start_pool(self.exe._h, self.pool._h, arr, len(binds), ctypes.byref(h)))
+            run = Execution(h, self.exe, self.pool)
+        except BaseException:
+            for knob, data in restore.items():
+                self.exe.knob(knob, data)
+            raise
+        run._outputs = dict(self._outputs)
+        run._restore = restore
+        return run
+
+
+class SyncPoint:
+    """``SyncPoint`` (run.rs:3015-3184): the launches one step queued."""
+
+    def __init__(self, run: "Execution", launches: int, t0: float, first: int):
+        self.run = run
+        self.launches = launches
+        self._t0 = t0
+        self._t1: Optional[float] = None
+        self._first = first
+
+    def block_on(self):
+        """``SyncPoint::block_on``: wait until the queued launches finished."""
+        import time
+        self.run.exe.ctx.sync()
+        self._t1 = time.perf_counter()
+
+    def finish(self):
+        """``SyncPoint::finish``."""
+        self.block_on()
+
+    def time_spent(self) -> float:
+        """``SyncPoint::time_spent``: seconds from the step to the end of
+        ``block_on`` (or to now, before it)."""
+        import time
+        return (self._t1 if self._t1 is not None else time.perf_counter()) - self._t0
+
+    def debug_mark(self) -> Optional[str]:
+        """``SyncPoint::debug_mark``: the first launch this step queued."""
+        if self.launches == 0:
+            return None
+        lines = [l for l in self.run.exe.describe().splitlines() if l.startswith("launch ")]
+        return lines[self._first] if self._first < len(lines) else None
+
+
+class Retire:
+    """``Retire`` (run.rs:2786-2998), from ``Execution.retire_gracefully``:
+    images and buffers back into the pool."""
+
+    def __init__(self, run: "Execution"):
+        self.run = run
+        self.pool = run.pool
+
+    def input(self, reg: int) -> int:
+        """``Retire::input``: a ``no_read`` input moved out for the launch
+        goes back under its key."""
+        return self.run.retire_input(reg)
+
+    def output(self, reg: int) -> int:
+        """``Retire::output``: the output's pool key (the bound key, else a
+        new one), with its bytes on the host."""
+        return self.run.retire_output(reg, to_host=True)
+
+    def render(self, reg: int) -> int:
+        """``Retire::render``: the render target's pool key, kept on the
+        device."""
+        return self.run.retire_output(reg, to_host=False)
+
+    def output_key(self, reg: int) -> Optional[int]:
+        """``Retire::output_key`` (run.rs:2860-2874): the key ``output``
+        will use, when the output was bound to one."""
+        self.run.exe.program.output_slot(reg)  # raises for a register that is no output
+        key = self.run._outputs.get(reg, 0)
+        return key or None
+
+    def retire_buffers(self) -> dict:
+        """``Retire::retire_buffers`` (run.rs:2876-2945): the execution's
+        device buffers go to the pool's cache at ``finish``; returns what it
+        used."""
+        return self.run.resources_used()
+
+    def prune(self):
+        """``Retire::prune``: give no buffer back to the pool's cache."""
+        self.run.prune()
+
+    def finish(self):
+        """``Retire::finish``."""
+        self.run.finish()
+
+    def finish_by_discarding(self):
+        """``Retire::finish_by_discarding``."""
+        self.run.finish_by_discarding()
 
 
 MLP_PARAMS = 4611

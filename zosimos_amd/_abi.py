@@ -98,6 +98,7 @@ _SIGNATURES = {
     "zh_program_describe": (i32, [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]),
     "zh_program_stats": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
     "zh_program_set_knob": (i32, [vp, i32, vp, u32]),
+    "zh_program_knob_data": (i32, [vp, i32, vp, u32, ctypes.POINTER(u32)]),
     "zh_program_launch_bytes": (i32, [vp, ctypes.POINTER(u64), u32, ctypes.POINTER(u32)]),
     "zh_program_launch": (i32, [vp, ctypes.POINTER(BindingC), u32]),
     "zh_program_graph_info": (i32, [vp, ctypes.POINTER(GraphInfoC)]),

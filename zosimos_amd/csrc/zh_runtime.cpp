@@ -770,6 +770,18 @@ zh_status zh_program_stats(zh_program* p, uint32_t* launches, uint64_t* algorith
   return ZH_OK;
 }
 
+zh_status zh_program_knob_data(zh_program* p, int32_t knob, void* out, uint32_t cap, uint32_t* len) {
+  if (!p || !len || (!out && cap)) return fail(ZH_ERR_INVALID_ARGUMENT, "null argument");
+  for (size_t i = 0; i < p->ops.size(); ++i)
+    if (p->ops[i].knob == knob) {
+      const std::vector<uint8_t>& b = p->params[i];
+      *len = (uint32_t)b.size();
+      if (cap && !b.empty()) std::memcpy(out, b.data(), std::min<size_t>(cap, b.size()));
+      return ZH_OK;
+    }
+  return fail(ZH_ERR_INVALID_ARGUMENT, "no op uses knob " + std::to_string(knob));
+}
+
 // A warp stage's kernel tag with or without the "_lds" suffix of the staged
 // kernels.
 static void set_staged_tag(std::string& text, bool staged) {
