@@ -130,3 +130,32 @@ def test_stepped_start_needs_a_device():
     kf = pool.insert(fg, np.zeros(fg.nbytes, np.uint8))
     expect(z.Status.NO_DEVICE, exe.start_pool, pool, {b: kb, f: kf}, {o: 0})
     expect(z.Status.MISSING_KEY, exe.start_pool, pool, {b: kb}, {o: 0})
+
+
+def test_environment_checks_before_launch():
+    """Environment::bind (run.rs:1171-1205) refuses a pool image whose layout
+    or texel differs from the register's and a register that is no input;
+    bind_render wants a device image; knob data must have the knob's
+    length (run.rs:1292-1298)."""
+    d = z.Descriptor.rgba8_srgb(16, 8)
+    cmd = z.CommandBuffer()
+    i = cmd.input(d)
+    c = cmd.color_convert(i, z.Color(primaries=z.Primaries.DisplayP3), d.texel)
+    knob = cmd.knob(c)
+    o, od = cmd.output(c)
+    exe = z.Linker.from_included().compile(cmd).lower_to(None)
+    pool = z.Pool()
+    good = pool.insert(d, np.zeros(d.nbytes, np.uint8))
+    taller = pool.insert(z.Descriptor.rgba8_srgb(16, 9), np.zeros(16 * 9 * 4, np.uint8))
+    half = pool.declare(z.Descriptor.rgba16f_linear(16, 8))
+    env = exe.from_pool(pool)
+    env.bind(i, good)
+    expect(z.Status.MISMATCHED_DESCRIPTOR, env.bind, i, taller)
+    expect(z.Status.MISMATCHED_DESCRIPTOR, env.bind, i, half)
+    expect(z.Status.INVALID_ARGUMENT, env.bind, o, good)  # an output register
+    env.bind_output(o, pool.declare(od))
+    expect(z.Status.INVALID_ARGUMENT, env.bind_render, o, good)  # host image
+    assert len(exe.knob_data(knob)) == 48
+    expect(z.Status.KNOB_LENGTH, env.knob, knob, b"\0" * 12)
+    env.knob(knob, bytes(48))
+    assert exe.knob_data(knob) != bytes(48)  # held for the run, not set yet
