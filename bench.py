@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--staged", action="store_true",
                     help="plan with ZH_PLAN_STAGED: separable one-layer resample stages read their taps from "
                          "LDS-staged source boxes instead of direct gathers (the A/B of the staged tiles)")
+    ap.add_argument("--pair", action="store_true",
+                    help="plan with ZH_PLAN_PAIR: the two warp forms of a batch as one grid dealing both tile "
+                         "queues alternately instead of two queues one after the other (the A/B of pairing)")
     ap.add_argument("--no-jit", action="store_true",
                     help="plan with ZH_PLAN_NO_JIT: chains without a registered kernel run on the interpreter")
     return ap.parse_args()
@@ -300,7 +303,8 @@ def main():
     stream = torch.cuda.Stream()  # default priority
     ctx = z.Context(0, stream=stream.cuda_stream)
     traffic = load_traffic()
-    flags = (z.PLAN_STAGED if args.staged else 0) | (z.PLAN_NO_JIT if args.no_jit else 0)
+    flags = ((z.PLAN_STAGED if args.staged else 0) | (z.PLAN_NO_JIT if args.no_jit else 0) |
+             (z.PLAN_PAIR if args.pair else 0))
     head_cfg = HEADLINE if args.config == "all" else args.config
     wl = W.copy_frame() if head_cfg == "copy" else {**W.CONFIGS, **W.EXTRA}[head_cfg]()
     head = measure(wl, ctx, stream, torch, args.steps, args.warmup, traffic, flags)

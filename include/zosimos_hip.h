@@ -305,6 +305,12 @@ typedef struct zh_op {
  * instead of gathering every tap directly.  Measured slower than the direct
  * gathers on MI355X at every scale from 1x to 4x, so not the default. */
 #define ZH_PLAN_STAGED 8u
+/* Launch the two warp forms of a run of independent resample stages (as a
+ * batch of RGBA8 and RGBA16F images) as one persistent grid that deals their
+ * tiles alternately (a kernel compiled at run time for the pair of forms),
+ * instead of two tile queues one after the other.  Measured slower on
+ * MI355X, so not the default. */
+#define ZH_PLAN_PAIR 16u
 
 /* Lower a High program (Program::lower_to, program.rs:1304).  `textures`
  * holds the descriptor of every texture id.  `ctx` may be NULL: the plan is

@@ -117,3 +117,31 @@ def test_staged_batch_with_autotune(ctx):
         check(f"staged_batch_img{k}", fmt, got[ro], want, dout.width, dout.height)
         if fmt == 0:
             assert np.abs(got[ro].astype(int) - plain[ro].astype(int)).max() <= 1
+
+
+def test_paired_forms_equal_two_queues(ctx):
+    """ZH_PLAN_PAIR: the RGBA8 and RGBA16F images of a ragged batch run as
+    one grid that deals the two forms' tile queues alternately (a kernel
+    compiled at run time for the pair); autotuned and relaunched, every
+    output equals the default plan's two queues byte for byte (the same
+    kernel bodies), and the plan has one launch."""
+    wl = W.c3(n=14, seed=21, max_side=300)
+    host = wl.make_inputs()
+    plain, _ = run(ctx, wl.commands, wl.inputs, host, wl.outputs)
+    exe = z.Linker.from_included().compile(wl.commands).lower_to(ctx, z.PLAN_PAIR)
+    text = exe.describe()
+    line = [l for l in text.splitlines() if l.startswith("launch 0:")][0]
+    assert "[kernel pair " in line and "w_s8m_s8_sep" in line and "w_hm_h_sep" in line, text
+    assert exe.stats()[0] == 1
+    imgs = {}
+    for reg, d in wl.inputs.items():
+        imgs[reg] = ctx.image(d)
+        imgs[reg].upload(host[reg])
+    outs = {reg: ctx.image(d) for reg, d in wl.outputs.items()}
+    exe.autotune(imgs, outs, iterations=2)
+    for _ in range(3):
+        exe.launch(imgs, outs)
+    ctx.sync()
+    for reg, img in outs.items():
+        assert np.array_equal(img.download(), plain[reg]), reg
+    report("paired_forms", launches=exe.stats()[0], images=len(outs))

@@ -122,14 +122,27 @@ def test_plan_c1_c5_resample_layers():
 
 def test_plan_c3_batches_into_one_tile_queue_per_form():
     """Independent resample stages batch by kernel form: one persistent
-    tile-queue launch per form, every stage in exactly one batch."""
+    tile-queue launch per form, every stage in exactly one batch; under
+    ZH_PLAN_PAIR the two forms share one grid that deals their queues
+    alternately, first form's stages first."""
     e = plan(W.c3(12, max_side=300))
     text = e.describe()
-    forms = set(re.findall(r"^stage .*\[kernel (\w+)", text, re.M))
+    forms = re.findall(r"^stage .*\[kernel (\w+)", text, re.M)
     batches = re.findall(r"batch: stages ([\d,]+) as one tile queue", text)
     ids = sorted(int(i) for b in batches for i in b.split(","))
-    assert ids == list(range(12)) and len(batches) == len(forms) <= 2
-    assert e.stats()[0] == len(forms)
+    assert ids == list(range(12)) and len(batches) == len(set(forms)) == 2
+    assert e.stats()[0] == 2
+    paired = plan(W.c3(12, max_side=300), z.PLAN_PAIR)
+    text = paired.describe()
+    assert paired.stats()[0] == 1
+    assert "[kernel pair w_s8m_s8_sep + w_hm_h_sep] over 12 stages" in text
+    order = re.findall(r"batch: stages ([\d,]+) as one grid dealing two tile queues alternately", text)
+    ids = [int(i) for i in order[0].split(",")]
+    assert sorted(ids) == list(range(12))
+    split = sum(1 for f in forms if f == forms[ids[0]])
+    assert {forms[i] for i in ids[:split]} == {forms[ids[0]]} and {forms[i] for i in ids[split:]} != {forms[ids[0]]}
+    # no pairs without run-time kernels
+    assert "pair" not in plan(W.c3(12, max_side=300), z.PLAN_PAIR | z.PLAN_NO_JIT).describe()
 
 
 def test_plan_strict_registers_quantises():

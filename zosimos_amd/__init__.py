@@ -695,6 +695,7 @@ PLAN_STRICT_REGISTERS = 1
 PLAN_NO_GRAPH = 2
 PLAN_NO_JIT = 4
 PLAN_STAGED = 8
+PLAN_PAIR = 16
 
 
 class Program:

@@ -79,6 +79,18 @@ zh_status jit_warp_kernels(int id, bool sep, bool staged, int device, hipFunctio
                            std::string* why);
 // Does form `id` have LDS-staged kernels (one-source forms)?
 bool jit_warp_has_staged(int id);
+// An affine layer-stack form as the warp bodies take it.
+struct WarpForm {
+  uint32_t src = 0, fmt = 0, mat = 0;
+  std::vector<uint16_t> post;
+  bool sep = false;
+};
+// The form behind a StageDesc::variant (registered or run-time); false for none.
+bool warp_form_of(int variant, WarpForm* f);
+// Two warp forms dealt by one persistent grid (warp_pair): the process-wide
+// id of the pair, and its kernel on `device`, compiled on first use.
+int jit_warp_pair(const WarpForm& a, const WarpForm& b);
+zh_status jit_pair_kernel(int id, int device, hipFunction_t* fn, std::string* why);
 // StageDesc::variant of a run-time warp form: kJitWarpBase + 2 id (+ 1 for
 // separable maps); registered forms are below it.
 constexpr int kJitWarpBase = 1 << 16;
@@ -164,6 +176,16 @@ struct Unit {
   bool warp = false;             // batch of one warp form (k_warp_batch<form>)
   bool spec = false;             // batch of one pointwise signature (k_spec_batch<signature>)
   bool staged = false;           // warp stages on the LDS-staged kernels
+  // Two warp forms in one grid (warp_pair): ids[0, split) are the first
+  // form's stages (the batch above), ids[split, ..) the second's (below).
+  bool pair = false;
+  uint32_t split = 0;
+  zh::BatchDesc* d_batch2 = nullptr;
+  uint32_t* d_prefix2 = nullptr;
+  uint32_t* d_ids2 = nullptr;
+  uint32_t total_tiles2 = 0;
+  hipFunction_t pair_fn = nullptr;
+  uint32_t pair_resident = 1;    // resident workgroups per CU of pair_fn
   std::vector<uint32_t> ids;     // stages of the unit (one unless batched)
   uint32_t* d_ids = nullptr;     // device copy of ids (batches)
   zh::BatchDesc* d_batch = nullptr;  // device

@@ -43,8 +43,12 @@ namespace {
 // form, source format, mat3 flag and the post program's signature.
 struct JitEntry {
   bool warp = false;
+  bool pair = false;  // two warp forms in one grid: the first in src..sig, the second in src2..sig2
   uint32_t src = 0, fmt = 0, mat = 0;
   std::vector<uint16_t> sig;
+  bool sep = false, sep2 = false;
+  uint32_t src2 = 0, fmt2 = 0, mat2 = 0;
+  std::vector<uint16_t> sig2;
   bool compiled = false;
   std::string error;  // why the compile failed (compiled and no kernels)
   // Kernels held for the process: pointwise {stage, batch}; warp {stage,
@@ -73,9 +77,28 @@ const char* const kSpecKernels[] = {"zh_jit_stage", "zh_jit_batch"};
 const char* const kWarpKernels[] = {"zh_jit_warp",       "zh_jit_warp_sep",    "zh_jit_warp_batch",
                                     "zh_jit_warp_batch_sep", "zh_jit_warp_staged", "zh_jit_warp_batch_staged"};
 
-int kernel_count(const JitEntry& e) { return !e.warp ? 2 : e.src == WSRC_ONE ? 6 : 4; }
+const char* const kPairKernels[] = {"zh_jit_warp_pair"};
+
+int kernel_count(const JitEntry& e) { return e.pair ? 1 : !e.warp ? 2 : e.src == WSRC_ONE ? 6 : 4; }
+
+std::string prog_list(const std::vector<uint16_t>& sig) {
+  std::string list;
+  for (size_t i = 0; i < sig.size(); ++i) list += (i ? ", " : "") + std::to_string(sig[i]);
+  return list;
+}
+
+std::string warp_type(uint32_t src, uint32_t fmt, uint32_t mat, const std::vector<uint16_t>& sig, bool sep) {
+  return "zh::spec::Warp<" + std::to_string(src) + "u, " + std::to_string(fmt) + "u, " + (mat ? "true" : "false") +
+         ", zh::spec::Prog<" + prog_list(sig) + ">, " + (sep ? "true" : "false") + ">";
+}
 
 std::string source_for(const JitEntry& e) {
+  if (e.pair)
+    return "#include \"zh_warp_kernels.h\"\n"
+           "typedef " + warp_type(e.src, e.fmt, e.mat, e.sig, e.sep) + " VA;\n"
+           "typedef " + warp_type(e.src2, e.fmt2, e.mat2, e.sig2, e.sep2) + " VB;\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void zh_jit_warp_pair(const zh::BatchDesc* a, "
+           "const zh::BatchDesc* b) { zh::spec::warp_pair<VA, VB>(a, b); }\n";
   std::string list;
   for (size_t i = 0; i < e.sig.size(); ++i) list += (i ? ", " : "") + std::to_string(e.sig[i]);
   const std::string k = "extern \"C\" __global__ __launch_bounds__(256) void ";
@@ -131,7 +154,7 @@ void compile(JitEntry& e) {
   std::vector<char> code(n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
-  const char* const* names = e.warp ? kWarpKernels : kSpecKernels;
+  const char* const* names = e.pair ? kPairKernels : e.warp ? kWarpKernels : kSpecKernels;
   const int count = kernel_count(e);
   for (int i = 0; i < count; ++i) {
     zh_shader* k = nullptr;
@@ -149,7 +172,9 @@ int find_or_add(const JitEntry& want) {
   std::vector<JitEntry>& es = entries();
   for (size_t i = 0; i < es.size(); ++i) {
     const JitEntry& e = es[i];
-    if (e.warp == want.warp && e.src == want.src && e.fmt == want.fmt && e.mat == want.mat && e.sig == want.sig)
+    if (e.warp == want.warp && e.pair == want.pair && e.src == want.src && e.fmt == want.fmt && e.mat == want.mat &&
+        e.sig == want.sig && e.sep == want.sep && e.src2 == want.src2 && e.fmt2 == want.fmt2 &&
+        e.mat2 == want.mat2 && e.sig2 == want.sig2 && e.sep2 == want.sep2)
       return (int)i;
   }
   es.push_back(want);
@@ -158,12 +183,12 @@ int find_or_add(const JitEntry& want) {
 
 // The kernels `which` of entry `id` on `device`, compiling the entry first.
 zh_status kernels_of(int id, bool warp, std::initializer_list<int> which, int device, hipFunction_t* out,
-                     std::string* why) {
+                     std::string* why, bool pair = false) {
   std::vector<zh_shader*> ks;
   {
     std::lock_guard<std::mutex> lock(g_mu);
     std::vector<JitEntry>& es = entries();
-    if (id < 0 || id >= (int)es.size() || es[(size_t)id].warp != warp)
+    if (id < 0 || id >= (int)es.size() || es[(size_t)id].warp != warp || es[(size_t)id].pair != pair)
       return fail(ZH_ERR_INVALID_ARGUMENT, "unknown run-time kernel");
     JitEntry& e = es[(size_t)id];
     if (!e.compiled) compile(e);
@@ -248,10 +273,55 @@ zh_status jit_warp_kernels(int id, bool sep, bool staged, int device, hipFunctio
   return st;
 }
 
+bool warp_form_of(int variant, WarpForm* f) {
+  if (variant < 0) return false;
+  if (variant >= kJitWarpBase) {
+    const int id = (variant - kJitWarpBase) / 2;
+    std::lock_guard<std::mutex> lock(g_mu);
+    const std::vector<JitEntry>& es = entries();
+    if (id >= (int)es.size() || !es[(size_t)id].warp || es[(size_t)id].pair) return false;
+    const JitEntry& e = es[(size_t)id];
+    f->src = e.src;
+    f->fmt = e.fmt;
+    f->mat = e.mat;
+    f->post = e.sig;
+    f->sep = (variant & 1) != 0;
+    return true;
+  }
+  const uint16_t* post = nullptr;
+  int n = 0;
+  if (!zh_warp_form_of(variant, &f->src, &f->fmt, &f->mat, &post, &n)) return false;
+  f->post.assign(post, post + n);
+  f->sep = (variant & 1) != 0;
+  return true;
+}
+
+int jit_warp_pair(const WarpForm& a, const WarpForm& b) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  JitEntry e;
+  e.warp = true;
+  e.pair = true;
+  e.src = a.src;
+  e.fmt = a.fmt;
+  e.mat = a.mat;
+  e.sig = a.post;
+  e.sep = a.sep;
+  e.src2 = b.src;
+  e.fmt2 = b.fmt;
+  e.mat2 = b.mat;
+  e.sig2 = b.post;
+  e.sep2 = b.sep;
+  return find_or_add(e);
+}
+
+zh_status jit_pair_kernel(int id, int device, hipFunction_t* fn, std::string* why) {
+  return kernels_of(id, true, {0}, device, fn, why, true);
+}
+
 bool jit_warp_has_staged(int id) {
   std::lock_guard<std::mutex> lock(g_mu);
   const std::vector<JitEntry>& es = entries();
-  return id >= 0 && id < (int)es.size() && es[(size_t)id].warp && es[(size_t)id].src == WSRC_ONE;
+  return id >= 0 && id < (int)es.size() && es[(size_t)id].warp && !es[(size_t)id].pair && es[(size_t)id].src == WSRC_ONE;
 }
 
 double jit_compile_seconds(int id) {

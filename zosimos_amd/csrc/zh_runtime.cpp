@@ -351,6 +351,9 @@ static void free_device(zh_program* p) {
     if (u.d_counter) hipFree(u.d_counter);
     if (u.d_prefix) hipFree(u.d_prefix);
     if (u.d_ids) hipFree(u.d_ids);
+    if (u.d_batch2) hipFree(u.d_batch2);
+    if (u.d_prefix2) hipFree(u.d_prefix2);
+    if (u.d_ids2) hipFree(u.d_ids2);
   }
   p->units.clear();
   for (void* b : p->d_intermediates) hipFree(b);
@@ -423,6 +426,22 @@ static void make_units(zh_program* p) {
       auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
       if (it == groups.end()) groups.push_back({key, {(uint32_t)k}});
       else it->second.push_back((uint32_t)k);
+    }
+    // Under ZH_PLAN_PAIR, two warp forms of one run (say, a batch of RGBA8
+    // and RGBA16F images) share one persistent grid that deals their tiles
+    // alternately.
+    const bool pair = fam == 1 && groups.size() == 2 && (p->flags & ZH_PLAN_PAIR) && !(p->flags & ZH_PLAN_NO_JIT) &&
+                      !p->arena &&
+                      !st[groups[0].second[0]].staged && !st[groups[1].second[0]].staged;
+    if (pair) {
+      Unit u;
+      u.ids = groups[0].second;
+      u.split = (uint32_t)u.ids.size();
+      u.ids.insert(u.ids.end(), groups[1].second.begin(), groups[1].second.end());
+      u.batch = u.warp = u.pair = true;
+      p->units.push_back(u);
+      i = j;
+      continue;
     }
     for (auto& g : groups) {
       Unit u;
@@ -510,6 +529,47 @@ static zh_status upload_buffers(zh_program* p) {
     ZH_HIP(hipMemcpy(b.ptr, p->params[plan.buffer_ops[k]].data(), b.bytes, hipMemcpyHostToDevice));
   }
   return ZH_OK;
+}
+
+// The tile queue of stages ids[0, n): their tile (or chunk) prefix, the ids
+// and the BatchDesc, in device memory.
+static zh_status alloc_batch(zh_program* p, const uint32_t* ids, size_t n, bool spec, uint32_t* counter,
+                             BatchDesc** d_batch, uint32_t** d_prefix, uint32_t** d_ids, uint32_t* total) {
+  std::vector<uint32_t> prefix(n + 1, 0);
+  for (size_t k = 0; k < n; ++k) {
+    const StageDesc& d = p->plan.stages[ids[k]].desc;
+    const uint64_t chunks = ((uint64_t)d.width * d.height + kSpecChunkPixels - 1) / kSpecChunkPixels;
+    prefix[k + 1] = prefix[k] + (spec ? (uint32_t)chunks : d.tiles_x * d.tiles_y);
+  }
+  ZH_ALLOC(p, sizeof(BatchDesc), d_batch);
+  ZH_ALLOC(p, prefix.size() * sizeof(uint32_t), d_prefix);
+  ZH_ALLOC(p, std::max<size_t>(n, 1) * sizeof(uint32_t), d_ids);
+  ZH_HIP(hipMemcpy(*d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  ZH_HIP(hipMemcpy(*d_ids, ids, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  BatchDesc b{};
+  b.n_stages = (uint32_t)n;
+  b.total_tiles = prefix.back();
+  b.counter = counter;
+  b.stages = p->d_stages;
+  b.stage_ids = *d_ids;
+  b.tile_prefix = *d_prefix;
+  ZH_HIP(hipMemcpy(*d_batch, &b, sizeof(b), hipMemcpyHostToDevice));
+  *total = b.total_tiles;
+  return ZH_OK;
+}
+
+// Unit ui, a pair of warp forms, as the two launches of its forms.
+static void split_pair(zh_program* p, size_t ui) {
+  const Unit u = p->units[ui];
+  Unit a, b;
+  a.ids.assign(u.ids.begin(), u.ids.begin() + u.split);
+  b.ids.assign(u.ids.begin() + u.split, u.ids.end());
+  for (Unit* x : {&a, &b}) {
+    x->batch = x->ids.size() >= 2;
+    x->warp = x->batch;
+  }
+  p->units[ui] = a;
+  p->units.insert(p->units.begin() + (ptrdiff_t)ui + 1, b);
 }
 
 static zh_status alloc_device(zh_program* p) {
@@ -600,6 +660,27 @@ static zh_status alloc_device(zh_program* p) {
     p->dyn_data.push_back(b);
   }
   make_units(p);
+  // The kernels of paired warp forms, compiled at run time; a pair whose
+  // kernel cannot be had launches its two forms one after the other.
+  for (size_t ui = 0; ui < p->units.size(); ++ui) {
+    Unit& u = p->units[ui];
+    if (!u.pair) continue;
+    WarpForm fa, fb;
+    std::string why;
+    hipFunction_t fn = nullptr;
+    const bool ok = warp_form_of(p->plan.stages[u.ids[0]].desc.variant, &fa) &&
+                    warp_form_of(p->plan.stages[u.ids[u.split]].desc.variant, &fb) &&
+                    jit_pair_kernel(jit_warp_pair(fa, fb), ctx->device, &fn, &why) == ZH_OK;
+    if (!ok) {
+      (void)hipGetLastError();
+      split_pair(p, ui);
+      continue;
+    }
+    int nb = 0;
+    u.pair_fn = fn;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kBlock, 0) == hipSuccess && nb > 0)
+      u.pair_resident = (uint32_t)nb;
+  }
   for (size_t s = 0; s < n; ++s) {
     zh_status st = tile_layer_masks(p, s);
     if (st != ZH_OK) return st;
@@ -618,33 +699,20 @@ static zh_status alloc_device(zh_program* p) {
   }
   for (Unit& u : p->units) {
     if (!u.batch) continue;
-    BatchDesc b{};
-    std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
-    for (size_t k = 0; k < u.ids.size(); ++k) {
-      const StageDesc& d = p->plan.stages[u.ids[k]].desc;
-      const uint64_t chunks = ((uint64_t)d.width * d.height + kSpecChunkPixels - 1) / kSpecChunkPixels;
-      prefix[k + 1] = prefix[k] + (u.spec ? (uint32_t)chunks : d.tiles_x * d.tiles_y);
-      u.lds = std::max(u.lds, p->plan.stages[u.ids[k]].lds_bytes);
-    }
-    ZH_ALLOC(p, sizeof(BatchDesc), &u.d_batch);
+    for (size_t k = 0; k < u.ids.size(); ++k) u.lds = std::max(u.lds, p->plan.stages[u.ids[k]].lds_bytes);
     ZH_ALLOC(p, sizeof(uint32_t), &u.d_counter);
-    ZH_ALLOC(p, prefix.size() * sizeof(uint32_t), &u.d_prefix);
-    ZH_ALLOC(p, u.ids.size() * sizeof(uint32_t), &u.d_ids);
-    ZH_HIP(hipMemcpy(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    ZH_HIP(hipMemcpy(u.d_ids, u.ids.data(), u.ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    b.n_stages = (uint32_t)u.ids.size();
-    b.total_tiles = prefix.back();
-    u.total_tiles = b.total_tiles;
-    b.counter = u.d_counter;
-    b.stages = p->d_stages;
-    b.stage_ids = u.d_ids;
-    b.tile_prefix = u.d_prefix;
-    ZH_HIP(hipMemcpy(u.d_batch, &b, sizeof(b), hipMemcpyHostToDevice));
+    const size_t na = u.pair ? u.split : u.ids.size();
+    zh_status st = alloc_batch(p, u.ids.data(), na, u.spec, u.d_counter, &u.d_batch, &u.d_prefix, &u.d_ids,
+                               &u.total_tiles);
+    if (st == ZH_OK && u.pair)
+      st = alloc_batch(p, u.ids.data() + na, u.ids.size() - na, false, u.d_counter, &u.d_batch2, &u.d_prefix2,
+                       &u.d_ids2, &u.total_tiles2);
+    if (st != ZH_OK) return st;
     // Persistent grid: enough workgroups to fill every CU at the kernel's
     // occupancy (2 per CU with a 64 KiB tile), capped by the work.
     hipDeviceProp_t prop;
     ZH_HIP(hipGetDeviceProperties(&prop, ctx->device));
-    u.grid = std::min<uint32_t>(b.total_tiles, (uint32_t)prop.multiProcessorCount * 2);
+    u.grid = std::min<uint32_t>(u.total_tiles, (uint32_t)prop.multiProcessorCount * 2);
     if (u.grid == 0) u.grid = 1;
   }
   return ZH_OK;
@@ -720,12 +788,25 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
     const Unit& u = p->units[i];
     const std::string& t = p->plan.stages[u.ids[0]].text;
     const size_t k = t.rfind("[kernel ");
-    o << "launch " << i << ": " << (k == std::string::npos ? std::string("stage ") + std::to_string(u.ids[0]) : t.substr(k))
-      << " over " << u.ids.size() << (u.ids.size() == 1 ? " stage" : " stages") << "\n";
+    std::string tag = k == std::string::npos ? std::string("stage ") + std::to_string(u.ids[0]) : t.substr(k);
+    if (u.pair) {
+      // "[kernel pair <first form> + <second form>]"
+      auto name = [&](uint32_t s) {
+        const std::string& x = p->plan.stages[s].text;
+        const size_t a = x.rfind("[kernel ");
+        const size_t b = a == std::string::npos ? a : x.find(']', a);
+        return b == std::string::npos ? std::string("?") : x.substr(a + 8, b - a - 8);
+      };
+      tag = "[kernel pair " + name(u.ids[0]) + " + " + name(u.ids[u.split]) + "]";
+    }
+    o << "launch " << i << ": " << tag << " over " << u.ids.size() << (u.ids.size() == 1 ? " stage" : " stages")
+      << "\n";
     if (p->units[i].batch) {
       o << "batch: stages ";
       for (size_t k = 0; k < p->units[i].ids.size(); ++k) o << (k ? "," : "") << p->units[i].ids[k];
-      o << (p->units[i].spec ? " as one launch\n" : " as one tile queue\n");
+      o << (p->units[i].spec ? " as one launch\n"
+                              : p->units[i].pair ? " as one grid dealing two tile queues alternately\n"
+                                                 : " as one tile queue\n");
     }
     if (p->units[i].blocks_per_cu)
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
@@ -1041,6 +1122,17 @@ static zh_status launch_unit(zh_program* p, const Unit& u) {
     }
     bool all_warp = u.warp;
     for (uint32_t id : u.ids) all_warp = all_warp && p->variant[id] == 3;
+    if (all_warp && u.pair) {
+      // Both forms' tiles in one persistent grid.
+      const uint32_t bpc = u.blocks_per_cu ? std::min(u.blocks_per_cu, u.pair_resident) : u.pair_resident;
+      const uint32_t blocks = std::min<uint32_t>(u.total_tiles + u.total_tiles2, bpc * (uint32_t)p->ctx->cus);
+      if (blocks == 0) return ZH_OK;
+      const BatchDesc* ba = u.d_batch;
+      const BatchDesc* bb = u.d_batch2;
+      void* args[] = {&ba, &bb};
+      ZH_HIP(hipModuleLaunchKernel(u.pair_fn, blocks, 1, 1, kBlock, 1, 1, 0, s, args, nullptr));
+      return ZH_OK;
+    }
     if (all_warp) {
       if (p->plan.stages[u.ids[0]].desc.variant >= kJitWarpBase) {
         const uint32_t first = u.ids[0], res = p->jit_batch_resident[first];
@@ -1381,7 +1473,7 @@ static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th)
   // The warp kernels split a tile's lanes by shifts: tw / 4 lanes per row,
   // a power of two, and 256 lanes per tile.
   if (tw < 4 || (tw & (tw - 1)) || tw * th != 1024) return fail(ZH_ERR_INVALID_ARGUMENT, "unsupported warp tile shape");
-  std::vector<uint32_t> prefix(u.ids.size() + 1, 0);
+  std::vector<uint32_t> tiles(u.ids.size(), 0);
   for (size_t k = 0; k < u.ids.size(); ++k) {
     const size_t s = u.ids[k];
     StagePlan& sp = p->plan.stages[s];
@@ -1406,17 +1498,29 @@ static zh_status set_warp_tile(zh_program* p, Unit& u, uint32_t tw, uint32_t th)
     h.tiles_x = d.tiles_x;
     h.tiles_y = d.tiles_y;
     h.tile_layers = d.tile_layers;
-    prefix[k + 1] = prefix[k] + d.tiles_x * d.tiles_y;
+    tiles[k] = d.tiles_x * d.tiles_y;
   }
   hipStream_t st = p->ctx->stream;
   for (uint32_t s : u.ids)
     ZH_HIP(hipMemcpyAsync(p->d_stages + s, p->h_stages + s, sizeof(StageDesc), hipMemcpyHostToDevice, st));
+  // The tile prefix of each queue (a pair's two forms have one each).
+  const size_t na = u.pair ? u.split : u.ids.size();
+  std::vector<uint32_t> first(na + 1, 0), second(u.ids.size() - na + 1, 0);
+  for (size_t k = 0; k < na; ++k) first[k + 1] = first[k] + tiles[k];
+  for (size_t k = na; k < u.ids.size(); ++k) second[k - na + 1] = second[k - na] + tiles[k];
   if (u.batch) {
-    ZH_HIP(hipMemcpyAsync(u.d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    u.total_tiles = prefix.back();
+    ZH_HIP(hipMemcpyAsync(u.d_prefix, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    u.total_tiles = first.back();
     ZH_HIP(hipMemcpyAsync((char*)u.d_batch + offsetof(BatchDesc, total_tiles), &u.total_tiles, sizeof(uint32_t),
                           hipMemcpyHostToDevice, st));
+    if (u.pair) {
+      ZH_HIP(hipMemcpyAsync(u.d_prefix2, second.data(), second.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+      u.total_tiles2 = second.back();
+      ZH_HIP(hipMemcpyAsync((char*)u.d_batch2 + offsetof(BatchDesc, total_tiles), &u.total_tiles2,
+                            sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
   }
+  // The host vectors and totals must outlive the copies.
   ZH_HIP(hipStreamSynchronize(st));
   return ZH_OK;
 }

@@ -106,4 +106,7 @@ int zh_warp_lookup(uint32_t src, uint32_t fmt, uint32_t mat, const uint16_t* pos
 const char* zh_warp_name(int index);
 // Does warp kernel `index` have an LDS-staged form (separable, one source)?
 int zh_warp_has_staged(int index);
+// The form of warp kernel `index` (source form, format, mat3 flag and post
+// signature; odd indices are the separable kernels); 0 for no such kernel.
+int zh_warp_form_of(int index, uint32_t* src, uint32_t* fmt, uint32_t* mat, const uint16_t** post, int* n);
 }

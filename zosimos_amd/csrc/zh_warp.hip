@@ -99,6 +99,17 @@ const char* zh_warp_name(int index) {
   return index >= 0 && index < zh::spec::kNumWarps ? zh::spec::kWarps[index].name : "";
 }
 
+int zh_warp_form_of(int index, uint32_t* src, uint32_t* fmt, uint32_t* mat, const uint16_t** post, int* n) {
+  if (index < 0 || index >= zh::spec::kNumWarps) return 0;
+  const auto& e = zh::spec::kWarps[index];
+  *src = e.src;
+  *fmt = e.fmt;
+  *mat = e.mat;
+  *post = e.post;
+  *n = e.n;
+  return 1;
+}
+
 int zh_warp_has_staged(int index) {
   return index >= 0 && index < zh::spec::kNumWarps && zh::spec::kWarps[index].staged != nullptr;
 }

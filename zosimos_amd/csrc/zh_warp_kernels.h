@@ -822,5 +822,67 @@ __device__ __forceinline__ void warp_batch(const BatchDesc* bb) {
   }
 }
 
+// Two warp forms in one persistent grid: the tiles of batch a (form VA) and
+// of batch b (form VB) are dealt alternately -- queue position 2 i is a's
+// tile i, 2 i + 1 is b's tile i -- while both have tiles, then the rest of
+// the larger batch in order.  Every compute unit then holds tiles of both
+// forms at once, so the vector work of one fills the other's memory waits.
+// Within each batch, tiles are dealt as warp_batch deals them (each
+// workgroup's tiles only move forward, so each batch's stage cursor only
+// walks forward).
+__device__ __forceinline__ uint32_t first_stage(CBatch b, uint32_t t) {
+  uint32_t lo = 0, hi = b->n_stages;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (cword(b->tile_prefix, mid) <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <class VA, class VB>
+__device__ __forceinline__ void warp_pair(const BatchDesc* ba, const BatchDesc* bb) {
+  const CBatch A = (CBatch)(uintptr_t)ba, B = (CBatch)(uintptr_t)bb;
+  __shared__ Luts luts;
+  if constexpr (needs_lut<VA>() || needs_lut<VB>()) {
+    load_lut(as_const(A->stages), &luts);
+    __syncthreads();
+  }
+  const uint32_t ta = A->total_tiles, tb = B->total_tiles;
+  const uint32_t m = min(ta, tb), total = ta + tb;
+  const uint32_t j0 = virtual_block();
+  if (j0 >= total) return;
+  // Queue position j -> (batch, tile of that batch).
+  auto which = [&](uint32_t j, uint32_t& t) {
+    if (j < 2 * m) {
+      t = j >> 1;
+      return (j & 1u) == 0u;
+    }
+    t = j - m;
+    return ta > tb;
+  };
+  // Both batches' tiles at queue positions j >= j0 are at least t_lb (j >> 1
+  // while both are dealt, j - m after): each cursor starts at that tile's
+  // stage.
+  const uint32_t t_lb = j0 < 2 * m ? j0 >> 1 : j0 - m;
+  uint32_t lo_a = first_stage(A, min(t_lb, ta ? ta - 1 : 0u));
+  uint32_t lo_b = first_stage(B, min(t_lb, tb ? tb - 1 : 0u));
+  for (uint32_t j = j0; j < total; j += gridDim.x) {
+    uint32_t t;
+    if (which(j, t)) {
+      while (cword(A->tile_prefix, lo_a + 1) <= t) ++lo_a;  // uniform
+      const CDesc d = as_const(A->stages + cword(A->stage_ids, lo_a));
+      uint32_t tx, ty;
+      tile_xy(d, t - cword(A->tile_prefix, lo_a), tx, ty);
+      warp_tile<VA>(d, tx, ty, &luts);
+    } else {
+      while (cword(B->tile_prefix, lo_b + 1) <= t) ++lo_b;  // uniform
+      const CDesc d = as_const(B->stages + cword(B->stage_ids, lo_b));
+      uint32_t tx, ty;
+      tile_xy(d, t - cword(B->tile_prefix, lo_b), tx, ty);
+      warp_tile<VB>(d, tx, ty, &luts);
+    }
+  }
+}
+
 }  // namespace spec
 }  // namespace zh
