@@ -347,7 +347,10 @@ __device__ __forceinline__ void layer_taps(const LayerArgs& a, const LaneMap& mp
       } else {
         float al = c.w;
         if constexpr (A8) al = (float)((PAIR ? pair_half<V::F>(p0[k][t >> 1], t & 1) : t0[k][t]) >> 24);
-        const float wa = w[k][t] * al;
+        // Half alphas: w * a as fma(a, w, +0), which compiles to one mixed-
+        // precision fma reading the f16 half directly (the same value but
+        // for the sign of a zero product).
+        const float wa = V::F == FMT_F16 ? __builtin_fmaf(al, w[k][t], 0.0f) : w[k][t] * al;
         s = make_float4(fmaf(c.x, wa, s.x), fmaf(c.y, wa, s.y), fmaf(c.z, wa, s.z), s.w + wa);
       }
     }
