@@ -138,6 +138,9 @@ def measure(wl, ctx, stream, torch, steps, warmup, traffic, flags=0):
                      "kernel": kernel[:160], "kernel_ms": round(dom_ms, 5), "launch_event_ms": round(unit_ms[dom], 5),
                      "algorithmic_bytes": dom_bytes},
         "graph": exe.graph_info()["state"],
+        # autotuned launch shapes (tile size, workgroups per CU, strip width)
+        "tuning": [l for l in text.splitlines()
+                   if l.startswith("launch ") and (": tiles" in l or "workgroups per CU" in l)],
         "wall_s": round(wall, 3),
     }
     del imgs, outs, keep, exe
@@ -330,6 +333,7 @@ def main():
         "hbm_frac": head["hbm_frac"],
         "roofline": head["roofline"],
         "graph": head["graph"],
+        "tuning": head["tuning"],
         "wall_s": head["wall_s"],
     }
     if args.config == "all":

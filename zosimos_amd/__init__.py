@@ -1290,8 +1290,9 @@ class SyncPoint:
         """``SyncPoint::debug_mark``: the first launch this step queued."""
         if self.launches == 0:
             return None
-        lines = [l for l in self.run.exe.describe().splitlines() if l.startswith("launch ")]
-        return lines[self._first] if self._first < len(lines) else None
+        head = f"launch {self._first}: "
+        lines = [l for l in self.run.exe.describe().splitlines() if l.startswith(head)]
+        return lines[0] if lines else None
 
 
 class Retire:

@@ -808,6 +808,8 @@ zh_status zh_program_describe(zh_program* p, char* buf, size_t len, size_t* need
                               : p->units[i].pair ? " as one grid dealing two tile queues alternately\n"
                                                  : " as one tile queue\n");
     }
+    const StageDesc& d0 = p->plan.stages[u.ids[0]].desc;
+    if (d0.kind == STAGE_RESAMPLE) o << "launch " << i << ": tiles " << d0.tile_w << "x" << d0.tile_h << "\n";
     if (p->units[i].blocks_per_cu)
       o << "launch " << i << ": " << p->units[i].blocks_per_cu << " workgroups per CU\n";
     if (p->units[i].strip_w)
@@ -1457,7 +1459,10 @@ zh_status zh_program_graph_info(zh_program* p, zh_graph_info* info) {
 
 // ---- autotuner ---------------------------------------------------------------
 // Tile shapes the warp kernels accept (tile_w / kPix * tile_h == kBlock).
-static const uint32_t kWarpTiles[][2] = {{64, 16}, {32, 32}, {128, 8}, {16, 64}};
+// The narrow ones put a workgroup's lanes down a column of rows: a round of
+// tiles in flight is then a short band of output rows across the image, whose
+// source rows every workgroup of the round shares.
+static const uint32_t kWarpTiles[][2] = {{64, 16}, {32, 32}, {128, 8}, {16, 64}, {8, 128}, {4, 256}};
 
 static bool warp_unit(const zh_program* p, const Unit& u) {
   for (uint32_t s : u.ids) {
