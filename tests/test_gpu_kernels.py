@@ -303,6 +303,9 @@ def test_autotune_keeps_parity(ctx):
     exe.autotune(imgs, outs, iterations=2)
     tiles = set(re.findall(r"tile (\d+x\d+)", exe.describe()))
     assert tiles and tiles <= {"64x16", "32x32", "128x8", "16x64"}, tiles
+    # each warp launch reports the shape its stages were tuned to
+    launch_tiles = set(re.findall(r"launch \d+: tiles (\d+x\d+)", exe.describe()))
+    assert launch_tiles and launch_tiles <= tiles, (launch_tiles, tiles)
     for _ in range(2):
         exe.launch(imgs, outs)
     ctx.sync()
